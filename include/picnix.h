@@ -1,0 +1,120 @@
+/*
+ * picnix.h -- C ABI of the MI355X (gfx950) implementation of pic-nix's
+ * per-timestep particle-in-cell hot path.
+ *
+ * One opaque handle owns every chunk of the domain (all on one GPU), the
+ * device buffers, one HIP stream and the options parsed from the same
+ * config.toml keys the reference reads.  Every call returns 0 on success or a
+ * negative PN_ERR_* code; pn_last_error() then describes the failure.  No C++
+ * exception crosses this boundary.  Host buffers are borrowed only for the
+ * duration of a call.  A handle is not re-entrant: calls on one handle must not
+ * overlap; the chunk loop lives inside each call.
+ *
+ * Each entry point names the reference interface it replaces.  The reference's
+ * PicChunk operators act on one chunk; these act on all chunks at once
+ * (chunk = -1), which is what PicApplication::push_openmp does in its
+ * `omp for` loops (pic/pic_application.cpp:219-292).
+ *
+ * Layouts at this boundary are the reference's:
+ *   particles : AoS [np][7] float64 (x, y, z, ux, uy, uz, id as int64 bits)
+ *               (nix/particle.hpp:19, nix/xtensor_particle.hpp:14-15)
+ *   uf        : [nz+2Nb][ny+2Nb][nx+2Nb][6] (Ex,Ey,Ez,Bx,By,Bz) per chunk
+ *   uj        : [nz+2Nb][ny+2Nb][nx+2Nb][4] (rho,Jx,Jy,Jz) per chunk
+ *               (pic/pic_chunk.cpp:106-122)
+ * Inside the library fields live in one global periodic array per component
+ * and particles in per-(chunk, species) SoA segments; see DESIGN.md.
+ */
+#ifndef PICNIX_H
+#define PICNIX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PN_OK 0
+#define PN_ERR_ARG -1        /* invalid argument                          */
+#define PN_ERR_CONFIG -2     /* invalid or unsupported configuration      */
+#define PN_ERR_HIP -3        /* HIP runtime error                         */
+#define PN_ERR_CAPACITY -4   /* particle segment capacity exceeded        */
+#define PN_ERR_UNSUPPORTED -5 /* option recognised but not implemented    */
+#define PN_ERR_STATE -6      /* call sequence invalid for current state   */
+
+/* boundary modes, pic/pic.hpp:40-46 */
+#define PN_BOUNDARY_EMF 0
+#define PN_BOUNDARY_CUR 1
+#define PN_BOUNDARY_MOM 2
+#define PN_BOUNDARY_PARTICLE 3
+
+/* field selectors for pn_get_fields / pn_set_fields */
+#define PN_FIELD_UF 0
+#define PN_FIELD_UJ 1
+
+typedef struct pn_sim pn_sim;
+
+/* Create a simulation from config.toml text (the reference's [application.option]
+ * and [parameter] keys, nix/cfgparser.hpp:145-217, pic/pic_chunk.cpp:135-262),
+ * allocate every chunk on the current HIP device and run the example loader
+ * (example/thermal/main.cpp:15-113, example/beam/main.cpp:15-117) followed by the
+ * initial sort (PicChunk::sort_particle, pic/pic_chunk.cpp:447-453) and the
+ * initial EMF exchange (PicApplication::setup_chunks, pic/pic_application.cpp:106-130).
+ * load = 0 skips the particle loader (empty chunks). */
+int pn_create(const char* toml, int load, pn_sim** out);
+/* Release every resource (reference: chunk destructors). */
+int pn_destroy(pn_sim* sim);
+/* Text of the last error on this handle ("" if none). */
+const char* pn_last_error(const pn_sim* sim);
+
+/* Geometry queries (nix::Chunk accessors, nix/chunk.hpp:96-352).
+ * info[0..2] chunk coord (cz,cy,cx); [3..5] cell offset (z,y,x);
+ * [6..8] ghost-inclusive extent (nz,ny,nx); [9] Nb; [10] Ng. */
+int pn_num_chunks(const pn_sim* sim, int* nchunk);
+int pn_num_species(const pn_sim* sim, int* ns);
+int pn_chunk_info(const pn_sim* sim, int chunk, int* info);
+
+/* Particles of one (chunk, species) as AoS [np][7] (XtensorParticle::xu,
+ * nix/xtensor_particle.hpp:14).  pn_get_particles writes at most cap records. */
+int pn_get_np(const pn_sim* sim, int chunk, int species, int64_t* np);
+int pn_get_particles(pn_sim* sim, int chunk, int species, double* xu7, int64_t cap,
+                     int64_t* np);
+int pn_set_particles(pn_sim* sim, int chunk, int species, const double* xu7, int64_t np);
+/* Old positions written by pn_push_position (XtensorParticle::xv, columns 0..2). */
+int pn_get_particles_old(pn_sim* sim, int chunk, int species, double* xv3, int64_t cap);
+/* Cell index of every particle from the last count (XtensorParticle::gindex,
+ * nix/xtensor_particle.hpp:16); out-of-chunk particles report Ng. */
+int pn_get_gindex(pn_sim* sim, int chunk, int species, int32_t* gindex, int64_t cap);
+/* First particle of every cell after the last sort (XtensorParticle::pindex, Ng+1 entries). */
+int pn_get_pindex(pn_sim* sim, int chunk, int species, int32_t* pindex);
+
+/* Ghost-inclusive per-chunk field arrays (PicChunk::uf / uj).  Ghost cells of
+ * uf read back as the periodic images held after an EMF exchange. */
+int pn_get_fields(pn_sim* sim, int chunk, int which, double* data);
+int pn_set_fields(pn_sim* sim, int chunk, int which, const double* data);
+
+/* Per-operator calls; chunk must be -1 (all chunks). */
+int pn_push_bfd(pn_sim* sim, int chunk, double delt);        /* PicChunk::push_bfd, pic/pic_chunk.cpp:544-550 */
+int pn_push_efd(pn_sim* sim, int chunk, double delt);        /* PicChunk::push_efd, pic/pic_chunk.cpp:536-542 */
+int pn_push_velocity(pn_sim* sim, int chunk, double delt);   /* PicChunk::push_velocity, pic/pic_chunk.cpp:501-512 */
+int pn_push_position(pn_sim* sim, int chunk, double delt);   /* PicChunk::push_position + count, pic/pic_chunk.cpp:491-499 */
+int pn_deposit_current(pn_sim* sim, int chunk, double delt); /* PicChunk::deposit_current, pic/pic_chunk.cpp:514-523 */
+int pn_sort_particle(pn_sim* sim, int chunk);                /* PicChunk::sort_particle, pic/pic_chunk.cpp:447-453 */
+/* Boundary exchange of one mode: pack+begin+end+unpack
+ * (PicChunk::set_boundary_{pack,begin,end,unpack}, pic/pic_chunk.cpp:287-405).
+ * PARTICLE migrates leavers to their neighbour chunk and sorts. */
+int pn_exchange(pn_sim* sim, int mode);
+/* One full timestep in the reference operator order (PicApplication::push_openmp,
+ * pic/pic_application.cpp:219-292). */
+int pn_step(pn_sim* sim, double delt);
+/* Field energy E^2/2, B^2/2 and per-species kinetic energy (PicChunk::get_energy,
+ * pic/pic_chunk.cpp:407-439, summed over chunks as HistoryDiag does). */
+int pn_get_energy(pn_sim* sim, double* efd, double* bfd, double* particle);
+/* Signed sums of div(E) - rho and div(B) (PicChunk::get_diverror, pic/pic_chunk.cpp:441-445). */
+int pn_get_diverror(pn_sim* sim, double* efd, double* bfd);
+/* Block until queued device work on the handle's stream is done. */
+int pn_synchronize(pn_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

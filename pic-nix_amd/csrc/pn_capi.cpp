@@ -1,0 +1,807 @@
+// pn_capi.cpp -- extern "C" entry points (include/picnix.h) and simulation set-up.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "../../include/picnix.h"
+#include "pn_sim.hpp"
+
+namespace picnix {
+std::vector<int> make_chunk_map(int Cz, int Cy, int Cx);
+}
+
+using namespace picnix;
+
+struct pn_sim {
+  Sim s;
+};
+
+namespace {
+
+struct HipError {
+  std::string msg;
+};
+
+void hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess)
+    throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+#define HIPCHK(call) hip_check((call), #call)
+
+// Run `f` and convert failures into an error code + message on the handle.
+template <typename F>
+int guarded(pn_sim* h, F f)
+{
+  if (!h)
+    return PN_ERR_ARG;
+  try {
+    int rc = f(h->s);
+    if (rc == PN_OK) {
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        h->s.err = std::string("HIP launch error: ") + hipGetErrorString(e);
+        return PN_ERR_HIP;
+      }
+    }
+    return rc;
+  } catch (const HipError& e) {
+    h->s.err = e.msg;
+    return PN_ERR_HIP;
+  } catch (const std::exception& e) {
+    h->s.err = e.what();
+    return PN_ERR_CONFIG;
+  }
+}
+
+int fail(Sim& s, int code, const std::string& msg)
+{
+  s.err = msg;
+  return code;
+}
+
+template <typename T>
+T* dalloc(size_t n)
+{
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+// ---------------------------------------------------------------------------
+// geometry + chunk tables (nix/application.cpp:245-309, nix/chunk.cpp:118-237)
+// ---------------------------------------------------------------------------
+void setup_geometry(Sim& s)
+{
+  const Config& cfg = s.cfg;
+  Geom&         g   = s.g;
+  for (int d = 0; d < 3; d++)
+    g.has[d] = !(cfg.N[d] == 1 && cfg.C[d] == 1);
+  g.dim   = g.has[0] ? 3 : 2;
+  g.order = cfg.order;
+  g.Nb    = (cfg.order + 3) / 2;
+  g.Ns    = cfg.Ns;
+  g.cc    = cfg.cc;
+  for (int d = 0; d < 3; d++) {
+    g.N[d]       = cfg.N[d];
+    g.C[d]       = cfg.C[d];
+    g.dims[d]    = cfg.N[d] / cfg.C[d];
+    g.pad[d]     = g.has[d] ? g.Nb : 0;
+    g.P[d]       = g.N[d] + 2 * g.pad[d];
+    g.del[d]     = cfg.delh;
+    g.glim[d][0] = 0.0;
+    g.glim[d][1] = g.N[d] * g.del[d];
+  }
+  g.plane  = (long)g.P[1] * g.P[2];
+  g.vol    = (long)g.P[0] * g.plane;
+  // flat bin strides, XtensorParticle::flatindex (nix/xtensor_particle.hpp:231-240);
+  // an absent dimension has Ub = Lb, so its stride factor is 2
+  int ex = g.has[2] ? g.dims[2] + 1 : 2;
+  int ey = g.has[1] ? g.dims[1] + 1 : 2;
+  int ez = g.has[0] ? g.dims[0] + 1 : 1;
+  g.bsy  = ex;
+  g.bsz  = ex * ey;
+  g.nbin = ex * ey * ez;
+  g.nchunk = cfg.C[0] * cfg.C[1] * cfg.C[2];
+  g.nseg   = g.nchunk * g.Ns;
+
+  s.id_of_coord = make_chunk_map(cfg.C[0], cfg.C[1], cfg.C[2]);
+  const int nc  = g.nchunk;
+  s.coord.assign(nc * 3, 0);
+  s.offset.assign(nc * 3, 0);
+  s.lim.assign(nc * 6, 0.0);
+  s.rng.assign(nc * 6, 0.0);
+  s.nb.assign(nc * 27, 0);
+  for (int cz = 0; cz < cfg.C[0]; cz++)
+    for (int cy = 0; cy < cfg.C[1]; cy++)
+      for (int cx = 0; cx < cfg.C[2]; cx++) {
+        int id             = s.id_of_coord[(cz * cfg.C[1] + cy) * cfg.C[2] + cx];
+        s.coord[id * 3 + 0] = cz;
+        s.coord[id * 3 + 1] = cy;
+        s.coord[id * 3 + 2] = cx;
+      }
+  for (int id = 0; id < nc; id++) {
+    for (int d = 0; d < 3; d++) {
+      int off              = s.coord[id * 3 + d] * cfg.N[d] / cfg.C[d];
+      s.offset[id * 3 + d] = off;
+      // nix/chunk.cpp:210-237 (set_coordinate)
+      double lo              = off * g.del[d];
+      double hi              = off * g.del[d] + g.dims[d] * g.del[d];
+      s.lim[id * 6 + 2 * d]  = lo;
+      s.lim[id * 6 + 2 * d + 1] = hi;
+      // get_{x,y,z}range: +-max for an absent dimension (nix/chunk.hpp:295-335)
+      s.rng[id * 6 + 2 * d]     = g.has[d] ? lo : -std::numeric_limits<double>::max();
+      s.rng[id * 6 + 2 * d + 1] = g.has[d] ? hi : +std::numeric_limits<double>::max();
+    }
+    // periodic neighbours (nix/chunkvector.hpp:56-81, nix/chunkmap.cpp:100-112)
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          int c3[3] = {s.coord[id * 3] + dz, s.coord[id * 3 + 1] + dy, s.coord[id * 3 + 2] + dx};
+          for (int d = 0; d < 3; d++) {
+            c3[d] = c3[d] >= 0 ? c3[d] : cfg.C[d] - 1;
+            c3[d] = c3[d] < cfg.C[d] ? c3[d] : 0;
+          }
+          s.nb[id * 27 + 9 * (dz + 1) + 3 * (dy + 1) + (dx + 1)] =
+              s.id_of_coord[(c3[0] * cfg.C[1] + c3[1]) * cfg.C[2] + c3[2]];
+        }
+  }
+  // species: example/thermal/main.cpp:86-88
+  int mp_max = 0;
+  for (int sp = 0; sp < g.Ns; sp++) {
+    const SpeciesConfig& sc = cfg.species[sp];
+    g.m[sp]                 = sc.ro / sc.np;
+    g.q[sp]                 = sc.qm * g.m[sp];
+    mp_max                  = std::max(mp_max, sc.np * g.dims[0] * g.dims[1] * g.dims[2]);
+  }
+  // capacity: round_up_alloc(mp * (1 + buffer_ratio)) (nix/particle.hpp:146-153)
+  int req = static_cast<int>(mp_max * (1 + cfg.buffer_ratio));
+  g.cap   = ((long)(req + 128) / 128) * 128;
+}
+
+void allocate(Sim& s)
+{
+  Geom&  g   = s.g;
+  size_t vol = (size_t)g.vol;
+  HIPCHK(hipStreamCreate(&s.stream));
+  s.d_E = dalloc<double>(3 * vol);
+  s.d_B = dalloc<double>(3 * vol);
+  s.d_J = dalloc<double>(4 * vol);
+  HIPCHK(hipMemset(s.d_E, 0, 3 * vol * sizeof(double)));
+  HIPCHK(hipMemset(s.d_B, 0, 3 * vol * sizeof(double)));
+  HIPCHK(hipMemset(s.d_J, 0, 4 * vol * sizeof(double)));
+
+  s.d_off = dalloc<int>(s.offset.size());
+  s.d_lim = dalloc<double>(s.lim.size());
+  s.d_rng = dalloc<double>(s.rng.size());
+  s.d_nb  = dalloc<int>(s.nb.size());
+  HIPCHK(hipMemcpy(s.d_off, s.offset.data(), s.offset.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s.d_lim, s.lim.data(), s.lim.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s.d_rng, s.rng.data(), s.rng.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s.d_nb, s.nb.data(), s.nb.size() * sizeof(int), hipMemcpyHostToDevice));
+
+  size_t np = (size_t)g.nseg * g.cap;
+  for (PArr* p : {&s.pa, &s.pb}) {
+    p->x  = dalloc<double>(np);
+    p->y  = dalloc<double>(np);
+    p->z  = dalloc<double>(np);
+    p->ux = dalloc<double>(np);
+    p->uy = dalloc<double>(np);
+    p->uz = dalloc<double>(np);
+    p->id = dalloc<int64_t>(np);
+  }
+  for (int d = 0; d < 3; d++)
+    s.d_xo[d] = dalloc<double>(np);
+  s.d_np = dalloc<int32_t>(g.nseg);
+  HIPCHK(hipMemset(s.d_np, 0, g.nseg * sizeof(int32_t)));
+  s.h_np.assign(g.nseg, 0);
+  s.np_host_valid = true;
+
+  SortScratch& ss = s.ss;
+  ss.ntile        = (int)((g.cap + 255) / 256);
+  ss.cellcnt      = dalloc<int32_t>((size_t)g.nseg * g.nbin);
+  ss.cellstart    = dalloc<int32_t>((size_t)g.nseg * (g.nbin + 1));
+  ss.key          = dalloc<int32_t>(np);
+  ss.slot         = dalloc<int32_t>(np);
+  ss.dseg         = dalloc<int32_t>(np);
+  ss.didx         = dalloc<int32_t>(np);
+  ss.mkey         = dalloc<int32_t>(np);
+  ss.tilecnt      = dalloc<int32_t>((size_t)g.nseg * ss.ntile * 27);
+  ss.nout         = dalloc<int32_t>((size_t)g.nseg * 27);
+  ss.recvoff      = dalloc<int32_t>((size_t)g.nseg * 27);
+  ss.npnew        = dalloc<int32_t>(g.nseg);
+  ss.errflag      = dalloc<int32_t>(4);
+  HIPCHK(hipMemset(ss.errflag, 0, 4 * sizeof(int32_t)));
+  HIPCHK(hipMemset(ss.cellstart, 0, (size_t)g.nseg * (g.nbin + 1) * sizeof(int32_t)));
+}
+
+void release(Sim& s)
+{
+  auto f = [](void* p) {
+    if (p)
+      (void)hipFree(p);
+  };
+  f(s.d_E);
+  f(s.d_B);
+  f(s.d_J);
+  f(s.d_off);
+  f(s.d_lim);
+  f(s.d_rng);
+  f(s.d_nb);
+  for (PArr* p : {&s.pa, &s.pb}) {
+    f(p->x);
+    f(p->y);
+    f(p->z);
+    f(p->ux);
+    f(p->uy);
+    f(p->uz);
+    f(p->id);
+  }
+  for (int d = 0; d < 3; d++)
+    f(s.d_xo[d]);
+  f(s.d_np);
+  SortScratch& ss = s.ss;
+  for (void* p : {(void*)ss.cellcnt, (void*)ss.cellstart, (void*)ss.key, (void*)ss.slot,
+                  (void*)ss.dseg, (void*)ss.didx, (void*)ss.mkey, (void*)ss.tilecnt,
+                  (void*)ss.nout, (void*)ss.recvoff, (void*)ss.npnew, (void*)ss.errflag})
+    f(p);
+  f(s.d_stage);
+  if (s.stream)
+    (void)hipStreamDestroy(s.stream);
+}
+
+void ensure_stage(Sim& s, size_t elems)
+{
+  if (elems <= s.stage_elems)
+    return;
+  if (s.d_stage)
+    HIPCHK(hipFree(s.d_stage));
+  s.d_stage     = dalloc<double>(elems);
+  s.stage_elems = elems;
+}
+
+void refresh_np(Sim& s)
+{
+  if (s.np_host_valid)
+    return;
+  HIPCHK(hipMemcpyAsync(s.h_np.data(), s.d_np, sizeof(int32_t) * s.g.nseg, hipMemcpyDeviceToHost,
+                        s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  s.np_host_valid = true;
+}
+
+int check_device_flags(Sim& s)
+{
+  int32_t flags[4];
+  HIPCHK(hipMemcpyAsync(flags, s.ss.errflag, sizeof(flags), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  if (flags[0])
+    return fail(s, PN_ERR_CAPACITY,
+                "particle segment capacity exceeded (raise [application.option] buffer_ratio)");
+  return PN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// loader: example/beam/main.cpp:67-116 (example/thermal is the zero-drift case),
+// with the reference's own generators: std::mt19937_64 seeded with the chunk id
+// (seed_type = 'fixed', pic/pic_chunk.cpp:242-255), uniform_real_distribution
+// and normal_distribution from libstdc++, mtp re-seeded per species.
+// ---------------------------------------------------------------------------
+void load_particles(Sim& s)
+{
+  const Geom& g   = s.g;
+  const int   nc  = g.nchunk;
+  size_t      tot = (size_t)g.nseg * g.cap;
+  std::vector<double>  hx(tot), hy(tot), hz(tot), hux(tot), huy(tot), huz(tot);
+  std::vector<int64_t> hid(tot);
+  std::vector<int32_t> hnp(g.nseg, 0);
+  std::vector<uint32_t> seeds(nc);
+  {
+    std::random_device rd;
+    for (int c = 0; c < nc; c++)
+      seeds[c] = s.cfg.seed_type == "fixed" ? (uint32_t)(c + s.cfg.seed_offset) : rd();
+  }
+  auto work = [&](int c) {
+    int                                    seed = (int)seeds[c];
+    std::mt19937_64                        mtp(seed);
+    std::mt19937_64                        mtv(seed);
+    std::uniform_real_distribution<double> uniform(0.0, 1.0);
+    std::normal_distribution<double>       normal(0.0, 1.0);
+    const double* lim = &s.lim[c * 6];
+    for (int sp = 0; sp < g.Ns; sp++) {
+      const SpeciesConfig& sc  = s.cfg.species[sp];
+      int                  mp  = sc.np * g.dims[0] * g.dims[1] * g.dims[2];
+      int64_t              id  = (int64_t)mp * c;
+      size_t               seg = (size_t)c * g.Ns + sp;
+      size_t               b   = seg * g.cap;
+      mtp.seed(seed);
+      for (int ip = 0; ip < mp; ip++) {
+        hx[b + ip]  = uniform(mtp) * (lim[5] - lim[4]) + lim[4];
+        hy[b + ip]  = uniform(mtp) * (lim[3] - lim[2]) + lim[2];
+        hz[b + ip]  = uniform(mtp) * (lim[1] - lim[0]) + lim[0];
+        hux[b + ip] = normal(mtv) * sc.vt + sc.vx;
+        huy[b + ip] = normal(mtv) * sc.vt + sc.vy;
+        huz[b + ip] = normal(mtv) * sc.vt + sc.vz;
+        hid[b + ip] = id + ip;
+      }
+      hnp[seg] = mp;
+    }
+  };
+  unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nthr; t++)
+    pool.emplace_back([&, t] {
+      for (int c = (int)t; c < nc; c += (int)nthr)
+        work(c);
+    });
+  for (auto& th : pool)
+    th.join();
+  auto up = [&](void* d, const void* h, size_t bytes) {
+    HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+  };
+  up(s.pa.x, hx.data(), tot * 8);
+  up(s.pa.y, hy.data(), tot * 8);
+  up(s.pa.z, hz.data(), tot * 8);
+  up(s.pa.ux, hux.data(), tot * 8);
+  up(s.pa.uy, huy.data(), tot * 8);
+  up(s.pa.uz, huz.data(), tot * 8);
+  up(s.pa.id, hid.data(), tot * 8);
+  up(s.d_np, hnp.data(), g.nseg * sizeof(int32_t));
+  s.h_np          = hnp;
+  s.np_host_valid = true;
+}
+
+void init_fields(Sim& s)
+{
+  const Geom&         g = s.g;
+  std::vector<double> h(3 * (size_t)g.vol, 0.0);
+  for (int which = 0; which < 2; which++) {
+    const double* v = which == 0 ? s.cfg.E0 : s.cfg.B0;
+    std::fill(h.begin(), h.end(), 0.0);
+    for (int z = g.pad[0]; z < g.pad[0] + g.N[0]; z++)
+      for (int y = g.pad[1]; y < g.pad[1] + g.N[1]; y++)
+        for (int x = g.pad[2]; x < g.pad[2] + g.N[2]; x++)
+          for (int k = 0; k < 3; k++)
+            h[k * g.vol + ((long)z * g.P[1] + y) * g.P[2] + x] = v[k];
+    HIPCHK(hipMemcpy(which == 0 ? s.d_E : s.d_B, h.data(), h.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+  }
+  launch_fill_ghost(s, s.d_E, 3);
+  launch_fill_ghost(s, s.d_B, 3);
+}
+
+int check_chunk_arg(Sim& s, int chunk)
+{
+  if (chunk != -1)
+    return fail(s, PN_ERR_UNSUPPORTED, "per-chunk operator calls are not supported; pass chunk = -1");
+  return PN_OK;
+}
+
+int check_seg(Sim& s, int chunk, int species)
+{
+  if (chunk < 0 || chunk >= s.g.nchunk || species < 0 || species >= s.g.Ns)
+    return fail(s, PN_ERR_ARG, "chunk or species index out of range");
+  return PN_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pn_create(const char* toml, int load, pn_sim** out)
+{
+  if (!toml || !out)
+    return PN_ERR_ARG;
+  *out   = nullptr;
+  auto h = new pn_sim();
+  Sim& s = h->s;
+  try {
+    TomlMap kv;
+    if (!parse_toml(toml, kv, s.err) || !build_config(kv, s.cfg, s.err)) {
+      // keep the handle so that pn_last_error can report the reason
+      *out = h;
+      return PN_ERR_CONFIG;
+    }
+    if (s.cfg.friedman != 0.0 || s.cfg.pusher != 0 || s.cfg.interp != 0 ||
+        s.cfg.precision != "fp64" || s.cfg.Ns > kMaxSpecies) {
+      s.err = "configuration uses an option this build does not implement yet "
+              "(supported: friedman = 0, pusher = Boris, interpolation = MC, precision = fp64, "
+              "Ns <= 8)";
+      *out  = h;
+      return PN_ERR_UNSUPPORTED;
+    }
+    setup_geometry(s);
+    allocate(s);
+    init_fields(s);
+    if (load) {
+      load_particles(s);
+      // initial sort, pic/pic_chunk.cpp:447-453 (count + sort per species)
+      launch_sort(s);
+    }
+    HIPCHK(hipStreamSynchronize(s.stream));
+    HIPCHK(hipGetLastError());
+  } catch (const HipError& e) {
+    s.err = e.msg;
+    *out  = h;
+    return PN_ERR_HIP;
+  }
+  *out = h;
+  return PN_OK;
+}
+
+int pn_destroy(pn_sim* h)
+{
+  if (!h)
+    return PN_ERR_ARG;
+  release(h->s);
+  delete h;
+  return PN_OK;
+}
+
+const char* pn_last_error(const pn_sim* h) { return h ? h->s.err.c_str() : "null handle"; }
+
+int pn_num_chunks(const pn_sim* h, int* n)
+{
+  if (!h || !n)
+    return PN_ERR_ARG;
+  *n = h->s.g.nchunk;
+  return PN_OK;
+}
+
+int pn_num_species(const pn_sim* h, int* ns)
+{
+  if (!h || !ns)
+    return PN_ERR_ARG;
+  *ns = h->s.g.Ns;
+  return PN_OK;
+}
+
+int pn_chunk_info(const pn_sim* h, int chunk, int* info)
+{
+  if (!h || !info || chunk < 0 || chunk >= h->s.g.nchunk)
+    return PN_ERR_ARG;
+  const Sim& s = h->s;
+  const Geom& g = s.g;
+  int ng = 1;
+  for (int d = 0; d < 3; d++) {
+    info[d]     = s.coord[chunk * 3 + d];
+    info[3 + d] = s.offset[chunk * 3 + d];
+    info[6 + d] = g.dims[d] + 2 * g.Nb;
+    ng *= g.dims[d] + 2 * g.Nb;
+  }
+  info[9]  = g.Nb;
+  info[10] = ng;
+  return PN_OK;
+}
+
+int pn_get_np(const pn_sim* hc, int chunk, int species, int64_t* np)
+{
+  pn_sim* h = const_cast<pn_sim*>(hc);
+  return guarded(h, [&](Sim& s) {
+    if (!np)
+      return PN_ERR_ARG;
+    int rc = check_seg(s, chunk, species);
+    if (rc)
+      return rc;
+    refresh_np(s);
+    *np = s.h_np[chunk * s.g.Ns + species];
+    return PN_OK;
+  });
+}
+
+int pn_get_particles(pn_sim* h, int chunk, int species, double* xu7, int64_t cap, int64_t* np)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_seg(s, chunk, species);
+    if (rc)
+      return rc;
+    if ((rc = check_device_flags(s)))
+      return rc;
+    refresh_np(s);
+    int     seg = chunk * s.g.Ns + species;
+    int64_t n   = s.h_np[seg];
+    if (np)
+      *np = n;
+    if (!xu7 || n == 0)
+      return PN_OK;
+    if (cap < n)
+      return fail(s, PN_ERR_ARG, "output buffer smaller than the particle count");
+    ensure_stage(s, (size_t)n * 7);
+    launch_soa_to_aos(s, seg, n);
+    HIPCHK(hipMemcpyAsync(xu7, s.d_stage, n * 7 * sizeof(double), hipMemcpyDeviceToHost,
+                          s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return PN_OK;
+  });
+}
+
+int pn_set_particles(pn_sim* h, int chunk, int species, const double* xu7, int64_t np)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_seg(s, chunk, species);
+    if (rc)
+      return rc;
+    if (np < 0 || np > s.g.cap || (np > 0 && !xu7))
+      return fail(s, PN_ERR_CAPACITY, "particle count exceeds the segment capacity");
+    refresh_np(s);
+    int seg = chunk * s.g.Ns + species;
+    if (np > 0) {
+      ensure_stage(s, (size_t)np * 7);
+      HIPCHK(hipMemcpyAsync(s.d_stage, xu7, np * 7 * sizeof(double), hipMemcpyHostToDevice,
+                            s.stream));
+      launch_aos_to_soa(s, seg, np);
+    }
+    s.h_np[seg] = (int32_t)np;
+    HIPCHK(hipMemcpyAsync(s.d_np + seg, &s.h_np[seg], sizeof(int32_t), hipMemcpyHostToDevice,
+                          s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    s.sorted  = false;
+    s.counted = false;
+    return PN_OK;
+  });
+}
+
+int pn_get_particles_old(pn_sim* h, int chunk, int species, double* xv3, int64_t cap)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_seg(s, chunk, species);
+    if (rc)
+      return rc;
+    refresh_np(s);
+    int     seg = chunk * s.g.Ns + species;
+    int64_t n   = s.h_np[seg];
+    if (!xv3 || cap < n)
+      return fail(s, PN_ERR_ARG, "output buffer smaller than the particle count");
+    std::vector<double> t(n);
+    for (int d = 0; d < 3; d++) {
+      HIPCHK(hipMemcpy(t.data(), s.d_xo[d] + (size_t)seg * s.g.cap, n * sizeof(double),
+                       hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; i++)
+        xv3[i * 3 + d] = t[i];
+    }
+    return PN_OK;
+  });
+}
+
+int pn_get_gindex(pn_sim* h, int chunk, int species, int32_t* gindex, int64_t cap)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_seg(s, chunk, species);
+    if (rc)
+      return rc;
+    if (!s.counted)
+      return fail(s, PN_ERR_STATE, "no fresh count: call pn_push_position first");
+    refresh_np(s);
+    int     seg = chunk * s.g.Ns + species;
+    int64_t n   = s.h_np[seg];
+    if (!gindex || cap < n)
+      return fail(s, PN_ERR_ARG, "output buffer smaller than the particle count");
+    HIPCHK(hipMemcpy(gindex, s.ss.key + (size_t)seg * s.g.cap, n * sizeof(int32_t),
+                     hipMemcpyDeviceToHost));
+    int ng = (s.g.dims[0] + 2 * s.g.Nb) * (s.g.dims[1] + 2 * s.g.Nb) * (s.g.dims[2] + 2 * s.g.Nb);
+    for (int64_t i = 0; i < n; i++)
+      if (gindex[i] < 0)
+        gindex[i] = ng;
+    return PN_OK;
+  });
+}
+
+int pn_get_pindex(pn_sim* h, int chunk, int species, int32_t* pindex)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_seg(s, chunk, species);
+    if (rc)
+      return rc;
+    if (!s.sorted)
+      return fail(s, PN_ERR_STATE, "particles are not sorted");
+    if (!pindex)
+      return PN_ERR_ARG;
+    int                  seg = chunk * s.g.Ns + species;
+    std::vector<int32_t> cs(s.g.nbin + 1);
+    HIPCHK(hipMemcpy(cs.data(), s.ss.cellstart + (size_t)seg * (s.g.nbin + 1),
+                     cs.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int ng = (s.g.dims[0] + 2 * s.g.Nb) * (s.g.dims[1] + 2 * s.g.Nb) * (s.g.dims[2] + 2 * s.g.Nb);
+    for (int i = 0; i <= ng; i++)
+      pindex[i] = i < s.g.nbin ? cs[i] : cs[s.g.nbin];
+    return PN_OK;
+  });
+}
+
+int pn_get_fields(pn_sim* h, int chunk, int which, double* data)
+{
+  return guarded(h, [&](Sim& s) {
+    if (chunk < 0 || chunk >= s.g.nchunk || !data || (which != PN_FIELD_UF && which != PN_FIELD_UJ))
+      return fail(s, PN_ERR_ARG, "invalid chunk, field selector or buffer");
+    const Geom& g  = s.g;
+    size_t      nn = (size_t)(g.dims[0] + 2 * g.Nb) * (g.dims[1] + 2 * g.Nb) * (g.dims[2] + 2 * g.Nb);
+    int         nc = which == PN_FIELD_UF ? 6 : 4;
+    ensure_stage(s, nn * nc);
+    launch_chunk_fields_get(s, chunk, which, s.d_stage);
+    HIPCHK(hipMemcpyAsync(data, s.d_stage, nn * nc * sizeof(double), hipMemcpyDeviceToHost,
+                          s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return PN_OK;
+  });
+}
+
+int pn_set_fields(pn_sim* h, int chunk, int which, const double* data)
+{
+  return guarded(h, [&](Sim& s) {
+    if (chunk < 0 || chunk >= s.g.nchunk || !data || (which != PN_FIELD_UF && which != PN_FIELD_UJ))
+      return fail(s, PN_ERR_ARG, "invalid chunk, field selector or buffer");
+    const Geom& g  = s.g;
+    size_t      nn = (size_t)(g.dims[0] + 2 * g.Nb) * (g.dims[1] + 2 * g.Nb) * (g.dims[2] + 2 * g.Nb);
+    int         nc = which == PN_FIELD_UF ? 6 : 4;
+    ensure_stage(s, nn * nc);
+    HIPCHK(hipMemcpyAsync(s.d_stage, data, nn * nc * sizeof(double), hipMemcpyHostToDevice,
+                          s.stream));
+    launch_chunk_fields_set(s, chunk, which, s.d_stage);
+    if (which == PN_FIELD_UF) {
+      launch_fill_ghost(s, s.d_E, 3);
+      launch_fill_ghost(s, s.d_B, 3);
+    }
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return PN_OK;
+  });
+}
+
+int pn_push_bfd(pn_sim* h, int chunk, double delt)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_chunk_arg(s, chunk);
+    if (rc)
+      return rc;
+    launch_push_bfd(s, delt);
+    return PN_OK;
+  });
+}
+
+int pn_push_efd(pn_sim* h, int chunk, double delt)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_chunk_arg(s, chunk);
+    if (rc)
+      return rc;
+    launch_push_efd(s, delt);
+    return PN_OK;
+  });
+}
+
+int pn_push_velocity(pn_sim* h, int chunk, double delt)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_chunk_arg(s, chunk);
+    if (rc)
+      return rc;
+    launch_push_velocity(s, delt);
+    return PN_OK;
+  });
+}
+
+int pn_push_position(pn_sim* h, int chunk, double delt)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_chunk_arg(s, chunk);
+    if (rc)
+      return rc;
+    bool was_sorted = s.sorted;
+    launch_push_position(s, delt);
+    // the particle order is unchanged, so cellstart still describes the old
+    // positions the deposit walks
+    s.sorted = was_sorted;
+    return PN_OK;
+  });
+}
+
+int pn_deposit_current(pn_sim* h, int chunk, double delt)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_chunk_arg(s, chunk);
+    if (rc)
+      return rc;
+    launch_deposit(s, delt);
+    return PN_OK;
+  });
+}
+
+int pn_sort_particle(pn_sim* h, int chunk)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_chunk_arg(s, chunk);
+    if (rc)
+      return rc;
+    launch_sort(s);
+    return PN_OK;
+  });
+}
+
+int pn_exchange(pn_sim* h, int mode)
+{
+  return guarded(h, [&](Sim& s) {
+    switch (mode) {
+    case PN_BOUNDARY_EMF:
+      launch_fill_ghost(s, s.d_E, 3);
+      launch_fill_ghost(s, s.d_B, 3);
+      return PN_OK;
+    case PN_BOUNDARY_CUR:
+      launch_fold_current(s);
+      return PN_OK;
+    case PN_BOUNDARY_PARTICLE:
+      if (!s.counted)
+        return fail(s, PN_ERR_STATE, "particle exchange needs the count of pn_push_position");
+      launch_migrate(s);
+      return PN_OK;
+    case PN_BOUNDARY_MOM:
+      return fail(s, PN_ERR_UNSUPPORTED, "moment exchange is not implemented");
+    default:
+      return fail(s, PN_ERR_ARG, "invalid boundary mode");
+    }
+  });
+}
+
+// PicApplication::push_openmp (pic/pic_application.cpp:219-292): per chunk
+// bfd(dt/2), velocity, position(+count), current, bfd(dt/2); then the current
+// fold, efd, particle migration + sort, EMF exchange (done by the ghost refresh
+// that follows every field update).
+int pn_step(pn_sim* h, double delt)
+{
+  return guarded(h, [&](Sim& s) {
+    launch_push_bfd(s, 0.5 * delt);
+    launch_push_velocity(s, delt);
+    bool was_sorted = s.sorted;
+    launch_push_position(s, delt);
+    s.sorted = was_sorted;
+    launch_deposit(s, delt);
+    launch_push_bfd(s, 0.5 * delt);
+    launch_fold_current(s);
+    launch_push_efd(s, delt);
+    launch_migrate(s);
+    return PN_OK;
+  });
+}
+
+int pn_get_energy(pn_sim* h, double* efd, double* bfd, double* particle)
+{
+  return guarded(h, [&](Sim& s) {
+    std::vector<double> out(4 + s.g.Ns);
+    launch_energy(s, out.data());
+    if (efd)
+      *efd = out[0];
+    if (bfd)
+      *bfd = out[1];
+    if (particle)
+      for (int sp = 0; sp < s.g.Ns; sp++)
+        particle[sp] = out[4 + sp];
+    return check_device_flags(s);
+  });
+}
+
+int pn_get_diverror(pn_sim* h, double* efd, double* bfd)
+{
+  return guarded(h, [&](Sim& s) {
+    std::vector<double> out(4 + s.g.Ns);
+    launch_energy(s, out.data());
+    if (efd)
+      *efd = out[2];
+    if (bfd)
+      *bfd = out[3];
+    return PN_OK;
+  });
+}
+
+int pn_synchronize(pn_sim* h)
+{
+  return guarded(h, [&](Sim& s) {
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return check_device_flags(s);
+  });
+}
+
+} // extern "C"

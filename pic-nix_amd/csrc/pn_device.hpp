@@ -1,0 +1,142 @@
+// pn_device.hpp -- device-side numerics shared by the gfx950 kernels.
+//
+// Every routine evaluates the same expression, in the same order, as the
+// reference routine cited next to it.  The FP64 kernels are compiled with
+// -ffp-contract=off, so each operation is one IEEE rounding and the results are
+// bit-identical to the reference algorithm evaluated without contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace picnix {
+
+constexpr int kSimdWidth = 8; // lane sub-bins of the counting sort (nix/nix.hpp:102-105)
+constexpr int kMaxSpecies = 8;
+
+// nix/primitives.hpp:46-58
+template <typename T>
+__device__ __forceinline__ int digitize(T x, T xmin, T rdx)
+{
+  return static_cast<int>(floor((x - xmin) * rdx));
+}
+
+// nix/primitives.hpp:157-161
+template <typename T>
+__device__ __forceinline__ T lorentz_factor(T ux, T uy, T uz, T rc)
+{
+  return sqrt(T(1) + (ux * ux + uy * uy + uz * uz) * rc * rc);
+}
+
+// nix/primitives.hpp:163-189
+template <typename T>
+__device__ __forceinline__ void push_boris(T& ux, T& uy, T& uz, T ex, T ey, T ez, T bx, T by,
+                                           T bz, T cc)
+{
+  ux += ex;
+  uy += ey;
+  uz += ez;
+  T gm = T(1) / sqrt(cc * cc + ux * ux + uy * uy + uz * uz);
+  bx *= gm;
+  by *= gm;
+  bz *= gm;
+  T bb = T(2.0) / (T(1.0) + bx * bx + by * by + bz * bz);
+  T vx = ux + (uy * bz - uz * by);
+  T vy = uy + (uz * bx - ux * bz);
+  T vz = uz + (ux * by - uy * bx);
+  ux += (vy * bz - vz * by) * bb + ex;
+  uy += (vz * bx - vx * bz) * bb + ey;
+  uz += (vx * by - vy * bx) * bb + ez;
+}
+
+// nix/primitives.hpp:256-328 (shape_mc1..4)
+template <int Order, typename T>
+__device__ __forceinline__ void shape_mc(T x, T X, T rdx, T* s)
+{
+  T delta = (x - X) * rdx;
+  if constexpr (Order == 1) {
+    s[0] = T(1) - delta;
+    s[1] = delta;
+  } else if constexpr (Order == 2) {
+    T w0 = delta;
+    T w1 = T(0.5) - w0;
+    T w2 = T(0.5) + w0;
+    s[0] = T(0.50) * w1 * w1;
+    s[1] = T(0.75) - w0 * w0;
+    s[2] = T(0.50) * w2 * w2;
+  } else if constexpr (Order == 3) {
+    const T a  = T(1) / T(6.0);
+    T       w1 = delta;
+    T       w2 = T(1) - delta;
+    T       p1 = w1 * w1;
+    T       p2 = w2 * w2;
+    T       q1 = p1 * w1;
+    T       q2 = p2 * w2;
+    s[0]       = a * q2;
+    s[1]       = a * (T(4) - T(6) * p1 + T(3) * q1);
+    s[2]       = a * (T(4) - T(6) * p2 + T(3) * q2);
+    s[3]       = a * q1;
+  } else {
+    const T a  = T(1) / T(384.0);
+    const T b  = T(1) / T(96.0);
+    const T c  = T(115) / T(192.0);
+    const T d  = T(1) / T(8.0);
+    T       w1 = T(1) + delta;
+    T       w2 = T(1) - delta;
+    T       w3 = T(1) + delta * T(2);
+    T       w4 = T(1) - delta * T(2);
+    T       p0 = delta * delta;
+    T       p1 = w1 * w1;
+    T       p2 = w2 * w2;
+    T       q1 = p1 * w1;
+    T       q2 = p2 * w2;
+    T       r1 = q1 * w1;
+    T       r2 = q2 * w2;
+    T       r3 = w3 * w3 * w3 * w3;
+    T       r4 = w4 * w4 * w4 * w4;
+    s[0]       = a * r4;
+    s[1]       = b * (T(55) + T(20) * w1 - T(120) * p1 + T(80) * q1 - T(16) * r1);
+    s[2]       = c + d * p0 * (T(2) * p0 - T(5));
+    s[3]       = b * (T(55) + T(20) * w2 - T(120) * p2 + T(80) * q2 - T(16) * r2);
+    s[4]       = a * r3;
+  }
+}
+
+// Device view of the domain: global padded periodic grid + chunk tables.
+struct Geom {
+  int    dim;      // 2 or 3
+  int    order;    // shape order
+  int    Nb;       // ghost margin (order + 3) / 2
+  int    Ns;       // species
+  int    N[3];     // global interior cells (z, y, x)
+  int    C[3];     // chunk counts (z, y, x)
+  int    dims[3];  // chunk interior cells (z, y, x)
+  int    pad[3];   // padding of the global arrays (Nb, or 0 for an absent z)
+  int    P[3];     // padded extents (z, y, x)
+  long   plane;    // P[1] * P[2]
+  long   vol;      // P[0] * P[1] * P[2]
+  double del[3];   // cell sizes (z, y, x)
+  double glim[3][2];
+  double cc;
+  int    nbin;     // sort bins per chunk ((nz+1)(ny+1)(nx+1), or 2D analogue)
+  int    bsy, bsz; // flat-bin strides (nix/xtensor_particle.hpp:231-240)
+  int    nchunk;
+  int    nseg;     // nchunk * Ns
+  long   cap;      // particle capacity of one (chunk, species) segment
+  double q[kMaxSpecies];
+  double m[kMaxSpecies];
+  bool   has[3];
+};
+
+// Structure-of-arrays particle storage; segment s occupies [s * cap, (s + 1) * cap).
+struct PArr {
+  double*  x;
+  double*  y;
+  double*  z;
+  double*  ux;
+  double*  uy;
+  double*  uz;
+  int64_t* id;
+};
+
+} // namespace picnix

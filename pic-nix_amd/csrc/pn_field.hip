@@ -1,0 +1,202 @@
+// pn_field.hip -- Yee FDTD updates, periodic ghost refresh and current fold on
+// the global padded grid.
+//
+// The reference updates every chunk's ghost-inclusive block and then copies
+// neighbour edges into ghosts (pic/engine/maxwell.hpp:383-496, EMF halo
+// nix/xtensor_halo3d.hpp:18-71).  Here the whole domain is one periodic array
+// with an Nb-wide padding that mirrors the opposite edge, so every interior
+// cell sees the same neighbour values as in the reference and gets the same
+// floating-point result; the chunk edges need no copies at all.
+#include "pn_sim.hpp"
+
+namespace picnix {
+
+namespace {
+
+__device__ __forceinline__ long gidx(const Geom& g, int z, int y, int x)
+{
+  return ((long)z * g.P[1] + y) * g.P[2] + x;
+}
+
+// B -= c dt curl E with Friedman theta = 0 (ff(...,0,k) == E exactly):
+// pic/engine/maxwell.hpp:442-496 (3D) and :292-348 (2D).  One thread per interior cell.
+template <int Dim>
+__global__ void __launch_bounds__(256) k_push_bfd(Geom g, const double* __restrict__ E,
+                                                  double* __restrict__ B, double cflx,
+                                                  double cfly, double cflz)
+{
+  long n   = (long)g.N[0] * g.N[1] * g.N[2];
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n)
+    return;
+  int  x  = (int)(tid % g.N[2]) + g.pad[2];
+  long r  = tid / g.N[2];
+  int  y  = (int)(r % g.N[1]) + g.pad[1];
+  int  z  = (int)(r / g.N[1]) + g.pad[0];
+  long i  = gidx(g, z, y, x);
+  long sx = 1, sy = g.P[2], sz = g.plane, v = g.vol;
+  const double* Ex = E;
+  const double* Ey = E + v;
+  const double* Ez = E + 2 * v;
+  if constexpr (Dim == 3) {
+    B[i] += (-cfly) * (Ez[i] - Ez[i - sy]) + (+cflz) * (Ey[i] - Ey[i - sz]);
+    B[v + i] += (-cflz) * (Ex[i] - Ex[i - sz]) + (+cflx) * (Ez[i] - Ez[i - sx]);
+    B[2 * v + i] += (-cflx) * (Ey[i] - Ey[i - sx]) + (+cfly) * (Ex[i] - Ex[i - sy]);
+  } else {
+    B[i] += (-cfly) * (Ez[i] - Ez[i - sy]);
+    B[v + i] += (+cflx) * (Ez[i] - Ez[i - sx]);
+    B[2 * v + i] += (-cflx) * (Ey[i] - Ey[i - sx]) + (+cfly) * (Ex[i] - Ex[i - sy]);
+  }
+}
+
+// E += c dt curl B - dt J: pic/engine/maxwell.hpp:383-440 (3D) and :231-290 (2D).
+template <int Dim>
+__global__ void __launch_bounds__(256) k_push_efd(Geom g, double* __restrict__ E,
+                                                  const double* __restrict__ B,
+                                                  const double* __restrict__ J, double cflx,
+                                                  double cfly, double cflz, double delt)
+{
+  long n   = (long)g.N[0] * g.N[1] * g.N[2];
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n)
+    return;
+  int  x  = (int)(tid % g.N[2]) + g.pad[2];
+  long r  = tid / g.N[2];
+  int  y  = (int)(r % g.N[1]) + g.pad[1];
+  int  z  = (int)(r / g.N[1]) + g.pad[0];
+  long i  = gidx(g, z, y, x);
+  long sx = 1, sy = g.P[2], sz = g.plane, v = g.vol;
+  const double* Bx = B;
+  const double* By = B + v;
+  const double* Bz = B + 2 * v;
+  if constexpr (Dim == 3) {
+    E[i] += (+cfly) * (Bz[i + sy] - Bz[i]) + (-cflz) * (By[i + sz] - By[i]) - delt * J[v + i];
+    E[v + i] +=
+        (+cflz) * (Bx[i + sz] - Bx[i]) + (-cflx) * (Bz[i + sx] - Bz[i]) - delt * J[2 * v + i];
+    E[2 * v + i] +=
+        (+cflx) * (By[i + sx] - By[i]) + (-cfly) * (Bx[i + sy] - Bx[i]) - delt * J[3 * v + i];
+  } else {
+    E[i] += (+cfly) * (Bz[i + sy] - Bz[i]) - delt * J[v + i];
+    E[v + i] += (-cflx) * (Bz[i + sx] - Bz[i]) - delt * J[2 * v + i];
+    E[2 * v + i] +=
+        (+cflx) * (By[i + sx] - By[i]) + (-cfly) * (Bx[i + sy] - Bx[i]) - delt * J[3 * v + i];
+  }
+}
+
+__device__ __forceinline__ int wrap(int i, int pad, int n)
+{
+  int j = i - pad;
+  j     = j < 0 ? j + n : (j >= n ? j - n : j);
+  return j + pad;
+}
+
+// Copy the periodic image into every padding cell (the EMF exchange,
+// nix/xtensor_halo3d.hpp:18-71, on a single periodic array).
+__global__ void __launch_bounds__(256) k_fill_ghost(Geom g, double* __restrict__ f, int ncomp)
+{
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= g.vol)
+    return;
+  int  x = (int)(tid % g.P[2]);
+  long r = tid / g.P[2];
+  int  y = (int)(r % g.P[1]);
+  int  z = (int)(r / g.P[1]);
+  bool in = x >= g.pad[2] && x < g.pad[2] + g.N[2] && y >= g.pad[1] && y < g.pad[1] + g.N[1] &&
+            z >= g.pad[0] && z < g.pad[0] + g.N[0];
+  if (in)
+    return;
+  long src = gidx(g, wrap(z, g.pad[0], g.N[0]), wrap(y, g.pad[1], g.N[1]),
+                  wrap(x, g.pad[2], g.N[2]));
+  for (int k = 0; k < ncomp; k++)
+    f[k * g.vol + tid] = f[k * g.vol + src];
+}
+
+// Fold the current deposited into the padding back onto the periodic image
+// (the Cur exchange adds a neighbour's ghost into the interior edge,
+// nix/xtensor_halo3d.hpp:76-129).  One pass per dimension; each thread owns one
+// line, so the additions happen in a fixed order (x, then y, then z).
+__global__ void __launch_bounds__(256) k_fold_line(Geom g, double* __restrict__ J, int dir)
+{
+  // dir: 2 = x lines over (z, y); 1 = y lines over (z, x); 0 = z lines over (y, x)
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int  n0, n1;
+  if (dir == 2) {
+    n0 = g.P[0];
+    n1 = g.P[1];
+  } else if (dir == 1) {
+    n0 = g.P[0];
+    n1 = g.P[2];
+  } else {
+    n0 = g.P[1];
+    n1 = g.P[2];
+  }
+  if (tid >= (long)n0 * n1)
+    return;
+  int  i0 = (int)(tid / n1), i1 = (int)(tid % n1);
+  int  pad = g.pad[dir], n = g.N[dir];
+  long stride = dir == 2 ? 1 : (dir == 1 ? g.P[2] : g.plane);
+  long base   = dir == 2 ? gidx(g, i0, i1, 0) : (dir == 1 ? gidx(g, i0, 0, i1) : gidx(g, 0, i0, i1));
+  for (int k = 0; k < 4; k++) {
+    double* f = J + k * g.vol + base;
+    for (int p = 0; p < pad; p++) {
+      // low padding p -> interior n + p ; high padding pad + n + p -> interior pad + p
+      f[(long)(n + p) * stride] += f[(long)p * stride];
+      f[(long)(pad + p) * stride] += f[(long)(pad + n + p) * stride];
+    }
+  }
+}
+
+inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+} // namespace
+
+void launch_push_bfd(Sim& s, double delt)
+{
+  const Geom& g    = s.g;
+  double      cflx = g.cc * delt / g.del[2];
+  double      cfly = g.cc * delt / g.del[1];
+  double      cflz = g.cc * delt / g.del[0];
+  long        n    = (long)g.N[0] * g.N[1] * g.N[2];
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_push_bfd<3>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_E,
+                       s.d_B, cflx, cfly, cflz);
+  else
+    hipLaunchKernelGGL(k_push_bfd<2>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_E,
+                       s.d_B, cflx, cfly, cflz);
+  launch_fill_ghost(s, s.d_B, 3);
+}
+
+void launch_push_efd(Sim& s, double delt)
+{
+  const Geom& g    = s.g;
+  double      cflx = g.cc * delt / g.del[2];
+  double      cfly = g.cc * delt / g.del[1];
+  double      cflz = g.cc * delt / g.del[0];
+  long        n    = (long)g.N[0] * g.N[1] * g.N[2];
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_push_efd<3>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_E,
+                       s.d_B, s.d_J, cflx, cfly, cflz, delt);
+  else
+    hipLaunchKernelGGL(k_push_efd<2>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_E,
+                       s.d_B, s.d_J, cflx, cfly, cflz, delt);
+  launch_fill_ghost(s, s.d_E, 3);
+}
+
+void launch_fill_ghost(Sim& s, double* f, int ncomp)
+{
+  hipLaunchKernelGGL(k_fill_ghost, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream, s.g, f,
+                     ncomp);
+}
+
+void launch_fold_current(Sim& s)
+{
+  const Geom& g = s.g;
+  for (int dir = 2; dir >= 0; dir--) {
+    if (!g.has[dir] || g.pad[dir] == 0)
+      continue;
+    long n = dir == 2 ? (long)g.P[0] * g.P[1] : (dir == 1 ? (long)g.P[0] * g.P[2] : g.plane);
+    hipLaunchKernelGGL(k_fold_line, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_J, dir);
+  }
+}
+
+} // namespace picnix

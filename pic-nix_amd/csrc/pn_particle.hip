@@ -1,0 +1,350 @@
+// pn_particle.hip -- field gather + Boris push, position push + count, and the
+// AoS <-> SoA transfers at the C-ABI boundary (FP64 plain path).
+#include "pn_sim.hpp"
+
+namespace picnix {
+
+namespace {
+
+inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+struct PushConsts {
+  double qmdt[kMaxSpecies]; // 0.5 * q / m * delt  (velocity.hpp:130)
+};
+
+// One thread per particle.  Weights follow BaseVelocity::weights{2,3}d
+// (pic/engine/velocity.hpp:262-359, MC interpolation) and the gather follows
+// push_scalar{2,3}d (:390-450) with nix::interp::interp{2,3}d_impl_sorted
+// (nix/interp.hpp:50-110): (Order+1) points per direction, x innermost.  Chunk-
+// local ghost-inclusive indices map onto the global padded grid by adding the
+// chunk's cell offset (the padding equals the ghost margin Nb).
+template <int Dim, int Order>
+__global__ void __launch_bounds__(256)
+    k_push_velocity(Geom g, PArr p, const int32_t* __restrict__ np, const int* __restrict__ off,
+                    const double* __restrict__ lim, const double* __restrict__ E,
+                    const double* __restrict__ B, PushConsts pc)
+{
+  constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
+  constexpr int S      = Order + 1;
+  const int     seg    = blockIdx.y;
+  const long    ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
+    return;
+  const int  c    = seg / g.Ns;
+  const int  sp   = seg - c * g.Ns;
+  const long i    = (long)seg * g.cap + ip;
+  const double qmdt = pc.qmdt[sp];
+
+  const double dx = g.del[2], dy = g.del[1], dz = g.del[0];
+  const double xmin = lim[c * 6 + 4], ymin = lim[c * 6 + 2], zmin = lim[c * 6 + 0];
+  const int    lb   = g.Nb;
+
+  double xu[6] = {p.x[i], p.y[i], p.z[i], p.ux[i], p.uy[i], p.uz[i]};
+
+  double ximin  = xmin + 0.5 * dx * is_odd;
+  double xhmin  = xmin + 0.5 * dx * is_odd - 0.5 * dx;
+  double yimin  = ymin + 0.5 * dy * is_odd;
+  double yhmin  = ymin + 0.5 * dy * is_odd - 0.5 * dy;
+  double xigrid = xmin + 0.5 * dx, xhgrid = xmin;
+  double yigrid = ymin + 0.5 * dy, yhgrid = ymin;
+  double rdx = 1 / dx, rdy = 1 / dy;
+
+  double wix[S], wiy[S], wiz[S], whx[S], why[S], whz[S];
+  int    ix0 = digitize(xu[0], ximin, rdx);
+  int    iy0 = digitize(xu[1], yimin, rdy);
+  int    hx0 = digitize(xu[0], xhmin, rdx);
+  int    hy0 = digitize(xu[1], yhmin, rdy);
+  shape_mc<Order>(xu[0], xigrid + static_cast<double>(ix0) * dx, rdx, wix);
+  shape_mc<Order>(xu[1], yigrid + static_cast<double>(iy0) * dy, rdy, wiy);
+  shape_mc<Order>(xu[0], xhgrid + static_cast<double>(hx0) * dx, rdx, whx);
+  shape_mc<Order>(xu[1], yhgrid + static_cast<double>(hy0) * dy, rdy, why);
+  // local ghost-inclusive start index -> global padded index
+  ix0 += lb - (Order / 2) + off[c * 3 + 2];
+  iy0 += lb - (Order / 2) + off[c * 3 + 1];
+  hx0 += lb - (Order / 2) + off[c * 3 + 2];
+  hy0 += lb - (Order / 2) + off[c * 3 + 1];
+
+  const long   sy = g.P[2], sz = g.plane, v = g.vol;
+  // stencils must lie inside the padded grid; a particle outside its chunk
+  // (possible only through pn_set_particles) is left untouched
+  if (min(ix0, hx0) < 0 || max(ix0, hx0) > g.P[2] - S || min(iy0, hy0) < 0 ||
+      max(iy0, hy0) > g.P[1] - S)
+    return;
+  double       eb[6];
+  if constexpr (Dim == 3) {
+    double zimin  = zmin + 0.5 * dz * is_odd;
+    double zhmin  = zmin + 0.5 * dz * is_odd - 0.5 * dz;
+    double zigrid = zmin + 0.5 * dz, zhgrid = zmin;
+    double rdz    = 1 / dz;
+    int    iz0    = digitize(xu[2], zimin, rdz);
+    int    hz0    = digitize(xu[2], zhmin, rdz);
+    shape_mc<Order>(xu[2], zigrid + static_cast<double>(iz0) * dz, rdz, wiz);
+    shape_mc<Order>(xu[2], zhgrid + static_cast<double>(hz0) * dz, rdz, whz);
+    iz0 += lb - (Order / 2) + off[c * 3 + 0];
+    hz0 += lb - (Order / 2) + off[c * 3 + 0];
+    if (min(iz0, hz0) < 0 || max(iz0, hz0) > g.P[0] - S)
+      return;
+    auto interp = [&](const double* F, int z0, int y0, int x0, const double* wz,
+                      const double* wy, const double* wx) {
+      double rz = 0;
+#pragma unroll
+      for (int jz = 0; jz < S; jz++) {
+        double ry = 0;
+#pragma unroll
+        for (int jy = 0; jy < S; jy++) {
+          const double* row = F + (z0 + jz) * sz + (y0 + jy) * sy + x0;
+          double        rx  = 0;
+#pragma unroll
+          for (int jx = 0; jx < S; jx++)
+            rx += row[jx] * wx[jx];
+          ry += rx * wy[jy];
+        }
+        rz += ry * wz[jz];
+      }
+      return rz * qmdt;
+    };
+    eb[0] = interp(E, iz0, iy0, hx0, wiz, wiy, whx);
+    eb[1] = interp(E + v, iz0, hy0, ix0, wiz, why, wix);
+    eb[2] = interp(E + 2 * v, hz0, iy0, ix0, whz, wiy, wix);
+    eb[3] = interp(B, hz0, hy0, ix0, whz, why, wix);
+    eb[4] = interp(B + v, hz0, iy0, hx0, whz, wiy, whx);
+    eb[5] = interp(B + 2 * v, iz0, hy0, hx0, wiz, why, whx);
+  } else {
+    (void)wiz;
+    (void)whz;
+    auto interp = [&](const double* F, int y0, int x0, const double* wy, const double* wx) {
+      double ry = 0;
+#pragma unroll
+      for (int jy = 0; jy < S; jy++) {
+        const double* row = F + (y0 + jy) * sy + x0;
+        double        rx  = 0;
+#pragma unroll
+        for (int jx = 0; jx < S; jx++)
+          rx += row[jx] * wx[jx];
+        ry += rx * wy[jy];
+      }
+      return ry * qmdt;
+    };
+    eb[0] = interp(E, iy0, hx0, wiy, whx);
+    eb[1] = interp(E + v, hy0, ix0, why, wix);
+    eb[2] = interp(E + 2 * v, iy0, ix0, wiy, wix);
+    eb[3] = interp(B, hy0, ix0, why, wix);
+    eb[4] = interp(B + v, iy0, hx0, wiy, whx);
+    eb[5] = interp(B + 2 * v, hy0, hx0, why, whx);
+  }
+  push_boris(xu[3], xu[4], xu[5], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], g.cc);
+  p.ux[i] = xu[3];
+  p.uy[i] = xu[4];
+  p.uz[i] = xu[5];
+}
+
+// Bin index of a position inside chunk c, or -1 when the particle lies outside
+// the chunk.  XtensorParticle::count (nix/xtensor_particle.hpp:324-357).
+__device__ __forceinline__ int bin_of(const Geom& g, const double* __restrict__ lim,
+                                      const double* __restrict__ rng, int c, double x, double y,
+                                      double z)
+{
+  const int    is_odd  = (g.order % 2 == 1) ? 1 : 0;
+  const double xoffset = lim[c * 6 + 4] - 0.5 * g.del[2] * is_odd;
+  const double yoffset = lim[c * 6 + 2] - 0.5 * g.del[1] * is_odd;
+  const double zoffset = lim[c * 6 + 0] - 0.5 * g.del[0] * is_odd;
+  const double rdx = 1 / g.del[2], rdy = 1 / g.del[1], rdz = 1 / g.del[0];
+  if (g.has[2] && (x < rng[c * 6 + 4] || x >= rng[c * 6 + 5]))
+    return -1;
+  if (g.has[1] && (y < rng[c * 6 + 2] || y >= rng[c * 6 + 3]))
+    return -1;
+  if (g.has[0] && (z < rng[c * 6 + 0] || z >= rng[c * 6 + 1]))
+    return -1;
+  int ix = g.has[2] ? digitize(x, xoffset, rdx) : 0;
+  int iy = g.has[1] ? digitize(y, yoffset, rdy) : 0;
+  int iz = g.has[0] ? digitize(z, zoffset, rdz) : 0;
+  return iz * g.bsz + iy * g.bsy + ix;
+}
+
+// Exchange direction slot (0..26, 13 = stays) of XtensorHaloParticle3D::pre_pack
+// (nix/xtensor_halo3d.hpp:230-244).
+__device__ __forceinline__ int dir_of(const double* __restrict__ rng, int c, double x, double y,
+                                      double z)
+{
+  int iz = (z >= rng[c * 6 + 1]) - (z < rng[c * 6 + 0]) + 1;
+  int iy = (y >= rng[c * 6 + 3]) - (y < rng[c * 6 + 2]) + 1;
+  int ix = (x >= rng[c * 6 + 5]) - (x < rng[c * 6 + 4]) + 1;
+  return 9 * iz + 3 * iy + ix;
+}
+
+// BasePosition::push (pic/engine/position.hpp:117-130) followed by count(reset)
+// (Position::set_boundary, pic/pic_engine.hpp:292-303).  Stayers take an
+// arbitrary slot in their bin; leavers record their exchange direction and the
+// block's per-direction totals (one 256-particle tile per block).
+__global__ void __launch_bounds__(256)
+    k_push_position(Geom g, PArr p, double* __restrict__ xo, double* __restrict__ yo,
+                    double* __restrict__ zo, const int32_t* __restrict__ np,
+                    const double* __restrict__ lim, const double* __restrict__ rng, double delt,
+                    double rc, int32_t* __restrict__ cellcnt, int32_t* __restrict__ key,
+                    int32_t* __restrict__ slot, int32_t* __restrict__ tilecnt, int ntile)
+{
+  __shared__ int cnt[27];
+  if (threadIdx.x < 27)
+    cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int  seg = blockIdx.y;
+  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip < np[seg]) {
+    const int  c = seg / g.Ns;
+    const long i = (long)seg * g.cap + ip;
+    double     x = p.x[i], y = p.y[i], z = p.z[i];
+    double     ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+    xo[i]        = x;
+    yo[i]        = y;
+    zo[i]        = z;
+    double gm    = lorentz_factor(ux, uy, uz, rc);
+    double dt    = delt / gm;
+    x += ux * dt;
+    y += uy * dt;
+    z += uz * dt;
+    p.x[i] = x;
+    p.y[i] = y;
+    p.z[i] = z;
+    int b  = bin_of(g, lim, rng, c, x, y, z);
+    if (b >= 0) {
+      slot[i] = atomicAdd(&cellcnt[(long)seg * g.nbin + b], 1);
+      key[i]  = b;
+    } else {
+      int d  = dir_of(rng, c, x, y, z);
+      key[i] = -1 - d;
+      atomicAdd(&cnt[d], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 27)
+    tilecnt[((long)seg * ntile + blockIdx.x) * 27 + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// count(reset) alone, for PicChunk::sort_particle: out-of-chunk particles are
+// dropped by the sort that follows (nix/xtensor_particle.hpp:317-320).
+__global__ void __launch_bounds__(256)
+    k_count(Geom g, PArr p, const int32_t* __restrict__ np, const double* __restrict__ lim,
+            const double* __restrict__ rng, int32_t* __restrict__ cellcnt,
+            int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ dseg,
+            int32_t* __restrict__ didx)
+{
+  const int  seg = blockIdx.y;
+  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
+    return;
+  const int  c = seg / g.Ns;
+  const long i = (long)seg * g.cap + ip;
+  int        b = bin_of(g, lim, rng, c, p.x[i], p.y[i], p.z[i]);
+  if (b >= 0) {
+    slot[i] = atomicAdd(&cellcnt[(long)seg * g.nbin + b], 1);
+    key[i]  = b;
+    dseg[i] = seg;
+    didx[i] = (int32_t)ip;
+  } else {
+    key[i]  = -1;
+    dseg[i] = -1;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_aos_to_soa(PArr p, long base, long n,
+                                                    const double* __restrict__ aos)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n)
+    return;
+  const double* r = aos + t * 7;
+  long          i = base + t;
+  p.x[i]          = r[0];
+  p.y[i]          = r[1];
+  p.z[i]          = r[2];
+  p.ux[i]         = r[3];
+  p.uy[i]         = r[4];
+  p.uz[i]         = r[5];
+  p.id[i]         = __double_as_longlong(r[6]);
+}
+
+__global__ void __launch_bounds__(256) k_soa_to_aos(PArr p, long base, long n,
+                                                    double* __restrict__ aos)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n)
+    return;
+  double* r = aos + t * 7;
+  long    i = base + t;
+  r[0]      = p.x[i];
+  r[1]      = p.y[i];
+  r[2]      = p.z[i];
+  r[3]      = p.ux[i];
+  r[4]      = p.uy[i];
+  r[5]      = p.uz[i];
+  r[6]      = __longlong_as_double(p.id[i]);
+}
+
+} // namespace
+
+void launch_push_velocity(Sim& s, double delt)
+{
+  const Geom& g = s.g;
+  PushConsts  pc{};
+  for (int sp = 0; sp < g.Ns; sp++)
+    pc.qmdt[sp] = 0.5 * g.q[sp] / g.m[sp] * delt;
+  dim3 grid(nblk(g.cap, 256), g.nseg);
+#define PN_VEL(D, O)                                                                             \
+  hipLaunchKernelGGL((k_push_velocity<D, O>), grid, dim3(256), 0, s.stream, g, s.pa, s.d_np,    \
+                     s.d_off, s.d_lim, s.d_E, s.d_B, pc)
+  if (g.dim == 3) {
+    switch (g.order) {
+    case 1: PN_VEL(3, 1); break;
+    case 2: PN_VEL(3, 2); break;
+    case 3: PN_VEL(3, 3); break;
+    default: PN_VEL(3, 4); break;
+    }
+  } else {
+    switch (g.order) {
+    case 1: PN_VEL(2, 1); break;
+    case 2: PN_VEL(2, 2); break;
+    case 3: PN_VEL(2, 3); break;
+    default: PN_VEL(2, 4); break;
+    }
+  }
+#undef PN_VEL
+}
+
+void launch_push_position(Sim& s, double delt)
+{
+  const Geom& g = s.g;
+  hipMemsetAsync(s.ss.cellcnt, 0, sizeof(int32_t) * (size_t)g.nseg * g.nbin, s.stream);
+  dim3 grid(s.ss.ntile, g.nseg);
+  hipLaunchKernelGGL(k_push_position, grid, dim3(256), 0, s.stream, g, s.pa, s.d_xo[0],
+                     s.d_xo[1], s.d_xo[2], s.d_np, s.d_lim, s.d_rng, delt, 1 / g.cc,
+                     s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.tilecnt, s.ss.ntile);
+  s.counted = true;
+  s.sorted  = false;
+}
+
+void launch_count(Sim& s)
+{
+  const Geom& g = s.g;
+  hipMemsetAsync(s.ss.cellcnt, 0, sizeof(int32_t) * (size_t)g.nseg * g.nbin, s.stream);
+  dim3 grid(s.ss.ntile, g.nseg);
+  hipLaunchKernelGGL(k_count, grid, dim3(256), 0, s.stream, g, s.pa, s.d_np, s.d_lim, s.d_rng,
+                     s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx);
+}
+
+void launch_aos_to_soa(Sim& s, int seg, int64_t np)
+{
+  if (np <= 0)
+    return;
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(np, 256)), dim3(256), 0, s.stream, s.pa,
+                     (long)seg * s.g.cap, (long)np, s.d_stage);
+}
+
+void launch_soa_to_aos(Sim& s, int seg, int64_t np)
+{
+  if (np <= 0)
+    return;
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(np, 256)), dim3(256), 0, s.stream, s.pa,
+                     (long)seg * s.g.cap, (long)np, s.d_stage);
+}
+
+} // namespace picnix

@@ -1,0 +1,90 @@
+// pn_sim.hpp -- host-side state of one simulation handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "pn_config.hpp"
+#include "pn_device.hpp"
+
+namespace picnix {
+
+// Per-step scratch of the counting sort and particle migration.
+struct SortScratch {
+  int32_t* cellcnt   = nullptr; // [nseg][nbin]   particles per bin (atomic slots)
+  int32_t* cellstart = nullptr; // [nseg][nbin+1] first particle of every bin
+  int32_t* key       = nullptr; // [nseg*cap]     bin of the particle in its destination
+  int32_t* slot      = nullptr; // [nseg*cap]     arbitrary slot inside that bin
+  int32_t* dseg      = nullptr; // [nseg*cap]     destination segment (-1 = dropped)
+  int32_t* didx      = nullptr; // [nseg*cap]     index in the destination's pre-sort array
+  int32_t* mkey      = nullptr; // [nseg*cap]     member list of every bin (pre-sort indices)
+  int32_t* tilecnt   = nullptr; // [nseg][ntile][27] leavers per 256-particle tile and direction
+  int32_t* nout      = nullptr; // [nseg][27]     leavers per direction
+  int32_t* recvoff   = nullptr; // [nseg][27]     first pre-sort index of each receive slot
+  int32_t* npnew     = nullptr; // [nseg]         particle count after the sort
+  int32_t* errflag   = nullptr; // [4]            device-side error flags
+  int      ntile     = 0;
+};
+
+struct Sim {
+  Config cfg;
+  Geom   g{};
+
+  // chunk tables (indexed by chunk id)
+  std::vector<int>    id_of_coord; // [Cz][Cy][Cx] -> id
+  std::vector<int>    coord;       // [id][3]
+  std::vector<int>    offset;      // [id][3] interior cell offset (z, y, x)
+  std::vector<double> lim;         // [id][3][2] physical range (z, y, x)
+  std::vector<double> rng;         // [id][3][2] range used for out-of-chunk tests
+  std::vector<int>    nb;          // [id][27] neighbour ids
+
+  int*    d_off = nullptr;
+  double* d_lim = nullptr;
+  double* d_rng = nullptr;
+  int*    d_nb  = nullptr;
+
+  hipStream_t stream = nullptr;
+
+  // global padded periodic field arrays, component-major
+  double* d_E = nullptr; // [3][vol]
+  double* d_B = nullptr; // [3][vol]
+  double* d_J = nullptr; // [4][vol]
+
+  // particles: A = current, Bv = sort destination
+  PArr     pa{}, pb{};
+  double*  d_xo[3] = {nullptr, nullptr, nullptr}; // old positions (XtensorParticle::xv)
+  int32_t* d_np    = nullptr;                     // [nseg]
+  std::vector<int32_t> h_np;                      // host mirror, valid when np_host_valid
+  bool     np_host_valid = false;
+
+  SortScratch ss{};
+  bool        counted = false; // keys hold a fresh count (push_position / count)
+  bool        sorted  = false; // particles are in sorted order and cellstart is valid
+
+  // staging for host <-> device particle transfers
+  double* d_stage     = nullptr;
+  size_t  stage_elems = 0;
+
+  std::string err;
+};
+
+// kernels / launchers (pn_field.hip, pn_particle.hip, pn_sort.hip, pn_deposit.hip, pn_diag.hip)
+void launch_push_bfd(Sim& s, double delt);
+void launch_push_efd(Sim& s, double delt);
+void launch_fill_ghost(Sim& s, double* f, int ncomp);
+void launch_fold_current(Sim& s);
+void launch_push_velocity(Sim& s, double delt);
+void launch_push_position(Sim& s, double delt);
+void launch_count(Sim& s);
+void launch_migrate(Sim& s);
+void launch_sort(Sim& s);
+void launch_deposit(Sim& s, double delt);
+void launch_energy(Sim& s, double* out /* host: efd, bfd, divE, divB, particle[Ns] */);
+void launch_aos_to_soa(Sim& s, int seg, int64_t np);
+void launch_soa_to_aos(Sim& s, int seg, int64_t np);
+void launch_chunk_fields_get(Sim& s, int chunk, int which, double* d_out);
+void launch_chunk_fields_set(Sim& s, int chunk, int which, const double* d_in);
+
+} // namespace picnix

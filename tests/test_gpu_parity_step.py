@@ -1,0 +1,120 @@
+"""GPU parity of the full FP64 step against the oracle (same seeded inputs).
+
+Bars (from the task's parity definition):
+  * particle order and cell indices: bit-exact;
+  * particle positions/momenta: bit-exact at step 1 (identical arithmetic), and
+    within 1e-12 relative afterwards (the current is summed in a different
+    order, which perturbs the fields at round-off);
+  * fields and currents: <= 1e-12 relative (max |diff| / max |value|) after 1 and
+    10 steps;
+  * div(E) - rho stays at round-off.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _sim(cfg):
+    import picnix_amd
+    return picnix_amd.Simulation(helpers.dumps(cfg))
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _compare_particles(o, g, exact):
+    worst = 0.0
+    for c in range(o.nchunks):
+        for s in range(o.Ns):
+            a = o.particles(c, s)
+            b = g.particles(c, s)
+            assert a.shape == b.shape, f"chunk {c} species {s}: np {a.shape[0]} vs {b.shape[0]}"
+            # ids identify particles: same order <=> identical sort
+            assert np.array_equal(a[:, 6].view(np.int64), b[:, 6].view(np.int64)), \
+                f"chunk {c} species {s}: particle order differs"
+            if exact:
+                assert np.array_equal(a[:, :6], b[:, :6]), f"chunk {c} species {s}: values differ"
+            else:
+                worst = max(worst, helpers.rel_err(b[:, :6], a[:, :6]))
+    return worst
+
+
+def _compare_fields(o, g, dim):
+    worst = {}
+    for c in range(o.nchunks):
+        info = o.chunk_info(c)
+        uf_o = helpers.interior(o.field(c, 0), info, dim)
+        uf_g = helpers.interior(g.field(c, 0), info, dim)
+        uj_o = helpers.interior(o.field(c, 1), info, dim)
+        uj_g = helpers.interior(g.field(c, 1), info, dim)
+        for k in range(6):
+            worst[f"uf{k}"] = max(worst.get(f"uf{k}", 0.0), np.max(np.abs(uf_g[..., k] - uf_o[..., k])))
+            worst[f"uf{k}_scale"] = max(worst.get(f"uf{k}_scale", 0.0), np.max(np.abs(uf_o[..., k])))
+        for k in range(4):
+            worst[f"uj{k}"] = max(worst.get(f"uj{k}", 0.0), np.max(np.abs(uj_g[..., k] - uj_o[..., k])))
+            worst[f"uj{k}_scale"] = max(worst.get(f"uj{k}_scale", 0.0), np.max(np.abs(uj_o[..., k])))
+    rel = {}
+    for k in [f"uf{i}" for i in range(6)] + [f"uj{i}" for i in range(4)]:
+        sc = worst[k + "_scale"]
+        rel[k] = worst[k] / sc if sc > 0 else worst[k]
+    return rel
+
+
+@pytest.mark.parametrize("kind", ["thermal3d_o2", "twostream2d_o1"])
+def test_step_parity(kind):
+    if kind == "thermal3d_o2":
+        cfg = helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=2)
+        dim = 3
+    else:
+        cfg = helpers.small_twostream(N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1)
+        dim = 2
+    dt = float(cfg["parameter"]["delt"])
+    _say(f"[parity] {kind}: create oracle + GPU")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    _say(f"[parity] {kind}: compare loader + initial sort")
+    _compare_particles(o, g, exact=True)
+
+    _say(f"[parity] {kind}: step 1")
+    o.step()
+    g.step(dt)
+    g.synchronize()
+    _compare_particles(o, g, exact=True)
+    rel = _compare_fields(o, g, dim)
+    _say(f"[parity] {kind}: step 1 field rel errors {rel}")
+    assert max(rel.values()) <= TOL
+
+    for k in range(9):
+        o.step()
+        g.step(dt)
+    g.synchronize()
+    _say(f"[parity] {kind}: step 10")
+    perr = _compare_particles(o, g, exact=False)
+    rel = _compare_fields(o, g, dim)
+    _say(f"[parity] {kind}: step 10 particle rel err {perr:.3e}, fields {rel}")
+    assert perr <= TOL
+    assert max(rel.values()) <= TOL
+
+    ho = o.history()
+    hg = g.history()
+    _say(f"[parity] {kind}: history oracle {ho} gpu {hg}")
+    # div(E) - rho: both sides hold the same value to round-off (it is 0 for the
+    # neutral thermal plasma; for the two-stream beams it is -sum(rho) because the
+    # immobile ions are a background that is not deposited); div(B) ~ 0
+    ncell = np.prod([cfg["parameter"][k] for k in ("Nx", "Ny", "Nz")])
+    assert abs(hg[0] - ho[0]) < 1e-11 * ncell
+    assert abs(hg[1]) < 1e-11 * ncell
+    assert np.allclose(hg[2:4], ho[2:4], rtol=1e-10, atol=1e-14)
+    assert np.allclose(hg[4:], ho[4:], rtol=1e-10)
+    g.close()
+    o.close()

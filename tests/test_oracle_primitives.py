@@ -1,0 +1,151 @@
+"""Pin the oracle's primitives against closed forms and invariants (CPU only).
+
+Each check is chosen so that a plausible restatement mistake (a dropped term,
+a wrong sign, an off-by-half-cell grid) fails it:
+  * shape_mc1..4 against the analytic B-splines used by the reference's own
+    tests (nix/unittest/test_primitives.cpp:271-530) and partition of unity;
+  * digitize against floor((x - xmin) / dx) cases (test_primitives.cpp:73-117);
+  * lorentz_factor against sqrt(1 + u^2 / c^2);
+  * push_boris against an independently written textbook Boris rotation, plus
+    |u| conservation for E = 0, the exact rotation angle 2 atan(|t|) and the
+    E x B drift.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def bspline(order, x):
+    """Centred cardinal B-spline of the given degree (support (order+1)/2)."""
+    ax = abs(x)
+    if order == 1:
+        return max(0.0, 1 - ax)
+    if order == 2:
+        if ax < 0.5:
+            return 0.75 - x * x
+        if ax < 1.5:
+            return (1.5 - ax) ** 2 / 2
+        return 0.0
+    if order == 3:
+        if ax < 1:
+            return 2.0 / 3 - ax * ax + ax ** 3 / 2
+        if ax < 2:
+            return (2 - ax) ** 3 / 6
+        return 0.0
+    if ax < 0.5:
+        return 115.0 / 192 - 5 * ax ** 2 / 8 + ax ** 4 / 4
+    if ax < 1.5:
+        return (55 + 20 * ax - 120 * ax ** 2 + 80 * ax ** 3 - 16 * ax ** 4) / 96
+    if ax < 2.5:
+        return (5 - 2 * ax) ** 4 / 384
+    return 0.0
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_shape_mc_closed_form(order):
+    L = ol.lib()
+    rng = np.random.default_rng(order)
+    # grid point offsets of the weights, from the shape_mc documentation
+    # (nix/primitives.hpp:482-512): odd orders X <= x < X + dx, even X - dx/2 <= x < X + dx/2
+    offs = {1: [0, 1], 2: [-1, 0, 1], 3: [-1, 0, 1, 2], 4: [-2, -1, 0, 1, 2]}[order]
+    lo, hi = (0.0, 1.0) if order % 2 == 1 else (-0.5, 0.5)
+    for _ in range(200):
+        d = rng.uniform(lo, hi)
+        s = np.zeros(order + 1)
+        X = 3.0
+        L.pno_shape_mc(order, X + d, X, 1.0, _dp(s))
+        ref = np.array([bspline(order, o - d) for o in offs])
+        assert np.max(np.abs(s - ref)) < 1e-14
+        assert abs(s.sum() - 1.0) < 1e-14
+
+
+def test_digitize():
+    L = ol.lib()
+    for x, xmin, dx in [(0.0, 0.0, 1.0), (0.999999, 0.0, 1.0), (1.0, 0.0, 1.0), (-0.1, 0.0, 1.0),
+                        (5.25, 0.5, 0.25), (-3.7, -1.0, 0.5)]:
+        assert L.pno_digitize(x, xmin, 1 / dx) == math.floor((x - xmin) * (1 / dx))
+
+
+def test_lorentz_factor():
+    L = ol.lib()
+    rng = np.random.default_rng(1)
+    for _ in range(100):
+        u = rng.normal(size=3) * 3
+        c = rng.uniform(0.5, 5)
+        ref = math.sqrt(1 + (u @ u) / c ** 2)
+        assert abs(L.pno_lorentz_factor(u[0], u[1], u[2], 1 / c) - ref) < 1e-14 * ref
+
+
+def boris_textbook(u, e, b, c):
+    """Birdsall & Langdon Boris step with the reference's normalisation:
+    e, b are already multiplied by (q/m) dt / 2; the rotation uses b / (gamma c)."""
+    um = u + e
+    gamma_c = math.sqrt(c * c + um @ um)
+    t = b / gamma_c
+    s = 2.0 / (1.0 + t @ t)
+    up = um + np.cross(um, t)
+    upp = um + s * np.cross(up, t)
+    return upp + e
+
+
+def _boris(u, eb, c):
+    L = ol.lib()
+    uu = np.array(u, dtype=np.float64)
+    ebb = np.array(eb, dtype=np.float64)
+    L.pno_push_boris(_dp(uu), _dp(ebb), c)
+    return uu
+
+
+def test_boris_matches_textbook():
+    rng = np.random.default_rng(7)
+    for _ in range(500):
+        u = rng.normal(size=3)
+        e = rng.normal(size=3) * 0.1
+        b = rng.normal(size=3) * 0.5
+        c = rng.uniform(0.5, 3)
+        got = _boris(u, np.concatenate([e, b]), c)
+        ref = boris_textbook(u, e, b, c)
+        assert np.max(np.abs(got - ref)) < 1e-13 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_boris_rotation_invariants():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        u = rng.normal(size=3)
+        b = rng.normal(size=3)
+        c = 1.3
+        got = _boris(u, np.concatenate([np.zeros(3), b]), c)
+        # |u| conserved for E = 0
+        assert abs(np.linalg.norm(got) - np.linalg.norm(u)) < 1e-14 * np.linalg.norm(u)
+        # rotation angle about b is 2 atan(|t|), t = b / (gamma c)
+        t = np.linalg.norm(b) / math.sqrt(c * c + u @ u)
+        bh = b / np.linalg.norm(b)
+        p0 = u - (u @ bh) * bh
+        p1 = got - (got @ bh) * bh
+        ang = math.atan2(np.linalg.norm(np.cross(p0, p1)), p0 @ p1)
+        assert abs(ang - 2 * math.atan(t)) < 1e-12
+
+
+def test_boris_exb_drift():
+    # non-relativistic E x B drift: <v> -> c E x B / B^2 (Gaussian units), with
+    # e, b scaled by (q/m) dt / 2 as the pusher expects
+    c = 100.0
+    qm_dt_half = 0.05
+    E = np.array([0.0, 1e-2, 0.0]) * c
+    B = np.array([0.0, 0.0, 1.0]) * c
+    u = np.zeros(3)
+    vs = []
+    for _ in range(20000):
+        u = _boris(u, np.concatenate([qm_dt_half * E, qm_dt_half * B]), c)
+        vs.append(u / math.sqrt(1 + (u @ u) / c ** 2))
+    vd = np.mean(vs, axis=0)
+    ref = c * np.cross(E, B) / (B @ B)
+    assert np.max(np.abs(vd - ref)) < 2e-3 * np.linalg.norm(ref)
