@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Throughput of the pic-nix per-timestep hot path on one MI355X.
+
+One "step" is PicApplication::push_openmp's full sequence (B half push, gather +
+Boris, position push + count, Esirkepov deposit, B half push, current fold, E push,
+particle migration + counting sort) over every chunk of the workload.  The
+metric is particle-steps/s = (mobile particles) x steps / wall time of the timed
+steps; inputs are resident in HBM before timing starts.
+
+Default workload: the 64^3 thermal plasma (64 chunks of 16^3, electrons +
+protons, 64 particles/cell/species = 33.5M particles, 2nd-order shape, FP64).
+
+The line also carries
+  * roofline: the dominant kernel's algorithmic bytes per launch over its mean
+    launch time (HIP events on the library's stream), against 8 TB/s;
+  * cpu_baseline: the FP64 CPU oracle (a restatement of the reference's OpenMP
+    path) timed on this host's cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pic-nix_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "particle-steps/s (push+deposit+sort+FDTD), FP64 plain vs FP32 batched; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "thermal64": "thermal plasma 3D 64^3 cells, 64 chunks of 16^3, electrons+protons "
+                 "(m_p/m_e=100, vth_e=0.1c), 64 particles/cell/species (33.5M), "
+                 "2nd-order shape, FP64 plain",
+    "thermal128": "thermal plasma 3D 128^3 cells, 512 chunks of 16^3, electrons+protons, "
+                  "32 particles/cell/species (134M), 2nd-order shape, FP64 plain",
+    "twostream2d": "two-stream 2D 1x16x512, 32 chunks of 1x16x16, two beams +-0.1c, "
+                   "64 particles/cell/beam, 1st-order shape, FP64 plain",
+}
+
+
+def algorithmic_bytes(phase, npart, ncell):
+    """Bytes each launch must move at minimum (FP64 operator-boundary model).
+
+    Per particle / per cell, one launch:
+      bfd      : read E, B; write B                      72 B/cell
+      velocity : read x, u; write u (72 B/particle) + one E, B gather per cell (48 B/cell)
+      position : read x, u; write x_old, x, cell index  100 B/particle
+      deposit  : read x_old, x (48 B/particle); write J (24 B/cell)
+      fold     : zero J and fold the periodic padding (32 B/cell)
+      efd      : read E, B, J; write E                   96 B/cell
+      sort     : read cell index + record, write record (116 B/particle) + counts (12 B/cell)
+      rho      : read x (24 B/particle); write rho (8 B/cell)  [3D: after the sort]
+    """
+    per_p = {"bfd": 0, "velocity": 72, "position": 100, "deposit": 48, "fold": 0, "efd": 0,
+             "sort": 116, "rho": 24}[phase]
+    per_c = {"bfd": 72, "velocity": 48, "position": 0, "deposit": 24, "fold": 32, "efd": 96,
+             "sort": 12, "rho": 8}[phase]
+    return per_p * npart + per_c * ncell
+
+
+def step_bytes(npart, ncell):
+    """Whole-step algorithmic bytes: 336 B/particle + 332 B/cell (FP64)."""
+    return 336 * npart + 332 * ncell
+
+
+def cpu_baseline(cfg_name, threads, steps):
+    """Time the FP64 oracle on the host: `steps` full steps of the same workload."""
+    import helpers
+    import oracle_lib
+    cfg = helpers.load_config(cfg_name)
+    oracle_lib.set_threads(threads)
+    o = oracle_lib.Oracle(cfg)
+    npart = sum(o.np(c, s) for c in range(o.nchunks) for s in range(o.Ns))
+    o.step(1)  # first touch of the old-position buffers
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {
+        "value": npart * steps / dt,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} full steps of the same {cfg_name} workload ({npart} particles) after "
+                  f"one untimed step; FP64 oracle, C++/OpenMP over chunks, scalar (not xsimd)",
+        "seconds": dt,
+    }
+
+
+def copy_probe(mb):
+    """One device-to-device hipMemcpy of `mb` MiB (ROCm's copy kernel): under
+    rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE calibrate the counters, since it
+    reads and writes exactly mb MiB."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="thermal64", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device copy of this size (calibrates HBM counters)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
+              "(see DESIGN.md) and --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import picnix_amd
+    sim = picnix_amd.Simulation.from_file(picnix_amd.config_path(args.config))
+    with open(picnix_amd.config_path(args.config)) as f:
+        import tomlmini
+        cfg = tomlmini.loads(f.read())
+    par = cfg["parameter"]
+    dt = float(par["delt"])
+    ncell = int(par["Nx"]) * int(par["Ny"]) * int(par["Nz"])
+    npart = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
+
+    sim.step(dt, args.warmup)
+    sim.synchronize()
+    sim.timing(True)
+    t0 = time.perf_counter()
+    sim.step(dt, args.steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    rep = sim.timing_report()
+    sim.timing(False)
+    npart_end = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
+    sim.close()
+    if args.copy_probe_mb:
+        copy_probe(args.copy_probe_mb)
+
+    elapsed = t1 - t0
+    value = npart * args.steps / elapsed
+    phases = {}
+    for ph, (ms, n) in rep.items():
+        if n == 0:
+            continue
+        per = ms / n
+        b = algorithmic_bytes(ph, npart, ncell)
+        phases[ph] = {"ms_per_launch": per, "launches_per_step": n / args.steps,
+                      "bytes_per_launch": b, "GBps": b / (per * 1e-3) / 1e9}
+    dom = max(phases, key=lambda p: phases[p]["ms_per_launch"] * phases[p]["launches_per_step"])
+    d = phases[dom]
+    roofline = {
+        "bound": "hbm",
+        "achieved": d["GBps"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": d["GBps"] / HBM_PEAK_GBS,
+        "traffic": None,
+        "kernel": dom,
+        "bytes_per_launch": d["bytes_per_launch"],
+    }
+    step_gbs = step_bytes(npart, ncell) * args.steps / elapsed / 1e9
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded loader of the reference's thermal example)",
+        "config": {"workload": WORKLOADS[args.config], "config": args.config,
+                   "particles": npart, "cells": ncell, "precision": "fp64 plain",
+                   "particles_conserved": npart_end == npart},
+        "roofline": roofline,
+        "step_roofline": {"bytes_per_step": step_bytes(npart, ncell), "achieved": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS},
+        "phases": phases,
+    }
+    if args.cpu_baseline == "on":
+        line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_threads, args.cpu_steps)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

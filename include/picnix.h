@@ -114,6 +114,24 @@ int pn_get_diverror(pn_sim* sim, double* efd, double* bfd);
 /* Block until queued device work on the handle's stream is done. */
 int pn_synchronize(pn_sim* sim);
 
+/* Phase timing with HIP events recorded on the handle's stream.  No counterpart
+ * in the reference beyond its per-step wall-clock "push" time
+ * (PicApplication::push, pic/pic_application.cpp:196-216); it exists so that a
+ * benchmark can report each kernel's launch duration.  pn_timing(sim, 1) clears
+ * the accumulators and starts recording; pn_timing_report() synchronises and
+ * returns the summed milliseconds and launch count of every phase. */
+#define PN_PHASE_BFD 0      /* B curl update + ghost refresh            */
+#define PN_PHASE_VELOCITY 1 /* gather + Boris                           */
+#define PN_PHASE_POSITION 2 /* position push + count                    */
+#define PN_PHASE_DEPOSIT 3  /* Esirkepov deposit kernel                 */
+#define PN_PHASE_FOLD 4     /* current zero + periodic fold             */
+#define PN_PHASE_EFD 5      /* E curl update + ghost refresh            */
+#define PN_PHASE_SORT 6     /* migration + counting sort                */
+#define PN_PHASE_RHO 7      /* charge density deposit + fold            */
+#define PN_NPHASE 8
+int pn_timing(pn_sim* sim, int enable);
+int pn_timing_report(pn_sim* sim, double* ms_total, int* launches);
+
 #ifdef __cplusplus
 }
 #endif

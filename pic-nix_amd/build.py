@@ -53,8 +53,15 @@ def build(jobs=8, verbose=True):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    # the config.toml driver, linked against the library with an rpath to it
+    app = os.path.join(HERE, "picnix_run")
+    cmd = [HIPCC, "-O2", "-std=c++17", os.path.join(HERE, "apps", "picnix_run.cpp"), "-o", app,
+           "-L" + HERE, "-lpicnix_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {app}")
     return OUT
 
 

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <random>
@@ -213,6 +214,7 @@ void allocate(Sim& s)
   ss.dseg         = dalloc<int32_t>(np);
   ss.didx         = dalloc<int32_t>(np);
   ss.mkey         = dalloc<int32_t>(np);
+  ss.msrc         = dalloc<int32_t>(np);
   ss.tilecnt      = dalloc<int32_t>((size_t)g.nseg * ss.ntile * 27);
   ss.nout         = dalloc<int32_t>((size_t)g.nseg * 27);
   ss.recvoff      = dalloc<int32_t>((size_t)g.nseg * 27);
@@ -249,7 +251,7 @@ void release(Sim& s)
   f(s.d_np);
   SortScratch& ss = s.ss;
   for (void* p : {(void*)ss.cellcnt, (void*)ss.cellstart, (void*)ss.key, (void*)ss.slot,
-                  (void*)ss.dseg, (void*)ss.didx, (void*)ss.mkey, (void*)ss.tilecnt,
+                  (void*)ss.dseg, (void*)ss.didx, (void*)ss.mkey, (void*)ss.msrc, (void*)ss.tilecnt,
                   (void*)ss.nout, (void*)ss.recvoff, (void*)ss.npnew, (void*)ss.errflag})
     f(p);
   f(s.d_stage);
@@ -418,6 +420,8 @@ int pn_create(const char* toml, int load, pn_sim** out)
       return PN_ERR_UNSUPPORTED;
     }
     setup_geometry(s);
+    if (const char* k = std::getenv("PICNIX_KERNELS"))
+      s.generic_kernels = std::string(k) == "generic";
     allocate(s);
     init_fields(s);
     if (load) {
@@ -758,11 +762,17 @@ int pn_step(pn_sim* h, double delt)
     bool was_sorted = s.sorted;
     launch_push_position(s, delt);
     s.sorted = was_sorted;
-    launch_deposit(s, delt);
+    // rho is deposited after the sort when the 3D plane kernel runs: the new
+    // positions are then grouped by cell, and the result is the same sum
+    const bool rho_later = launch_deposit(s, delt, true);
     launch_push_bfd(s, 0.5 * delt);
-    launch_fold_current(s);
+    launch_fold_current(s, rho_later ? 1 : 0, 4);
     launch_push_efd(s, delt);
     launch_migrate(s);
+    if (rho_later) {
+      launch_rho(s, true);
+      launch_fold_current(s, 0, 1);
+    }
     return PN_OK;
   });
 }
@@ -801,6 +811,84 @@ int pn_synchronize(pn_sim* h)
   return guarded(h, [&](Sim& s) {
     HIPCHK(hipStreamSynchronize(s.stream));
     return check_device_flags(s);
+  });
+}
+
+} // extern "C"
+
+namespace picnix {
+
+static hipEvent_t take_event(Sim& s)
+{
+  if (!s.ev_pool.empty()) {
+    hipEvent_t e = s.ev_pool.back();
+    s.ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIPCHK(hipEventCreate(&e));
+  return e;
+}
+
+PhaseTimer::PhaseTimer(Sim& s_, int ph) : s(s_), phase(ph)
+{
+  if (!s.timing)
+    return;
+  a = take_event(s);
+  b = take_event(s);
+  HIPCHK(hipEventRecord(a, s.stream));
+}
+
+PhaseTimer::~PhaseTimer()
+{
+  if (!a)
+    return;
+  (void)hipEventRecord(b, s.stream);
+  s.ev_pending.push_back({phase, a, b});
+}
+
+static void drain_timing(Sim& s)
+{
+  HIPCHK(hipStreamSynchronize(s.stream));
+  for (auto& t : s.ev_pending) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, t.start, t.stop));
+    s.ph_ms[t.phase] += ms;
+    s.ph_count[t.phase] += 1;
+    s.ev_pool.push_back(t.start);
+    s.ev_pool.push_back(t.stop);
+  }
+  s.ev_pending.clear();
+}
+
+} // namespace picnix
+
+extern "C" {
+
+int pn_timing(pn_sim* h, int enable)
+{
+  return guarded(h, [&](Sim& s) {
+    drain_timing(s);
+    for (int i = 0; i < PN_NPHASE; i++) {
+      s.ph_ms[i]    = 0;
+      s.ph_count[i] = 0;
+    }
+    s.timing = enable != 0;
+    return PN_OK;
+  });
+}
+
+int pn_timing_report(pn_sim* h, double* ms_total, int* launches)
+{
+  return guarded(h, [&](Sim& s) {
+    drain_timing(s);
+    for (int i = 0; i < PN_NPHASE; i++) {
+      if (ms_total)
+        ms_total[i] = s.ph_ms[i];
+      if (launches)
+        launches[i] = s.ph_count[i];
+    }
+    return PN_OK;
   });
 }
 

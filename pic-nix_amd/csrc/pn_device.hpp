@@ -102,6 +102,30 @@ __device__ __forceinline__ void shape_mc(T x, T X, T rdx, T* s)
   }
 }
 
+// Slot allocation with one atomic per distinct counter in the wave: lanes that
+// want a slot from the same counter share a single atomicAdd and take
+// consecutive slots in lane order.  Every lane of the wave must call this with
+// uniform control flow; lanes with want = false get -1.
+__device__ __forceinline__ int wave_slot(int32_t* counters, int key, bool want)
+{
+  const int          lane    = (int)__lane_id();
+  unsigned long long pending = __ballot(want);
+  int                slot    = -1;
+  while (pending) {
+    const int          leader = __builtin_ctzll(pending);
+    const int          lk     = __shfl(key, leader);
+    unsigned long long m      = __ballot(want && key == lk);
+    int                base   = 0;
+    if (lane == leader)
+      base = atomicAdd(&counters[lk], __popcll(m));
+    base = __shfl(base, leader);
+    if ((m >> lane) & 1ull)
+      slot = base + __popcll(m & ((1ull << lane) - 1ull));
+    pending &= ~m;
+  }
+  return slot;
+}
+
 // Device view of the domain: global padded periodic grid + chunk tables.
 struct Geom {
   int    dim;      // 2 or 3

@@ -115,7 +115,8 @@ __global__ void __launch_bounds__(256) k_fill_ghost(Geom g, double* __restrict__
 // (the Cur exchange adds a neighbour's ghost into the interior edge,
 // nix/xtensor_halo3d.hpp:76-129).  One pass per dimension; each thread owns one
 // line, so the additions happen in a fixed order (x, then y, then z).
-__global__ void __launch_bounds__(256) k_fold_line(Geom g, double* __restrict__ J, int dir)
+__global__ void __launch_bounds__(256) k_fold_line(Geom g, double* __restrict__ J, int dir, int k0,
+                                                   int k1)
 {
   // dir: 2 = x lines over (z, y); 1 = y lines over (z, x); 0 = z lines over (y, x)
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(256) k_fold_line(Geom g, double* __restrict__ 
   int  pad = g.pad[dir], n = g.N[dir];
   long stride = dir == 2 ? 1 : (dir == 1 ? g.P[2] : g.plane);
   long base   = dir == 2 ? gidx(g, i0, i1, 0) : (dir == 1 ? gidx(g, i0, 0, i1) : gidx(g, 0, i0, i1));
-  for (int k = 0; k < 4; k++) {
+  for (int k = k0; k < k1; k++) {
     double* f = J + k * g.vol + base;
     for (int p = 0; p < pad; p++) {
       // low padding p -> interior n + p ; high padding pad + n + p -> interior pad + p
@@ -152,6 +153,7 @@ inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 void launch_push_bfd(Sim& s, double delt)
 {
+  PhaseTimer  pt(s, 0);
   const Geom& g    = s.g;
   double      cflx = g.cc * delt / g.del[2];
   double      cfly = g.cc * delt / g.del[1];
@@ -168,6 +170,7 @@ void launch_push_bfd(Sim& s, double delt)
 
 void launch_push_efd(Sim& s, double delt)
 {
+  PhaseTimer  pt(s, 5);
   const Geom& g    = s.g;
   double      cflx = g.cc * delt / g.del[2];
   double      cfly = g.cc * delt / g.del[1];
@@ -188,14 +191,16 @@ void launch_fill_ghost(Sim& s, double* f, int ncomp)
                      ncomp);
 }
 
-void launch_fold_current(Sim& s)
+void launch_fold_current(Sim& s, int k0, int k1)
 {
+  PhaseTimer  pt(s, k0 == 0 && k1 == 1 ? 7 : 4);
   const Geom& g = s.g;
   for (int dir = 2; dir >= 0; dir--) {
     if (!g.has[dir] || g.pad[dir] == 0)
       continue;
     long n = dir == 2 ? (long)g.P[0] * g.P[1] : (dir == 1 ? (long)g.P[0] * g.P[2] : g.plane);
-    hipLaunchKernelGGL(k_fold_line, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_J, dir);
+    hipLaunchKernelGGL(k_fold_line, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_J, dir, k0,
+                       k1);
   }
 }
 

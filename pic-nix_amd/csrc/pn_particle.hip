@@ -138,6 +138,196 @@ __global__ void __launch_bounds__(256)
   p.uz[i] = xu[5];
 }
 
+// Row form of the gather: one block per (bin row, segment).  The E and B
+// values every particle of the row can touch -- (Order+2) nodes around the
+// row's (z, y) bin and its whole x extent -- are staged in LDS once; each
+// particle then gathers exactly as k_push_velocity does (same weights, same
+// summation order), reading LDS instead of global memory.  A particle whose
+// stencil leaves the staged region (a rounding edge of the bin test) reads
+// global memory.
+template <int Dim, int Order>
+__global__ void __launch_bounds__(256)
+    k_push_velocity_rows(Geom g, PArr p, const int32_t* __restrict__ cellstart,
+                         const int* __restrict__ off, const double* __restrict__ lim,
+                         const double* __restrict__ E, const double* __restrict__ B,
+                         PushConsts pc)
+{
+  constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
+  constexpr int S      = Order + 1;
+  constexpr int T      = Order + 2;
+  extern __shared__ double ft[]; // [6][TZ][T][TXv]
+  const int seg = blockIdx.y;
+  const int c   = seg / g.Ns;
+  const int sp  = seg - c * g.Ns;
+  const int nbx = g.bsy;
+  int       bz = 0, by, b0;
+  if constexpr (Dim == 3) {
+    const int nby = g.bsz / g.bsy;
+    bz            = blockIdx.x / nby;
+    by            = blockIdx.x - bz * nby;
+    b0            = bz * g.bsz + by * g.bsy;
+  } else {
+    by = blockIdx.x;
+    b0 = by * g.bsy;
+  }
+  const int32_t* cs = cellstart + (long)seg * (g.nbin + 1);
+  const int      p0 = cs[b0], p1 = cs[b0 + nbx];
+  if (p1 <= p0)
+    return;
+  const int  lb  = g.Nb;
+  const int  TZ  = Dim == 3 ? T : 1;
+  const int  tz0 = Dim == 3 ? bz - is_odd + lb - (Order / 2) : lb;
+  const int  ty0 = by - is_odd + lb - (Order / 2);
+  const int  tx0 = -is_odd + lb - (Order / 2);
+  const int  TXv = nbx + Order + 1;
+  const int  pc1 = TZ * T * TXv; // per component
+  const long sy = g.P[2], sz = g.plane, v = g.vol;
+  const int  offz = off[c * 3 + 0], offy = off[c * 3 + 1], offx = off[c * 3 + 2];
+  for (int t = threadIdx.x; t < 6 * pc1; t += blockDim.x) {
+    int k  = t / pc1;
+    int r  = t - k * pc1;
+    int tz = r / (T * TXv);
+    int r2 = r - tz * T * TXv;
+    int ty = r2 / TXv;
+    int tx = r2 - ty * TXv;
+    int gz = Dim == 3 ? offz + tz0 + tz : 0;
+    const double* F = k < 3 ? E + k * v : B + (k - 3) * v;
+    ft[t] = F[gz * sz + (offy + ty0 + ty) * sy + offx + tx0 + tx];
+  }
+  __syncthreads();
+
+  const double qmdt = pc.qmdt[sp];
+  const double dx = g.del[2], dy = g.del[1], dz = g.del[0];
+  const double xmin = lim[c * 6 + 4], ymin = lim[c * 6 + 2], zmin = lim[c * 6 + 0];
+  const double ximin  = xmin + 0.5 * dx * is_odd;
+  const double xhmin  = xmin + 0.5 * dx * is_odd - 0.5 * dx;
+  const double yimin  = ymin + 0.5 * dy * is_odd;
+  const double yhmin  = ymin + 0.5 * dy * is_odd - 0.5 * dy;
+  const double xigrid = xmin + 0.5 * dx, xhgrid = xmin;
+  const double yigrid = ymin + 0.5 * dy, yhgrid = ymin;
+  const double rdx = 1 / dx, rdy = 1 / dy;
+
+  // (Order+1)^Dim-point sum of nix::interp::interp{2,3}d_impl_sorted; base[z*bs_z + y*bs_y + x]
+  auto interp3 = [&](const double* base, long bs_z, long bs_y, int z0, int y0, int x0,
+                     const double* wz, const double* wy, const double* wx) {
+    double rz = 0;
+#pragma unroll
+    for (int jz = 0; jz < S; jz++) {
+      double ry = 0;
+#pragma unroll
+      for (int jy = 0; jy < S; jy++) {
+        const double* row = base + (z0 + jz) * bs_z + (y0 + jy) * bs_y + x0;
+        double        rx  = 0;
+#pragma unroll
+        for (int jx = 0; jx < S; jx++)
+          rx += row[jx] * wx[jx];
+        ry += rx * wy[jy];
+      }
+      rz += ry * wz[jz];
+    }
+    return rz * qmdt;
+  };
+  auto interp2 = [&](const double* base, long bs_y, int y0, int x0, const double* wy,
+                     const double* wx) {
+    double ry = 0;
+#pragma unroll
+    for (int jy = 0; jy < S; jy++) {
+      const double* row = base + (y0 + jy) * bs_y + x0;
+      double        rx  = 0;
+#pragma unroll
+      for (int jx = 0; jx < S; jx++)
+        rx += row[jx] * wx[jx];
+      ry += rx * wy[jy];
+    }
+    return ry * qmdt;
+  };
+
+  for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+    const long i     = (long)seg * g.cap + q;
+    double     xu[6] = {p.x[i], p.y[i], p.z[i], p.ux[i], p.uy[i], p.uz[i]};
+    double     wix[S], wiy[S], wiz[S], whx[S], why[S], whz[S];
+    int        ix0 = digitize(xu[0], ximin, rdx);
+    int        iy0 = digitize(xu[1], yimin, rdy);
+    int        hx0 = digitize(xu[0], xhmin, rdx);
+    int        hy0 = digitize(xu[1], yhmin, rdy);
+    shape_mc<Order>(xu[0], xigrid + static_cast<double>(ix0) * dx, rdx, wix);
+    shape_mc<Order>(xu[1], yigrid + static_cast<double>(iy0) * dy, rdy, wiy);
+    shape_mc<Order>(xu[0], xhgrid + static_cast<double>(hx0) * dx, rdx, whx);
+    shape_mc<Order>(xu[1], yhgrid + static_cast<double>(hy0) * dy, rdy, why);
+    ix0 += lb - (Order / 2);
+    iy0 += lb - (Order / 2);
+    hx0 += lb - (Order / 2);
+    hy0 += lb - (Order / 2);
+    bool in = min(ix0, hx0) >= tx0 && max(ix0, hx0) + S <= tx0 + TXv && min(iy0, hy0) >= ty0 &&
+              max(iy0, hy0) + S <= ty0 + T;
+    double eb[6];
+    if constexpr (Dim == 3) {
+      const double zimin  = zmin + 0.5 * dz * is_odd;
+      const double zhmin  = zmin + 0.5 * dz * is_odd - 0.5 * dz;
+      const double zigrid = zmin + 0.5 * dz, zhgrid = zmin;
+      const double rdz    = 1 / dz;
+      int          iz0    = digitize(xu[2], zimin, rdz);
+      int          hz0    = digitize(xu[2], zhmin, rdz);
+      shape_mc<Order>(xu[2], zigrid + static_cast<double>(iz0) * dz, rdz, wiz);
+      shape_mc<Order>(xu[2], zhgrid + static_cast<double>(hz0) * dz, rdz, whz);
+      iz0 += lb - (Order / 2);
+      hz0 += lb - (Order / 2);
+      in = in && min(iz0, hz0) >= tz0 && max(iz0, hz0) + S <= tz0 + T;
+      if (in) {
+        const long bsz = (long)T * TXv, bsy = TXv;
+        const int  Z = iz0 - tz0, HZ = hz0 - tz0, Y = iy0 - ty0, HY = hy0 - ty0;
+        const int  X = ix0 - tx0, HX = hx0 - tx0;
+        eb[0] = interp3(ft + 0 * pc1, bsz, bsy, Z, Y, HX, wiz, wiy, whx);
+        eb[1] = interp3(ft + 1 * pc1, bsz, bsy, Z, HY, X, wiz, why, wix);
+        eb[2] = interp3(ft + 2 * pc1, bsz, bsy, HZ, Y, X, whz, wiy, wix);
+        eb[3] = interp3(ft + 3 * pc1, bsz, bsy, HZ, HY, X, whz, why, wix);
+        eb[4] = interp3(ft + 4 * pc1, bsz, bsy, HZ, Y, HX, whz, wiy, whx);
+        eb[5] = interp3(ft + 5 * pc1, bsz, bsy, Z, HY, HX, wiz, why, whx);
+      } else {
+        const int Z = iz0 + offz, HZ = hz0 + offz, Y = iy0 + offy, HY = hy0 + offy;
+        const int X = ix0 + offx, HX = hx0 + offx;
+        if (min(Z, HZ) < 0 || max(Z, HZ) > g.P[0] - S || min(Y, HY) < 0 ||
+            max(Y, HY) > g.P[1] - S || min(X, HX) < 0 || max(X, HX) > g.P[2] - S)
+          continue;
+        eb[0] = interp3(E, sz, sy, Z, Y, HX, wiz, wiy, whx);
+        eb[1] = interp3(E + v, sz, sy, Z, HY, X, wiz, why, wix);
+        eb[2] = interp3(E + 2 * v, sz, sy, HZ, Y, X, whz, wiy, wix);
+        eb[3] = interp3(B, sz, sy, HZ, HY, X, whz, why, wix);
+        eb[4] = interp3(B + v, sz, sy, HZ, Y, HX, whz, wiy, whx);
+        eb[5] = interp3(B + 2 * v, sz, sy, Z, HY, HX, wiz, why, whx);
+      }
+    } else {
+      (void)wiz;
+      (void)whz;
+      if (in) {
+        const long bsy = TXv;
+        const int  Y = iy0 - ty0, HY = hy0 - ty0, X = ix0 - tx0, HX = hx0 - tx0;
+        eb[0] = interp2(ft + 0 * pc1, bsy, Y, HX, wiy, whx);
+        eb[1] = interp2(ft + 1 * pc1, bsy, HY, X, why, wix);
+        eb[2] = interp2(ft + 2 * pc1, bsy, Y, X, wiy, wix);
+        eb[3] = interp2(ft + 3 * pc1, bsy, HY, X, why, wix);
+        eb[4] = interp2(ft + 4 * pc1, bsy, Y, HX, wiy, whx);
+        eb[5] = interp2(ft + 5 * pc1, bsy, HY, HX, why, whx);
+      } else {
+        const int Y = iy0 + offy, HY = hy0 + offy, X = ix0 + offx, HX = hx0 + offx;
+        if (min(Y, HY) < 0 || max(Y, HY) > g.P[1] - S || min(X, HX) < 0 ||
+            max(X, HX) > g.P[2] - S)
+          continue;
+        eb[0] = interp2(E, sy, Y, HX, wiy, whx);
+        eb[1] = interp2(E + v, sy, HY, X, why, wix);
+        eb[2] = interp2(E + 2 * v, sy, Y, X, wiy, wix);
+        eb[3] = interp2(B, sy, HY, X, why, wix);
+        eb[4] = interp2(B + v, sy, Y, HX, wiy, whx);
+        eb[5] = interp2(B + 2 * v, sy, HY, HX, why, whx);
+      }
+    }
+    push_boris(xu[3], xu[4], xu[5], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], g.cc);
+    p.ux[i] = xu[3];
+    p.uy[i] = xu[4];
+    p.uz[i] = xu[5];
+  }
+}
+
 // Bin index of a position inside chunk c, or -1 when the particle lies outside
 // the chunk.  XtensorParticle::count (nix/xtensor_particle.hpp:324-357).
 __device__ __forceinline__ int bin_of(const Geom& g, const double* __restrict__ lim,
@@ -173,9 +363,10 @@ __device__ __forceinline__ int dir_of(const double* __restrict__ rng, int c, dou
 }
 
 // BasePosition::push (pic/engine/position.hpp:117-130) followed by count(reset)
-// (Position::set_boundary, pic/pic_engine.hpp:292-303).  Stayers take an
-// arbitrary slot in their bin; leavers record their exchange direction and the
-// block's per-direction totals (one 256-particle tile per block).
+// (Position::set_boundary, pic/pic_engine.hpp:292-303).  Stayers take a slot in
+// their bin (one atomic per distinct bin in the wave); leavers record their
+// exchange direction and the block's per-direction totals (one 256-particle tile
+// per block).
 __global__ void __launch_bounds__(256)
     k_push_position(Geom g, PArr p, double* __restrict__ xo, double* __restrict__ yo,
                     double* __restrict__ zo, const int32_t* __restrict__ np,
@@ -187,33 +378,38 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x < 27)
     cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int  seg = blockIdx.y;
-  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip < np[seg]) {
-    const int  c = seg / g.Ns;
-    const long i = (long)seg * g.cap + ip;
-    double     x = p.x[i], y = p.y[i], z = p.z[i];
-    double     ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
-    xo[i]        = x;
-    yo[i]        = y;
-    zo[i]        = z;
-    double gm    = lorentz_factor(ux, uy, uz, rc);
-    double dt    = delt / gm;
+  const int  seg    = blockIdx.y;
+  const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = ip < np[seg];
+  const int  c      = seg / g.Ns;
+  const long i      = (long)seg * g.cap + ip;
+  int        b      = -1;
+  if (active) {
+    double x = p.x[i], y = p.y[i], z = p.z[i];
+    double ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+    xo[i]     = x;
+    yo[i]     = y;
+    zo[i]     = z;
+    double gm = lorentz_factor(ux, uy, uz, rc);
+    double dt = delt / gm;
     x += ux * dt;
     y += uy * dt;
     z += uz * dt;
     p.x[i] = x;
     p.y[i] = y;
     p.z[i] = z;
-    int b  = bin_of(g, lim, rng, c, x, y, z);
-    if (b >= 0) {
-      slot[i] = atomicAdd(&cellcnt[(long)seg * g.nbin + b], 1);
-      key[i]  = b;
-    } else {
+    b      = bin_of(g, lim, rng, c, x, y, z);
+    if (b < 0) {
       int d  = dir_of(rng, c, x, y, z);
       key[i] = -1 - d;
       atomicAdd(&cnt[d], 1);
     }
+  }
+  const bool stay = active && b >= 0;
+  int        sl   = wave_slot(cellcnt + (long)seg * g.nbin, b, stay);
+  if (stay) {
+    slot[i] = sl;
+    key[i]  = b;
   }
   __syncthreads();
   if (threadIdx.x < 27)
@@ -228,15 +424,18 @@ __global__ void __launch_bounds__(256)
             int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ dseg,
             int32_t* __restrict__ didx)
 {
-  const int  seg = blockIdx.y;
-  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= np[seg])
+  const int  seg    = blockIdx.y;
+  const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = ip < np[seg];
+  const int  c      = seg / g.Ns;
+  const long i      = (long)seg * g.cap + ip;
+  const int  b      = active ? bin_of(g, lim, rng, c, p.x[i], p.y[i], p.z[i]) : -1;
+  const bool stay   = active && b >= 0;
+  int        sl     = wave_slot(cellcnt + (long)seg * g.nbin, b, stay);
+  if (!active)
     return;
-  const int  c = seg / g.Ns;
-  const long i = (long)seg * g.cap + ip;
-  int        b = bin_of(g, lim, rng, c, p.x[i], p.y[i], p.z[i]);
-  if (b >= 0) {
-    slot[i] = atomicAdd(&cellcnt[(long)seg * g.nbin + b], 1);
+  if (stay) {
+    slot[i] = sl;
     key[i]  = b;
     dseg[i] = seg;
     didx[i] = (int32_t)ip;
@@ -284,14 +483,25 @@ __global__ void __launch_bounds__(256) k_soa_to_aos(PArr p, long base, long n,
 
 void launch_push_velocity(Sim& s, double delt)
 {
+  PhaseTimer  pt(s, 1);
   const Geom& g = s.g;
   PushConsts  pc{};
   for (int sp = 0; sp < g.Ns; sp++)
     pc.qmdt[sp] = 0.5 * g.q[sp] / g.m[sp] * delt;
-  dim3 grid(nblk(g.cap, 256), g.nseg);
+  dim3         grid(nblk(g.cap, 256), g.nseg);
+  const int    T    = g.order + 2;
+  const size_t lds  = sizeof(double) * 6 * (g.dim == 3 ? T : 1) * T * (g.bsy + g.order + 1);
+  const bool   rows = s.sorted && !s.generic_kernels && lds <= 65536;
+  const int    nrow = g.nbin / g.bsy;
 #define PN_VEL(D, O)                                                                             \
-  hipLaunchKernelGGL((k_push_velocity<D, O>), grid, dim3(256), 0, s.stream, g, s.pa, s.d_np,    \
-                     s.d_off, s.d_lim, s.d_E, s.d_B, pc)
+  do {                                                                                           \
+    if (rows)                                                                                    \
+      hipLaunchKernelGGL((k_push_velocity_rows<D, O>), dim3(nrow, g.nseg), dim3(256), lds,       \
+                         s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc); \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_push_velocity<D, O>), grid, dim3(256), 0, s.stream, g, s.pa,         \
+                         s.d_np, s.d_off, s.d_lim, s.d_E, s.d_B, pc);                            \
+  } while (0)
   if (g.dim == 3) {
     switch (g.order) {
     case 1: PN_VEL(3, 1); break;
@@ -312,6 +522,7 @@ void launch_push_velocity(Sim& s, double delt)
 
 void launch_push_position(Sim& s, double delt)
 {
+  PhaseTimer  pt(s, 2);
   const Geom& g = s.g;
   hipMemsetAsync(s.ss.cellcnt, 0, sizeof(int32_t) * (size_t)g.nseg * g.nbin, s.stream);
   dim3 grid(s.ss.ntile, g.nseg);

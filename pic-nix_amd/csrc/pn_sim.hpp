@@ -20,6 +20,7 @@ struct SortScratch {
   int32_t* dseg      = nullptr; // [nseg*cap]     destination segment (-1 = dropped)
   int32_t* didx      = nullptr; // [nseg*cap]     index in the destination's pre-sort array
   int32_t* mkey      = nullptr; // [nseg*cap]     member list of every bin (pre-sort indices)
+  int32_t* msrc      = nullptr; // [nseg*cap]     where each member's record currently is
   int32_t* tilecnt   = nullptr; // [nseg][ntile][27] leavers per 256-particle tile and direction
   int32_t* nout      = nullptr; // [nseg][27]     leavers per direction
   int32_t* recvoff   = nullptr; // [nseg][27]     first pre-sort index of each receive slot
@@ -62,25 +63,51 @@ struct Sim {
   SortScratch ss{};
   bool        counted = false; // keys hold a fresh count (push_position / count)
   bool        sorted  = false; // particles are in sorted order and cellstart is valid
+  bool        generic_kernels = false; // PICNIX_KERNELS=generic: one-thread-per-particle forms
 
   // staging for host <-> device particle transfers
   double* d_stage     = nullptr;
   size_t  stage_elems = 0;
 
+  // phase timing (pn_timing); events are recorded on `stream`
+  bool                         timing = false;
+  std::vector<hipEvent_t>      ev_pool;
+  struct TimedPhase {
+    int        phase;
+    hipEvent_t start, stop;
+  };
+  std::vector<TimedPhase> ev_pending;
+  double                  ph_ms[8]    = {0};
+  int                     ph_count[8] = {0};
+
   std::string err;
+};
+
+// Records HIP events around one phase when timing is enabled.
+struct PhaseTimer {
+  Sim&       s;
+  int        phase;
+  hipEvent_t a = nullptr, b = nullptr;
+  PhaseTimer(Sim& s_, int ph);
+  ~PhaseTimer();
 };
 
 // kernels / launchers (pn_field.hip, pn_particle.hip, pn_sort.hip, pn_deposit.hip, pn_diag.hip)
 void launch_push_bfd(Sim& s, double delt);
 void launch_push_efd(Sim& s, double delt);
 void launch_fill_ghost(Sim& s, double* f, int ncomp);
-void launch_fold_current(Sim& s);
+// fold the padding of J components [k0, k1) onto the periodic interior
+void launch_fold_current(Sim& s, int k0 = 0, int k1 = 4);
 void launch_push_velocity(Sim& s, double delt);
 void launch_push_position(Sim& s, double delt);
 void launch_count(Sim& s);
 void launch_migrate(Sim& s);
 void launch_sort(Sim& s);
-void launch_deposit(Sim& s, double delt);
+// Esirkepov current.  With defer_rho the 3D plane path leaves rho (component 0)
+// untouched and returns true: the caller deposits it later with launch_rho.
+bool launch_deposit(Sim& s, double delt, bool defer_rho = false);
+// rho from the current positions; sorted_by_new_cell: cellstart describes them
+void launch_rho(Sim& s, bool sorted_by_new_cell);
 void launch_energy(Sim& s, double* out /* host: efd, bfd, divE, divB, particle[Ns] */);
 void launch_aos_to_soa(Sim& s, int seg, int64_t np);
 void launch_soa_to_aos(Sim& s, int seg, int64_t np);

@@ -19,23 +19,52 @@ namespace {
 
 inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-// Exclusive prefix over tiles of the per-direction leaver counts of each segment.
-__global__ void k_tile_scan(Geom g, const int32_t* __restrict__ np, int32_t* __restrict__ tilecnt,
-                            int32_t* __restrict__ nout, int ntile)
+// Exclusive prefix over tiles of the per-direction leaver counts of each
+// segment: one block per segment, each thread owns a contiguous run of tiles.
+__global__ void __launch_bounds__(256)
+    k_tile_scan(Geom g, const int32_t* __restrict__ np, int32_t* __restrict__ tilecnt,
+                int32_t* __restrict__ nout, int ntile)
 {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.nseg * 27)
-    return;
-  int seg = t / 27, d = t % 27;
-  int nt  = (np[seg] + 255) / 256;
-  int run = 0;
-  for (int k = 0; k < nt; k++) {
-    long a      = ((long)seg * ntile + k) * 27 + d;
-    int  v      = tilecnt[a];
-    tilecnt[a]  = run;
-    run        += v;
+  __shared__ int part[27][257];
+  const int seg = blockIdx.x;
+  const int nt  = (np[seg] + 255) / 256;
+  const int per = (nt + 255) / 256;
+  const int t0  = threadIdx.x * per;
+  const int t1  = min(t0 + per, nt);
+  int32_t*  tc  = tilecnt + (long)seg * ntile * 27;
+  int       loc[27];
+  for (int d = 0; d < 27; d++)
+    loc[d] = 0;
+  for (int t = t0; t < t1; t++)
+    for (int d = 0; d < 27; d++)
+      loc[d] += tc[(long)t * 27 + d];
+  for (int d = 0; d < 27; d++)
+    part[d][threadIdx.x + 1] = loc[d];
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 27; d++)
+      part[d][0] = 0;
+  __syncthreads();
+  // inclusive scan of part[d][1..256] (Hillis-Steele), 27 independent lines
+  for (int off = 1; off < 256; off <<= 1) {
+    int v[27];
+    for (int d = 0; d < 27; d++)
+      v[d] = (int)threadIdx.x >= off ? part[d][threadIdx.x + 1 - off] : 0;
+    __syncthreads();
+    for (int d = 0; d < 27; d++)
+      part[d][threadIdx.x + 1] += v[d];
+    __syncthreads();
   }
-  nout[seg * 27 + d] = run;
+  for (int d = 0; d < 27; d++) {
+    int run = part[d][threadIdx.x];
+    for (int t = t0; t < t1; t++) {
+      long a = (long)t * 27 + d;
+      int  v = tc[a];
+      tc[a]  = run;
+      run += v;
+    }
+  }
+  if (threadIdx.x < 27)
+    nout[seg * 27 + threadIdx.x] = part[threadIdx.x][256];
 }
 
 // Receive offsets: slot order (iz, iy, ix) lexicographic, skipping the centre;
@@ -100,6 +129,35 @@ __global__ void __launch_bounds__(256)
   int        d      = (active && k < 0) ? (-1 - k) : -1;
   dirs[threadIdx.x] = d;
   __syncthreads();
+  // arrivals: destination segment, pre-sort index, wrapped position and bin
+  int  sR = -1, b = -1, idx = 0;
+  if (d >= 0) {
+    int rank = tilecnt[((long)seg * ntile + blockIdx.x) * 27 + d];
+    for (int t = 0; t < (int)threadIdx.x; t++)
+      rank += (dirs[t] == d);
+    const int c  = seg / g.Ns;
+    const int sp = seg - c * g.Ns;
+    const int R  = nb[c * 27 + d];
+    sR           = R * g.Ns + sp;
+    idx          = recvoff[sR * 27 + (26 - d)] + rank;
+
+    const double X1 = g.glim[2][0], X2 = g.glim[2][1];
+    const double Y1 = g.glim[1][0], Y2 = g.glim[1][1];
+    const double Z1 = g.glim[0][0], Z2 = g.glim[0][1];
+    const double X  = g.has[2] * (X2 - X1);
+    const double Y  = g.has[1] * (Y2 - Y1);
+    const double Z  = g.has[0] * (Z2 - Z1);
+    double       x = p.x[i], y = p.y[i], z = p.z[i];
+    x += (x < X1) * X - (x >= X2) * X;
+    y += (y < Y1) * Y - (y >= Y2) * Y;
+    z += (z < Z1) * Z - (z >= Z2) * Z;
+    p.x[i] = x;
+    p.y[i] = y;
+    p.z[i] = z;
+    b      = bin_in(g, lim, rng, R, x, y, z);
+  }
+  const bool arrive = d >= 0 && b >= 0;
+  int        sl     = wave_slot(cellcnt, arrive ? sR * g.nbin + b : 0, arrive);
   if (!active)
     return;
   if (d < 0) {
@@ -107,34 +165,11 @@ __global__ void __launch_bounds__(256)
     didx[i] = (int32_t)ip;
     return;
   }
-  int rank = tilecnt[((long)seg * ntile + blockIdx.x) * 27 + d];
-  for (int t = 0; t < (int)threadIdx.x; t++)
-    rank += (dirs[t] == d);
-  const int c  = seg / g.Ns;
-  const int sp = seg - c * g.Ns;
-  const int R  = nb[c * 27 + d];
-  const int sR = R * g.Ns + sp;
-  const int idx = recvoff[sR * 27 + (26 - d)] + rank;
-
-  const double X1 = g.glim[2][0], X2 = g.glim[2][1];
-  const double Y1 = g.glim[1][0], Y2 = g.glim[1][1];
-  const double Z1 = g.glim[0][0], Z2 = g.glim[0][1];
-  const double X  = g.has[2] * (X2 - X1);
-  const double Y  = g.has[1] * (Y2 - Y1);
-  const double Z  = g.has[0] * (Z2 - Z1);
-  double       x = p.x[i], y = p.y[i], z = p.z[i];
-  x += (x < X1) * X - (x >= X2) * X;
-  y += (y < Y1) * Y - (y >= Y2) * Y;
-  z += (z < Z1) * Z - (z >= Z2) * Z;
-  p.x[i] = x;
-  p.y[i] = y;
-  p.z[i] = z;
-  int b  = bin_in(g, lim, rng, R, x, y, z);
-  if (b < 0) {
+  if (!arrive) {
     dseg[i] = -1; // outside the receiver as well: discarded by its sort
     return;
   }
-  slot[i] = atomicAdd(&cellcnt[(long)sR * g.nbin + b], 1);
+  slot[i] = sl;
   key[i]  = b;
   dseg[i] = sR;
   didx[i] = idx;
@@ -176,12 +211,13 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Member lists: every bin receives the pre-sort indices of its particles.
+// Member lists: every bin receives the pre-sort index of each of its particles
+// (the sort key) and where the particle's record currently lives.
 __global__ void __launch_bounds__(256)
     k_members(Geom g, const int32_t* __restrict__ np, const int32_t* __restrict__ cellstart,
               const int32_t* __restrict__ key, const int32_t* __restrict__ slot,
               const int32_t* __restrict__ dseg, const int32_t* __restrict__ didx,
-              int32_t* __restrict__ mkey)
+              int32_t* __restrict__ mkey, int32_t* __restrict__ msrc)
 {
   const int  seg = blockIdx.y;
   const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -192,51 +228,64 @@ __global__ void __launch_bounds__(256)
   if (ds < 0)
     return;
   long pos = (long)cellstart[(long)ds * (g.nbin + 1) + key[i]] + slot[i];
-  if (pos < g.cap)
+  if (pos < g.cap) {
     mkey[(long)ds * g.cap + pos] = didx[i];
+    msrc[(long)ds * g.cap + pos] = (int32_t)i;
+  }
 }
 
-// Rank inside the bin by (idx mod 8, idx) and move the record.
-__global__ void __launch_bounds__(256)
-    k_rank_move(Geom g, PArr src, PArr dst, const int32_t* __restrict__ np,
-                const int32_t* __restrict__ cellstart, const int32_t* __restrict__ key,
-                const int32_t* __restrict__ dseg, const int32_t* __restrict__ didx,
-                const int32_t* __restrict__ mkey)
+// One wavefront per (segment, bin): rank the bin's members by
+// (idx mod 8, idx) -- the order of XtensorParticle::sort -- by comparing every
+// pair in registers, then move each record to start(bin) + rank.
+__device__ __forceinline__ unsigned int sort_key(int idx)
 {
-  const int  seg = blockIdx.y;
-  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= np[seg])
+  return ((unsigned int)(idx & (kSimdWidth - 1)) << 28) | ((unsigned int)idx >> 3);
+}
+
+__global__ void __launch_bounds__(256)
+    k_rank_bin(Geom g, PArr src, PArr dst, const int32_t* __restrict__ cellstart,
+               const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc)
+{
+  const int  lane = threadIdx.x & 63;
+  const long w    = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= (long)g.nseg * g.nbin)
     return;
-  const long i  = (long)seg * g.cap + ip;
-  const int  ds = dseg[i];
-  if (ds < 0)
-    return;
-  const int32_t* cs   = cellstart + (long)ds * (g.nbin + 1);
-  const int      b    = key[i];
+  const int      seg  = (int)(w / g.nbin);
+  const int      b    = (int)(w - (long)seg * g.nbin);
+  const int32_t* cs   = cellstart + (long)seg * (g.nbin + 1);
   const int      base = cs[b];
-  if (base >= g.cap)
-    return; // capacity exceeded (flagged by k_cell_scan)
-  const int      n    = min(cs[b + 1], (int)g.cap) - base;
-  const int      me   = didx[i];
-  const int      ml   = me & (kSimdWidth - 1);
-  const int32_t* mk   = mkey + (long)ds * g.cap + base;
-  int            rank = 0;
-  for (int q = 0; q < n; q++) {
-    int o  = mk[q];
-    int ol = o & (kSimdWidth - 1);
-    rank += (ol < ml) || (ol == ml && o < me);
-  }
-  long dst_i = (long)base + rank;
-  if (dst_i >= g.cap)
+  const int      end  = min(cs[b + 1], (int)g.cap);
+  if (base >= end)
     return;
-  dst_i += (long)ds * g.cap;
-  dst.x[dst_i]  = src.x[i];
-  dst.y[dst_i]  = src.y[i];
-  dst.z[dst_i]  = src.z[i];
-  dst.ux[dst_i] = src.ux[i];
-  dst.uy[dst_i] = src.uy[i];
-  dst.uz[dst_i] = src.uz[i];
-  dst.id[dst_i] = src.id[i];
+  const int      n  = end - base;
+  const int32_t* mk = mkey + (long)seg * g.cap + base;
+  const int32_t* ms = msrc + (long)seg * g.cap + base;
+  for (int c0 = 0; c0 < n; c0 += 64) {
+    const int          q     = c0 + lane;
+    const bool         valid = q < n;
+    const unsigned int mine  = valid ? sort_key(mk[q]) : 0xFFFFFFFFu;
+    int                rank  = 0;
+    for (int c1 = 0; c1 < n; c1 += 64) {
+      const int          q1    = c1 + lane;
+      const unsigned int other = q1 < n ? sort_key(mk[q1]) : 0xFFFFFFFFu;
+#pragma unroll
+      for (int j = 0; j < 64; j++) {
+        unsigned int o = (unsigned int)__builtin_amdgcn_readlane((int)other, j);
+        rank += (o < mine);
+      }
+    }
+    if (valid) {
+      const long si = ms[q];
+      const long di = (long)seg * g.cap + base + rank;
+      dst.x[di]     = src.x[si];
+      dst.y[di]     = src.y[si];
+      dst.z[di]     = src.z[si];
+      dst.ux[di]    = src.ux[si];
+      dst.uy[di]    = src.uy[si];
+      dst.uz[di]    = src.uz[si];
+      dst.id[di]    = src.id[si];
+    }
+  }
 }
 
 void finish_sort(Sim& s)
@@ -246,9 +295,10 @@ void finish_sort(Sim& s)
                      s.ss.cellstart, s.ss.npnew, s.ss.errflag);
   dim3 grid(s.ss.ntile, g.nseg);
   hipLaunchKernelGGL(k_members, grid, dim3(256), 0, s.stream, g, s.d_np, s.ss.cellstart,
-                     s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx, s.ss.mkey);
-  hipLaunchKernelGGL(k_rank_move, grid, dim3(256), 0, s.stream, g, s.pa, s.pb, s.d_np,
-                     s.ss.cellstart, s.ss.key, s.ss.dseg, s.ss.didx, s.ss.mkey);
+                     s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx, s.ss.mkey, s.ss.msrc);
+  long nwave = (long)g.nseg * g.nbin;
+  hipLaunchKernelGGL(k_rank_bin, dim3(nblk(nwave, 4)), dim3(256), 0, s.stream, g, s.pa, s.pb,
+                     s.ss.cellstart, s.ss.mkey, s.ss.msrc);
   hipMemcpyAsync(s.d_np, s.ss.npnew, sizeof(int32_t) * g.nseg, hipMemcpyDeviceToDevice,
                  s.stream);
   std::swap(s.pa, s.pb);
@@ -261,9 +311,10 @@ void finish_sort(Sim& s)
 
 void launch_migrate(Sim& s)
 {
+  PhaseTimer  pt(s, 6);
   const Geom& g = s.g;
-  hipLaunchKernelGGL(k_tile_scan, dim3(nblk((long)g.nseg * 27, 256)), dim3(256), 0, s.stream, g,
-                     s.d_np, s.ss.tilecnt, s.ss.nout, s.ss.ntile);
+  hipLaunchKernelGGL(k_tile_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.d_np,
+                     s.ss.tilecnt, s.ss.nout, s.ss.ntile);
   hipLaunchKernelGGL(k_recv_offsets, dim3(nblk(g.nseg, 256)), dim3(256), 0, s.stream, g, s.d_np,
                      s.d_nb, s.ss.nout, s.ss.recvoff);
   dim3 grid(s.ss.ntile, g.nseg);
@@ -275,6 +326,7 @@ void launch_migrate(Sim& s)
 
 void launch_sort(Sim& s)
 {
+  PhaseTimer pt(s, 6);
   launch_count(s);
   finish_sort(s);
 }

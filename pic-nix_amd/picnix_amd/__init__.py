@@ -30,8 +30,10 @@ EXPORTS = [
     "pn_get_particles_old", "pn_get_gindex", "pn_get_pindex", "pn_get_fields",
     "pn_set_fields", "pn_push_bfd", "pn_push_efd", "pn_push_velocity", "pn_push_position",
     "pn_deposit_current", "pn_sort_particle", "pn_exchange", "pn_step", "pn_get_energy",
-    "pn_get_diverror", "pn_synchronize",
+    "pn_get_diverror", "pn_synchronize", "pn_timing", "pn_timing_report",
 ]
+
+PHASES = ["bfd", "velocity", "position", "deposit", "fold", "efd", "sort", "rho"]
 
 
 class PicnixError(RuntimeError):
@@ -85,6 +87,8 @@ def load_library(path=LIB_PATH):
         "pn_get_energy": [P, dp, dp, dp],
         "pn_get_diverror": [P, dp, dp],
         "pn_synchronize": [P],
+        "pn_timing": [P, i],
+        "pn_timing_report": [P, dp, ip],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -220,6 +224,17 @@ class Simulation:
 
     def synchronize(self):
         self._chk(self.L.pn_synchronize(self.h))
+
+    def timing(self, enable=True):
+        self._chk(self.L.pn_timing(self.h, 1 if enable else 0))
+
+    def timing_report(self):
+        """{phase: (total_ms, launches)} since the last timing(True)."""
+        ms = np.zeros(len(PHASES), dtype=np.float64)
+        n = np.zeros(len(PHASES), dtype=np.int32)
+        self._chk(self.L.pn_timing_report(self.h, _dp(ms),
+                                          n.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return {p: (float(ms[i]), int(n[i])) for i, p in enumerate(PHASES)}
 
     def energy(self):
         e, b = ctypes.c_double(), ctypes.c_double()

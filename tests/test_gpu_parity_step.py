@@ -70,14 +70,20 @@ def _compare_fields(o, g, dim):
     return rel
 
 
-@pytest.mark.parametrize("kind", ["thermal3d_o2", "twostream2d_o1"])
+CASES = {
+    "thermal3d_o2": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=2)),
+    "thermal3d_o1": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=1)),
+    "thermal3d_o3": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=4, order=3)),
+    "thermal3d_o4": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=4, order=4)),
+    "twostream2d_o1": (2, lambda: helpers.small_twostream(N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1)),
+    "twostream2d_o2": (2, lambda: helpers.small_twostream(N=(1, 16, 64), C=(1, 2, 4), np_=16, order=2)),
+}
+
+
+@pytest.mark.parametrize("kind", list(CASES))
 def test_step_parity(kind):
-    if kind == "thermal3d_o2":
-        cfg = helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=2)
-        dim = 3
-    else:
-        cfg = helpers.small_twostream(N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1)
-        dim = 2
+    dim, make = CASES[kind]
+    cfg = make()
     dt = float(cfg["parameter"]["delt"])
     _say(f"[parity] {kind}: create oracle + GPU")
     o = oracle_lib.Oracle(cfg)
