@@ -422,6 +422,8 @@ int pn_create(const char* toml, int load, pn_sim** out)
     setup_geometry(s);
     if (const char* k = std::getenv("PICNIX_KERNELS"))
       s.generic_kernels = std::string(k) == "generic";
+    if (const char* k = std::getenv("PICNIX_MARCH_ROWS"))
+      s.march_rows = std::max(0, std::atoi(k));
     allocate(s);
     init_fields(s);
     if (load) {

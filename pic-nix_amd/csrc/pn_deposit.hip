@@ -240,6 +240,18 @@ __device__ __forceinline__ long gnode_checked(const Geom& g, const int* __restri
   return ((long)gz * g.P[1] + gy) * g.P[2] + gx;
 }
 
+// Value of lane (lane ^ M) within each quad of lanes, M = 1 or 2 (DPP
+// quad_perm; no LDS traffic).
+template <int M>
+__device__ __forceinline__ double quad_xor(double x)
+{
+  constexpr int ctrl = M == 1 ? 0xB1 /* quad_perm [1,0,3,2] */ : 0x4E /* [2,3,0,1] */;
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo     = __builtin_amdgcn_mov_dpp(lo, ctrl, 0xF, 0xF, true);
+  hi     = __builtin_amdgcn_mov_dpp(hi, ctrl, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
 // Generic form: one thread per particle, contributions go straight to global J.
 template <int Dim, int Order>
 __global__ void __launch_bounds__(256)
@@ -352,75 +364,113 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Plane kernel (3D current): one block per (bin row (bz, by), segment); each
-// wavefront takes whole cells of the row from a block-level counter.  All
-// particles of a bin share the tile origin (the old cell, pic/engine/
-// current.hpp:390-392), so a cell's contributions are summed in registers.
-// Jx, Jy and Jz have the same structure: a plane weight
-//   w = -((s0F[f] + A dsF[f]) s0S[s] + (A s0F[f] + B dsF[f]) dsS[s]) q d/dt
-// with (F, S) = (y, z), (z, x), (x, y), followed by a prefix sum along x, y, z
-// (nix/esirkepov.hpp:168-232).  A lane owns one plane of one component -- N rows
-// of N values, the rows running across the plane and the prefix along each row
-// -- for one particle of a slot of PPS particles, so every active lane evaluates
-// the same expression on its own operands.  Shape weights are computed
-// lane-parallel per batch and read back from LDS.  Each value added is the
-// reference's per-particle tile entry; per lane the additions follow the sorted
-// particle order.  After the cell, the PPS partial planes are summed with xor
-// shuffles and added to the block's LDS tile; the tile goes to global J once per
-// block.  rho is deposited after the sort (k_rho_cells) or by k_rho_global.
+// March kernel (3D current): one block per (bin column by, chunk, range of bin
+// rows bz).  The block walks its rows in z order; for each row its wavefronts
+// take whole cells, of every species of the chunk, from a block-level counter.
+// All particles of a bin share the tile origin (the old cell,
+// pic/engine/current.hpp:390-392), so the contributions of a cell are summed
+// in registers.  Jx, Jy and Jz have one structure (nix/esirkepov.hpp:168-232):
+// a plane weight
+//   w = -((1 s0F[f] + A dsF[f]) s0S[s] + (A s0F[f] + B dsF[f]) dsS[s]) q d/dt
+// with (F, S) = (y, z), (z, x), (x, y), and the running sum along C = x, y, z
+//   J(f, s, c + 1) += sum_{c' <= c} dsC[c'] w  =  P[c] w,
+// P being the prefix sum of dsC.  The kernel evaluates P[c] w, one fused
+// multiply-add per node; the reference adds the products one at a time.  The
+// two agree to rounding (a few ulp of each contribution), the tolerance the
+// current is held to (the sum over particles runs in another order anyway).
+// A lane owns the plane F = f of one component (N rows s of N-1 nodes) for one
+// particle of a slot of PPS particles, so all lanes evaluate the same
+// expression on their own operands.  The shape weights are computed by one lane
+// per (particle, axis) and handed over in LDS; the coordinates of the next batch
+// are loaded while the current one is processed.  After a cell, the PPS partial
+// planes are summed across lanes (DPP) and added to the block's LDS tile, a ring
+// of N z-planes: when the block moves on to the next row, the lowest z-plane
+// holds all of this block's contributions and goes to global J (one atomic per
+// non-zero node), and its slot becomes the new top plane.
+// rho is deposited after the sort (k_rho_march) or by k_rho_global.
 template <int Order>
-struct PlanePlan {
+struct MarchPlan {
   static constexpr int N     = Order + 3;
   static constexpr int NPL   = 3 * N; // planes per particle (3 components x N)
   static constexpr int PPS   = 64 / NPL >= 4 ? 4 : (64 / NPL >= 2 ? 2 : 1);
-  static constexpr int BATCH = 16;
-  static constexpr int PAIRS = 0;                   // (s0[j], ds[j]) per axis: [3][N][2]
-  static constexpr int DSC   = 6 * N;               // ds per axis, contiguous: [3][N]
-  static constexpr int FLAG  = 9 * N;               // 1: origin of the bin, 0: own origin
-  static constexpr int ORG   = 9 * N + 1;           // own tile origin (x, y, z), local nodes
-  static constexpr int REC   = (9 * N + 5) & ~1;    // doubles per particle record
+  static constexpr int BATCH = 16;    // phase 1 uses 3 x BATCH lanes
+  static constexpr int NS    = N & ~1;                // even stride of the prefix sums (>= N-1)
+  static constexpr int PAIRS = 0;                     // (s0[l], ds[l]) per axis: [3][N][2]
+  static constexpr int PRE   = 6 * N;                 // prefix of ds per axis: [3][NS]
+  static constexpr int FLAG  = PRE + 3 * NS;          // 1: origin of the bin, 0: own origin
+  static constexpr int ORG   = FLAG + 1;              // own tile origin (x, y, z), local nodes
+  static constexpr int REC   = (ORG + 3 + 1) & ~1;    // doubles per particle record (even)
+  static constexpr int NWAVE = 4;
+  static constexpr size_t WLDS = sizeof(double) * NWAVE * BATCH * REC;
 };
+
+// Dynamic LDS of the march kernels: the ring tile (doubles), then the bin
+// offsets of the current row for every species (ints).
+inline size_t march_lds(const Geom& g, int ntile)
+{
+  return sizeof(double) * (size_t)ntile + sizeof(int) * (size_t)g.Ns * (g.bsy + 1);
+}
 
 template <int Order>
 __global__ void __launch_bounds__(256)
-    k_deposit_planes(Geom g, const PArr p, const double* __restrict__ xo,
-                     const double* __restrict__ yo, const double* __restrict__ zo,
-                     const int32_t* __restrict__ cellstart, const int* __restrict__ off,
-                     const double* __restrict__ lim, double* __restrict__ J, DepConsts dc)
+    k_deposit_march(Geom g, const PArr p, const double* __restrict__ xo,
+                    const double* __restrict__ yo, const double* __restrict__ zo,
+                    const int32_t* __restrict__ cellstart, const int* __restrict__ off,
+                    const double* __restrict__ lim, double* __restrict__ J, DepConsts dc,
+                    int zrows)
 {
-  using PP             = PlanePlan<Order>;
-  constexpr int N      = PP::N;
-  constexpr int NPL    = PP::NPL;
-  constexpr int PPS    = PP::PPS;
-  constexpr int BATCH  = PP::BATCH;
-  constexpr int REC    = PP::REC;
+  using MP             = MarchPlan<Order>;
+  constexpr int N      = MP::N;
+  constexpr int NPL    = MP::NPL;
+  constexpr int PPS    = MP::PPS;
+  constexpr int BATCH  = MP::BATCH;
+  constexpr int NS     = MP::NS;
+  constexpr int REC    = MP::REC;
   constexpr int is_odd = Order % 2;
-  constexpr int NWAVE  = 4;
+  constexpr int NWAVE  = MP::NWAVE;
 
-  __shared__ double wts[NWAVE][BATCH][REC];
-  __shared__ int    next_cell;
-  extern __shared__ double tile[]; // [3][N][N][TX] (Jx, Jy, Jz)
+  __shared__ __align__(16) double wts[NWAVE][BATCH][REC];
+  __shared__ int                  next_cell;
+  extern __shared__ double        tile[]; // [3][N z-ring][N][TX] (Jx, Jy, Jz), then rcs
 
-  const int seg = blockIdx.y;
-  const int c   = seg / g.Ns;
-  const int sp  = seg - c * g.Ns;
+  const int c   = blockIdx.y;
+  const int ns  = g.Ns;
   const int nbx = g.bsy;
   const int nby = g.bsz / g.bsy;
-  const int bz  = blockIdx.x / nby;
-  const int by  = blockIdx.x - bz * nby;
-  const int32_t* cs = cellstart + (long)seg * (g.nbin + 1);
-  const int      b0 = bz * g.bsz + by * g.bsy;
-  if (cs[b0 + nbx] <= cs[b0])
-    return; // empty row (uniform for the block)
+  const int nbz = g.nbin / g.bsz;
+  const int by  = blockIdx.x % nby;
+  const int bz0 = (blockIdx.x / nby) * zrows;
+  const int bz1 = min(nbz, bz0 + zrows);
+
+  // nothing to do when every row of the range is empty for every species
+  int any = 0;
+  for (int q = threadIdx.x; q < ns * (bz1 - bz0); q += blockDim.x) {
+    const int      sp = q % ns;
+    const int      bz = bz0 + q / ns;
+    const int32_t* cs = cellstart + (long)(c * ns + sp) * (g.nbin + 1);
+    const int      b0 = bz * g.bsz + by * g.bsy;
+    any |= cs[b0 + nbx] > cs[b0] ? 1 : 0;
+  }
+  if (!__syncthreads_or(any))
+    return; // uniform for the block
 
   const int lb  = g.Nb;
-  const int zlo = bz - is_odd + lb - (Order / 2) - 1; // tile origin (local node index)
-  const int ylo = by - is_odd + lb - (Order / 2) - 1;
+  const int ylo = by - is_odd + lb - (Order / 2) - 1; // tile origin (local node index)
   const int xlo = -is_odd + lb - (Order / 2) - 1;
   const int TX  = nbx + N - 1;
-  const int tsz = N * N * TX;
-  for (int t = threadIdx.x; t < 3 * tsz; t += blockDim.x)
-    tile[t] = 0.0;
+  const int psz = N * TX; // one z-plane of one component
+  const int tsz = N * psz;
+  int*      rcs = reinterpret_cast<int*>(tile + 3 * tsz); // [ns][nbx + 1]
+  for (int q = threadIdx.x; q < 3 * tsz; q += blockDim.x)
+    tile[q] = 0.0;
+  auto load_row = [&](int bz) {
+    for (int q = threadIdx.x; q < ns * (nbx + 1); q += blockDim.x) {
+      const int sp = q / (nbx + 1);
+      const int i  = q - sp * (nbx + 1);
+      rcs[q] = cellstart[(long)(c * ns + sp) * (g.nbin + 1) + bz * g.bsz + by * g.bsy + i];
+    }
+  };
+  load_row(bz0);
   if (threadIdx.x == 0)
     next_cell = 0;
   __syncthreads();
@@ -429,71 +479,100 @@ __global__ void __launch_bounds__(256)
   const int    wv   = threadIdx.x >> 6;
   const long   v    = g.vol;
   const double xmin = lim[c * 6 + 4], ymin = lim[c * 6 + 2], zmin = lim[c * 6 + 0];
-  const double qs   = dc.q[sp];
   const double A    = 1.0 / 2;
   const double B    = 1.0 / 3;
 
-  // lane -> (plane, slot particle k); plane -> (component, plane index t)
+  // phase-2 lane -> (plane, slot particle k); plane -> (component, index t along F)
   const bool   act   = lane < NPL * PPS;
   const int    plane = act ? lane / PPS : 0;
-  const int    k     = lane - (lane / PPS) * PPS;
-  const int    comp  = plane / N;            // 0 Jx, 1 Jy, 2 Jz
+  const int    k     = lane & (PPS - 1);
+  const int    comp  = plane / N; // 0 Jx, 1 Jy, 2 Jz
   const int    t     = plane - comp * N;
-  // axes (0 x, 1 y, 2 z): F, S of the plane weight, C the prefix direction
-  const int    axF   = comp == 0 ? 1 : (comp == 1 ? 2 : 0);
+  const int    axF   = comp == 0 ? 1 : (comp == 1 ? 2 : 0); // axes: 0 x, 1 y, 2 z
   const int    axS   = comp == 0 ? 2 : (comp == 1 ? 0 : 1);
   const int    axC   = comp;
-  const bool   pIsS  = comp != 1;            // the plane index runs along S (else F)
-  const int    axP   = pIsS ? axS : axF;
-  const int    axR   = pIsS ? axF : axS;
-  const double qd    = qs * (comp == 0 ? dc.dxdt : (comp == 1 ? dc.dydt : dc.dzdt));
+  const double ddt   = comp == 0 ? dc.dxdt : (comp == 1 ? dc.dydt : dc.dzdt);
+  // node offsets (z, y, x) of plane t, row r, prefix j: F = t, S = r, C = j + 1
+  // phase-1 lane -> (axis, batch particle)
+  const int     a1    = lane / BATCH; // 3: idle
+  const int     k1    = lane - a1 * BATCH;
+  const double* X0    = a1 == 0 ? xo : (a1 == 1 ? yo : zo);
+  const double* X1    = a1 == 0 ? p.x : (a1 == 1 ? p.y : p.z);
+  const double  d1    = a1 == 0 ? g.del[2] : (a1 == 1 ? g.del[1] : g.del[0]);
+  const double  mn1   = a1 == 0 ? xmin : (a1 == 1 ? ymin : zmin);
+  const double  rd1   = 1 / d1;
+  const double  imin1 = mn1 + 0.5 * d1 * is_odd;
+  const double  grid1 = mn1 + 0.5 * d1;
+  double*       W     = &wts[wv][0][0];
 
-  double* W = &wts[wv][0][0];
+  // claim the next non-empty cell of the row (uniform per wave); false when done
+  auto claim = [&](int& sp, int& bx, int& p0, int& p1) {
+    for (;;) {
+      int q = 0;
+      if (lane == 0)
+        q = atomicAdd(&next_cell, 1);
+      q = __builtin_amdgcn_readfirstlane(q);
+      if (q >= ns * nbx)
+        return false;
+      sp = q / nbx;
+      bx = q - sp * nbx;
+      p0 = rcs[sp * (nbx + 1) + bx];
+      p1 = rcs[sp * (nbx + 1) + bx + 1];
+      if (p1 > p0)
+        return true;
+    }
+  };
+  // coordinates of batch particle k1 (axis a1) of [pb, p1) in segment seg
+  auto fetch = [&](int seg, int pb, int p1, double& xv, double& xu) {
+    if (a1 < 3 && pb + k1 < p1) {
+      const long i = (long)seg * g.cap + pb + k1;
+      xv           = X0[i];
+      xu           = X1[i];
+    }
+  };
 
-  for (;;) {
-    int bx = 0;
-    if (lane == 0)
-      bx = atomicAdd(&next_cell, 1);
-    bx = __shfl(bx, 0);
-    if (bx >= nbx)
-      break;
-    const int p0 = cs[b0 + bx], p1 = cs[b0 + bx + 1];
-    if (p0 >= p1)
-      continue;
-    const int ox = bx + xlo;
+  for (int bz = bz0; bz < bz1; bz++) {
+    const int zlo = bz - is_odd + lb - (Order / 2) - 1;
+    const int zs  = ((zlo % N) + N) % N; // ring slot of z-plane zlo
+    int       sp, bx, p0, p1;
+    bool      have = claim(sp, bx, p0, p1);
+    double    nxv = 0, nxu = 0; // prefetched coordinates
+    if (have)
+      fetch(c * ns + sp, p0, p1, nxv, nxu);
+    while (have) {
+      int        spn, bxn, p0n, p1n;
+      const bool more = claim(spn, bxn, p0n, p1n);
+      const int  seg  = c * ns + sp;
+      const int  ox   = bx + xlo;
+      const double qd = dc.q[sp] * ddt; // qs * dxdt etc. (esirkepov.hpp:172-174)
 
-    double acc[N][N];
+      double acc[N][N - 1];
 #pragma unroll
-    for (int r = 0; r < N; r++)
+      for (int r = 0; r < N; r++)
 #pragma unroll
-      for (int j = 0; j < N; j++)
-        acc[r][j] = 0.0;
+        for (int j = 0; j < N - 1; j++)
+          acc[r][j] = 0.0;
 
-    for (int pb = p0; pb < p1; pb += BATCH) {
-      const int nbt = min(BATCH, p1 - pb);
-      // ---- phase 1: shape weights, lane-parallel ----
-      if (lane < nbt) {
-        const long   i       = (long)seg * g.cap + pb + lane;
-        double       xv[3]   = {xo[i], yo[i], zo[i]};
-        double       xu[3]   = {p.x[i], p.y[i], p.z[i]};
-        const double d3[3]   = {g.del[2], g.del[1], g.del[0]};
-        const double mn[3]   = {xmin, ymin, zmin};
-        const int    want[3] = {ox, ylo, zlo};
-        double*      rec     = W + lane * REC;
-        bool         same    = true;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-          const double rd    = 1 / d3[a];
-          const double imin  = mn[a] + 0.5 * d3[a] * is_odd;
-          const double grid0 = mn[a] + 0.5 * d3[a];
-          double       s0[N], s1[N];
+      for (int pb = p0; pb < p1; pb += BATCH) {
+        const int    nbt = min(BATCH, p1 - pb);
+        const double xv  = nxv, xu = nxu;
+        if (pb + BATCH < p1)
+          fetch(seg, pb + BATCH, p1, nxv, nxu);
+        else if (more)
+          fetch(c * ns + spn, p0n, p1n, nxv, nxu);
+        // ---- phase 1: shape weights, one lane per (particle, axis) ----
+        const bool on1  = a1 < 3 && k1 < nbt;
+        bool       same = false;
+        if (on1) {
+          double s0[N], s1[N];
 #pragma unroll
           for (int l = 0; l < N; l++)
             s0[l] = s1[l] = 0.0;
-          int i0 = digitize(xv[a], imin, rd);
-          int i1 = digitize(xu[a], imin, rd);
-          shape_mc<Order>(xv[a], grid0 + static_cast<double>(i0) * d3[a], rd, &s0[1]);
-          shape_mc<Order>(xu[a], grid0 + static_cast<double>(i1) * d3[a], rd, &s1[1]);
+          const int i0 = digitize(xv, imin1, rd1);
+          const int i1 = digitize(xu, imin1, rd1);
+          shape_mc<Order>(xv, grid1 + static_cast<double>(i0) * d1, rd1, &s0[1]);
+          shape_mc<Order>(xu, grid1 + static_cast<double>(i1) * d1, rd1, &s1[1]);
+          // shift_weights (nix/esirkepov.hpp:239-270)
           if (i1 - i0 < 0) {
 #pragma unroll
             for (int ii = 0; ii < Order + 2; ii++)
@@ -504,153 +583,187 @@ __global__ void __launch_bounds__(256)
               s1[ii] = s1[ii - 1];
           }
           const int org  = i0 + lb - (Order / 2) - 1;
-          same           = same && (org == want[a]);
-          rec[PP::ORG + a] = static_cast<double>(org);
+          const int want = a1 == 0 ? ox : (a1 == 1 ? ylo : zlo);
+          same           = org == want;
+          double* rec    = W + k1 * REC;
+          rec[MP::ORG + a1] = static_cast<double>(org);
+          double2* pr       = reinterpret_cast<double2*>(rec + MP::PAIRS) + a1 * N;
+          double   pre      = 0;
 #pragma unroll
           for (int l = 0; l < N; l++) {
             const double ds = s1[l] - s0[l]; // DS (esirkepov.hpp:157-166)
-            rec[PP::PAIRS + (a * N + l) * 2]     = s0[l];
-            rec[PP::PAIRS + (a * N + l) * 2 + 1] = ds;
-            rec[PP::DSC + a * N + l]             = ds;
+            pr[l]           = make_double2(s0[l], ds);
+            pre += ds;
+            if (l < N - 1)
+              rec[MP::PRE + a1 * NS + l] = pre;
           }
         }
-        rec[PP::FLAG] = same ? 1.0 : 0.0;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // ---- phase 2: each lane accumulates its plane ----
-      for (int k0 = 0; k0 < nbt; k0 += PPS) {
-        const int kk = k0 + k;
-        if (act && kk < nbt) {
-          const double* rec = W + kk * REC;
-          const double  pa0 = rec[PP::PAIRS + (axP * N + t) * 2];     // s0 along P at t
-          const double  pa1 = rec[PP::PAIRS + (axP * N + t) * 2 + 1]; // ds along P at t
-          double        dsC[N - 1];
+        {
+          const unsigned long long m = __ballot(on1 && same);
+          if (lane < nbt) {
+            const bool all3 =
+                ((m >> lane) & (m >> (lane + BATCH)) & (m >> (lane + 2 * BATCH)) & 1ull) != 0;
+            W[lane * REC + MP::FLAG] = all3 ? 1.0 : 0.0;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // ---- phase 2: each lane accumulates its plane ----
+        for (int k0 = 0; k0 < nbt; k0 += PPS) {
+          const int kk = k0 + k;
+          if (act && kk < nbt) {
+            const double*  rec = W + kk * REC;
+            const double2* pr  = reinterpret_cast<const double2*>(rec + MP::PAIRS);
+            const double2  pf  = pr[axF * N + t]; // (s0F[t], dsF[t])
+            const double   aF  = pf.x + A * pf.y;
+            const double   bF  = A * pf.x + B * pf.y;
+            const double2* pc  = reinterpret_cast<const double2*>(rec + MP::PRE + axC * NS);
+            double         P[NS];
 #pragma unroll
-          for (int j = 0; j < N - 1; j++)
-            dsC[j] = rec[PP::DSC + axC * N + j];
-          if (rec[PP::FLAG] != 0.0) {
-#pragma unroll
-            for (int r = 0; r < N; r++) {
-              const double ra0 = rec[PP::PAIRS + (axR * N + r) * 2];
-              const double ra1 = rec[PP::PAIRS + (axR * N + r) * 2 + 1];
-              const double s0F = pIsS ? ra0 : pa0, dsF = pIsS ? ra1 : pa1;
-              const double s0S = pIsS ? pa0 : ra0, dsS = pIsS ? pa1 : ra1;
-              const double w   = -((1 * s0F + A * dsF) * s0S + (A * s0F + B * dsF) * dsS) * qd;
-              double       ww  = 0;
-#pragma unroll
-              for (int j = 0; j < N - 1; j++) {
-                ww += dsC[j] * w;
-                acc[r][j + 1] += ww;
-              }
+            for (int j = 0; j < NS / 2; j++) {
+              const double2 d2 = pc[j];
+              P[2 * j]         = d2.x;
+              P[2 * j + 1]     = d2.y;
             }
-          } else {
-            // the particle's tile origin differs from its bin's (odd orders: the
-            // bin and shape digitize offsets differ by half a cell, and the two
-            // roundings can disagree): same values, added at its own nodes
-            const int pox = (int)rec[PP::ORG + 0];
-            const int poy = (int)rec[PP::ORG + 1];
-            const int poz = (int)rec[PP::ORG + 2];
-            for (int r = 0; r < N; r++) {
-              const double ra0 = rec[PP::PAIRS + (axR * N + r) * 2];
-              const double ra1 = rec[PP::PAIRS + (axR * N + r) * 2 + 1];
-              const double s0F = pIsS ? ra0 : pa0, dsF = pIsS ? ra1 : pa1;
-              const double s0S = pIsS ? pa0 : ra0, dsS = pIsS ? pa1 : ra1;
-              const double w   = -((1 * s0F + A * dsF) * s0S + (A * s0F + B * dsF) * dsS) * qd;
-              double       ww  = 0;
-              for (int j = 0; j < N - 1; j++) {
-                ww += dsC[j] * w;
-                int lz, ly, lx;
-                if (comp == 0) {
-                  lz = poz + t, ly = poy + r, lx = pox + j + 1;
-                } else if (comp == 1) {
-                  lz = poz + t, ly = poy + j + 1, lx = pox + r;
-                } else {
-                  lz = poz + j + 1, ly = poy + t, lx = pox + r;
+            if (rec[MP::FLAG] != 0.0) {
+#pragma unroll
+              for (int r = 0; r < N; r++) {
+                const double2 ps = pr[axS * N + r]; // (s0S[r], dsS[r])
+                const double  w  = -(aF * ps.x + bF * ps.y) * qd;
+#pragma unroll
+                for (int j = 0; j < N - 1; j++)
+                  acc[r][j] = fma(P[j], w, acc[r][j]);
+              }
+            } else {
+              // the particle's tile origin differs from its bin's (odd orders: the
+              // bin and shape digitize offsets differ by half a cell, and the two
+              // roundings can disagree): same values, added at its own nodes
+              const int pox = (int)rec[MP::ORG + 0];
+              const int poy = (int)rec[MP::ORG + 1];
+              const int poz = (int)rec[MP::ORG + 2];
+#pragma unroll 1
+              for (int r = 0; r < N; r++) {
+                const double2 ps = pr[axS * N + r];
+                const double  w  = -(aF * ps.x + bF * ps.y) * qd;
+#pragma unroll 1
+                for (int j = 0; j < N - 1; j++) {
+                  const double val = P[j] * w;
+                  const int    oz  = comp == 0 ? r : (comp == 1 ? t : j + 1);
+                  const int    oy  = comp == 0 ? t : (comp == 1 ? j + 1 : r);
+                  const int    oxx = comp == 0 ? j + 1 : (comp == 1 ? r : t);
+                  const int    lz = poz + oz, ly = poy + oy, lx = pox + oxx;
+                  const int    tz = lz - zlo, ty = ly - ylo, tx = lx - xlo;
+                  if ((unsigned)tz < (unsigned)N && (unsigned)ty < (unsigned)N &&
+                      (unsigned)tx < (unsigned)TX) {
+                    const int sl = zs + tz >= N ? zs + tz - N : zs + tz;
+                    atomicAdd(&tile[comp * tsz + sl * psz + ty * TX + tx], val);
+                  } else if (long gi = gnode_checked(g, off, c, lz, ly, lx); gi >= 0) {
+                    unsafeAtomicAdd(&J[(comp + 1) * v + gi], val);
+                  }
                 }
-                const int tz = lz - zlo, ty = ly - ylo, tx = lx - xlo;
-                if ((unsigned)tz < (unsigned)N && (unsigned)ty < (unsigned)N &&
-                    (unsigned)tx < (unsigned)TX)
-                  atomicAdd(&tile[((comp * N + tz) * N + ty) * TX + tx], ww);
-                else if (long gi = gnode_checked(g, off, c, lz, ly, lx); gi >= 0)
-                  unsafeAtomicAdd(&J[(comp + 1) * v + gi], ww);
               }
             }
           }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    // ---- reduce the PPS partial planes and add them to the block tile ----
-#pragma unroll
-    for (int r = 0; r < N; r++)
-#pragma unroll
-      for (int j = 1; j < N; j++) {
-        double a = acc[r][j];
-        if (PPS >= 2)
-          a += __shfl_xor(a, 1);
-        if (PPS >= 4)
-          a += __shfl_xor(a, 2);
-        acc[r][j] = a;
-      }
-    if (act && k == 0) {
-      const int tx0 = ox - xlo;
+      // ---- sum the PPS partial planes, add them to the ring tile ----
 #pragma unroll
       for (int r = 0; r < N; r++)
 #pragma unroll
-        for (int j = 1; j < N; j++) {
-          // node: P-axis = t, R-axis = r, C-axis = j
-          int idx;
-          if (comp == 0) // Jx: (z = t, y = r, x = j)
-            idx = ((0 * N + t) * N + r) * TX + tx0 + j;
-          else if (comp == 1) // Jy: (z = t, y = j, x = r)
-            idx = ((1 * N + t) * N + j) * TX + tx0 + r;
-          else // Jz: (z = j, y = t, x = r)
-            idx = ((2 * N + j) * N + t) * TX + tx0 + r;
-          atomicAdd(&tile[idx], acc[r][j]);
+        for (int j = 0; j < N - 1; j++) {
+          double s = acc[r][j];
+          if (PPS >= 2)
+            s += quad_xor<1>(s);
+          if (PPS >= 4)
+            s += quad_xor<2>(s);
+          acc[r][j] = s;
         }
+      if (act && k == 0) {
+#pragma unroll
+        for (int r = 0; r < N; r++)
+#pragma unroll
+          for (int j = 0; j < N - 1; j++) {
+            const int oz  = comp == 0 ? r : (comp == 1 ? t : j + 1);
+            const int oy  = comp == 0 ? t : (comp == 1 ? j + 1 : r);
+            const int oxx = comp == 0 ? j + 1 : (comp == 1 ? r : t);
+            const int sl  = zs + oz >= N ? zs + oz - N : zs + oz;
+            atomicAdd(&tile[comp * tsz + sl * psz + oy * TX + bx + oxx], acc[r][j]);
+          }
+      }
+      have = more;
+      sp   = spn;
+      bx   = bxn;
+      p0   = p0n;
+      p1   = p1n;
     }
+    __syncthreads();
+    // z-plane zlo is complete for this block: flush and clear its ring slot
+    for (int q = threadIdx.x; q < 3 * psz; q += blockDim.x) {
+      const int    kc  = q / psz;
+      const int    r2  = q - kc * psz;
+      const int    idx = kc * tsz + zs * psz + r2;
+      const double val = tile[idx];
+      if (val != 0.0) {
+        const int  ty = r2 / TX;
+        const int  tx = r2 - ty * TX;
+        const long gi = gnode_checked(g, off, c, zlo, ylo + ty, xlo + tx);
+        if (gi >= 0)
+          unsafeAtomicAdd(&J[(kc + 1) * v + gi], val);
+        tile[idx] = 0.0;
+      }
+    }
+    if (bz + 1 < bz1)
+      load_row(bz + 1);
+    if (threadIdx.x == 0)
+      next_cell = 0;
+    __syncthreads();
   }
-  __syncthreads();
-  for (int q = threadIdx.x; q < 3 * tsz; q += blockDim.x) {
-    double val = tile[q];
+  // the N-1 planes above the last row's zlo
+  const int zl = (bz1 - 1) - is_odd + lb - (Order / 2) - 1;
+  const int zs = ((zl % N) + N) % N;
+  for (int q = threadIdx.x; q < 3 * (N - 1) * psz; q += blockDim.x) {
+    const int    kc  = q / ((N - 1) * psz);
+    const int    r   = q - kc * (N - 1) * psz;
+    const int    dz  = 1 + r / psz;
+    const int    r2  = r - (dz - 1) * psz;
+    const int    sl  = zs + dz >= N ? zs + dz - N : zs + dz;
+    const double val = tile[kc * tsz + sl * psz + r2];
     if (val == 0.0)
       continue;
-    int kc = q / tsz;
-    int r2 = q - kc * tsz;
-    int tz = r2 / (N * TX);
-    int r3 = r2 - tz * N * TX;
-    int ty = r3 / TX;
-    int tx = r3 - ty * TX;
-    const long gi = gnode_checked(g, off, c, tz + zlo, ty + ylo, tx + xlo);
+    const int  ty = r2 / TX;
+    const int  tx = r2 - ty * TX;
+    const long gi = gnode_checked(g, off, c, zl + dz, ylo + ty, xlo + tx);
     if (gi >= 0)
       unsafeAtomicAdd(&J[(kc + 1) * v + gi], val);
   }
 }
 
-// rho from the new positions (3D), one block per (bin row, segment) after the
-// sort: the reference's rho term qs S1x S1y S1z (ro3d, nix/esirkepov.hpp:145-155)
-// uses the new-position weights, which -- once shifted back -- are the (Order+1)
-// unshifted weights around the new cell.  Lanes own (jz plane, slot particle);
-// PPS_R particles per slot; partials are xor-shuffle reduced per cell.
+// rho from the new positions (3D), after the sort: the reference's rho term
+// qs S1x S1y S1z (ro3d, nix/esirkepov.hpp:145-155) uses the new-position
+// weights, which -- undoing the shift -- are the Order+1 unshifted weights around
+// the new cell.  Same block walk as k_deposit_march, with a ring of S = Order+1
+// z-planes; lanes own (z plane t, slot particle k), PPS particles per slot.
 template <int Order>
 struct RhoPlan {
   static constexpr int S     = Order + 1;
   static constexpr int PPS   = 64 / S >= 32 ? 32 : (64 / S >= 16 ? 16 : 8);
   static constexpr int BATCH = 64;
-  static constexpr int FLAG  = 3 * S;      // 1: origin of the bin, 0: own origin
-  static constexpr int ORG   = 3 * S + 1;  // own origin (x, y, z), local nodes
+  static constexpr int FLAG  = 3 * S;     // 1: origin of the bin, 0: own origin
+  static constexpr int ORG   = 3 * S + 1; // own origin (x, y, z), local nodes
   static constexpr int REC   = (3 * S + 5) & ~1;
+  static constexpr int NWAVE = 4;
+  static constexpr size_t WLDS = sizeof(double) * NWAVE * BATCH * REC;
 };
 
 template <int Order>
 __global__ void __launch_bounds__(256)
-    k_rho_cells(Geom g, const PArr p, const int32_t* __restrict__ cellstart,
+    k_rho_march(Geom g, const PArr p, const int32_t* __restrict__ cellstart,
                 const int* __restrict__ off, const double* __restrict__ lim,
-                double* __restrict__ J)
+                double* __restrict__ J, int zrows)
 {
   using RP             = RhoPlan<Order>;
   constexpr int S      = RP::S;
@@ -658,158 +771,235 @@ __global__ void __launch_bounds__(256)
   constexpr int BATCH  = RP::BATCH;
   constexpr int REC    = RP::REC;
   constexpr int is_odd = Order % 2;
-  constexpr int NWAVE  = 4;
+  constexpr int NWAVE  = RP::NWAVE;
 
   __shared__ double wts[NWAVE][BATCH][REC];
   __shared__ int    next_cell;
-  extern __shared__ double tile[]; // [S][S][TX]
+  extern __shared__ double tile[]; // [S z-ring][S][TX], then rcs
 
-  const int seg = blockIdx.y;
-  const int c   = seg / g.Ns;
-  const int sp  = seg - c * g.Ns;
+  const int c   = blockIdx.y;
+  const int ns  = g.Ns;
   const int nbx = g.bsy;
   const int nby = g.bsz / g.bsy;
-  const int bz  = blockIdx.x / nby;
-  const int by  = blockIdx.x - bz * nby;
-  const int32_t* cs = cellstart + (long)seg * (g.nbin + 1);
-  const int      b0 = bz * g.bsz + by * g.bsy;
-  if (cs[b0 + nbx] <= cs[b0])
+  const int nbz = g.nbin / g.bsz;
+  const int by  = blockIdx.x % nby;
+  const int bz0 = (blockIdx.x / nby) * zrows;
+  const int bz1 = min(nbz, bz0 + zrows);
+
+  int any = 0;
+  for (int q = threadIdx.x; q < ns * (bz1 - bz0); q += blockDim.x) {
+    const int      sp = q % ns;
+    const int      bz = bz0 + q / ns;
+    const int32_t* cs = cellstart + (long)(c * ns + sp) * (g.nbin + 1);
+    const int      b0 = bz * g.bsz + by * g.bsy;
+    any |= cs[b0 + nbx] > cs[b0] ? 1 : 0;
+  }
+  if (!__syncthreads_or(any))
     return;
 
   const int lb  = g.Nb;
-  const int zlo = bz - is_odd + lb - (Order / 2);
   const int ylo = by - is_odd + lb - (Order / 2);
   const int xlo = -is_odd + lb - (Order / 2);
   const int TX  = nbx + Order;
-  const int tsz = S * S * TX;
+  const int psz = S * TX;
+  const int tsz = S * psz;
+  int*      rcs = reinterpret_cast<int*>(tile + tsz); // [ns][nbx + 1]
   for (int q = threadIdx.x; q < tsz; q += blockDim.x)
     tile[q] = 0.0;
+  auto load_row = [&](int bz) {
+    for (int q = threadIdx.x; q < ns * (nbx + 1); q += blockDim.x) {
+      const int sp = q / (nbx + 1);
+      const int i  = q - sp * (nbx + 1);
+      rcs[q] = cellstart[(long)(c * ns + sp) * (g.nbin + 1) + bz * g.bsz + by * g.bsy + i];
+    }
+  };
+  load_row(bz0);
   if (threadIdx.x == 0)
     next_cell = 0;
   __syncthreads();
 
   const int    lane = threadIdx.x & 63;
   const int    wv   = threadIdx.x >> 6;
-  const long   v    = g.vol;
   const double xmin = lim[c * 6 + 4], ymin = lim[c * 6 + 2], zmin = lim[c * 6 + 0];
-  const double qs   = g.q[sp];
   const bool   act  = lane < S * PPS;
-  const int    t    = act ? lane / PPS : 0; // jz plane
-  const int    k    = lane - (lane / PPS) * PPS;
+  const int    t    = act ? lane / PPS : 0; // z plane
+  const int    k    = lane & (PPS - 1);
   double*      W    = &wts[wv][0][0];
 
-  for (;;) {
-    int bx = 0;
-    if (lane == 0)
-      bx = atomicAdd(&next_cell, 1);
-    bx = __shfl(bx, 0);
-    if (bx >= nbx)
-      break;
-    const int p0 = cs[b0 + bx], p1 = cs[b0 + bx + 1];
-    if (p0 >= p1)
-      continue;
-    double acc[S][S];
+  auto claim = [&](int& sp, int& bx, int& p0, int& p1) {
+    for (;;) {
+      int q = 0;
+      if (lane == 0)
+        q = atomicAdd(&next_cell, 1);
+      q = __builtin_amdgcn_readfirstlane(q);
+      if (q >= ns * nbx)
+        return false;
+      sp = q / nbx;
+      bx = q - sp * nbx;
+      p0 = rcs[sp * (nbx + 1) + bx];
+      p1 = rcs[sp * (nbx + 1) + bx + 1];
+      if (p1 > p0)
+        return true;
+    }
+  };
+  auto fetch = [&](int seg, int pb, int p1, double* x3) {
+    if (pb + lane < p1) {
+      const long i = (long)seg * g.cap + pb + lane;
+      x3[0]        = p.x[i];
+      x3[1]        = p.y[i];
+      x3[2]        = p.z[i];
+    }
+  };
+
+  for (int bz = bz0; bz < bz1; bz++) {
+    const int zlo = bz - is_odd + lb - (Order / 2);
+    const int zs  = ((zlo % S) + S) % S;
+    int       sp, bx, p0, p1;
+    bool      have   = claim(sp, bx, p0, p1);
+    double    nx3[3] = {0, 0, 0};
+    if (have)
+      fetch(c * ns + sp, p0, p1, nx3);
+    while (have) {
+      int          spn, bxn, p0n, p1n;
+      const bool   more = claim(spn, bxn, p0n, p1n);
+      const int    seg  = c * ns + sp;
+      const double qs   = g.q[sp];
+      double       acc[S][S];
 #pragma unroll
-    for (int a = 0; a < S; a++)
+      for (int a = 0; a < S; a++)
 #pragma unroll
-      for (int b = 0; b < S; b++)
-        acc[a][b] = 0.0;
-    for (int pb = p0; pb < p1; pb += BATCH) {
-      const int nbt = min(BATCH, p1 - pb);
-      if (lane < nbt) {
-        const long   i       = (long)seg * g.cap + pb + lane;
-        double       xu[3]   = {p.x[i], p.y[i], p.z[i]};
-        const double d3[3]   = {g.del[2], g.del[1], g.del[0]};
-        const double mn[3]   = {xmin, ymin, zmin};
-        const int    want[3] = {bx + xlo, ylo, zlo};
-        double*      rec     = W + lane * REC;
-        bool         same    = true;
+        for (int e = 0; e < S; e++)
+          acc[a][e] = 0.0;
+      for (int pb = p0; pb < p1; pb += BATCH) {
+        const int    nbt   = min(BATCH, p1 - pb);
+        const double xu[3] = {nx3[0], nx3[1], nx3[2]};
+        if (pb + BATCH < p1)
+          fetch(seg, pb + BATCH, p1, nx3);
+        else if (more)
+          fetch(c * ns + spn, p0n, p1n, nx3);
+        if (lane < nbt) {
+          const double d3[3]   = {g.del[2], g.del[1], g.del[0]};
+          const double mn[3]   = {xmin, ymin, zmin};
+          const int    want[3] = {bx + xlo, ylo, zlo};
+          double*      rec     = W + lane * REC;
+          bool         same    = true;
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-          const double rd    = 1 / d3[a];
-          const double imin  = mn[a] + 0.5 * d3[a] * is_odd;
-          const double grid0 = mn[a] + 0.5 * d3[a];
-          const int    cell  = digitize(xu[a], imin, rd);
-          shape_mc<Order>(xu[a], grid0 + static_cast<double>(cell) * d3[a], rd, &rec[a * S]);
-          const int org    = cell + lb - (Order / 2);
-          same             = same && (org == want[a]);
-          rec[RP::ORG + a] = static_cast<double>(org);
+          for (int a = 0; a < 3; a++) {
+            const double rd    = 1 / d3[a];
+            const double imin  = mn[a] + 0.5 * d3[a] * is_odd;
+            const double grid0 = mn[a] + 0.5 * d3[a];
+            const int    cell  = digitize(xu[a], imin, rd);
+            shape_mc<Order>(xu[a], grid0 + static_cast<double>(cell) * d3[a], rd, &rec[a * S]);
+            const int org    = cell + lb - (Order / 2);
+            same             = same && (org == want[a]);
+            rec[RP::ORG + a] = static_cast<double>(org);
+          }
+          rec[RP::FLAG] = same ? 1.0 : 0.0;
         }
-        rec[RP::FLAG] = same ? 1.0 : 0.0;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (int k0 = 0; k0 < nbt; k0 += PPS) {
-        const int kk = k0 + k;
-        if (act && kk < nbt) {
-          const double* rec = W + kk * REC;
-          const double  sz  = rec[2 * S + t];
-          double        sx[S];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k0 = 0; k0 < nbt; k0 += PPS) {
+          const int kk = k0 + k;
+          if (act && kk < nbt) {
+            const double* rec = W + kk * REC;
+            const double  sz  = rec[2 * S + t];
+            double        sx[S];
 #pragma unroll
-          for (int jx = 0; jx < S; jx++)
-            sx[jx] = rec[jx];
-          if (rec[RP::FLAG] != 0.0) {
+            for (int jx = 0; jx < S; jx++)
+              sx[jx] = rec[jx];
+            if (rec[RP::FLAG] != 0.0) {
 #pragma unroll
-            for (int jy = 0; jy < S; jy++) {
-              const double sy = rec[S + jy];
+              for (int jy = 0; jy < S; jy++) {
+                const double sy = rec[S + jy];
 #pragma unroll
-              for (int jx = 0; jx < S; jx++)
-                acc[jy][jx] += qs * sx[jx] * sy * sz;
-            }
-          } else {
-            // own origin (odd orders, see k_deposit_planes)
-            const int pox = (int)rec[RP::ORG + 0];
-            const int poy = (int)rec[RP::ORG + 1];
-            const int poz = (int)rec[RP::ORG + 2];
-            for (int jy = 0; jy < S; jy++) {
-              const double sy = rec[S + jy];
-              for (int jx = 0; jx < S; jx++) {
-                const double val = qs * sx[jx] * sy * sz;
-                const int    lz = poz + t, ly = poy + jy, lx = pox + jx;
-                const int    tz = lz - zlo, ty = ly - ylo, tx = lx - xlo;
-                if ((unsigned)tz < (unsigned)S && (unsigned)ty < (unsigned)S &&
-                    (unsigned)tx < (unsigned)TX)
-                  atomicAdd(&tile[(tz * S + ty) * TX + tx], val);
-                else if (long gi = gnode_checked(g, off, c, lz, ly, lx); gi >= 0)
-                  unsafeAtomicAdd(&J[gi], val);
+                for (int jx = 0; jx < S; jx++)
+                  acc[jy][jx] += qs * sx[jx] * sy * sz;
+              }
+            } else {
+              // own origin (odd orders, as in k_deposit_march)
+              const int pox = (int)rec[RP::ORG + 0];
+              const int poy = (int)rec[RP::ORG + 1];
+              const int poz = (int)rec[RP::ORG + 2];
+              for (int jy = 0; jy < S; jy++) {
+                const double sy = rec[S + jy];
+                for (int jx = 0; jx < S; jx++) {
+                  const double val = qs * sx[jx] * sy * sz;
+                  const int    lz = poz + t, ly = poy + jy, lx = pox + jx;
+                  const int    tz = lz - zlo, ty = ly - ylo, tx = lx - xlo;
+                  if ((unsigned)tz < (unsigned)S && (unsigned)ty < (unsigned)S &&
+                      (unsigned)tx < (unsigned)TX) {
+                    const int sl = zs + tz >= S ? zs + tz - S : zs + tz;
+                    atomicAdd(&tile[sl * psz + ty * TX + tx], val);
+                  } else if (long gi = gnode_checked(g, off, c, lz, ly, lx); gi >= 0) {
+                    unsafeAtomicAdd(&J[gi], val);
+                  }
+                }
               }
             }
           }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
 #pragma unroll
-    for (int a = 0; a < S; a++)
+      for (int a = 0; a < S; a++)
 #pragma unroll
-      for (int b = 0; b < S; b++) {
-        double x = acc[a][b];
+        for (int e = 0; e < S; e++) {
+          double x = acc[a][e];
+          x += quad_xor<1>(x);
+          x += quad_xor<2>(x);
 #pragma unroll
-        for (int m = 1; m < PPS; m <<= 1)
-          x += __shfl_xor(x, m);
-        acc[a][b] = x;
+          for (int m = 4; m < PPS; m <<= 1)
+            x += __shfl_xor(x, m);
+          acc[a][e] = x;
+        }
+      if (act && k == 0) {
+        const int sl = zs + t >= S ? zs + t - S : zs + t;
+#pragma unroll
+        for (int jy = 0; jy < S; jy++)
+#pragma unroll
+          for (int jx = 0; jx < S; jx++)
+            atomicAdd(&tile[sl * psz + jy * TX + bx + jx], acc[jy][jx]);
       }
-    if (act && k == 0) {
-#pragma unroll
-      for (int jy = 0; jy < S; jy++)
-#pragma unroll
-        for (int jx = 0; jx < S; jx++)
-          atomicAdd(&tile[(t * S + jy) * TX + bx + jx], acc[jy][jx]);
+      have = more;
+      sp   = spn;
+      bx   = bxn;
+      p0   = p0n;
+      p1   = p1n;
     }
+    __syncthreads();
+    for (int q = threadIdx.x; q < psz; q += blockDim.x) {
+      const int    idx = zs * psz + q;
+      const double val = tile[idx];
+      if (val != 0.0) {
+        const int  ty = q / TX;
+        const int  tx = q - ty * TX;
+        const long gi = gnode_checked(g, off, c, zlo, ylo + ty, xlo + tx);
+        if (gi >= 0)
+          unsafeAtomicAdd(&J[gi], val);
+        tile[idx] = 0.0;
+      }
+    }
+    if (bz + 1 < bz1)
+      load_row(bz + 1);
+    if (threadIdx.x == 0)
+      next_cell = 0;
+    __syncthreads();
   }
-  __syncthreads();
-  for (int q = threadIdx.x; q < tsz; q += blockDim.x) {
-    double val = tile[q];
+  const int zl = (bz1 - 1) - is_odd + lb - (Order / 2);
+  const int zs = ((zl % S) + S) % S;
+  for (int q = threadIdx.x; q < (S - 1) * psz; q += blockDim.x) {
+    const int    dz  = 1 + q / psz;
+    const int    r2  = q - (dz - 1) * psz;
+    const int    sl  = zs + dz >= S ? zs + dz - S : zs + dz;
+    const double val = tile[sl * psz + r2];
     if (val == 0.0)
       continue;
-    int tz = q / (S * TX);
-    int r3 = q - tz * S * TX;
-    int ty = r3 / TX;
-    int tx = r3 - ty * TX;
-    const long gi = gnode_checked(g, off, c, tz + zlo, ty + ylo, tx + xlo);
+    const int  ty = r2 / TX;
+    const int  tx = r2 - ty * TX;
+    const long gi = gnode_checked(g, off, c, zl + dz, ylo + ty, xlo + tx);
     if (gi >= 0)
       unsafeAtomicAdd(&J[gi], val);
   }
@@ -854,7 +1044,20 @@ __global__ void __launch_bounds__(256)
       }
 }
 
-// Returns true when rho still has to be deposited (3D plane path).
+// Rows of bins per block of the march kernels: whole bin columns unless the
+// grid would then hold fewer than ~2048 blocks (8 per CU), in which case each
+// column is cut into nzr ranges of zrows rows.  `fixed` > 0 sets zrows.
+void march_split(const Geom& g, int fixed, int& zrows, int& nzr)
+{
+  const int  nby  = g.bsz / g.bsy;
+  const int  nbz  = g.nbin / g.bsz;
+  const long cols = (long)nby * g.nchunk;
+  const int  split = (int)std::min<long>(nbz, std::max<long>(1, (2048 + cols - 1) / cols));
+  zrows            = fixed > 0 ? std::min(fixed, nbz) : (nbz + split - 1) / split;
+  nzr              = (nbz + zrows - 1) / zrows;
+}
+
+// Returns true when rho still has to be deposited (3D march path).
 template <int Dim, int Order>
 bool deposit_impl(Sim& s, const DepConsts& dc)
 {
@@ -870,13 +1073,13 @@ bool deposit_impl(Sim& s, const DepConsts& dc)
     lds   = sizeof(double) * 4 * (size_t)N * TX;
     nslab = g.dims[1] + 1;
   }
-  const size_t tlds = sizeof(double) * 3 * (size_t)N * N * (g.bsy + N - 1);
-  const size_t wlds = sizeof(double) * 4 * PlanePlan<Order>::BATCH * PlanePlan<Order>::REC;
-  if (Dim == 3 && s.sorted && !s.generic_kernels && tlds + wlds + 64 <= 65536) {
-    const int nrow = g.nbin / g.bsy;
-    hipLaunchKernelGGL((k_deposit_planes<Order>), dim3(nrow, g.nseg), dim3(256), tlds, s.stream,
-                       g, s.pa, s.d_xo[0], s.d_xo[1], s.d_xo[2], s.ss.cellstart, s.d_off,
-                       s.d_lim, s.d_J, dc);
+  const size_t mlds = march_lds(g, 3 * N * N * (g.bsy + N - 1));
+  if (Dim == 3 && s.sorted && !s.generic_kernels && mlds + MarchPlan<Order>::WLDS + 64 <= 65536) {
+    int zrows, nzr;
+    march_split(g, s.march_rows, zrows, nzr);
+    hipLaunchKernelGGL((k_deposit_march<Order>), dim3(nzr * (g.bsz / g.bsy), g.nchunk), dim3(256),
+                       mlds, s.stream, g, s.pa, s.d_xo[0], s.d_xo[1], s.d_xo[2], s.ss.cellstart,
+                       s.d_off, s.d_lim, s.d_J, dc, zrows);
     return true;
   }
   if (s.sorted && lds <= 65536 && !s.generic_kernels) {
@@ -896,11 +1099,13 @@ void rho_impl(Sim& s, bool sorted_by_new_cell)
 {
   const Geom& g = s.g;
   const int   S = Order + 1;
-  const size_t tlds = sizeof(double) * (size_t)S * S * (g.bsy + Order);
-  if (Dim == 3 && sorted_by_new_cell && !s.generic_kernels && tlds <= 32768) {
-    const int nrow = g.nbin / g.bsy;
-    hipLaunchKernelGGL((k_rho_cells<Order>), dim3(nrow, g.nseg), dim3(256), tlds, s.stream, g,
-                       s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_J);
+  const size_t mlds = march_lds(g, S * S * (g.bsy + Order));
+  if (Dim == 3 && sorted_by_new_cell && !s.generic_kernels &&
+      mlds + RhoPlan<Order>::WLDS + 64 <= 65536) {
+    int zrows, nzr;
+    march_split(g, s.march_rows, zrows, nzr);
+    hipLaunchKernelGGL((k_rho_march<Order>), dim3(nzr * (g.bsz / g.bsy), g.nchunk), dim3(256),
+                       mlds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_J, zrows);
   } else {
     hipLaunchKernelGGL((k_rho_global<Dim, Order>), dim3(nblk(g.cap, 256), g.nseg), dim3(256), 0,
                        s.stream, g, s.pa, s.d_np, s.d_off, s.d_lim, s.d_J);

@@ -126,6 +126,19 @@ __device__ __forceinline__ int wave_slot(int32_t* counters, int key, bool want)
   return slot;
 }
 
+// Workgroups of a launch are dispatched to the 8 XCDs of an MI355X round-robin
+// (id % 8), and each XCD has its own L2.  xcd_block maps the hardware block id
+// of a grid of xcd_grid(nblocks) blocks so that every XCD runs one contiguous
+// range of logical ids: blocks that touch neighbouring data share an L2.  The
+// result is >= nblocks for the few padding blocks, which must exit.
+constexpr int kNumXcd = 8;
+inline long xcd_grid(long nblocks) { return (nblocks + kNumXcd - 1) / kNumXcd * kNumXcd; }
+__device__ __forceinline__ long xcd_block(long hw, long nblocks)
+{
+  const long per = (nblocks + kNumXcd - 1) / kNumXcd;
+  return (hw % kNumXcd) * per + hw / kNumXcd;
+}
+
 // Device view of the domain: global padded periodic grid + chunk tables.
 struct Geom {
   int    dim;      // 2 or 3

@@ -64,6 +64,7 @@ struct Sim {
   bool        counted = false; // keys hold a fresh count (push_position / count)
   bool        sorted  = false; // particles are in sorted order and cellstart is valid
   bool        generic_kernels = false; // PICNIX_KERNELS=generic: one-thread-per-particle forms
+  int         march_rows      = 0;     // PICNIX_MARCH_ROWS: bin rows per march block (0: auto)
 
   // staging for host <-> device particle transfers
   double* d_stage     = nullptr;

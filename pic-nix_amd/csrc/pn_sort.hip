@@ -244,10 +244,15 @@ __device__ __forceinline__ unsigned int sort_key(int idx)
 
 __global__ void __launch_bounds__(256)
     k_rank_bin(Geom g, PArr src, PArr dst, const int32_t* __restrict__ cellstart,
-               const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc)
+               const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc, long nblocks)
 {
+  // consecutive bins on one XCD: records moving between neighbouring bins are
+  // then read through the same L2
+  const long blk = xcd_block(blockIdx.x, nblocks);
+  if (blk >= nblocks)
+    return;
   const int  lane = threadIdx.x & 63;
-  const long w    = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long w    = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= (long)g.nseg * g.nbin)
     return;
   const int      seg  = (int)(w / g.nbin);
@@ -296,9 +301,10 @@ void finish_sort(Sim& s)
   dim3 grid(s.ss.ntile, g.nseg);
   hipLaunchKernelGGL(k_members, grid, dim3(256), 0, s.stream, g, s.d_np, s.ss.cellstart,
                      s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx, s.ss.mkey, s.ss.msrc);
-  long nwave = (long)g.nseg * g.nbin;
-  hipLaunchKernelGGL(k_rank_bin, dim3(nblk(nwave, 4)), dim3(256), 0, s.stream, g, s.pa, s.pb,
-                     s.ss.cellstart, s.ss.mkey, s.ss.msrc);
+  const long nwave = (long)g.nseg * g.nbin;
+  const long nrb   = nblk(nwave, 4);
+  hipLaunchKernelGGL(k_rank_bin, dim3(xcd_grid(nrb)), dim3(256), 0, s.stream, g, s.pa, s.pb,
+                     s.ss.cellstart, s.ss.mkey, s.ss.msrc, nrb);
   hipMemcpyAsync(s.d_np, s.ss.npnew, sizeof(int32_t) * g.nseg, hipMemcpyDeviceToDevice,
                  s.stream);
   std::swap(s.pa, s.pb);

@@ -3,9 +3,11 @@
 The sequence is the one PicApplication::push_openmp runs for a step
 (pic/pic_application.cpp:219-292), issued operator by operator through the C
 ABI, with a comparison after each operator:
-  * push_bfd / push_efd: interior fields bit-exact (one expression per cell);
-  * push_velocity / push_position: particles bit-exact, cell indices
-    (XtensorParticle::gindex) bit-exact;
+  * push_bfd / push_efd: interior fields bit-exact (one expression per cell)
+    while the inputs are; once the current has entered E (it is summed in
+    another order), within 1e-12 relative;
+  * push_velocity / push_position: particles bit-exact on bit-exact fields
+    (within 1e-12 afterwards), cell indices (XtensorParticle::gindex) bit-exact;
   * deposit_current + Cur exchange: uj interior <= 1e-12 relative (the sum
     over particles runs in another order);
   * Particle exchange (migration + sort): particle order (ids) and pindex
@@ -51,7 +53,7 @@ def _fields_rel(o, g, which, dim, ncomp):
     return np.where(scale > 0, worst / np.where(scale > 0, scale, 1), worst)
 
 
-def _particles_equal(o, g, what):
+def _particles_equal(o, g, what, exact=True):
     for c in range(o.nchunks):
         for s in range(o.Ns):
             a = o.particles(c, s)
@@ -59,7 +61,12 @@ def _particles_equal(o, g, what):
             assert a.shape == b.shape, f"{what}: chunk {c} species {s}: np differs"
             assert np.array_equal(a[:, 6].view(np.int64), b[:, 6].view(np.int64)), \
                 f"{what}: chunk {c} species {s}: particle order differs"
-            assert np.array_equal(a[:, :6], b[:, :6]), f"{what}: chunk {c} species {s}: values differ"
+            if exact:
+                assert np.array_equal(a[:, :6], b[:, :6]), \
+                    f"{what}: chunk {c} species {s}: values differ"
+            else:
+                assert helpers.rel_err(b[:, :6], a[:, :6]) <= TOL, \
+                    f"{what}: chunk {c} species {s}: values differ beyond {TOL}"
 
 
 def _gindex_equal(o, g, what):
@@ -86,21 +93,22 @@ CASES = {
 EMF, CUR, MOM, PARTICLE = 0, 1, 2, 3
 
 
-def _run_sequence(o, g, dim, dt, tag):
+def _run_sequence(o, g, dim, dt, tag, exact):
+    """One step, operator by operator; `exact`: the fields still match bit for bit."""
     o.push_bfd(0.5 * dt)
     g.push_bfd(0.5 * dt)
     rel = _fields_rel(o, g, 0, dim, 6)
     _say(f"[ops] {tag}: push_bfd(dt/2) rel {rel}")
-    assert np.all(rel == 0.0)
+    assert np.all(rel == 0.0) if exact else np.all(rel <= TOL)
 
     o.push_velocity(dt)
     g.push_velocity(dt)
-    _particles_equal(o, g, f"{tag} push_velocity")
+    _particles_equal(o, g, f"{tag} push_velocity", exact)
     _say(f"[ops] {tag}: push_velocity ok")
 
     o.push_position(dt)
     g.push_position(dt)
-    _particles_equal(o, g, f"{tag} push_position")
+    _particles_equal(o, g, f"{tag} push_position", exact)
     _gindex_equal(o, g, f"{tag} count")
     _say(f"[ops] {tag}: push_position + count ok")
 
@@ -124,7 +132,7 @@ def _run_sequence(o, g, dim, dt, tag):
 
     o.exchange(PARTICLE)
     g.exchange(PARTICLE)
-    _particles_equal(o, g, f"{tag} particle exchange")
+    _particles_equal(o, g, f"{tag} particle exchange", exact)
     _pindex_equal(o, g, f"{tag} particle exchange")
     _say(f"[ops] {tag}: particle exchange + sort ok")
 
@@ -138,7 +146,7 @@ def test_operator_sequence(kind):
     o = oracle_lib.Oracle(cfg)
     g = _sim(cfg)
     for it in range(2):
-        _run_sequence(o, g, dim, dt, f"{kind} step {it + 1}")
+        _run_sequence(o, g, dim, dt, f"{kind} step {it + 1}", exact=(it == 0))
     g.close()
     o.close()
 
@@ -218,7 +226,7 @@ def test_placed_particles(kind):
     _particles_equal(o, g, f"{kind} placed exchange")
     _pindex_equal(o, g, f"{kind} placed exchange")
     # sorted now: one more sequence on the LDS kernels
-    _run_sequence(o, g, dim, dt, f"{kind} placed sorted")
+    _run_sequence(o, g, dim, dt, f"{kind} placed sorted", exact=False)
     g.close()
     o.close()
 

@@ -124,3 +124,28 @@ def test_step_parity(kind):
     assert np.allclose(hg[4:], ho[4:], rtol=1e-10)
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("rows", [2, 1000])
+def test_step_parity_march_rows(rows, monkeypatch):
+    """The 3D deposit blocks walk `rows` bin rows each (PICNIX_MARCH_ROWS; 1000 =
+    whole columns), so the z-ring of the LDS tile wraps and flushes mid-block."""
+    monkeypatch.setenv("PICNIX_MARCH_ROWS", str(rows))
+    for kind in ("thermal3d_o2", "thermal3d_o3"):
+        dim, make = CASES[kind]
+        cfg = make()
+        dt = float(cfg["parameter"]["delt"])
+        _say(f"[parity] march rows {rows} {kind}: create")
+        o = oracle_lib.Oracle(cfg)
+        g = _sim(cfg)
+        for k in range(4):
+            o.step()
+            g.step(dt)
+        g.synchronize()
+        perr = _compare_particles(o, g, exact=False)
+        rel = _compare_fields(o, g, dim)
+        _say(f"[parity] march rows {rows} {kind}: step 4 particle rel err {perr:.3e}, fields {rel}")
+        assert perr <= TOL
+        assert max(rel.values()) <= TOL
+        g.close()
+        o.close()
