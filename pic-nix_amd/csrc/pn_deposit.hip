@@ -394,14 +394,17 @@ struct MarchPlan {
   static constexpr int NPL   = 3 * N; // planes per particle (3 components x N)
   static constexpr int PPS   = 64 / NPL >= 4 ? 4 : (64 / NPL >= 2 ? 2 : 1);
   static constexpr int BATCH = 16;    // phase 1 uses 3 x BATCH lanes
-  static constexpr int NS    = N & ~1;                // even stride of the prefix sums (>= N-1)
-  static constexpr int PAIRS = 0;                     // (s0[l], ds[l]) per axis: [3][N][2]
-  static constexpr int PRE   = 6 * N;                 // prefix of ds per axis: [3][NS]
-  static constexpr int FLAG  = PRE + 3 * NS;          // 1: origin of the bin, 0: own origin
-  static constexpr int ORG   = FLAG + 1;              // own tile origin (x, y, z), local nodes
-  static constexpr int REC   = (ORG + 3 + 1) & ~1;    // doubles per particle record (even)
+  static constexpr int NS    = N & ~1;   // prefix sums per axis, padded to pairs (>= N-1)
+  // Weights live in LDS as pairs of doubles, [axis][row][particle]: rows 0..N-1 hold
+  // (s0[l], ds[l]), rows N.. the prefix sums of ds two by two.  The particle pitch
+  // PITCH = BATCH + 4 pairs puts the 4 particles of a slot, and the neighbouring
+  // rows of one particle, on distinct LDS banks.
+  static constexpr int ROWS  = N + NS / 2;
+  static constexpr int PITCH = BATCH + 4;
+  static constexpr int WPAIR = 3 * ROWS * PITCH; // pairs per wave
   static constexpr int NWAVE = 4;
-  static constexpr size_t WLDS = sizeof(double) * NWAVE * BATCH * REC;
+  static constexpr size_t WLDS =
+      (sizeof(double) * 2 * WPAIR + sizeof(int) * 4 * BATCH) * NWAVE;
 };
 
 // Dynamic LDS of the march kernels: the ring tile (doubles), then the bin
@@ -425,13 +428,15 @@ __global__ void __launch_bounds__(256)
   constexpr int PPS    = MP::PPS;
   constexpr int BATCH  = MP::BATCH;
   constexpr int NS     = MP::NS;
-  constexpr int REC    = MP::REC;
+  constexpr int ROWS   = MP::ROWS;
+  constexpr int PITCH  = MP::PITCH;
   constexpr int is_odd = Order % 2;
   constexpr int NWAVE  = MP::NWAVE;
 
-  __shared__ __align__(16) double wts[NWAVE][BATCH][REC];
-  __shared__ int                  next_cell;
-  extern __shared__ double        tile[]; // [3][N z-ring][N][TX] (Jx, Jy, Jz), then rcs
+  __shared__ double2 wts[NWAVE][MP::WPAIR];
+  __shared__ int4    forg[NWAVE][BATCH]; // (same origin as the bin, own origin x, y, z)
+  __shared__ int     next_cell;
+  extern __shared__ double tile[]; // [3][N z-ring][N][TX] (Jx, Jy, Jz), then rcs
 
   const int c   = blockIdx.y;
   const int ns  = g.Ns;
@@ -503,7 +508,8 @@ __global__ void __launch_bounds__(256)
   const double  rd1   = 1 / d1;
   const double  imin1 = mn1 + 0.5 * d1 * is_odd;
   const double  grid1 = mn1 + 0.5 * d1;
-  double*       W     = &wts[wv][0][0];
+  double2*      W     = &wts[wv][0];
+  int4*         FO    = &forg[wv][0];
 
   // claim the next non-empty cell of the row (uniform per wave); false when done
   auto claim = [&](int& sp, int& bx, int& p0, int& p1) {
@@ -563,6 +569,7 @@ __global__ void __launch_bounds__(256)
         // ---- phase 1: shape weights, one lane per (particle, axis) ----
         const bool on1  = a1 < 3 && k1 < nbt;
         bool       same = false;
+        int        org  = 0;
         if (on1) {
           double s0[N], s1[N];
 #pragma unroll
@@ -582,28 +589,34 @@ __global__ void __launch_bounds__(256)
             for (int ii = Order + 2; ii > 0; ii--)
               s1[ii] = s1[ii - 1];
           }
-          const int org  = i0 + lb - (Order / 2) - 1;
+          org            = i0 + lb - (Order / 2) - 1;
           const int want = a1 == 0 ? ox : (a1 == 1 ? ylo : zlo);
           same           = org == want;
-          double* rec    = W + k1 * REC;
-          rec[MP::ORG + a1] = static_cast<double>(org);
-          double2* pr       = reinterpret_cast<double2*>(rec + MP::PAIRS) + a1 * N;
-          double   pre      = 0;
+          double2* pr    = W + a1 * ROWS * PITCH + k1;
+          double   ds[N];
 #pragma unroll
           for (int l = 0; l < N; l++) {
-            const double ds = s1[l] - s0[l]; // DS (esirkepov.hpp:157-166)
-            pr[l]           = make_double2(s0[l], ds);
-            pre += ds;
-            if (l < N - 1)
-              rec[MP::PRE + a1 * NS + l] = pre;
+            ds[l]           = s1[l] - s0[l]; // DS (esirkepov.hpp:157-166)
+            pr[l * PITCH] = make_double2(s0[l], ds[l]);
           }
+          double pre[NS];
+          pre[0] = ds[0];
+#pragma unroll
+          for (int l = 1; l < NS; l++)
+            pre[l] = l < N - 1 ? pre[l - 1] + ds[l] : 0.0;
+#pragma unroll
+          for (int l = 0; l < NS / 2; l++)
+            pr[(N + l) * PITCH] = make_double2(pre[2 * l], pre[2 * l + 1]);
         }
         {
-          const unsigned long long m = __ballot(on1 && same);
+          // per particle: all three axes at the bin's origin?  and its own origin
+          const unsigned long long m    = __ballot(on1 && same);
+          const int                orgy = __shfl(org, (lane + BATCH) & 63);
+          const int                orgz = __shfl(org, (lane + 2 * BATCH) & 63);
           if (lane < nbt) {
             const bool all3 =
                 ((m >> lane) & (m >> (lane + BATCH)) & (m >> (lane + 2 * BATCH)) & 1ull) != 0;
-            W[lane * REC + MP::FLAG] = all3 ? 1.0 : 0.0;
+            FO[lane] = make_int4(all3 ? 1 : 0, org, orgy, orgz);
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -613,23 +626,22 @@ __global__ void __launch_bounds__(256)
         for (int k0 = 0; k0 < nbt; k0 += PPS) {
           const int kk = k0 + k;
           if (act && kk < nbt) {
-            const double*  rec = W + kk * REC;
-            const double2* pr  = reinterpret_cast<const double2*>(rec + MP::PAIRS);
-            const double2  pf  = pr[axF * N + t]; // (s0F[t], dsF[t])
+            const double2* pp  = W + kk;
+            const double2  pf  = pp[(axF * ROWS + t) * PITCH]; // (s0F[t], dsF[t])
             const double   aF  = pf.x + A * pf.y;
             const double   bF  = A * pf.x + B * pf.y;
-            const double2* pc  = reinterpret_cast<const double2*>(rec + MP::PRE + axC * NS);
             double         P[NS];
 #pragma unroll
             for (int j = 0; j < NS / 2; j++) {
-              const double2 d2 = pc[j];
+              const double2 d2 = pp[(axC * ROWS + N + j) * PITCH];
               P[2 * j]         = d2.x;
               P[2 * j + 1]     = d2.y;
             }
-            if (rec[MP::FLAG] != 0.0) {
+            const int4 fo = FO[kk];
+            if (fo.x != 0) {
 #pragma unroll
               for (int r = 0; r < N; r++) {
-                const double2 ps = pr[axS * N + r]; // (s0S[r], dsS[r])
+                const double2 ps = pp[(axS * ROWS + r) * PITCH]; // (s0S[r], dsS[r])
                 const double  w  = -(aF * ps.x + bF * ps.y) * qd;
 #pragma unroll
                 for (int j = 0; j < N - 1; j++)
@@ -639,12 +651,12 @@ __global__ void __launch_bounds__(256)
               // the particle's tile origin differs from its bin's (odd orders: the
               // bin and shape digitize offsets differ by half a cell, and the two
               // roundings can disagree): same values, added at its own nodes
-              const int pox = (int)rec[MP::ORG + 0];
-              const int poy = (int)rec[MP::ORG + 1];
-              const int poz = (int)rec[MP::ORG + 2];
+              const int pox = fo.y;
+              const int poy = fo.z;
+              const int poz = fo.w;
 #pragma unroll 1
               for (int r = 0; r < N; r++) {
-                const double2 ps = pr[axS * N + r];
+                const double2 ps = pp[(axS * ROWS + r) * PITCH];
                 const double  w  = -(aF * ps.x + bF * ps.y) * qd;
 #pragma unroll 1
                 for (int j = 0; j < N - 1; j++) {
