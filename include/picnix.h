@@ -50,6 +50,7 @@ extern "C" {
 /* field selectors for pn_get_fields / pn_set_fields */
 #define PN_FIELD_UF 0
 #define PN_FIELD_UJ 1
+#define PN_FIELD_UM 3 /* moments [z][y][x][Ns][14] (PicChunk::um); read only */
 
 typedef struct pn_sim pn_sim;
 
@@ -99,9 +100,15 @@ int pn_push_velocity(pn_sim* sim, int chunk, double delt);   /* PicChunk::push_v
 int pn_push_position(pn_sim* sim, int chunk, double delt);   /* PicChunk::push_position + count, pic/pic_chunk.cpp:491-499 */
 int pn_deposit_current(pn_sim* sim, int chunk, double delt); /* PicChunk::deposit_current, pic/pic_chunk.cpp:514-523 */
 int pn_sort_particle(pn_sim* sim, int chunk);                /* PicChunk::sort_particle, pic/pic_chunk.cpp:447-453 */
+int pn_deposit_moment(pn_sim* sim, int chunk);               /* PicChunk::deposit_moment, pic/pic_chunk.cpp:525-534 */
+/* Reset the Friedman-filter history to the current E (PicChunk::init_friedman,
+ * pic/pic_chunk.cpp:264-268; pn_create already does it once).  No-op for
+ * friedman = 0. */
+int pn_init_friedman(pn_sim* sim);
 /* Boundary exchange of one mode: pack+begin+end+unpack
  * (PicChunk::set_boundary_{pack,begin,end,unpack}, pic/pic_chunk.cpp:287-405).
- * PARTICLE migrates leavers to their neighbour chunk and sorts. */
+ * PARTICLE migrates leavers to their neighbour chunk and sorts; MOM adds the
+ * moment ghosts into the neighbours (after pn_deposit_moment). */
 int pn_exchange(pn_sim* sim, int mode);
 /* One full timestep in the reference operator order (PicApplication::push_openmp,
  * pic/pic_application.cpp:219-292). */

@@ -1722,6 +1722,21 @@ void pno_push_boris(double* u, const double* eb, double cc)
   push_boris(u[0], u[1], u[2], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], cc);
 }
 
+void pno_push_vay(double* u, const double* eb, double cc)
+{
+  push_vay(u[0], u[1], u[2], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], cc);
+}
+
+void pno_push_higuera_cary(double* u, const double* eb, double cc)
+{
+  push_higuera_cary(u[0], u[1], u[2], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], cc);
+}
+
+void pno_init_friedman(pno_sim* h)
+{
+  for_chunks(h->s, [&](Chunk& c) { init_friedman(c); });
+}
+
 void pno_esirkepov3d(int order, const double* xv, const double* xu, double q, double delt,
                      double dx, double dy, double dz, double* tile, int* origin)
 {

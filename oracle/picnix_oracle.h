@@ -55,7 +55,8 @@ void     pno_get_particles_old(const pno_sim* s, int chunk, int species, double*
 void     pno_set_particles(pno_sim* s, int chunk, int species, const double* xu7, int64_t np);
 void     pno_get_gindex(const pno_sim* s, int chunk, int species, int32_t* out);
 void     pno_get_pindex(const pno_sim* s, int chunk, int species, int32_t* out);
-/* which: 0 = uf [z][y][x][6], 1 = uj [z][y][x][4], 2 = ff [z][y][x][3][6] */
+/* which: 0 = uf [z][y][x][6], 1 = uj [z][y][x][4], 2 = ff [z][y][x][3][3] (the
+ * reference allocates [3][6] and uses components 0..2 only), 3 = um [z][y][x][Ns][14] */
 void     pno_get_field(const pno_sim* s, int chunk, int which, double* out);
 void     pno_set_field(pno_sim* s, int chunk, int which, const double* in);
 
@@ -79,6 +80,10 @@ void   pno_shape_mc(int order, double x, double X, double rdx, double* s);
 int    pno_digitize(double x, double xmin, double rdx);
 double pno_lorentz_factor(double ux, double uy, double uz, double rc);
 void   pno_push_boris(double* u, const double* eb, double cc);
+void   pno_push_vay(double* u, const double* eb, double cc);
+void   pno_push_higuera_cary(double* u, const double* eb, double cc);
+/* PicChunk::init_friedman on every chunk: all Friedman levels = E */
+void   pno_init_friedman(pno_sim* s);
 /* single-particle Esirkepov: ss_old/new from positions, returns tile (O+3)^3*4 */
 void   pno_esirkepov3d(int order, const double* xv, const double* xu, double q, double delt,
                        double dx, double dy, double dz, double* tile, int* origin);

@@ -175,6 +175,8 @@ void allocate(Sim& s)
   s.d_E = dalloc<double>(3 * vol);
   s.d_B = dalloc<double>(3 * vol);
   s.d_J = dalloc<double>(4 * vol);
+  if (s.cfg.friedman != 0.0)
+    s.d_ff = dalloc<double>(9 * vol);
   HIPCHK(hipMemset(s.d_E, 0, 3 * vol * sizeof(double)));
   HIPCHK(hipMemset(s.d_B, 0, 3 * vol * sizeof(double)));
   HIPCHK(hipMemset(s.d_J, 0, 4 * vol * sizeof(double)));
@@ -233,6 +235,8 @@ void release(Sim& s)
   f(s.d_E);
   f(s.d_B);
   f(s.d_J);
+  f(s.d_M);
+  f(s.d_ff);
   f(s.d_off);
   f(s.d_lim);
   f(s.d_rng);
@@ -411,11 +415,9 @@ int pn_create(const char* toml, int load, pn_sim** out)
       *out = h;
       return PN_ERR_CONFIG;
     }
-    if (s.cfg.friedman != 0.0 || s.cfg.pusher != 0 || s.cfg.interp != 0 ||
-        s.cfg.precision != "fp64" || s.cfg.Ns > kMaxSpecies) {
+    if (s.cfg.interp != 0 || s.cfg.precision != "fp64" || s.cfg.Ns > kMaxSpecies) {
       s.err = "configuration uses an option this build does not implement yet "
-              "(supported: friedman = 0, pusher = Boris, interpolation = MC, precision = fp64, "
-              "Ns <= 8)";
+              "(supported: interpolation = MC, precision = fp64, Ns <= 8)";
       *out  = h;
       return PN_ERR_UNSUPPORTED;
     }
@@ -426,6 +428,7 @@ int pn_create(const char* toml, int load, pn_sim** out)
       s.march_rows = std::max(0, std::atoi(k));
     allocate(s);
     init_fields(s);
+    launch_init_friedman(s); // PicChunk::init_friedman in the examples' setup
     if (load) {
       load_particles(s);
       // initial sort, pic/pic_chunk.cpp:447-453 (count + sort per species)
@@ -623,11 +626,14 @@ int pn_get_pindex(pn_sim* h, int chunk, int species, int32_t* pindex)
 int pn_get_fields(pn_sim* h, int chunk, int which, double* data)
 {
   return guarded(h, [&](Sim& s) {
-    if (chunk < 0 || chunk >= s.g.nchunk || !data || (which != PN_FIELD_UF && which != PN_FIELD_UJ))
+    if (chunk < 0 || chunk >= s.g.nchunk || !data ||
+        (which != PN_FIELD_UF && which != PN_FIELD_UJ && which != PN_FIELD_UM))
       return fail(s, PN_ERR_ARG, "invalid chunk, field selector or buffer");
+    if (which == PN_FIELD_UM && !s.d_M)
+      return fail(s, PN_ERR_STATE, "no moments: call pn_deposit_moment first");
     const Geom& g  = s.g;
     size_t      nn = (size_t)(g.dims[0] + 2 * g.Nb) * (g.dims[1] + 2 * g.Nb) * (g.dims[2] + 2 * g.Nb);
-    int         nc = which == PN_FIELD_UF ? 6 : 4;
+    int         nc = which == PN_FIELD_UF ? 6 : (which == PN_FIELD_UJ ? 4 : 14 * g.Ns);
     ensure_stage(s, nn * nc);
     launch_chunk_fields_get(s, chunk, which, s.d_stage);
     HIPCHK(hipMemcpyAsync(data, s.d_stage, nn * nc * sizeof(double), hipMemcpyDeviceToHost,
@@ -717,6 +723,27 @@ int pn_deposit_current(pn_sim* h, int chunk, double delt)
   });
 }
 
+int pn_init_friedman(pn_sim* h)
+{
+  return guarded(h, [&](Sim& s) {
+    launch_init_friedman(s);
+    return PN_OK;
+  });
+}
+
+int pn_deposit_moment(pn_sim* h, int chunk)
+{
+  return guarded(h, [&](Sim& s) {
+    int rc = check_chunk_arg(s, chunk);
+    if (rc)
+      return rc;
+    if (!s.d_M)
+      s.d_M = dalloc<double>(14 * (size_t)s.g.Ns * s.g.vol);
+    launch_deposit_moment(s);
+    return PN_OK;
+  });
+}
+
 int pn_sort_particle(pn_sim* h, int chunk)
 {
   return guarded(h, [&](Sim& s) {
@@ -745,7 +772,10 @@ int pn_exchange(pn_sim* h, int mode)
       launch_migrate(s);
       return PN_OK;
     case PN_BOUNDARY_MOM:
-      return fail(s, PN_ERR_UNSUPPORTED, "moment exchange is not implemented");
+      if (!s.d_M)
+        return fail(s, PN_ERR_STATE, "moment exchange needs pn_deposit_moment first");
+      launch_fold(s, s.d_M, 0, 14 * s.g.Ns, PN_PHASE_RHO);
+      return PN_OK;
     default:
       return fail(s, PN_ERR_ARG, "invalid boundary mode");
     }

@@ -49,6 +49,70 @@ __device__ __forceinline__ void push_boris(T& ux, T& uy, T& uz, T ex, T ey, T ez
   uz += (vx * by - vy * bx) * bb + ez;
 }
 
+// Vay (2008) pusher, nix/primitives.hpp:191-220 (gamma of Vay's Eq. 11)
+template <typename T>
+__device__ __forceinline__ void push_vay(T& ux, T& uy, T& uz, T ex, T ey, T ez, T bx, T by, T bz,
+                                         T cc)
+{
+  T gm = T(1) / sqrt(cc * cc + ux * ux + uy * uy + uz * uz);
+  T vx = ux + 2 * ex + gm * (uy * bz - uz * by);
+  T vy = uy + 2 * ey + gm * (uz * bx - ux * bz);
+  T vz = uz + 2 * ez + gm * (ux * by - uy * bx);
+  gm   = (cc * cc + vx * vx + vy * vy + vz * vz);
+  T bb = bx * bx + by * by + bz * bz;
+  T bu = bx * vx + by * vy + bz * vz;
+  T xx = gm - bb;
+  T yy = bb + bu * bu;
+  gm   = T(1) / sqrt(T(0.5) * (xx + sqrt(xx * xx + 4 * yy)));
+  bx *= gm;
+  by *= gm;
+  bz *= gm;
+  bu = bx * vx + by * vy + bz * vz;
+  bb = T(1.0) / (T(1.0) + bx * bx + by * by + bz * bz);
+  ux = (vx + bu * bx + (vy * bz - vz * by)) * bb;
+  uy = (vy + bu * by + (vz * bx - vx * bz)) * bb;
+  uz = (vz + bu * bz + (vx * by - vy * bx)) * bb;
+}
+
+// Higuera-Cary (2017) pusher, nix/primitives.hpp:222-253 (gamma of their Eq. 20)
+template <typename T>
+__device__ __forceinline__ void push_higuera_cary(T& ux, T& uy, T& uz, T ex, T ey, T ez, T bx,
+                                                  T by, T bz, T cc)
+{
+  ux += ex;
+  uy += ey;
+  uz += ez;
+  T gm = cc * cc + ux * ux + uy * uy + uz * uz;
+  T bb = bx * bx + by * by + bz * bz;
+  T bu = bx * ux + by * uy + bz * uz;
+  T xx = gm - bb;
+  T yy = bb + bu * bu;
+  gm   = T(1) / sqrt(T(0.5) * (xx + sqrt(xx * xx + 4 * yy)));
+  bx *= gm;
+  by *= gm;
+  bz *= gm;
+  bb   = T(2.0) / (T(1.0) + bx * bx + by * by + bz * bz);
+  T vx = ux + (uy * bz - uz * by);
+  T vy = uy + (uz * bx - ux * bz);
+  T vz = uz + (ux * by - uy * bx);
+  ux += (vy * bz - vz * by) * bb + ex;
+  uy += (vz * bx - vx * bz) * bb + ey;
+  uz += (vx * by - vy * bx) * bb + ez;
+}
+
+// BaseVelocity::push_impl (pic/engine/velocity.hpp:81-99): 0 Boris, 1 Vay,
+// 2 Higuera-Cary; `pusher` is uniform across a launch.
+template <typename T>
+__device__ __forceinline__ void push_momentum(int pusher, T& ux, T& uy, T& uz, const T eb[6], T cc)
+{
+  if (pusher == 0)
+    push_boris(ux, uy, uz, eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], cc);
+  else if (pusher == 1)
+    push_vay(ux, uy, uz, eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], cc);
+  else
+    push_higuera_cary(ux, uy, uz, eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], cc);
+}
+
 // nix/primitives.hpp:256-328 (shape_mc1..4)
 template <int Order, typename T>
 __device__ __forceinline__ void shape_mc(T x, T X, T rdx, T* s)

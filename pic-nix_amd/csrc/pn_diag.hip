@@ -2,6 +2,7 @@
 // global fields for the C-ABI boundary.
 #include <vector>
 
+#include "../../include/picnix.h"
 #include "pn_sim.hpp"
 
 namespace picnix {
@@ -189,9 +190,13 @@ void launch_chunk_fields_get(Sim& s, int chunk, int which, double* d_out)
                        o[2], d_out);
     hipLaunchKernelGGL(k_chunk_get, grid, dim3(256), 0, s.stream, g, s.d_B, 3, 3, 6, o[0], o[1],
                        o[2], d_out);
-  } else {
+  } else if (which == PN_FIELD_UJ) {
     hipLaunchKernelGGL(k_chunk_get, grid, dim3(256), 0, s.stream, g, s.d_J, 4, 0, 4, o[0], o[1],
                        o[2], d_out);
+  } else {
+    // [z][y][x][Ns][14], PicChunk::um
+    hipLaunchKernelGGL(k_chunk_get, grid, dim3(256), 0, s.stream, g, s.d_M, 14 * g.Ns, 0,
+                       14 * g.Ns, o[0], o[1], o[2], d_out);
   }
 }
 

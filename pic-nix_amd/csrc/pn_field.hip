@@ -18,8 +18,10 @@ __device__ __forceinline__ long gidx(const Geom& g, int z, int y, int x)
   return ((long)z * g.P[1] + y) * g.P[2] + x;
 }
 
-// B -= c dt curl E with Friedman theta = 0 (ff(...,0,k) == E exactly):
-// pic/engine/maxwell.hpp:442-496 (3D) and :292-348 (2D).  One thread per interior cell.
+// B -= c dt curl F: pic/engine/maxwell.hpp:442-496 (3D) and :292-348 (2D), where
+// F = ff(...,0,k) is the Friedman-filtered E (k_friedman_combine), or E itself
+// for theta = 0 (then ff(...,0,k) = 1 E + (-0) ff1 + 0 ff2 == E exactly).  One
+// thread per interior cell.
 template <int Dim>
 __global__ void __launch_bounds__(256) k_push_bfd(Geom g, const double* __restrict__ E,
                                                   double* __restrict__ B, double cflx,
@@ -47,6 +49,56 @@ __global__ void __launch_bounds__(256) k_push_bfd(Geom g, const double* __restri
     B[v + i] += (+cflx) * (Ez[i] - Ez[i - sx]);
     B[2 * v + i] += (-cflx) * (Ey[i] - Ey[i - sx]) + (+cfly) * (Ex[i] - Ex[i - sy]);
   }
+}
+
+// Friedman filter (pic/engine/maxwell.hpp:442-457 / :292-307):
+//   ff0 = A E + B ff1 + C ff2,  A = 1 + theta/2, B = -theta (1 - theta/2),
+//   C = theta (1 - theta)^2 / 2,
+// over every padded cell (the reference covers the ghosts too; the padding holds
+// periodic images, so each padding cell gets its image's value).  ff is
+// [level][component][vol].
+__global__ void __launch_bounds__(256) k_friedman_combine(Geom g, const double* __restrict__ E,
+                                                          double* __restrict__ ff, double A,
+                                                          double B, double C)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.vol)
+    return;
+  const long v = g.vol;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    ff[k * v + i] = A * E[k * v + i] + B * ff[(3 + k) * v + i] + C * ff[(6 + k) * v + i];
+}
+
+// History shift before the E update (maxwell.hpp:389-404 / :237-252):
+//   ff2 = ff1 + theta ff2;  ff1 = E.
+__global__ void __launch_bounds__(256) k_friedman_shift(Geom g, const double* __restrict__ E,
+                                                        double* __restrict__ ff, double theta)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.vol)
+    return;
+  const long v = g.vol;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ff[(6 + k) * v + i] = ff[(3 + k) * v + i] + theta * ff[(6 + k) * v + i];
+    ff[(3 + k) * v + i] = E[k * v + i];
+  }
+}
+
+// init_friedman (maxwell.hpp:44-60): every level = E.
+__global__ void __launch_bounds__(256) k_friedman_init(Geom g, const double* __restrict__ E,
+                                                       double* __restrict__ ff)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.vol)
+    return;
+  const long v = g.vol;
+#pragma unroll
+  for (int l = 0; l < 3; l++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      ff[(3 * l + k) * v + i] = E[k * v + i];
 }
 
 // E += c dt curl B - dt J: pic/engine/maxwell.hpp:383-440 (3D) and :231-290 (2D).
@@ -159,12 +211,22 @@ void launch_push_bfd(Sim& s, double delt)
   double      cfly = g.cc * delt / g.del[1];
   double      cflz = g.cc * delt / g.del[0];
   long        n    = (long)g.N[0] * g.N[1] * g.N[2];
+  const double* F  = s.d_E;
+  if (s.d_ff) {
+    const double th = s.cfg.friedman;
+    const double A  = 1 + 0.5 * th;
+    const double B  = -th * (1 - 0.5 * th);
+    const double C  = 0.5 * th * (1 - th) * (1 - th);
+    hipLaunchKernelGGL(k_friedman_combine, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g,
+                       s.d_E, s.d_ff, A, B, C);
+    F = s.d_ff;
+  }
   if (g.dim == 3)
-    hipLaunchKernelGGL(k_push_bfd<3>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_E,
-                       s.d_B, cflx, cfly, cflz);
+    hipLaunchKernelGGL(k_push_bfd<3>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, s.d_B,
+                       cflx, cfly, cflz);
   else
-    hipLaunchKernelGGL(k_push_bfd<2>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_E,
-                       s.d_B, cflx, cfly, cflz);
+    hipLaunchKernelGGL(k_push_bfd<2>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, s.d_B,
+                       cflx, cfly, cflz);
   launch_fill_ghost(s, s.d_B, 3);
 }
 
@@ -176,6 +238,9 @@ void launch_push_efd(Sim& s, double delt)
   double      cfly = g.cc * delt / g.del[1];
   double      cflz = g.cc * delt / g.del[0];
   long        n    = (long)g.N[0] * g.N[1] * g.N[2];
+  if (s.d_ff)
+    hipLaunchKernelGGL(k_friedman_shift, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g,
+                       s.d_E, s.d_ff, s.cfg.friedman);
   if (g.dim == 3)
     hipLaunchKernelGGL(k_push_efd<3>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_E,
                        s.d_B, s.d_J, cflx, cfly, cflz, delt);
@@ -185,23 +250,35 @@ void launch_push_efd(Sim& s, double delt)
   launch_fill_ghost(s, s.d_E, 3);
 }
 
+void launch_init_friedman(Sim& s)
+{
+  if (s.d_ff)
+    hipLaunchKernelGGL(k_friedman_init, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream, s.g,
+                       s.d_E, s.d_ff);
+}
+
 void launch_fill_ghost(Sim& s, double* f, int ncomp)
 {
   hipLaunchKernelGGL(k_fill_ghost, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream, s.g, f,
                      ncomp);
 }
 
-void launch_fold_current(Sim& s, int k0, int k1)
+void launch_fold(Sim& s, double* F, int k0, int k1, int phase)
 {
-  PhaseTimer  pt(s, k0 == 0 && k1 == 1 ? 7 : 4);
+  PhaseTimer  pt(s, phase);
   const Geom& g = s.g;
   for (int dir = 2; dir >= 0; dir--) {
     if (!g.has[dir] || g.pad[dir] == 0)
       continue;
     long n = dir == 2 ? (long)g.P[0] * g.P[1] : (dir == 1 ? (long)g.P[0] * g.P[2] : g.plane);
-    hipLaunchKernelGGL(k_fold_line, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_J, dir, k0,
+    hipLaunchKernelGGL(k_fold_line, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, dir, k0,
                        k1);
   }
+}
+
+void launch_fold_current(Sim& s, int k0, int k1)
+{
+  launch_fold(s, s.d_J, k0, k1, k0 == 0 && k1 == 1 ? 7 : 4);
 }
 
 } // namespace picnix

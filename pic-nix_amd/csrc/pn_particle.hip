@@ -10,6 +10,7 @@ inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 struct PushConsts {
   double qmdt[kMaxSpecies]; // 0.5 * q / m * delt  (velocity.hpp:130)
+  int    pusher;            // 0 Boris, 1 Vay, 2 Higuera-Cary
 };
 
 // One thread per particle.  Weights follow BaseVelocity::weights{2,3}d
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(256)
     eb[4] = interp(B + v, iy0, hx0, wiy, whx);
     eb[5] = interp(B + 2 * v, hy0, hx0, why, whx);
   }
-  push_boris(xu[3], xu[4], xu[5], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], g.cc);
+  push_momentum(pc.pusher, xu[3], xu[4], xu[5], eb, g.cc);
   p.ux[i] = xu[3];
   p.uy[i] = xu[4];
   p.uz[i] = xu[5];
@@ -321,7 +322,7 @@ __global__ void __launch_bounds__(256)
         eb[5] = interp2(B + 2 * v, sy, HY, HX, why, whx);
       }
     }
-    push_boris(xu[3], xu[4], xu[5], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], g.cc);
+    push_momentum(pc.pusher, xu[3], xu[4], xu[5], eb, g.cc);
     p.ux[i] = xu[3];
     p.uy[i] = xu[4];
     p.uz[i] = xu[5];
@@ -488,6 +489,7 @@ void launch_push_velocity(Sim& s, double delt)
   PushConsts  pc{};
   for (int sp = 0; sp < g.Ns; sp++)
     pc.qmdt[sp] = 0.5 * g.q[sp] / g.m[sp] * delt;
+  pc.pusher = s.cfg.pusher;
   dim3         grid(nblk(g.cap, 256), g.nseg);
   const int    T    = g.order + 2;
   const size_t lds  = sizeof(double) * 6 * (g.dim == 3 ? T : 1) * T * (g.bsy + g.order + 1);

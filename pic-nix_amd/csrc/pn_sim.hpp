@@ -52,6 +52,8 @@ struct Sim {
   double* d_E = nullptr; // [3][vol]
   double* d_B = nullptr; // [3][vol]
   double* d_J = nullptr; // [4][vol]
+  double* d_M = nullptr; // [Ns][14][vol] moments (PicChunk::um), allocated on first use
+  double* d_ff = nullptr; // [3 levels][3][vol] Friedman history, only for theta != 0
 
   // particles: A = current, Bv = sort destination
   PArr     pa{}, pb{};
@@ -97,8 +99,13 @@ struct PhaseTimer {
 void launch_push_bfd(Sim& s, double delt);
 void launch_push_efd(Sim& s, double delt);
 void launch_fill_ghost(Sim& s, double* f, int ncomp);
-// fold the padding of J components [k0, k1) onto the periodic interior
+void launch_init_friedman(Sim& s); // every Friedman level = E (no-op for theta = 0)
+// fold the padding of components [k0, k1) of a [ncomp][vol] global array onto
+// the periodic interior (the Cur / Mom exchanges); phase: timing slot
+void launch_fold(Sim& s, double* F, int k0, int k1, int phase);
 void launch_fold_current(Sim& s, int k0 = 0, int k1 = 4);
+// moment deposit into d_M (allocated on first use), without the Mom exchange
+void launch_deposit_moment(Sim& s);
 void launch_push_velocity(Sim& s, double delt);
 void launch_push_position(Sim& s, double delt);
 void launch_count(Sim& s);
@@ -112,6 +119,7 @@ void launch_rho(Sim& s, bool sorted_by_new_cell);
 void launch_energy(Sim& s, double* out /* host: efd, bfd, divE, divB, particle[Ns] */);
 void launch_aos_to_soa(Sim& s, int seg, int64_t np);
 void launch_soa_to_aos(Sim& s, int seg, int64_t np);
+// which: PN_FIELD_UF, PN_FIELD_UJ or PN_FIELD_UM
 void launch_chunk_fields_get(Sim& s, int chunk, int which, double* d_out);
 void launch_chunk_fields_set(Sim& s, int chunk, int which, const double* d_in);
 

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libpicnix_amd.so")
 CONFIG_DIR = os.path.join(PKG_DIR, "configs")
 
 BOUNDARY_EMF, BOUNDARY_CUR, BOUNDARY_MOM, BOUNDARY_PARTICLE = 0, 1, 2, 3
-FIELD_UF, FIELD_UJ = 0, 1
+FIELD_UF, FIELD_UJ, FIELD_UM = 0, 1, 3
 
 ERRORS = {-1: "PN_ERR_ARG", -2: "PN_ERR_CONFIG", -3: "PN_ERR_HIP", -4: "PN_ERR_CAPACITY",
           -5: "PN_ERR_UNSUPPORTED", -6: "PN_ERR_STATE"}
@@ -29,7 +29,8 @@ EXPORTS = [
     "pn_chunk_info", "pn_get_np", "pn_get_particles", "pn_set_particles",
     "pn_get_particles_old", "pn_get_gindex", "pn_get_pindex", "pn_get_fields",
     "pn_set_fields", "pn_push_bfd", "pn_push_efd", "pn_push_velocity", "pn_push_position",
-    "pn_deposit_current", "pn_sort_particle", "pn_exchange", "pn_step", "pn_get_energy",
+    "pn_deposit_current", "pn_deposit_moment", "pn_init_friedman", "pn_sort_particle", "pn_exchange", "pn_step",
+    "pn_get_energy",
     "pn_get_diverror", "pn_synchronize", "pn_timing", "pn_timing_report",
 ]
 
@@ -82,6 +83,8 @@ def load_library(path=LIB_PATH):
         "pn_push_position": [P, i, d],
         "pn_deposit_current": [P, i, d],
         "pn_sort_particle": [P, i],
+        "pn_deposit_moment": [P, i],
+        "pn_init_friedman": [P],
         "pn_exchange": [P, i],
         "pn_step": [P, d],
         "pn_get_energy": [P, dp, dp, dp],
@@ -188,7 +191,8 @@ class Simulation:
     def field(self, c, which):
         info = self.chunk_info(c)
         nz, ny, nx = (int(v) for v in info[6:9])
-        out = np.zeros((nz, ny, nx, 6 if which == FIELD_UF else 4), dtype=np.float64)
+        ncomp = {FIELD_UF: 6, FIELD_UJ: 4, FIELD_UM: 14 * self.Ns}[which]
+        out = np.zeros((nz, ny, nx, ncomp), dtype=np.float64)
         self._chk(self.L.pn_get_fields(self.h, c, which, _dp(out)))
         return out
 
@@ -211,6 +215,13 @@ class Simulation:
 
     def deposit_current(self, dt):
         self._chk(self.L.pn_deposit_current(self.h, -1, dt))
+
+    def deposit_moment(self):
+        """PicChunk::deposit_moment; follow with exchange(BOUNDARY_MOM)."""
+        self._chk(self.L.pn_deposit_moment(self.h, -1))
+
+    def init_friedman(self):
+        self._chk(self.L.pn_init_friedman(self.h))
 
     def sort_particle(self):
         self._chk(self.L.pn_sort_particle(self.h, -1))

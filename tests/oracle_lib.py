@@ -66,6 +66,7 @@ def lib():
         L.pno_sort_particle.argtypes = [P]
         L.pno_exchange.argtypes = [P, i]
         L.pno_step.argtypes = [P]
+        L.pno_init_friedman.argtypes = [P]
         L.pno_deposit_moment.argtypes = [P]
         L.pno_history.argtypes = [P, dp]
         L.pno_set_threads.argtypes = [i]
@@ -74,6 +75,8 @@ def lib():
         L.pno_lorentz_factor.restype = d
         L.pno_lorentz_factor.argtypes = [d, d, d, d]
         L.pno_push_boris.argtypes = [dp, dp, d]
+        L.pno_push_vay.argtypes = [dp, dp, d]
+        L.pno_push_higuera_cary.argtypes = [dp, dp, d]
         L.pno_esirkepov3d.argtypes = [i, dp, dp, d, d, d, d, d, dp, ip]
         L.pno_esirkepov2d.argtypes = [i, dp, dp, d, d, d, d, dp, ip]
         L.pno_mt19937_64.restype = ctypes.c_uint64
@@ -205,6 +208,13 @@ class Oracle:
 
     def deposit_current(self, dt):
         self.L.pno_deposit_current(self.h, dt)
+
+    def init_friedman(self):
+        self.L.pno_init_friedman(self.h)
+
+    def deposit_moment(self):
+        """Moments + Mom exchange (PicChunk::deposit_moment, set_boundary(Mom))."""
+        self.L.pno_deposit_moment(self.h)
 
     def sort_particle(self):
         self.L.pno_sort_particle(self.h)

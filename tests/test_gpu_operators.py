@@ -253,3 +253,49 @@ def test_sort_drops_outside(kind):
             assert g.np(c, s) == 50
     g.close()
     o.close()
+
+
+def _moments_rel(o, g, dim):
+    worst, scale = 0.0, 0.0
+    for c in range(o.nchunks):
+        info = o.chunk_info(c)
+        a = helpers.interior(o.field(c, 3), info, dim)
+        b = helpers.interior(g.field(c, 3), info, dim)
+        worst = max(worst, float(np.max(np.abs(a - b))))
+        scale = max(scale, float(np.max(np.abs(a))))
+    return worst / scale if scale > 0 else worst
+
+
+@pytest.mark.parametrize("kind", ["thermal3d_o2", "thermal3d_o1", "twostream2d_o1"])
+def test_moment_deposit(kind):
+    """PicChunk::deposit_moment + Mom exchange on sorted particles (LDS row
+    kernel) and on unsorted ones (after push_position: one thread per particle)."""
+    import picnix_amd
+    dim, make = CASES[kind]
+    cfg = make()
+    dt = float(cfg["parameter"]["delt"])
+    _say(f"[ops] {kind} moment: create")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    for _ in range(2):
+        o.step()
+        g.step(dt)
+    o.deposit_moment()
+    g.deposit_moment()
+    g.exchange(picnix_amd.BOUNDARY_MOM)
+    rel = _moments_rel(o, g, dim)
+    _say(f"[ops] {kind} moment (sorted): rel {rel:.3e}")
+    assert rel <= TOL
+    # unsorted: after a position push the particles are out of bin order
+    o.push_velocity(dt)
+    g.push_velocity(dt)
+    o.push_position(dt)
+    g.push_position(dt)
+    o.deposit_moment()
+    g.deposit_moment()
+    g.exchange(picnix_amd.BOUNDARY_MOM)
+    rel = _moments_rel(o, g, dim)
+    _say(f"[ops] {kind} moment (unsorted): rel {rel:.3e}")
+    assert rel <= TOL
+    g.close()
+    o.close()

@@ -149,3 +149,58 @@ def test_step_parity_march_rows(rows, monkeypatch):
         assert max(rel.values()) <= TOL
         g.close()
         o.close()
+
+
+def _with_fields(cfg, E=(0.0, 0.0, 0.0), B=(0.0, 0.0, 0.0)):
+    p = cfg["parameter"]
+    p.update(Ex=E[0], Ey=E[1], Ez=E[2], Bx=B[0], By=B[1], Bz=B[2])
+    return cfg
+
+
+OPTION_CASES = {
+    # pushers under a background field strong enough to rotate the electrons
+    "vay_3d_o2": (3, lambda: _with_fields(helpers.small_thermal(
+        N=(16, 16, 16), C=(2, 2, 2), np_=8, order=2, pusher="Vay"),
+        E=(0.002, 0.0, -0.001), B=(0.3, 0.1, 0.5))),
+    "hc_2d_o1": (2, lambda: _with_fields(helpers.small_twostream(
+        N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1, pusher="HigueraCary"),
+        B=(0.0, 0.2, 0.4))),
+    # Friedman filter theta > 0 (ff history kept on the GPU)
+    "friedman_3d_o1": (3, lambda: helpers.small_thermal(
+        N=(16, 16, 16), C=(2, 2, 2), np_=8, order=1, friedman=0.1)),
+    "friedman_vay_2d_o2": (2, lambda: _with_fields(helpers.small_twostream(
+        N=(1, 16, 64), C=(1, 2, 4), np_=16, order=2, friedman=0.2, pusher="Vay"),
+        B=(0.1, 0.0, 0.3))),
+}
+
+
+@pytest.mark.parametrize("kind", list(OPTION_CASES))
+def test_step_parity_options(kind):
+    """Vay / Higuera-Cary pushers (nix/primitives.hpp:191-253) and the Friedman
+    filter (pic/engine/maxwell.hpp) against the oracle: particles bit-exact at
+    step 1, everything within 1e-12 at step 10."""
+    dim, make = OPTION_CASES[kind]
+    cfg = make()
+    dt = float(cfg["parameter"]["delt"])
+    _say(f"[parity] options {kind}: create")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    _compare_particles(o, g, exact=True)
+    o.step()
+    g.step(dt)
+    g.synchronize()
+    _compare_particles(o, g, exact=True)
+    rel = _compare_fields(o, g, dim)
+    _say(f"[parity] options {kind}: step 1 fields {rel}")
+    assert max(rel.values()) <= TOL
+    for k in range(9):
+        o.step()
+        g.step(dt)
+    g.synchronize()
+    perr = _compare_particles(o, g, exact=False)
+    rel = _compare_fields(o, g, dim)
+    _say(f"[parity] options {kind}: step 10 particle rel err {perr:.3e}, fields {rel}")
+    assert perr <= TOL
+    assert max(rel.values()) <= TOL
+    g.close()
+    o.close()
