@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256)
 // summation order), reading LDS instead of global memory.  A particle whose
 // stencil leaves the staged region (a rounding edge of the bin test) reads
 // global memory.
-template <int Dim, int Order>
+template <int Dim, int Order, int RS>
 __global__ void __launch_bounds__(256)
     k_push_velocity_rows(Geom g, PArr p, const int32_t* __restrict__ cellstart,
                          const int* __restrict__ off, const double* __restrict__ lim,
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256)
   constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
   constexpr int S      = Order + 1;
   constexpr int T      = Order + 2;
-  extern __shared__ double ft[]; // [6][TZ][T][TXv]
+  extern __shared__ double ft[]; // [6][TZ][T][RS]; RS >= TXv, a compile-time row stride
   const int seg = blockIdx.y;
   const int c   = seg / g.Ns;
   const int sp  = seg - c * g.Ns;
@@ -180,20 +180,20 @@ __global__ void __launch_bounds__(256)
   const int  tz0 = Dim == 3 ? bz - is_odd + lb - (Order / 2) : lb;
   const int  ty0 = by - is_odd + lb - (Order / 2);
   const int  tx0 = -is_odd + lb - (Order / 2);
-  const int  TXv = nbx + Order + 1;
-  const int  pc1 = TZ * T * TXv; // per component
-  const long sy = g.P[2], sz = g.plane, v = g.vol;
-  const int  offz = off[c * 3 + 0], offy = off[c * 3 + 1], offx = off[c * 3 + 2];
+  const int     TXv = nbx + Order + 1;         // valid row length
+  constexpr int pc1 = (Dim == 3 ? T : 1) * T * RS; // per component
+  const long    sy = g.P[2], sz = g.plane, v = g.vol;
+  const int     offz = off[c * 3 + 0], offy = off[c * 3 + 1], offx = off[c * 3 + 2];
   for (int t = threadIdx.x; t < 6 * pc1; t += blockDim.x) {
     int k  = t / pc1;
     int r  = t - k * pc1;
-    int tz = r / (T * TXv);
-    int r2 = r - tz * T * TXv;
-    int ty = r2 / TXv;
-    int tx = r2 - ty * TXv;
+    int tz = r / (T * RS);
+    int r2 = r - tz * T * RS;
+    int ty = r2 / RS;
+    int tx = r2 - ty * RS;
     int gz = Dim == 3 ? offz + tz0 + tz : 0;
     const double* F = k < 3 ? E + k * v : B + (k - 3) * v;
-    ft[t] = F[gz * sz + (offy + ty0 + ty) * sy + offx + tx0 + tx];
+    ft[t] = tx < TXv ? F[gz * sz + (offy + ty0 + ty) * sy + offx + tx0 + tx] : 0.0;
   }
   __syncthreads();
 
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(256)
       hz0 += lb - (Order / 2);
       in = in && min(iz0, hz0) >= tz0 && max(iz0, hz0) + S <= tz0 + T;
       if (in) {
-        const long bsz = (long)T * TXv, bsy = TXv;
+        constexpr long bsz = (long)T * RS, bsy = RS;
         const int  Z = iz0 - tz0, HZ = hz0 - tz0, Y = iy0 - ty0, HY = hy0 - ty0;
         const int  X = ix0 - tx0, HX = hx0 - tx0;
         eb[0] = interp3(ft + 0 * pc1, bsz, bsy, Z, Y, HX, wiz, wiy, whx);
@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(256)
       (void)wiz;
       (void)whz;
       if (in) {
-        const long bsy = TXv;
+        constexpr long bsy = RS;
         const int  Y = iy0 - ty0, HY = hy0 - ty0, X = ix0 - tx0, HX = hx0 - tx0;
         eb[0] = interp2(ft + 0 * pc1, bsy, Y, HX, wiy, whx);
         eb[1] = interp2(ft + 1 * pc1, bsy, HY, X, why, wix);
@@ -490,19 +490,29 @@ void launch_push_velocity(Sim& s, double delt)
   for (int sp = 0; sp < g.Ns; sp++)
     pc.qmdt[sp] = 0.5 * g.q[sp] / g.m[sp] * delt;
   pc.pusher = s.cfg.pusher;
-  dim3         grid(nblk(g.cap, 256), g.nseg);
-  const int    T    = g.order + 2;
-  const size_t lds  = sizeof(double) * 6 * (g.dim == 3 ? T : 1) * T * (g.bsy + g.order + 1);
-  const bool   rows = s.sorted && !s.generic_kernels && lds <= 65536;
+  dim3      grid(nblk(g.cap, 256), g.nseg);
+  const int T    = g.order + 2;
+  const int need = g.bsy + g.order + 1; // row of the staged field tile
+  // compile-time row stride of the LDS tile (folds every stencil offset into
+  // the LDS instruction's immediate): the smallest of 16, 24, 32 that fits
+  const int    rs   = need <= 16 ? 16 : (need <= 24 ? 24 : (need <= 32 ? 32 : 0));
+  const size_t lds  = sizeof(double) * 6 * (g.dim == 3 ? T : 1) * T * rs;
+  const bool   rows = s.sorted && !s.generic_kernels && rs > 0 && lds <= 65536;
   const int    nrow = g.nbin / g.bsy;
-#define PN_VEL(D, O)                                                                             \
-  do {                                                                                           \
-    if (rows)                                                                                    \
-      hipLaunchKernelGGL((k_push_velocity_rows<D, O>), dim3(nrow, g.nseg), dim3(256), lds,       \
-                         s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc); \
-    else                                                                                         \
-      hipLaunchKernelGGL((k_push_velocity<D, O>), grid, dim3(256), 0, s.stream, g, s.pa,         \
-                         s.d_np, s.d_off, s.d_lim, s.d_E, s.d_B, pc);                            \
+#define PN_VEL_RS(D, O, R)                                                                    \
+  hipLaunchKernelGGL((k_push_velocity_rows<D, O, R>), dim3(nrow, g.nseg), dim3(256), lds,      \
+                     s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc)
+#define PN_VEL(D, O)                                                                          \
+  do {                                                                                        \
+    if (rows && rs == 16)                                                                     \
+      PN_VEL_RS(D, O, 16);                                                                    \
+    else if (rows && rs == 24)                                                                \
+      PN_VEL_RS(D, O, 24);                                                                    \
+    else if (rows)                                                                            \
+      PN_VEL_RS(D, O, 32);                                                                    \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_push_velocity<D, O>), grid, dim3(256), 0, s.stream, g, s.pa,      \
+                         s.d_np, s.d_off, s.d_lim, s.d_E, s.d_B, pc);                         \
   } while (0)
   if (g.dim == 3) {
     switch (g.order) {
@@ -520,6 +530,7 @@ void launch_push_velocity(Sim& s, double delt)
     }
   }
 #undef PN_VEL
+#undef PN_VEL_RS
 }
 
 void launch_push_position(Sim& s, double delt)

@@ -234,27 +234,20 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// One wavefront per (segment, bin): rank the bin's members by
-// (idx mod 8, idx) -- the order of XtensorParticle::sort -- by comparing every
-// pair in registers, then move each record to start(bin) + rank.
+// Total order of the members of one bin: (idx mod 8, idx).
 __device__ __forceinline__ unsigned int sort_key(int idx)
 {
   return ((unsigned int)(idx & (kSimdWidth - 1)) << 28) | ((unsigned int)idx >> 3);
 }
 
-__global__ void __launch_bounds__(256)
-    k_rank_bin(Geom g, PArr src, PArr dst, const int32_t* __restrict__ cellstart,
-               const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc, long nblocks)
+// Rank the members of bin w (segment-major flat index) by (idx mod 8, idx) --
+// the order of XtensorParticle::sort -- by comparing every pair in registers,
+// then move each record to start(bin) + rank.  One wavefront; all lanes call.
+__device__ __forceinline__ void rank_one_bin(const Geom& g, const PArr& src, const PArr& dst,
+                                             const int32_t* __restrict__ cellstart,
+                                             const int32_t* __restrict__ mkey,
+                                             const int32_t* __restrict__ msrc, long w, int lane)
 {
-  // consecutive bins on one XCD: records moving between neighbouring bins are
-  // then read through the same L2
-  const long blk = xcd_block(blockIdx.x, nblocks);
-  if (blk >= nblocks)
-    return;
-  const int  lane = threadIdx.x & 63;
-  const long w    = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= (long)g.nseg * g.nbin)
-    return;
   const int      seg  = (int)(w / g.nbin);
   const int      b    = (int)(w - (long)seg * g.nbin);
   const int32_t* cs   = cellstart + (long)seg * (g.nbin + 1);
@@ -293,6 +286,26 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// A block owns kRankBins consecutive bins and its wavefronts take them in turn,
+// so that a launch is a few thousand blocks of real work rather than one short
+// block per four bins; consecutive blocks run on one XCD (xcd_block), so
+// records moving between neighbouring bins are read through the same L2.
+constexpr int kRankBins = 64;
+
+__global__ void __launch_bounds__(256)
+    k_rank_bin(Geom g, PArr src, PArr dst, const int32_t* __restrict__ cellstart,
+               const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc, long nblocks)
+{
+  const long blk = xcd_block(blockIdx.x, nblocks);
+  if (blk >= nblocks)
+    return;
+  const int  lane  = threadIdx.x & 63;
+  const long nbins = (long)g.nseg * g.nbin;
+  const long wend  = min(nbins, (blk + 1) * kRankBins);
+  for (long w = blk * kRankBins + (threadIdx.x >> 6); w < wend; w += blockDim.x >> 6)
+    rank_one_bin(g, src, dst, cellstart, mkey, msrc, w, lane);
+}
+
 void finish_sort(Sim& s)
 {
   const Geom& g = s.g;
@@ -302,7 +315,7 @@ void finish_sort(Sim& s)
   hipLaunchKernelGGL(k_members, grid, dim3(256), 0, s.stream, g, s.d_np, s.ss.cellstart,
                      s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx, s.ss.mkey, s.ss.msrc);
   const long nwave = (long)g.nseg * g.nbin;
-  const long nrb   = nblk(nwave, 4);
+  const long nrb   = nblk(nwave, kRankBins);
   hipLaunchKernelGGL(k_rank_bin, dim3(xcd_grid(nrb)), dim3(256), 0, s.stream, g, s.pa, s.pb,
                      s.ss.cellstart, s.ss.mkey, s.ss.msrc, nrb);
   hipMemcpyAsync(s.d_np, s.ss.npnew, sizeof(int32_t) * g.nseg, hipMemcpyDeviceToDevice,
