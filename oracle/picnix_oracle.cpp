@@ -175,6 +175,120 @@ inline void shape_mc(double x, double X, double rdx, double* s)
   }
 }
 
+template <typename T>
+inline T wt_abs(T v)
+{
+  return std::fabs(v);
+}
+
+// WT shape functions (nix/primitives.hpp:331-566; Lu et al., J. Comput. Phys. 413,
+// 109388 (2020)): the shape of order O is the B-spline of degree O-1 smeared by
+// a top hat of half width dt = c dt / dx.  Left/middle/right branches selected
+// by the 0/1 factors t1..t4 exactly as the reference combines them.
+template <int Order, typename T>
+inline void shape_wt(T x, T X, T rdx, T dt, T rdt, T* s)
+{
+  const T delta = (x - X) * rdx;
+  if constexpr (Order == 1) {
+    // top-hat (half width dt) smeared nearest-grid-point weight
+    T ss = T(0.25) * rdt * (1 + 2 * dt - 2 * delta);
+    ss   = T(0) < ss ? ss : T(0); // max(0, .) then min(1, .), as std::max / std::min
+    ss   = ss < T(1) ? ss : T(1);
+    s[0] = ss;
+    s[1] = 1 - ss;
+  } else if constexpr (Order == 2) {
+    const T t1 = delta < -dt ? T(1) : T(0);
+    const T t2 = 1 - t1;
+    const T t3 = delta < +dt ? T(1) : T(0);
+    const T t4 = 1 - t3;
+    const T w0 = wt_abs(delta);
+    const T w1 = dt - delta;
+    const T w2 = dt + delta;
+    // left (t1), middle (t2 t3) and right (t4) branches of the piecewise weight
+    const T l[3] = {w0, 1 - w0, T(0)};
+    const T m[3] = {T(0.25) * rdt * w1 * w1, T(0.50) * rdt * (dt * (2 - dt) - w0 * w0),
+                    T(0.25) * rdt * w2 * w2};
+    const T r[3] = {l[2], l[1], l[0]};
+    for (int k = 0; k < 3; k++)
+      s[k] = l[k] * t1 + m[k] * t2 * t3 + r[k] * t4;
+  } else if constexpr (Order == 3) {
+    const T a    = T(1) / T(96.0);
+    const T b    = T(1) / T(24.0);
+    const T c    = T(1) / T(12.0);
+    const T adt  = a * rdt;
+    const T t1   = delta < T(0.5) - dt ? T(1) : T(0);
+    const T t2   = 1 - t1;
+    const T t3   = delta < T(0.5) + dt ? T(1) : T(0);
+    const T t4   = 1 - t3;
+    const T w0   = delta;
+    const T w1   = 1 - delta;
+    const T w3   = 1 - 2 * delta;
+    const T w4   = 1 + 2 * delta;
+    const T w5   = 2 * dt + w3;
+    const T w6   = 2 * dt - w3;
+    const T w7   = 3 - 2 * delta;
+    const T w0p2 = w0 * w0;
+    const T w1p2 = w1 * w1;
+    const T w3p2 = w3 * w3;
+    const T w3p3 = w3p2 * w3;
+    const T w4p2 = w4 * w4;
+    const T w5p3 = w5 * w5 * w5;
+    const T w6p3 = w6 * w6 * w6;
+    const T w7p2 = w7 * w7;
+    const T dtp2 = dt * dt;
+    const T dtp3 = dtp2 * dt;
+    const T dt4  = 4 * dtp2;
+    const T sodd  = adt * (-8 * dtp3 - 6 * dt * w3p2);
+    const T seven = adt * (-36 * dtp2 * w3 - 3 * w3p3);
+    const T l[4] = {b * (dt4 + 3 * w3p2), c * (9 - dt4 - 12 * w0p2), b * (dt4 + 3 * w4p2), T(0)};
+    const T m[4] = {adt * w5p3, sodd + seven + w1, sodd - seven + w0, adt * w6p3};
+    const T r[4] = {T(0), b * (dt4 + 3 * w7p2), c * (9 - dt4 - 12 * w1p2), b * (dt4 + 3 * w3p2)};
+    for (int k = 0; k < 4; k++)
+      s[k] = l[k] * t1 + m[k] * t2 * t3 + r[k] * t4;
+  } else {
+    const T a    = T(1) / T(48.0);
+    const T b    = T(1) / T(24.0);
+    const T c    = T(1) / T(12.0);
+    const T d    = T(1) / T(6.0);
+    const T adt  = a * rdt;
+    const T bdt  = b * rdt;
+    const T cdt  = c * rdt;
+    const T t1   = delta < -dt ? T(1) : T(0);
+    const T t2   = 1 - t1;
+    const T t3   = delta < +dt ? T(1) : T(0);
+    const T t4   = 1 - t3;
+    const T w0   = wt_abs(delta);
+    const T w1   = 1 - w0;
+    const T w2   = 1 - delta;
+    const T w3   = 1 + delta;
+    const T w4   = dt - delta;
+    const T w5   = dt + delta;
+    const T w0p2 = w0 * w0;
+    const T w0p3 = w0p2 * w0;
+    const T w0p4 = w0p3 * w0;
+    const T w1p2 = w1 * w1;
+    const T w1p3 = w1p2 * w1;
+    const T w2p3 = w2 * w2 * w2;
+    const T w3p3 = w3 * w3 * w3;
+    const T w4p4 = w4 * w4 * w4 * w4;
+    const T w5p4 = w5 * w5 * w5 * w5;
+    const T dtp2 = dt * dt;
+    const T dtp3 = dtp2 * dt;
+    const T dtp4 = dtp3 * dt;
+    const T ss1  = -dtp4 - 6 * w0p2 * dtp2 - w0p4;
+    const T ss2  = 3 * dtp4 - 8 * dtp3 + 18 * w0p2 * dtp2 + (16 - 24 * w0p2) * dt + 3 * w0p4;
+    const T l[5] = {d * w0 * (w0p2 + dtp2), d * (4 - 6 * w1p2 + 3 * w1p3 + (1 - 3 * w0) * dtp2),
+                    d * (4 - 6 * w0p2 + 3 * w0p3 - (2 - 3 * w0) * dtp2), d * w1 * (w1p2 + dtp2),
+                    T(0)};
+    const T m[5] = {adt * w4p4, cdt * (ss1 + 2 * dtp3 * w3 + 2 * dt * (-6 * delta + w3p3)),
+                    bdt * ss2, cdt * (ss1 + 2 * dtp3 * w2 + 2 * dt * (+6 * delta + w2p3)),
+                    adt * w5p4};
+    const T r[5] = {l[4], l[3], l[2], l[1], l[0]};
+    for (int k = 0; k < 5; k++)
+      s[k] = l[k] * t1 + m[k] * t2 * t3 + r[k] * t4;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Esirkepov primitives (nix/esirkepov.hpp)
 // ---------------------------------------------------------------------------
@@ -959,7 +1073,7 @@ struct Geo {
 // weights (velocity.hpp:223-359, MC interpolation) and the scalar gather
 // (velocity.hpp:420-450 push_scalar3d; :390-418 push_scalar2d; nix/interp.hpp:14-130)
 template <int Dim, int Order>
-void push_velocity_chunk(Chunk& c, int Ns, double cc, int pusher, double delt)
+void push_velocity_chunk(Chunk& c, int Ns, double cc, int pusher, int interp, double delt)
 {
   constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
   constexpr int S      = Order + 1; // points used by the scalar gather (Stencil + 2)
@@ -987,8 +1101,15 @@ void push_velocity_chunk(Chunk& c, int Ns, double cc, int pusher, double delt)
       int iy0 = digitize(xu[1], yimin, rdy);
       int hx0 = digitize(xu[0], xhmin, rdx);
       int hy0 = digitize(xu[1], yhmin, rdy);
-      shape_mc<Order>(xu[0], xigrid + static_cast<double>(ix0) * g.dx, rdx, wix);
-      shape_mc<Order>(xu[1], yigrid + static_cast<double>(iy0) * g.dy, rdy, wiy);
+      // integer-grid weights: MC, or WT with c dt / dx (velocity.hpp:240-250, 283-293)
+      const double cdtx = cc * delt / g.dx, cdty = cc * delt / g.dy;
+      if (interp == 1) {
+        shape_wt<Order>(xu[0], xigrid + static_cast<double>(ix0) * g.dx, rdx, cdtx, 1 / cdtx, wix);
+        shape_wt<Order>(xu[1], yigrid + static_cast<double>(iy0) * g.dy, rdy, cdty, 1 / cdty, wiy);
+      } else {
+        shape_mc<Order>(xu[0], xigrid + static_cast<double>(ix0) * g.dx, rdx, wix);
+        shape_mc<Order>(xu[1], yigrid + static_cast<double>(iy0) * g.dy, rdy, wiy);
+      }
       shape_mc<Order>(xu[0], xhgrid + static_cast<double>(hx0) * g.dx, rdx, whx);
       shape_mc<Order>(xu[1], yhgrid + static_cast<double>(hy0) * g.dy, rdy, why);
       ix0 += g.lbx - (Order / 2);
@@ -1000,7 +1121,11 @@ void push_velocity_chunk(Chunk& c, int Ns, double cc, int pusher, double delt)
       if constexpr (Dim == 3) {
         int iz0 = digitize(xu[2], zimin, rdz);
         int hz0 = digitize(xu[2], zhmin, rdz);
-        shape_mc<Order>(xu[2], zigrid + static_cast<double>(iz0) * g.dz, rdz, wiz);
+        const double cdtz = cc * delt / g.dz;
+        if (interp == 1)
+          shape_wt<Order>(xu[2], zigrid + static_cast<double>(iz0) * g.dz, rdz, cdtz, 1 / cdtz, wiz);
+        else
+          shape_mc<Order>(xu[2], zigrid + static_cast<double>(iz0) * g.dz, rdz, wiz);
         shape_mc<Order>(xu[2], zhgrid + static_cast<double>(hz0) * g.dz, rdz, whz);
         iz0 += g.lbz - (Order / 2);
         hz0 += g.lbz - (Order / 2);
@@ -1372,9 +1497,9 @@ void dispatch(int dim, int order, Args&&... args)
 
 template <int D, int O>
 struct VelocityOp {
-  static void run(Chunk& c, int Ns, double cc, int pusher, double delt)
+  static void run(Chunk& c, int Ns, double cc, int pusher, int interp, double delt)
   {
-    push_velocity_chunk<D, O>(c, Ns, cc, pusher, delt);
+    push_velocity_chunk<D, O>(c, Ns, cc, pusher, interp, delt);
   }
 };
 template <int D, int O>
@@ -1616,7 +1741,7 @@ void pno_push_velocity(pno_sim* h, double delt)
 {
   Sim& s = h->s;
   for_chunks(s, [&](Chunk& c) {
-    dispatch<VelocityOp>(s.dim, s.order, c, s.Ns, s.p.cc, s.p.pusher, delt);
+    dispatch<VelocityOp>(s.dim, s.order, c, s.Ns, s.p.cc, s.p.pusher, s.p.interp, delt);
   });
 }
 
@@ -1665,7 +1790,7 @@ void pno_step(pno_sim* h)
   const double delt = s.p.delt;
   for_chunks(s, [&](Chunk& c) {
     push_bfd(c, s.dim, s.p.cc, s.p.friedman, 0.5 * delt);
-    dispatch<VelocityOp>(s.dim, s.order, c, s.Ns, s.p.cc, s.p.pusher, delt);
+    dispatch<VelocityOp>(s.dim, s.order, c, s.Ns, s.p.cc, s.p.pusher, s.p.interp, delt);
     push_position_chunk(c, s.Ns, s.p.cc, s.order, delt);
     dispatch<CurrentOp>(s.dim, s.order, c, s.Ns, delt);
     push_bfd(c, s.dim, s.p.cc, s.p.friedman, 0.5 * delt);
@@ -1707,6 +1832,16 @@ void pno_shape_mc(int order, double x, double X, double rdx, double* s)
   case 2: shape_mc<2>(x, X, rdx, s); break;
   case 3: shape_mc<3>(x, X, rdx, s); break;
   default: shape_mc<4>(x, X, rdx, s); break;
+  }
+}
+
+void pno_shape_wt(int order, double x, double X, double rdx, double dt, double rdt, double* s)
+{
+  switch (order) {
+  case 1: shape_wt<1>(x, X, rdx, dt, rdt, s); break;
+  case 2: shape_wt<2>(x, X, rdx, dt, rdt, s); break;
+  case 3: shape_wt<3>(x, X, rdx, dt, rdt, s); break;
+  default: shape_wt<4>(x, X, rdx, dt, rdt, s); break;
   }
 }
 

@@ -39,6 +39,7 @@ typedef struct pno_params {
   int    np[PNO_MAX_SPECIES];
   double qm[PNO_MAX_SPECIES], ro[PNO_MAX_SPECIES], vt[PNO_MAX_SPECIES];
   double vx[PNO_MAX_SPECIES], vy[PNO_MAX_SPECIES], vz[PNO_MAX_SPECIES];
+  int    interp;         /* 0 MC, 1 WT ([application.option] interpolation) */
 } pno_params;
 
 typedef struct pno_sim pno_sim;
@@ -77,6 +78,8 @@ void pno_set_threads(int n);
 
 /* ---- primitive entry points (for closed-form and invariant tests) ---- */
 void   pno_shape_mc(int order, double x, double X, double rdx, double* s);
+/* WT weights (nix/primitives.hpp:331-566); dt = c dt / dx, rdt = 1 / dt */
+void   pno_shape_wt(int order, double x, double X, double rdx, double dt, double rdt, double* s);
 int    pno_digitize(double x, double xmin, double rdx);
 double pno_lorentz_factor(double ux, double uy, double uz, double rc);
 void   pno_push_boris(double* u, const double* eb, double cc);

@@ -415,9 +415,9 @@ int pn_create(const char* toml, int load, pn_sim** out)
       *out = h;
       return PN_ERR_CONFIG;
     }
-    if (s.cfg.interp != 0 || s.cfg.precision != "fp64" || s.cfg.Ns > kMaxSpecies) {
+    if (s.cfg.precision != "fp64" || s.cfg.Ns > kMaxSpecies) {
       s.err = "configuration uses an option this build does not implement yet "
-              "(supported: interpolation = MC, precision = fp64, Ns <= 8)";
+              "(supported: precision = fp64, Ns <= 8)";
       *out  = h;
       return PN_ERR_UNSUPPORTED;
     }

@@ -166,6 +166,120 @@ __device__ __forceinline__ void shape_mc(T x, T X, T rdx, T* s)
   }
 }
 
+template <typename T>
+__device__ __forceinline__ T wt_abs(T v)
+{
+  return fabs(v);
+}
+
+// WT shape functions (nix/primitives.hpp:331-566; Lu et al., J. Comput. Phys. 413,
+// 109388 (2020)): the B-spline of degree Order-1 smeared by a top hat of half
+// width dt = c dt / dx; rdt = 1 / dt.  The branch factors t1..t4 combine the
+// left/middle/right pieces exactly as the reference does.
+template <int Order, typename T>
+__device__ __forceinline__ void shape_wt(T x, T X, T rdx, T dt, T rdt, T* s)
+{
+  const T delta = (x - X) * rdx;
+  if constexpr (Order == 1) {
+    // top-hat (half width dt) smeared nearest-grid-point weight
+    T ss = T(0.25) * rdt * (1 + 2 * dt - 2 * delta);
+    ss   = T(0) < ss ? ss : T(0); // max(0, .) then min(1, .), as std::max / std::min
+    ss   = ss < T(1) ? ss : T(1);
+    s[0] = ss;
+    s[1] = 1 - ss;
+  } else if constexpr (Order == 2) {
+    const T t1 = delta < -dt ? T(1) : T(0);
+    const T t2 = 1 - t1;
+    const T t3 = delta < +dt ? T(1) : T(0);
+    const T t4 = 1 - t3;
+    const T w0 = wt_abs(delta);
+    const T w1 = dt - delta;
+    const T w2 = dt + delta;
+    // left (t1), middle (t2 t3) and right (t4) branches of the piecewise weight
+    const T l[3] = {w0, 1 - w0, T(0)};
+    const T m[3] = {T(0.25) * rdt * w1 * w1, T(0.50) * rdt * (dt * (2 - dt) - w0 * w0),
+                    T(0.25) * rdt * w2 * w2};
+    const T r[3] = {l[2], l[1], l[0]};
+    for (int k = 0; k < 3; k++)
+      s[k] = l[k] * t1 + m[k] * t2 * t3 + r[k] * t4;
+  } else if constexpr (Order == 3) {
+    const T a    = T(1) / T(96.0);
+    const T b    = T(1) / T(24.0);
+    const T c    = T(1) / T(12.0);
+    const T adt  = a * rdt;
+    const T t1   = delta < T(0.5) - dt ? T(1) : T(0);
+    const T t2   = 1 - t1;
+    const T t3   = delta < T(0.5) + dt ? T(1) : T(0);
+    const T t4   = 1 - t3;
+    const T w0   = delta;
+    const T w1   = 1 - delta;
+    const T w3   = 1 - 2 * delta;
+    const T w4   = 1 + 2 * delta;
+    const T w5   = 2 * dt + w3;
+    const T w6   = 2 * dt - w3;
+    const T w7   = 3 - 2 * delta;
+    const T w0p2 = w0 * w0;
+    const T w1p2 = w1 * w1;
+    const T w3p2 = w3 * w3;
+    const T w3p3 = w3p2 * w3;
+    const T w4p2 = w4 * w4;
+    const T w5p3 = w5 * w5 * w5;
+    const T w6p3 = w6 * w6 * w6;
+    const T w7p2 = w7 * w7;
+    const T dtp2 = dt * dt;
+    const T dtp3 = dtp2 * dt;
+    const T dt4  = 4 * dtp2;
+    const T sodd  = adt * (-8 * dtp3 - 6 * dt * w3p2);
+    const T seven = adt * (-36 * dtp2 * w3 - 3 * w3p3);
+    const T l[4] = {b * (dt4 + 3 * w3p2), c * (9 - dt4 - 12 * w0p2), b * (dt4 + 3 * w4p2), T(0)};
+    const T m[4] = {adt * w5p3, sodd + seven + w1, sodd - seven + w0, adt * w6p3};
+    const T r[4] = {T(0), b * (dt4 + 3 * w7p2), c * (9 - dt4 - 12 * w1p2), b * (dt4 + 3 * w3p2)};
+    for (int k = 0; k < 4; k++)
+      s[k] = l[k] * t1 + m[k] * t2 * t3 + r[k] * t4;
+  } else {
+    const T a    = T(1) / T(48.0);
+    const T b    = T(1) / T(24.0);
+    const T c    = T(1) / T(12.0);
+    const T d    = T(1) / T(6.0);
+    const T adt  = a * rdt;
+    const T bdt  = b * rdt;
+    const T cdt  = c * rdt;
+    const T t1   = delta < -dt ? T(1) : T(0);
+    const T t2   = 1 - t1;
+    const T t3   = delta < +dt ? T(1) : T(0);
+    const T t4   = 1 - t3;
+    const T w0   = wt_abs(delta);
+    const T w1   = 1 - w0;
+    const T w2   = 1 - delta;
+    const T w3   = 1 + delta;
+    const T w4   = dt - delta;
+    const T w5   = dt + delta;
+    const T w0p2 = w0 * w0;
+    const T w0p3 = w0p2 * w0;
+    const T w0p4 = w0p3 * w0;
+    const T w1p2 = w1 * w1;
+    const T w1p3 = w1p2 * w1;
+    const T w2p3 = w2 * w2 * w2;
+    const T w3p3 = w3 * w3 * w3;
+    const T w4p4 = w4 * w4 * w4 * w4;
+    const T w5p4 = w5 * w5 * w5 * w5;
+    const T dtp2 = dt * dt;
+    const T dtp3 = dtp2 * dt;
+    const T dtp4 = dtp3 * dt;
+    const T ss1  = -dtp4 - 6 * w0p2 * dtp2 - w0p4;
+    const T ss2  = 3 * dtp4 - 8 * dtp3 + 18 * w0p2 * dtp2 + (16 - 24 * w0p2) * dt + 3 * w0p4;
+    const T l[5] = {d * w0 * (w0p2 + dtp2), d * (4 - 6 * w1p2 + 3 * w1p3 + (1 - 3 * w0) * dtp2),
+                    d * (4 - 6 * w0p2 + 3 * w0p3 - (2 - 3 * w0) * dtp2), d * w1 * (w1p2 + dtp2),
+                    T(0)};
+    const T m[5] = {adt * w4p4, cdt * (ss1 + 2 * dtp3 * w3 + 2 * dt * (-6 * delta + w3p3)),
+                    bdt * ss2, cdt * (ss1 + 2 * dtp3 * w2 + 2 * dt * (+6 * delta + w2p3)),
+                    adt * w5p4};
+    const T r[5] = {l[4], l[3], l[2], l[1], l[0]};
+    for (int k = 0; k < 5; k++)
+      s[k] = l[k] * t1 + m[k] * t2 * t3 + r[k] * t4;
+  }
+}
+
 // Slot allocation with one atomic per distinct counter in the wave: lanes that
 // want a slot from the same counter share a single atomicAdd and take
 // consecutive slots in lane order.  Every lane of the wave must call this with

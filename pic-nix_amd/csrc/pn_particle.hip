@@ -10,8 +10,22 @@ inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 struct PushConsts {
   double qmdt[kMaxSpecies]; // 0.5 * q / m * delt  (velocity.hpp:130)
+  double cdt[3], rcdt[3];   // c dt / d(x, y, z) and inverses, for WT (velocity.hpp:104-107)
   int    pusher;            // 0 Boris, 1 Vay, 2 Higuera-Cary
+  int    wt;                // integer-grid weights: 0 MC, 1 WT (velocity.hpp:240-250)
 };
+
+// Integer-grid weights of one axis: shape_mc, or shape_wt with c dt / d
+// (BaseVelocity::weights{2,3}d, pic/engine/velocity.hpp:262-359).
+template <int Order>
+__device__ __forceinline__ void int_weights(const PushConsts& pc, int axis, double x, double X,
+                                            double rd, double* w)
+{
+  if (pc.wt)
+    shape_wt<Order>(x, X, rd, pc.cdt[axis], pc.rcdt[axis], w);
+  else
+    shape_mc<Order>(x, X, rd, w);
+}
 
 // One thread per particle.  Weights follow BaseVelocity::weights{2,3}d
 // (pic/engine/velocity.hpp:262-359, MC interpolation) and the gather follows
@@ -55,8 +69,8 @@ __global__ void __launch_bounds__(256)
   int    iy0 = digitize(xu[1], yimin, rdy);
   int    hx0 = digitize(xu[0], xhmin, rdx);
   int    hy0 = digitize(xu[1], yhmin, rdy);
-  shape_mc<Order>(xu[0], xigrid + static_cast<double>(ix0) * dx, rdx, wix);
-  shape_mc<Order>(xu[1], yigrid + static_cast<double>(iy0) * dy, rdy, wiy);
+  int_weights<Order>(pc, 0, xu[0], xigrid + static_cast<double>(ix0) * dx, rdx, wix);
+  int_weights<Order>(pc, 1, xu[1], yigrid + static_cast<double>(iy0) * dy, rdy, wiy);
   shape_mc<Order>(xu[0], xhgrid + static_cast<double>(hx0) * dx, rdx, whx);
   shape_mc<Order>(xu[1], yhgrid + static_cast<double>(hy0) * dy, rdy, why);
   // local ghost-inclusive start index -> global padded index
@@ -79,7 +93,7 @@ __global__ void __launch_bounds__(256)
     double rdz    = 1 / dz;
     int    iz0    = digitize(xu[2], zimin, rdz);
     int    hz0    = digitize(xu[2], zhmin, rdz);
-    shape_mc<Order>(xu[2], zigrid + static_cast<double>(iz0) * dz, rdz, wiz);
+    int_weights<Order>(pc, 2, xu[2], zigrid + static_cast<double>(iz0) * dz, rdz, wiz);
     shape_mc<Order>(xu[2], zhgrid + static_cast<double>(hz0) * dz, rdz, whz);
     iz0 += lb - (Order / 2) + off[c * 3 + 0];
     hz0 += lb - (Order / 2) + off[c * 3 + 0];
@@ -251,8 +265,8 @@ __global__ void __launch_bounds__(256)
     int        iy0 = digitize(xu[1], yimin, rdy);
     int        hx0 = digitize(xu[0], xhmin, rdx);
     int        hy0 = digitize(xu[1], yhmin, rdy);
-    shape_mc<Order>(xu[0], xigrid + static_cast<double>(ix0) * dx, rdx, wix);
-    shape_mc<Order>(xu[1], yigrid + static_cast<double>(iy0) * dy, rdy, wiy);
+    int_weights<Order>(pc, 0, xu[0], xigrid + static_cast<double>(ix0) * dx, rdx, wix);
+    int_weights<Order>(pc, 1, xu[1], yigrid + static_cast<double>(iy0) * dy, rdy, wiy);
     shape_mc<Order>(xu[0], xhgrid + static_cast<double>(hx0) * dx, rdx, whx);
     shape_mc<Order>(xu[1], yhgrid + static_cast<double>(hy0) * dy, rdy, why);
     ix0 += lb - (Order / 2);
@@ -269,7 +283,7 @@ __global__ void __launch_bounds__(256)
       const double rdz    = 1 / dz;
       int          iz0    = digitize(xu[2], zimin, rdz);
       int          hz0    = digitize(xu[2], zhmin, rdz);
-      shape_mc<Order>(xu[2], zigrid + static_cast<double>(iz0) * dz, rdz, wiz);
+      int_weights<Order>(pc, 2, xu[2], zigrid + static_cast<double>(iz0) * dz, rdz, wiz);
       shape_mc<Order>(xu[2], zhgrid + static_cast<double>(hz0) * dz, rdz, whz);
       iz0 += lb - (Order / 2);
       hz0 += lb - (Order / 2);
@@ -490,6 +504,11 @@ void launch_push_velocity(Sim& s, double delt)
   for (int sp = 0; sp < g.Ns; sp++)
     pc.qmdt[sp] = 0.5 * g.q[sp] / g.m[sp] * delt;
   pc.pusher = s.cfg.pusher;
+  pc.wt     = s.cfg.interp;
+  for (int a = 0; a < 3; a++) {
+    pc.cdt[a]  = g.cc * delt / g.del[2 - a];
+    pc.rcdt[a] = 1 / pc.cdt[a];
+  }
   dim3      grid(nblk(g.cap, 256), g.nseg);
   const int T    = g.order + 2;
   const int need = g.bsy + g.order + 1; // row of the staged field tile

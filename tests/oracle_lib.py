@@ -28,6 +28,7 @@ class Params(ctypes.Structure):
         ("qm", ctypes.c_double * MAXS), ("ro", ctypes.c_double * MAXS),
         ("vt", ctypes.c_double * MAXS), ("vx", ctypes.c_double * MAXS),
         ("vy", ctypes.c_double * MAXS), ("vz", ctypes.c_double * MAXS),
+        ("interp", ctypes.c_int),
     ]
 
 
@@ -112,6 +113,7 @@ def make_params(cfg, load=True):
     p.delt = float(par["delt"])
     p.delh = float(par["delh"])
     p.friedman = float(opt.get("friedman", 0.0))
+    p.interp = {"MC": 0, "WT": 1}[opt.get("interpolation", "MC")]
     p.buffer_ratio = float(opt.get("buffer_ratio", 0.2))
     for k in ("Ex", "Ey", "Ez", "Bx", "By", "Bz"):
         setattr(p, k, float(par.get(k, 0.0)))

@@ -171,14 +171,21 @@ OPTION_CASES = {
     "friedman_vay_2d_o2": (2, lambda: _with_fields(helpers.small_twostream(
         N=(1, 16, 64), C=(1, 2, 4), np_=16, order=2, friedman=0.2, pusher="Vay"),
         B=(0.1, 0.0, 0.3))),
+    # WT interpolation of the integer-grid weights (nix/primitives.hpp:331-566)
+    "wt_3d_o2": (3, lambda: _with_fields(helpers.small_thermal(
+        N=(16, 16, 16), C=(2, 2, 2), np_=8, order=2, interpolation="WT"),
+        E=(0.001, 0.002, 0.0), B=(0.2, 0.0, 0.1))),
+    "wt_hc_2d_o3": (2, lambda: _with_fields(helpers.small_twostream(
+        N=(1, 16, 64), C=(1, 1, 4), np_=16, order=3, interpolation="WT", pusher="HigueraCary"),
+        E=(0.0, 0.001, 0.0), B=(0.0, 0.1, 0.2))),
 }
 
 
 @pytest.mark.parametrize("kind", list(OPTION_CASES))
 def test_step_parity_options(kind):
-    """Vay / Higuera-Cary pushers (nix/primitives.hpp:191-253) and the Friedman
-    filter (pic/engine/maxwell.hpp) against the oracle: particles bit-exact at
-    step 1, everything within 1e-12 at step 10."""
+    """Vay / Higuera-Cary pushers (nix/primitives.hpp:191-253), the Friedman
+    filter (pic/engine/maxwell.hpp) and WT interpolation against the oracle:
+    particles bit-exact at step 1, everything within 1e-12 at step 10."""
     dim, make = OPTION_CASES[kind]
     cfg = make()
     dt = float(cfg["parameter"]["delt"])

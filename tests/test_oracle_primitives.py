@@ -244,3 +244,49 @@ def test_vay_hc_relativistic_exb_is_exact(name):
     # Boris drifts off at gamma ~ 7
     got = _boris(u, np.concatenate([e, b]), c)
     assert np.max(np.abs(got - u)) > 1e-6 * np.linalg.norm(u)
+
+
+# --- WT shape functions (nix/primitives.hpp:331-566) ---
+
+def _box(x):
+    return 1.0 if -0.5 <= x < 0.5 else 0.0
+
+
+def _wt_reference(order, delta, dt):
+    """Weights of the WT shape of the given order at its nodes, by direct
+    quadrature: the degree-(order-1) centred B-spline smeared by a top hat of
+    half width dt (Lu et al., J. Comput. Phys. 413, 109388 (2020))."""
+    from scipy.integrate import quad
+    nodes = {1: [0, 1], 2: [-1, 0, 1], 3: [-1, 0, 1, 2], 4: [-2, -1, 0, 1, 2]}[order]
+    deg = order - 1
+    out = []
+    for n in nodes:
+        f = (lambda u, n=n: _box(delta + u - n)) if deg == 0 else \
+            (lambda u, n=n: bspline(deg, delta + u - n))
+        # breakpoints of the piecewise polynomial inside [-dt, dt]
+        bps = sorted({b for k in range(-4, 6) for b in (k - 0.5 - delta + n, k - delta + n)
+                      if -dt < b < dt})
+        val, _ = quad(f, -dt, dt, points=bps or None, epsabs=1e-15, epsrel=1e-13, limit=200)
+        out.append(val / (2 * dt))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_shape_wt_is_smeared_bspline(order):
+    L = ol.lib()
+    L.pno_shape_wt.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_double,
+                               ctypes.POINTER(ctypes.c_double)]
+    rng = np.random.default_rng(order)
+    lo, hi = (0.0, 1.0) if order % 2 == 1 else (-0.5, 0.5)
+    for dt in (0.1, 0.3, 0.45, 0.5):
+        for delta in np.concatenate([rng.uniform(lo, hi, 40), [lo, 0.5 - dt, 0.5 + dt - 1e-9,
+                                                                 -dt, dt - 1e-9]]):
+            if not (lo <= delta < hi):
+                continue
+            s = np.zeros(order + 1)
+            # X = 0, dx = 1: delta = x
+            L.pno_shape_wt(order, float(delta), 0.0, 1.0, dt, 1 / dt, _dp(s))
+            ref = _wt_reference(order, float(delta), dt)
+            assert np.max(np.abs(s - ref)) < 1e-12, (order, dt, delta, s, ref)
+            assert abs(s.sum() - 1) < 1e-13
