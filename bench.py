@@ -60,6 +60,34 @@ def algorithmic_bytes(phase, npart, ncell):
     return per_p * npart + per_c * ncell
 
 
+# kernel that carries each timed phase (its name prefix in rocprofv3 output)
+PHASE_KERNEL = {"bfd": "k_push_bfd", "velocity": "k_push_velocity_rows",
+                "position": "k_push_position", "deposit": "k_deposit_march",
+                "fold": "k_fold_line", "efd": "k_push_efd", "sort": "k_rank_bin",
+                "rho": "k_rho_march"}
+
+
+def profile_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    profiles/hbm_traffic_<config>_fp64.txt: FETCH_SIZE + WRITE_SIZE per dispatch,
+    each scaled by the copy-probe calibration recorded in the same file.
+    None when the summary or the kernel is missing."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}_fp64.txt")
+    if not os.path.exists(path):
+        return None
+    total, seen = 0.0, 0
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 4 and parts[0].startswith(kernel):
+                try:
+                    total += float(parts[-1]) * 1e6  # calibrated MB per dispatch
+                    seen += 1
+                except ValueError:
+                    pass
+    return total if seen == 2 else None
+
+
 def step_bytes(npart, ncell):
     """Whole-step algorithmic bytes: 336 B/particle + 332 B/cell (FP64)."""
     return 336 * npart + 332 * ncell
@@ -167,8 +195,9 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": d["GBps"] / HBM_PEAK_GBS,
-        "traffic": None,
-        "kernel": dom,
+        "traffic": profile_traffic(args.config, PHASE_KERNEL.get(dom, dom)),
+        "kernel": PHASE_KERNEL.get(dom, dom),
+        "phase": dom,
         "bytes_per_launch": d["bytes_per_launch"],
     }
     step_gbs = step_bytes(npart, ncell) * args.steps / elapsed / 1e9

@@ -113,6 +113,12 @@ int pn_exchange(pn_sim* sim, int mode);
 /* One full timestep in the reference operator order (PicApplication::push_openmp,
  * pic/pic_application.cpp:219-292). */
 int pn_step(pn_sim* sim, double delt);
+/* Replay pn_step as a captured hipGraph (enable = 1) or launch it eagerly (0,
+ * the default).  No counterpart in the reference (its step is a loop of OpenMP
+ * phases); for callers that run many steps back to back.  The graph is recorded
+ * at the first step that starts from sorted particles, once per buffer parity
+ * and time step; phase timing (pn_timing) runs steps eagerly. */
+int pn_set_step_graph(pn_sim* sim, int enable);
 /* Field energy E^2/2, B^2/2 and per-species kinetic energy (PicChunk::get_energy,
  * pic/pic_chunk.cpp:407-439, summed over chunks as HistoryDiag does). */
 int pn_get_energy(pn_sim* sim, double* efd, double* bfd, double* particle);

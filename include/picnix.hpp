@@ -73,6 +73,8 @@ public:
   void set_boundary(BoundaryMode mode) { check(pn_exchange(sim_, mode)); }
   // PicApplication::push: one full timestep
   void push(double delt) { check(pn_step(sim_, delt)); }
+  // replay push() as a captured hipGraph (no reference counterpart)
+  void set_step_graph(bool enable) { check(pn_set_step_graph(sim_, enable ? 1 : 0)); }
   void synchronize() { check(pn_synchronize(sim_)); }
 
   // PicChunk::get_energy / get_diverror, summed over chunks (pic/diag/history.hpp:44-61)

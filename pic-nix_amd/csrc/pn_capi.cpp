@@ -228,6 +228,9 @@ void allocate(Sim& s)
 
 void release(Sim& s)
 {
+  for (int k = 0; k < 2; k++)
+    if (s.gexec[k])
+      (void)hipGraphExecDestroy(s.gexec[k]);
   auto f = [](void* p) {
     if (p)
       (void)hipFree(p);
@@ -786,25 +789,84 @@ int pn_exchange(pn_sim* h, int mode)
 // bfd(dt/2), velocity, position(+count), current, bfd(dt/2); then the current
 // fold, efd, particle migration + sort, EMF exchange (done by the ghost refresh
 // that follows every field update).
+namespace {
+
+void step_launches(Sim& s, double delt)
+{
+  launch_push_bfd(s, 0.5 * delt);
+  launch_push_velocity(s, delt);
+  bool was_sorted = s.sorted;
+  launch_push_position(s, delt);
+  s.sorted = was_sorted;
+  // rho is deposited after the sort when the 3D march kernel runs: the new
+  // positions are then grouped by cell, and the result is the same sum
+  const bool rho_later = launch_deposit(s, delt, true);
+  launch_push_bfd(s, 0.5 * delt);
+  launch_fold_current(s, rho_later ? 1 : 0, 4);
+  launch_push_efd(s, delt);
+  launch_migrate(s);
+  if (rho_later) {
+    launch_rho(s, true);
+    launch_fold_current(s, 0, 1);
+  }
+}
+
+void drop_graphs(Sim& s)
+{
+  for (int k = 0; k < 2; k++)
+    if (s.gexec[k]) {
+      (void)hipGraphExecDestroy(s.gexec[k]);
+      s.gexec[k] = nullptr;
+    }
+}
+
+} // namespace
+
 int pn_step(pn_sim* h, double delt)
 {
   return guarded(h, [&](Sim& s) {
-    launch_push_bfd(s, 0.5 * delt);
-    launch_push_velocity(s, delt);
-    bool was_sorted = s.sorted;
-    launch_push_position(s, delt);
-    s.sorted = was_sorted;
-    // rho is deposited after the sort when the 3D plane kernel runs: the new
-    // positions are then grouped by cell, and the result is the same sum
-    const bool rho_later = launch_deposit(s, delt, true);
-    launch_push_bfd(s, 0.5 * delt);
-    launch_fold_current(s, rho_later ? 1 : 0, 4);
-    launch_push_efd(s, delt);
-    launch_migrate(s);
-    if (rho_later) {
-      launch_rho(s, true);
-      launch_fold_current(s, 0, 1);
+    // a replayed graph needs the state the capture saw: sorted particles and
+    // no phase timing (event records are host-side bookkeeping)
+    if (!s.step_graph || !s.sorted || s.timing) {
+      step_launches(s, delt);
+      return PN_OK;
     }
+    if (!s.buf0_x)
+      s.buf0_x = s.pa.x;
+    const int k = s.pa.x == s.buf0_x ? 0 : 1;
+    if (!s.gexec[k] || s.graph_dt[k] != delt) {
+      if (s.gexec[k]) {
+        (void)hipGraphExecDestroy(s.gexec[k]);
+        s.gexec[k] = nullptr;
+      }
+      // record the launches (nothing runs), then run the recording; the host
+      // state (buffer swap, flags) has advanced exactly as an eager step would
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+      step_launches(s, delt);
+      HIPCHK(hipStreamEndCapture(s.stream, &graph));
+      hipError_t e = hipGraphInstantiate(&s.gexec[k], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(e);
+      s.graph_dt[k] = delt;
+    } else {
+      // replay: advance the host-side state as the recorded step does
+      std::swap(s.pa, s.pb);
+      s.np_host_valid = false;
+      s.counted       = false;
+      s.sorted        = true;
+    }
+    HIPCHK(hipGraphLaunch(s.gexec[k], s.stream));
+    return PN_OK;
+  });
+}
+
+int pn_set_step_graph(pn_sim* h, int enable)
+{
+  return guarded(h, [&](Sim& s) {
+    s.step_graph = enable != 0;
+    if (!s.step_graph)
+      drop_graphs(s);
     return PN_OK;
   });
 }

@@ -1138,7 +1138,7 @@ bool launch_deposit(Sim& s, double delt, bool defer_rho)
   dc.dzdt = g.del[0] / delt;
   {
     PhaseTimer pz(s, 4);
-    hipMemsetAsync(s.d_J, 0, sizeof(double) * 4 * (size_t)g.vol, s.stream);
+    launch_zero(s, s.d_J, 4 * g.vol);
   }
   bool need_rho;
   {

@@ -7,6 +7,8 @@
 // with an Nb-wide padding that mirrors the opposite edge, so every interior
 // cell sees the same neighbour values as in the reference and gets the same
 // floating-point result; the chunk edges need no copies at all.
+#include <algorithm>
+
 #include "pn_sim.hpp"
 
 namespace picnix {
@@ -201,7 +203,40 @@ __global__ void __launch_bounds__(256) k_fold_line(Geom g, double* __restrict__ 
 
 inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// Plain fills and copies as kernels: a step that may be captured into a
+// hipGraph records no memset or memcpy nodes.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(T* __restrict__ p, T v, long n)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_copy_i32(int32_t* __restrict__ d, const int32_t* __restrict__ s,
+                                                  long n)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+inline int fill_grid(long n) { return (int)std::min<long>(std::max<long>(nblk(n, 256), 1), 4096); }
+
 } // namespace
+
+void launch_zero(Sim& s, double* p, long n)
+{
+  hipLaunchKernelGGL(k_fill<double>, dim3(fill_grid(n)), dim3(256), 0, s.stream, p, 0.0, n);
+}
+
+void launch_zero(Sim& s, int32_t* p, long n)
+{
+  hipLaunchKernelGGL(k_fill<int32_t>, dim3(fill_grid(n)), dim3(256), 0, s.stream, p, 0, n);
+}
+
+void launch_copy(Sim& s, int32_t* dst, const int32_t* src, long n)
+{
+  hipLaunchKernelGGL(k_copy_i32, dim3(fill_grid(n)), dim3(256), 0, s.stream, dst, src, n);
+}
 
 void launch_push_bfd(Sim& s, double delt)
 {

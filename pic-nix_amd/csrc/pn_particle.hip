@@ -556,7 +556,7 @@ void launch_push_position(Sim& s, double delt)
 {
   PhaseTimer  pt(s, 2);
   const Geom& g = s.g;
-  hipMemsetAsync(s.ss.cellcnt, 0, sizeof(int32_t) * (size_t)g.nseg * g.nbin, s.stream);
+  launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
   dim3 grid(s.ss.ntile, g.nseg);
   hipLaunchKernelGGL(k_push_position, grid, dim3(256), 0, s.stream, g, s.pa, s.d_xo[0],
                      s.d_xo[1], s.d_xo[2], s.d_np, s.d_lim, s.d_rng, delt, 1 / g.cc,
@@ -568,7 +568,7 @@ void launch_push_position(Sim& s, double delt)
 void launch_count(Sim& s)
 {
   const Geom& g = s.g;
-  hipMemsetAsync(s.ss.cellcnt, 0, sizeof(int32_t) * (size_t)g.nseg * g.nbin, s.stream);
+  launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
   dim3 grid(s.ss.ntile, g.nseg);
   hipLaunchKernelGGL(k_count, grid, dim3(256), 0, s.stream, g, s.pa, s.d_np, s.d_lim, s.d_rng,
                      s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx);

@@ -68,6 +68,13 @@ struct Sim {
   bool        generic_kernels = false; // PICNIX_KERNELS=generic: one-thread-per-particle forms
   int         march_rows      = 0;     // PICNIX_MARCH_ROWS: bin rows per march block (0: auto)
 
+  // pn_step as a replayed hipGraph (pn_set_step_graph).  The particle arrays
+  // swap roles every step, so there is one graph per buffer parity.
+  bool            step_graph  = false;
+  hipGraphExec_t  gexec[2]    = {nullptr, nullptr};
+  double          graph_dt[2] = {0, 0};
+  double*         buf0_x      = nullptr; // pa.x of parity 0
+
   // staging for host <-> device particle transfers
   double* d_stage     = nullptr;
   size_t  stage_elems = 0;
@@ -99,6 +106,10 @@ struct PhaseTimer {
 void launch_push_bfd(Sim& s, double delt);
 void launch_push_efd(Sim& s, double delt);
 void launch_fill_ghost(Sim& s, double* f, int ncomp);
+// zero fills and copies as kernels (no memset/memcpy nodes under graph capture)
+void launch_zero(Sim& s, double* p, long n);
+void launch_zero(Sim& s, int32_t* p, long n);
+void launch_copy(Sim& s, int32_t* dst, const int32_t* src, long n);
 void launch_init_friedman(Sim& s); // every Friedman level = E (no-op for theta = 0)
 // fold the padding of components [k0, k1) of a [ncomp][vol] global array onto
 // the periodic interior (the Cur / Mom exchanges); phase: timing slot

@@ -318,8 +318,7 @@ void finish_sort(Sim& s)
   const long nrb   = nblk(nwave, kRankBins);
   hipLaunchKernelGGL(k_rank_bin, dim3(xcd_grid(nrb)), dim3(256), 0, s.stream, g, s.pa, s.pb,
                      s.ss.cellstart, s.ss.mkey, s.ss.msrc, nrb);
-  hipMemcpyAsync(s.d_np, s.ss.npnew, sizeof(int32_t) * g.nseg, hipMemcpyDeviceToDevice,
-                 s.stream);
+  launch_copy(s, s.d_np, s.ss.npnew, g.nseg);
   std::swap(s.pa, s.pb);
   s.np_host_valid = false;
   s.counted       = false;

@@ -29,7 +29,8 @@ EXPORTS = [
     "pn_chunk_info", "pn_get_np", "pn_get_particles", "pn_set_particles",
     "pn_get_particles_old", "pn_get_gindex", "pn_get_pindex", "pn_get_fields",
     "pn_set_fields", "pn_push_bfd", "pn_push_efd", "pn_push_velocity", "pn_push_position",
-    "pn_deposit_current", "pn_deposit_moment", "pn_init_friedman", "pn_sort_particle", "pn_exchange", "pn_step",
+    "pn_deposit_current", "pn_deposit_moment", "pn_init_friedman", "pn_set_step_graph",
+    "pn_sort_particle", "pn_exchange", "pn_step",
     "pn_get_energy",
     "pn_get_diverror", "pn_synchronize", "pn_timing", "pn_timing_report",
 ]
@@ -85,6 +86,7 @@ def load_library(path=LIB_PATH):
         "pn_sort_particle": [P, i],
         "pn_deposit_moment": [P, i],
         "pn_init_friedman": [P],
+        "pn_set_step_graph": [P, i],
         "pn_exchange": [P, i],
         "pn_step": [P, d],
         "pn_get_energy": [P, dp, dp, dp],
@@ -219,6 +221,10 @@ class Simulation:
     def deposit_moment(self):
         """PicChunk::deposit_moment; follow with exchange(BOUNDARY_MOM)."""
         self._chk(self.L.pn_deposit_moment(self.h, -1))
+
+    def step_graph(self, enable=True):
+        """Replay pn_step as a captured hipGraph (pn_set_step_graph)."""
+        self._chk(self.L.pn_set_step_graph(self.h, 1 if enable else 0))
 
     def init_friedman(self):
         self._chk(self.L.pn_init_friedman(self.h))
