@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256)
 // summation order), reading LDS instead of global memory.  A particle whose
 // stencil leaves the staged region (a rounding edge of the bin test) reads
 // global memory.
-template <int Dim, int Order, int RS>
+template <int Dim, int Order, bool Plain>
 __global__ void __launch_bounds__(256)
     k_push_velocity_rows(Geom g, PArr p, const int32_t* __restrict__ cellstart,
                          const int* __restrict__ off, const double* __restrict__ lim,
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(256)
   constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
   constexpr int S      = Order + 1;
   constexpr int T      = Order + 2;
-  extern __shared__ double ft[]; // [6][TZ][T][RS]; RS >= TXv, a compile-time row stride
+  extern __shared__ double ft[]; // [6][TZ][T][TXv]
   const int seg = blockIdx.y;
   const int c   = seg / g.Ns;
   const int sp  = seg - c * g.Ns;
@@ -194,20 +194,20 @@ __global__ void __launch_bounds__(256)
   const int  tz0 = Dim == 3 ? bz - is_odd + lb - (Order / 2) : lb;
   const int  ty0 = by - is_odd + lb - (Order / 2);
   const int  tx0 = -is_odd + lb - (Order / 2);
-  const int     TXv = nbx + Order + 1;         // valid row length
-  constexpr int pc1 = (Dim == 3 ? T : 1) * T * RS; // per component
-  const long    sy = g.P[2], sz = g.plane, v = g.vol;
-  const int     offz = off[c * 3 + 0], offy = off[c * 3 + 1], offx = off[c * 3 + 2];
+  const int  TXv = nbx + Order + 1;
+  const int  pc1 = TZ * T * TXv; // per component
+  const long sy = g.P[2], sz = g.plane, v = g.vol;
+  const int  offz = off[c * 3 + 0], offy = off[c * 3 + 1], offx = off[c * 3 + 2];
   for (int t = threadIdx.x; t < 6 * pc1; t += blockDim.x) {
     int k  = t / pc1;
     int r  = t - k * pc1;
-    int tz = r / (T * RS);
-    int r2 = r - tz * T * RS;
-    int ty = r2 / RS;
-    int tx = r2 - ty * RS;
+    int tz = r / (T * TXv);
+    int r2 = r - tz * T * TXv;
+    int ty = r2 / TXv;
+    int tx = r2 - ty * TXv;
     int gz = Dim == 3 ? offz + tz0 + tz : 0;
     const double* F = k < 3 ? E + k * v : B + (k - 3) * v;
-    ft[t] = tx < TXv ? F[gz * sz + (offy + ty0 + ty) * sy + offx + tx0 + tx] : 0.0;
+    ft[t] = F[gz * sz + (offy + ty0 + ty) * sy + offx + tx0 + tx];
   }
   __syncthreads();
 
@@ -265,8 +265,13 @@ __global__ void __launch_bounds__(256)
     int        iy0 = digitize(xu[1], yimin, rdy);
     int        hx0 = digitize(xu[0], xhmin, rdx);
     int        hy0 = digitize(xu[1], yhmin, rdy);
-    int_weights<Order>(pc, 0, xu[0], xigrid + static_cast<double>(ix0) * dx, rdx, wix);
-    int_weights<Order>(pc, 1, xu[1], yigrid + static_cast<double>(iy0) * dy, rdy, wiy);
+    if constexpr (Plain) {
+      shape_mc<Order>(xu[0], xigrid + static_cast<double>(ix0) * dx, rdx, wix);
+      shape_mc<Order>(xu[1], yigrid + static_cast<double>(iy0) * dy, rdy, wiy);
+    } else {
+      int_weights<Order>(pc, 0, xu[0], xigrid + static_cast<double>(ix0) * dx, rdx, wix);
+      int_weights<Order>(pc, 1, xu[1], yigrid + static_cast<double>(iy0) * dy, rdy, wiy);
+    }
     shape_mc<Order>(xu[0], xhgrid + static_cast<double>(hx0) * dx, rdx, whx);
     shape_mc<Order>(xu[1], yhgrid + static_cast<double>(hy0) * dy, rdy, why);
     ix0 += lb - (Order / 2);
@@ -283,13 +288,16 @@ __global__ void __launch_bounds__(256)
       const double rdz    = 1 / dz;
       int          iz0    = digitize(xu[2], zimin, rdz);
       int          hz0    = digitize(xu[2], zhmin, rdz);
-      int_weights<Order>(pc, 2, xu[2], zigrid + static_cast<double>(iz0) * dz, rdz, wiz);
+      if constexpr (Plain)
+        shape_mc<Order>(xu[2], zigrid + static_cast<double>(iz0) * dz, rdz, wiz);
+      else
+        int_weights<Order>(pc, 2, xu[2], zigrid + static_cast<double>(iz0) * dz, rdz, wiz);
       shape_mc<Order>(xu[2], zhgrid + static_cast<double>(hz0) * dz, rdz, whz);
       iz0 += lb - (Order / 2);
       hz0 += lb - (Order / 2);
       in = in && min(iz0, hz0) >= tz0 && max(iz0, hz0) + S <= tz0 + T;
       if (in) {
-        constexpr long bsz = (long)T * RS, bsy = RS;
+        const long bsz = (long)T * TXv, bsy = TXv;
         const int  Z = iz0 - tz0, HZ = hz0 - tz0, Y = iy0 - ty0, HY = hy0 - ty0;
         const int  X = ix0 - tx0, HX = hx0 - tx0;
         eb[0] = interp3(ft + 0 * pc1, bsz, bsy, Z, Y, HX, wiz, wiy, whx);
@@ -315,7 +323,7 @@ __global__ void __launch_bounds__(256)
       (void)wiz;
       (void)whz;
       if (in) {
-        constexpr long bsy = RS;
+        const long bsy = TXv;
         const int  Y = iy0 - ty0, HY = hy0 - ty0, X = ix0 - tx0, HX = hx0 - tx0;
         eb[0] = interp2(ft + 0 * pc1, bsy, Y, HX, wiy, whx);
         eb[1] = interp2(ft + 1 * pc1, bsy, HY, X, why, wix);
@@ -336,7 +344,10 @@ __global__ void __launch_bounds__(256)
         eb[5] = interp2(B + 2 * v, sy, HY, HX, why, whx);
       }
     }
-    push_momentum(pc.pusher, xu[3], xu[4], xu[5], eb, g.cc);
+    if constexpr (Plain)
+      push_boris(xu[3], xu[4], xu[5], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], g.cc);
+    else
+      push_momentum(pc.pusher, xu[3], xu[4], xu[5], eb, g.cc);
     p.ux[i] = xu[3];
     p.uy[i] = xu[4];
     p.uz[i] = xu[5];
@@ -509,26 +520,21 @@ void launch_push_velocity(Sim& s, double delt)
     pc.cdt[a]  = g.cc * delt / g.del[2 - a];
     pc.rcdt[a] = 1 / pc.cdt[a];
   }
-  dim3      grid(nblk(g.cap, 256), g.nseg);
-  const int T    = g.order + 2;
-  const int need = g.bsy + g.order + 1; // row of the staged field tile
-  // compile-time row stride of the LDS tile (folds every stencil offset into
-  // the LDS instruction's immediate): the smallest of 16, 24, 32 that fits
-  const int    rs   = need <= 16 ? 16 : (need <= 24 ? 24 : (need <= 32 ? 32 : 0));
-  const size_t lds  = sizeof(double) * 6 * (g.dim == 3 ? T : 1) * T * rs;
-  const bool   rows = s.sorted && !s.generic_kernels && rs > 0 && lds <= 65536;
-  const int    nrow = g.nbin / g.bsy;
-#define PN_VEL_RS(D, O, R)                                                                    \
-  hipLaunchKernelGGL((k_push_velocity_rows<D, O, R>), dim3(nrow, g.nseg), dim3(256), lds,      \
+  dim3         grid(nblk(g.cap, 256), g.nseg);
+  const int    T     = g.order + 2;
+  const size_t lds   = sizeof(double) * 6 * (g.dim == 3 ? T : 1) * T * (g.bsy + g.order + 1);
+  const bool   rows  = s.sorted && !s.generic_kernels && lds <= 65536;
+  const int    nrow  = g.nbin / g.bsy;
+  const bool   plain = pc.pusher == 0 && pc.wt == 0; // Boris + MC: no option branches
+#define PN_VEL_ROWS(D, O, PL)                                                                 \
+  hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL>), dim3(nrow, g.nseg), dim3(256), lds,     \
                      s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc)
 #define PN_VEL(D, O)                                                                          \
   do {                                                                                        \
-    if (rows && rs == 16)                                                                     \
-      PN_VEL_RS(D, O, 16);                                                                    \
-    else if (rows && rs == 24)                                                                \
-      PN_VEL_RS(D, O, 24);                                                                    \
+    if (rows && plain)                                                                        \
+      PN_VEL_ROWS(D, O, true);                                                                \
     else if (rows)                                                                            \
-      PN_VEL_RS(D, O, 32);                                                                    \
+      PN_VEL_ROWS(D, O, false);                                                               \
     else                                                                                      \
       hipLaunchKernelGGL((k_push_velocity<D, O>), grid, dim3(256), 0, s.stream, g, s.pa,      \
                          s.d_np, s.d_off, s.d_lim, s.d_E, s.d_B, pc);                         \
@@ -549,7 +555,7 @@ void launch_push_velocity(Sim& s, double delt)
     }
   }
 #undef PN_VEL
-#undef PN_VEL_RS
+#undef PN_VEL_ROWS
 }
 
 void launch_push_position(Sim& s, double delt)

@@ -286,11 +286,11 @@ __device__ __forceinline__ void rank_one_bin(const Geom& g, const PArr& src, con
   }
 }
 
-// A block owns kRankBins consecutive bins and its wavefronts take them in turn,
-// so that a launch is a few thousand blocks of real work rather than one short
-// block per four bins; consecutive blocks run on one XCD (xcd_block), so
-// records moving between neighbouring bins are read through the same L2.
-constexpr int kRankBins = 64;
+// A block owns kRankBins consecutive bins, one per wavefront (more bins per
+// block, walked in turn, measured slower: fewer bins in flight per CU);
+// consecutive blocks run on one XCD (xcd_block), so records moving between
+// neighbouring bins are read through the same L2.
+constexpr int kRankBins = 4;
 
 __global__ void __launch_bounds__(256)
     k_rank_bin(Geom g, PArr src, PArr dst, const int32_t* __restrict__ cellstart,
