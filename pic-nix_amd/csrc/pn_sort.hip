@@ -259,10 +259,23 @@ __device__ __forceinline__ void rank_one_bin(const Geom& g, const PArr& src, con
   const int32_t* mk = mkey + (long)seg * g.cap + base;
   const int32_t* ms = msrc + (long)seg * g.cap + base;
   for (int c0 = 0; c0 < n; c0 += 64) {
-    const int          q     = c0 + lane;
-    const bool         valid = q < n;
-    const unsigned int mine  = valid ? sort_key(mk[q]) : 0xFFFFFFFFu;
-    int                rank  = 0;
+    const int  q     = c0 + lane;
+    const bool valid = q < n;
+    // the record is loaded first, so that its latency overlaps the ranking
+    double  rx = 0, ry = 0, rz = 0, rux = 0, ruy = 0, ruz = 0;
+    int64_t rid = 0;
+    if (valid) {
+      const long si = ms[q];
+      rx            = src.x[si];
+      ry            = src.y[si];
+      rz            = src.z[si];
+      rux           = src.ux[si];
+      ruy           = src.uy[si];
+      ruz           = src.uz[si];
+      rid           = src.id[si];
+    }
+    const unsigned int mine = valid ? sort_key(mk[q]) : 0xFFFFFFFFu;
+    int                rank = 0;
     for (int c1 = 0; c1 < n; c1 += 64) {
       const int          q1    = c1 + lane;
       const unsigned int other = q1 < n ? sort_key(mk[q1]) : 0xFFFFFFFFu;
@@ -273,15 +286,14 @@ __device__ __forceinline__ void rank_one_bin(const Geom& g, const PArr& src, con
       }
     }
     if (valid) {
-      const long si = ms[q];
       const long di = (long)seg * g.cap + base + rank;
-      dst.x[di]     = src.x[si];
-      dst.y[di]     = src.y[si];
-      dst.z[di]     = src.z[si];
-      dst.ux[di]    = src.ux[si];
-      dst.uy[di]    = src.uy[si];
-      dst.uz[di]    = src.uz[si];
-      dst.id[di]    = src.id[si];
+      dst.x[di]     = rx;
+      dst.y[di]     = ry;
+      dst.z[di]     = rz;
+      dst.ux[di]    = rux;
+      dst.uy[di]    = ruy;
+      dst.uz[di]    = ruz;
+      dst.id[di]    = rid;
     }
   }
 }
