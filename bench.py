@@ -143,6 +143,8 @@ def main():
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--fp32", default="on", choices=["on", "off"],
+                    help="also time the FP32 form on the same workload")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device copy of this size (calibrates HBM counters)")
     args = ap.parse_args()
@@ -154,30 +156,58 @@ def main():
         sys.exit(2)
 
     import picnix_amd
-    sim = picnix_amd.Simulation.from_file(picnix_amd.config_path(args.config))
+    import tomlmini
     with open(picnix_amd.config_path(args.config)) as f:
-        import tomlmini
-        cfg = tomlmini.loads(f.read())
+        toml_text = f.read()
+    cfg = tomlmini.loads(toml_text)
     par = cfg["parameter"]
     dt = float(par["delt"])
     ncell = int(par["Nx"]) * int(par["Ny"]) * int(par["Nz"])
-    npart = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
 
-    sim.step(dt, args.warmup)
-    sim.synchronize()
-    sim.timing(True)
-    t0 = time.perf_counter()
-    sim.step(dt, args.steps)
-    sim.synchronize()
-    t1 = time.perf_counter()
-    rep = sim.timing_report()
-    sim.timing(False)
-    npart_end = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
-    sim.close()
+    def run_form(precision):
+        """Warm-up, then K timed steps of one precision form; returns the wall
+        time, the particle counts, the per-phase report and the final history."""
+        text = toml_text
+        if precision == "fp32":
+            text = text.replace("[application.option]",
+                                "[application.option]\n    precision = 'fp32'", 1)
+        sim = picnix_amd.Simulation(text)
+        npart = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
+        sim.step(dt, args.warmup)
+        sim.synchronize()
+        sim.timing(True)
+        t0 = time.perf_counter()
+        sim.step(dt, args.steps)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        rep = sim.timing_report()
+        sim.timing(False)
+        npart_end = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
+        hist = sim.history()
+        sim.close()
+        return t1 - t0, npart, npart_end, rep, hist
+
+    elapsed, npart, npart_end, rep, hist64 = run_form("fp64")
+    fp32 = None
+    if args.fp32 == "on":
+        e32, n32, n32_end, rep32, hist32 = run_form("fp32")
+        # error of the FP32 form against the FP64 form after the same steps
+        names = ["div(E)-rho", "div(B)", "E^2/2", "B^2/2"] + \
+                [f"kinetic[{k}]" for k in range(len(hist64) - 4)]
+        err = {names[k]: (abs(hist32[k] - hist64[k]) / abs(hist64[k]) if hist64[k] != 0 else
+                          abs(hist32[k])) for k in range(2, len(hist64))}
+        err["div(E)-rho (abs)"] = abs(hist32[0] - hist64[0])
+        fp32 = {"value": n32 * args.steps / e32, "ms_per_step": e32 / args.steps * 1e3,
+                "speedup_vs_fp64": (n32 * args.steps / e32) / (npart * args.steps / elapsed),
+                "particles_conserved": n32_end == n32, "dtype": "f32",
+                "rel_error_vs_fp64_after_steps": args.warmup + args.steps,
+                "rel_error_vs_fp64": err,
+                "phases_ms_per_launch": {k: (ms / n if n else 0.0) for k, (ms, n) in rep32.items()},
+                "form": "FP32 cell-relative particles and fields, FP64 LDS current tiles, "
+                        "FP64 energy/charge reductions"}
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb)
 
-    elapsed = t1 - t0
     value = npart * args.steps / elapsed
     phases = {}
     for ph, (ms, n) in rep.items():
@@ -222,6 +252,7 @@ def main():
         "step_roofline": {"bytes_per_step": step_bytes(npart, ncell), "achieved": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "phases": phases,
+        "fp32_form": fp32,
     }
     if args.cpu_baseline == "on":
         line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_threads, args.cpu_steps)

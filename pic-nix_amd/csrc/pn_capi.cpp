@@ -192,6 +192,8 @@ void allocate(Sim& s)
 
   size_t np = (size_t)g.nseg * g.cap;
   for (PArr* p : {&s.pa, &s.pb}) {
+    if (s.fp32 && p == &s.pb)
+      continue; // the FP32 form sorts P32 records
     p->x  = dalloc<double>(np);
     p->y  = dalloc<double>(np);
     p->z  = dalloc<double>(np);
@@ -202,6 +204,29 @@ void allocate(Sim& s)
   }
   for (int d = 0; d < 3; d++)
     s.d_xo[d] = dalloc<double>(np);
+  if (s.fp32) {
+    // FP32 state; pa, d_xo and the FP64 fields above stay as boundary mirrors
+    s.f_E = dalloc<float>(3 * vol);
+    s.f_B = dalloc<float>(3 * vol);
+    s.f_J = dalloc<float>(4 * vol);
+    HIPCHK(hipMemset(s.f_E, 0, 3 * vol * sizeof(float)));
+    HIPCHK(hipMemset(s.f_B, 0, 3 * vol * sizeof(float)));
+    HIPCHK(hipMemset(s.f_J, 0, 4 * vol * sizeof(float)));
+    if (s.cfg.friedman != 0.0)
+      s.f_ff = dalloc<float>(9 * vol);
+    for (P32* q : {&s.qa, &s.qb}) {
+      q->rx   = dalloc<float>(np);
+      q->ry   = dalloc<float>(np);
+      q->rz   = dalloc<float>(np);
+      q->ux   = dalloc<float>(np);
+      q->uy   = dalloc<float>(np);
+      q->uz   = dalloc<float>(np);
+      q->id   = dalloc<int64_t>(np);
+      q->cell = dalloc<int32_t>(np);
+    }
+    for (int d = 0; d < 3; d++)
+      s.f_ro[d] = dalloc<float>(np);
+  }
   s.d_np = dalloc<int32_t>(g.nseg);
   HIPCHK(hipMemset(s.d_np, 0, g.nseg * sizeof(int32_t)));
   s.h_np.assign(g.nseg, 0);
@@ -255,6 +280,22 @@ void release(Sim& s)
   }
   for (int d = 0; d < 3; d++)
     f(s.d_xo[d]);
+  f(s.f_E);
+  f(s.f_B);
+  f(s.f_J);
+  f(s.f_ff);
+  for (P32* q : {&s.qa, &s.qb}) {
+    f(q->rx);
+    f(q->ry);
+    f(q->rz);
+    f(q->ux);
+    f(q->uy);
+    f(q->uz);
+    f(q->id);
+    f(q->cell);
+  }
+  for (int d = 0; d < 3; d++)
+    f(s.f_ro[d]);
   f(s.d_np);
   SortScratch& ss = s.ss;
   for (void* p : {(void*)ss.cellcnt, (void*)ss.cellstart, (void*)ss.key, (void*)ss.slot,
@@ -365,6 +406,8 @@ void load_particles(Sim& s)
   up(s.d_np, hnp.data(), g.nseg * sizeof(int32_t));
   s.h_np          = hnp;
   s.np_host_valid = true;
+  if (s.fp32)
+    launch32_from_f64(s, 0, g.nseg);
 }
 
 void init_fields(Sim& s)
@@ -384,6 +427,8 @@ void init_fields(Sim& s)
   }
   launch_fill_ghost(s, s.d_E, 3);
   launch_fill_ghost(s, s.d_B, 3);
+  if (s.fp32)
+    launch32_fields_from_f64(s, PN_FIELD_UF);
 }
 
 int check_chunk_arg(Sim& s, int chunk)
@@ -418,12 +463,13 @@ int pn_create(const char* toml, int load, pn_sim** out)
       *out = h;
       return PN_ERR_CONFIG;
     }
-    if (s.cfg.precision != "fp64" || s.cfg.Ns > kMaxSpecies) {
+    if ((s.cfg.precision != "fp64" && s.cfg.precision != "fp32") || s.cfg.Ns > kMaxSpecies) {
       s.err = "configuration uses an option this build does not implement yet "
-              "(supported: precision = fp64, Ns <= 8)";
+              "(supported: precision = fp64 or fp32, Ns <= 8)";
       *out  = h;
       return PN_ERR_UNSUPPORTED;
     }
+    s.fp32 = s.cfg.precision == "fp32";
     setup_geometry(s);
     if (const char* k = std::getenv("PICNIX_KERNELS"))
       s.generic_kernels = std::string(k) == "generic";
@@ -435,7 +481,10 @@ int pn_create(const char* toml, int load, pn_sim** out)
     if (load) {
       load_particles(s);
       // initial sort, pic/pic_chunk.cpp:447-453 (count + sort per species)
-      launch_sort(s);
+      if (s.fp32)
+        launch32_sort(s);
+      else
+        launch_sort(s);
     }
     HIPCHK(hipStreamSynchronize(s.stream));
     HIPCHK(hipGetLastError());
@@ -526,6 +575,8 @@ int pn_get_particles(pn_sim* h, int chunk, int species, double* xu7, int64_t cap
     if (cap < n)
       return fail(s, PN_ERR_ARG, "output buffer smaller than the particle count");
     ensure_stage(s, (size_t)n * 7);
+    if (s.fp32)
+      launch32_to_f64(s, seg, 1, false);
     launch_soa_to_aos(s, seg, n);
     HIPCHK(hipMemcpyAsync(xu7, s.d_stage, n * 7 * sizeof(double), hipMemcpyDeviceToHost,
                           s.stream));
@@ -553,6 +604,8 @@ int pn_set_particles(pn_sim* h, int chunk, int species, const double* xu7, int64
     s.h_np[seg] = (int32_t)np;
     HIPCHK(hipMemcpyAsync(s.d_np + seg, &s.h_np[seg], sizeof(int32_t), hipMemcpyHostToDevice,
                           s.stream));
+    if (s.fp32 && np > 0)
+      launch32_from_f64(s, seg, 1);
     HIPCHK(hipStreamSynchronize(s.stream));
     s.sorted  = false;
     s.counted = false;
@@ -572,6 +625,10 @@ int pn_get_particles_old(pn_sim* h, int chunk, int species, double* xv3, int64_t
     if (!xv3 || cap < n)
       return fail(s, PN_ERR_ARG, "output buffer smaller than the particle count");
     std::vector<double> t(n);
+    if (s.fp32) {
+      launch32_to_f64(s, seg, 1, true);
+      HIPCHK(hipStreamSynchronize(s.stream));
+    }
     for (int d = 0; d < 3; d++) {
       HIPCHK(hipMemcpy(t.data(), s.d_xo[d] + (size_t)seg * s.g.cap, n * sizeof(double),
                        hipMemcpyDeviceToHost));
@@ -638,6 +695,8 @@ int pn_get_fields(pn_sim* h, int chunk, int which, double* data)
     size_t      nn = (size_t)(g.dims[0] + 2 * g.Nb) * (g.dims[1] + 2 * g.Nb) * (g.dims[2] + 2 * g.Nb);
     int         nc = which == PN_FIELD_UF ? 6 : (which == PN_FIELD_UJ ? 4 : 14 * g.Ns);
     ensure_stage(s, nn * nc);
+    if (s.fp32 && which != PN_FIELD_UM)
+      launch32_fields_to_f64(s, which);
     launch_chunk_fields_get(s, chunk, which, s.d_stage);
     HIPCHK(hipMemcpyAsync(data, s.d_stage, nn * nc * sizeof(double), hipMemcpyDeviceToHost,
                           s.stream));
@@ -657,11 +716,15 @@ int pn_set_fields(pn_sim* h, int chunk, int which, const double* data)
     ensure_stage(s, nn * nc);
     HIPCHK(hipMemcpyAsync(s.d_stage, data, nn * nc * sizeof(double), hipMemcpyHostToDevice,
                           s.stream));
+    if (s.fp32)
+      launch32_fields_to_f64(s, which); // the mirror holds the other chunks' values
     launch_chunk_fields_set(s, chunk, which, s.d_stage);
     if (which == PN_FIELD_UF) {
       launch_fill_ghost(s, s.d_E, 3);
       launch_fill_ghost(s, s.d_B, 3);
     }
+    if (s.fp32)
+      launch32_fields_from_f64(s, which);
     HIPCHK(hipStreamSynchronize(s.stream));
     return PN_OK;
   });
@@ -695,7 +758,10 @@ int pn_push_velocity(pn_sim* h, int chunk, double delt)
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
-    launch_push_velocity(s, delt);
+    if (s.fp32)
+      launch32_push_velocity(s, delt);
+    else
+      launch_push_velocity(s, delt);
     return PN_OK;
   });
 }
@@ -707,7 +773,10 @@ int pn_push_position(pn_sim* h, int chunk, double delt)
     if (rc)
       return rc;
     bool was_sorted = s.sorted;
-    launch_push_position(s, delt);
+    if (s.fp32)
+      launch32_push_position(s, delt);
+    else
+      launch_push_position(s, delt);
     // the particle order is unchanged, so cellstart still describes the old
     // positions the deposit walks
     s.sorted = was_sorted;
@@ -721,7 +790,10 @@ int pn_deposit_current(pn_sim* h, int chunk, double delt)
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
-    launch_deposit(s, delt);
+    if (s.fp32)
+      launch32_deposit(s, delt, false);
+    else
+      launch_deposit(s, delt);
     return PN_OK;
   });
 }
@@ -740,6 +812,8 @@ int pn_deposit_moment(pn_sim* h, int chunk)
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
+    if (s.fp32)
+      return fail(s, PN_ERR_UNSUPPORTED, "moments are not built for precision = fp32");
     if (!s.d_M)
       s.d_M = dalloc<double>(14 * (size_t)s.g.Ns * s.g.vol);
     launch_deposit_moment(s);
@@ -753,7 +827,10 @@ int pn_sort_particle(pn_sim* h, int chunk)
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
-    launch_sort(s);
+    if (s.fp32)
+      launch32_sort(s);
+    else
+      launch_sort(s);
     return PN_OK;
   });
 }
@@ -763,8 +840,13 @@ int pn_exchange(pn_sim* h, int mode)
   return guarded(h, [&](Sim& s) {
     switch (mode) {
     case PN_BOUNDARY_EMF:
-      launch_fill_ghost(s, s.d_E, 3);
-      launch_fill_ghost(s, s.d_B, 3);
+      if (s.fp32) {
+        launch_fill_ghost(s, s.f_E, 3);
+        launch_fill_ghost(s, s.f_B, 3);
+      } else {
+        launch_fill_ghost(s, s.d_E, 3);
+        launch_fill_ghost(s, s.d_B, 3);
+      }
       return PN_OK;
     case PN_BOUNDARY_CUR:
       launch_fold_current(s);
@@ -772,9 +854,14 @@ int pn_exchange(pn_sim* h, int mode)
     case PN_BOUNDARY_PARTICLE:
       if (!s.counted)
         return fail(s, PN_ERR_STATE, "particle exchange needs the count of pn_push_position");
-      launch_migrate(s);
+      if (s.fp32)
+        launch32_migrate(s);
+      else
+        launch_migrate(s);
       return PN_OK;
     case PN_BOUNDARY_MOM:
+      if (s.fp32)
+        return fail(s, PN_ERR_UNSUPPORTED, "moments are not built for precision = fp32");
       if (!s.d_M)
         return fail(s, PN_ERR_STATE, "moment exchange needs pn_deposit_moment first");
       launch_fold(s, s.d_M, 0, 14 * s.g.Ns, PN_PHASE_RHO);
@@ -791,8 +878,28 @@ int pn_exchange(pn_sim* h, int mode)
 // that follows every field update).
 namespace {
 
+void step_launches32(Sim& s, double delt)
+{
+  launch_push_bfd(s, 0.5 * delt);
+  launch32_push_velocity(s, delt);
+  bool was_sorted = s.sorted;
+  launch32_push_position(s, delt);
+  s.sorted = was_sorted;
+  launch32_deposit(s, delt, true);
+  launch_push_bfd(s, 0.5 * delt);
+  launch_fold_current(s, 1, 4);
+  launch_push_efd(s, delt);
+  launch32_migrate(s);
+  launch32_rho(s);
+  launch_fold_current(s, 0, 1);
+}
+
 void step_launches(Sim& s, double delt)
 {
+  if (s.fp32) {
+    step_launches32(s, delt);
+    return;
+  }
   launch_push_bfd(s, 0.5 * delt);
   launch_push_velocity(s, delt);
   bool was_sorted = s.sorted;
@@ -831,9 +938,10 @@ int pn_step(pn_sim* h, double delt)
       step_launches(s, delt);
       return PN_OK;
     }
+    void* cur = s.fp32 ? (void*)s.qa.rx : (void*)s.pa.x; // buffer parity
     if (!s.buf0_x)
-      s.buf0_x = s.pa.x;
-    const int k = s.pa.x == s.buf0_x ? 0 : 1;
+      s.buf0_x = cur;
+    const int k = cur == s.buf0_x ? 0 : 1;
     if (!s.gexec[k] || s.graph_dt[k] != delt) {
       if (s.gexec[k]) {
         (void)hipGraphExecDestroy(s.gexec[k]);
@@ -851,7 +959,10 @@ int pn_step(pn_sim* h, double delt)
       s.graph_dt[k] = delt;
     } else {
       // replay: advance the host-side state as the recorded step does
-      std::swap(s.pa, s.pb);
+      if (s.fp32)
+        std::swap(s.qa, s.qb);
+      else
+        std::swap(s.pa, s.pb);
       s.np_host_valid = false;
       s.counted       = false;
       s.sorted        = true;

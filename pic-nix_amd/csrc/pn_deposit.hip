@@ -375,9 +375,10 @@ __global__ void __launch_bounds__(256)
 // with (F, S) = (y, z), (z, x), (x, y), and the running sum along C = x, y, z
 //   J(f, s, c + 1) += sum_{c' <= c} dsC[c'] w  =  P[c] w,
 // P being the prefix sum of dsC.  The kernel evaluates P[c] w, one fused
-// multiply-add per node; the reference adds the products one at a time.  The
-// two agree to rounding (a few ulp of each contribution), the tolerance the
-// current is held to (the sum over particles runs in another order anyway).
+// multiply-add per node, and forms w and its factors with fused multiply-adds
+// too; the reference adds the products one at a time.  The two agree to
+// rounding (a few ulp of each contribution), the tolerance the current is held
+// to (the sum over particles runs in another order anyway).
 // A lane owns the plane F = f of one component (N rows s of N-1 nodes) for one
 // particle of a slot of PPS particles, so all lanes evaluate the same
 // expression on their own operands.  The shape weights are computed by one lane
@@ -633,8 +634,8 @@ __global__ void __launch_bounds__(256)
           if (act && kk < nbt) {
             const double2* pp  = W + kk;
             const double2  pf  = pp[(axF * ROWS + t) * PITCH]; // (s0F[t], dsF[t])
-            const double   aF  = pf.x + A * pf.y;
-            const double   bF  = A * pf.x + B * pf.y;
+            const double   aF  = fma(A, pf.y, pf.x);                 // 1 s0F + A dsF
+            const double   bF  = fma(B, pf.y, A * pf.x);             // A s0F + B dsF
             double         P[NS];
 #pragma unroll
             for (int j = 0; j < NS / 2; j++) {
@@ -647,7 +648,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
               for (int r = 0; r < N; r++) {
                 const double2 ps = pp[(axS * ROWS + r) * PITCH]; // (s0S[r], dsS[r])
-                const double  w  = -(aF * ps.x + bF * ps.y) * qd;
+                const double  w  = -fma(aF, ps.x, bF * ps.y) * qd;
 #pragma unroll
                 for (int j = 0; j < N - 1; j++)
                   acc[r][j] = fma(P[j], w, acc[r][j]);
@@ -662,7 +663,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll 1
               for (int r = 0; r < N; r++) {
                 const double2 ps = pp[(axS * ROWS + r) * PITCH];
-                const double  w  = -(aF * ps.x + bF * ps.y) * qd;
+                const double  w  = -fma(aF, ps.x, bF * ps.y) * qd;
 #pragma unroll 1
                 for (int j = 0; j < N - 1; j++) {
                   const double val = P[j] * w;
@@ -931,7 +932,7 @@ __global__ void __launch_bounds__(256)
                 const double sy = rec[S + jy];
 #pragma unroll
                 for (int jx = 0; jx < S; jx++)
-                  acc[jy][jx] += qs * sx[jx] * sy * sz;
+                  acc[jy][jx] = fma(qs * sx[jx] * sy, sz, acc[jy][jx]);
               }
             } else {
               // own origin (odd orders, as in k_deposit_march)

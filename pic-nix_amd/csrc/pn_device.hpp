@@ -343,6 +343,21 @@ struct Geom {
   bool   has[3];
 };
 
+// FP32 particle storage (precision = "fp32"): position as the cell index (the
+// chunk-local bin, packed like the sort bins) plus the offset inside the cell
+// in cell units, r in [0, 1) after every sort; between the position push and
+// the sort, r holds the new position relative to the old cell (in [-1, 2)).
+struct P32 {
+  float*   rx;
+  float*   ry;
+  float*   rz;
+  float*   ux;
+  float*   uy;
+  float*   uz;
+  int64_t* id;
+  int32_t* cell;
+};
+
 // Structure-of-arrays particle storage; segment s occupies [s * cap, (s + 1) * cap).
 struct PArr {
   double*  x;

@@ -155,6 +155,12 @@ void launch_energy(Sim& s, double* out)
   int         nt  = s.ss.ntile;
   size_t      np1 = 4 * (size_t)nb1, np2 = (size_t)g.nseg * nt;
   double*     d_part = nullptr;
+  if (s.fp32) {
+    // FP64 reductions over the FP64 mirrors of the FP32 state
+    launch32_fields_to_f64(s, PN_FIELD_UF);
+    launch32_fields_to_f64(s, PN_FIELD_UJ);
+    launch32_to_f64(s, 0, g.nseg, false);
+  }
   hipMallocAsync((void**)&d_part, sizeof(double) * (np1 + np2), s.stream);
   hipMemsetAsync(d_part, 0, sizeof(double) * (np1 + np2), s.stream);
   if (g.dim == 3)

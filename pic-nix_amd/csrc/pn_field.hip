@@ -24,10 +24,9 @@ __device__ __forceinline__ long gidx(const Geom& g, int z, int y, int x)
 // F = ff(...,0,k) is the Friedman-filtered E (k_friedman_combine), or E itself
 // for theta = 0 (then ff(...,0,k) = 1 E + (-0) ff1 + 0 ff2 == E exactly).  One
 // thread per interior cell.
-template <int Dim>
-__global__ void __launch_bounds__(256) k_push_bfd(Geom g, const double* __restrict__ E,
-                                                  double* __restrict__ B, double cflx,
-                                                  double cfly, double cflz)
+template <int Dim, typename T>
+__global__ void __launch_bounds__(256) k_push_bfd(Geom g, const T* __restrict__ E,
+                                                  T* __restrict__ B, T cflx, T cfly, T cflz)
 {
   long n   = (long)g.N[0] * g.N[1] * g.N[2];
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -39,9 +38,9 @@ __global__ void __launch_bounds__(256) k_push_bfd(Geom g, const double* __restri
   int  z  = (int)(r / g.N[1]) + g.pad[0];
   long i  = gidx(g, z, y, x);
   long sx = 1, sy = g.P[2], sz = g.plane, v = g.vol;
-  const double* Ex = E;
-  const double* Ey = E + v;
-  const double* Ez = E + 2 * v;
+  const T* Ex = E;
+  const T* Ey = E + v;
+  const T* Ez = E + 2 * v;
   if constexpr (Dim == 3) {
     B[i] += (-cfly) * (Ez[i] - Ez[i - sy]) + (+cflz) * (Ey[i] - Ey[i - sz]);
     B[v + i] += (-cflz) * (Ex[i] - Ex[i - sz]) + (+cflx) * (Ez[i] - Ez[i - sx]);
@@ -59,9 +58,9 @@ __global__ void __launch_bounds__(256) k_push_bfd(Geom g, const double* __restri
 // over every padded cell (the reference covers the ghosts too; the padding holds
 // periodic images, so each padding cell gets its image's value).  ff is
 // [level][component][vol].
-__global__ void __launch_bounds__(256) k_friedman_combine(Geom g, const double* __restrict__ E,
-                                                          double* __restrict__ ff, double A,
-                                                          double B, double C)
+template <typename T>
+__global__ void __launch_bounds__(256) k_friedman_combine(Geom g, const T* __restrict__ E,
+                                                          T* __restrict__ ff, T A, T B, T C)
 {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.vol)
@@ -74,8 +73,9 @@ __global__ void __launch_bounds__(256) k_friedman_combine(Geom g, const double* 
 
 // History shift before the E update (maxwell.hpp:389-404 / :237-252):
 //   ff2 = ff1 + theta ff2;  ff1 = E.
-__global__ void __launch_bounds__(256) k_friedman_shift(Geom g, const double* __restrict__ E,
-                                                        double* __restrict__ ff, double theta)
+template <typename T>
+__global__ void __launch_bounds__(256) k_friedman_shift(Geom g, const T* __restrict__ E,
+                                                        T* __restrict__ ff, T theta)
 {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.vol)
@@ -89,8 +89,9 @@ __global__ void __launch_bounds__(256) k_friedman_shift(Geom g, const double* __
 }
 
 // init_friedman (maxwell.hpp:44-60): every level = E.
-__global__ void __launch_bounds__(256) k_friedman_init(Geom g, const double* __restrict__ E,
-                                                       double* __restrict__ ff)
+template <typename T>
+__global__ void __launch_bounds__(256) k_friedman_init(Geom g, const T* __restrict__ E,
+                                                       T* __restrict__ ff)
 {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.vol)
@@ -104,11 +105,11 @@ __global__ void __launch_bounds__(256) k_friedman_init(Geom g, const double* __r
 }
 
 // E += c dt curl B - dt J: pic/engine/maxwell.hpp:383-440 (3D) and :231-290 (2D).
-template <int Dim>
-__global__ void __launch_bounds__(256) k_push_efd(Geom g, double* __restrict__ E,
-                                                  const double* __restrict__ B,
-                                                  const double* __restrict__ J, double cflx,
-                                                  double cfly, double cflz, double delt)
+template <int Dim, typename T>
+__global__ void __launch_bounds__(256) k_push_efd(Geom g, T* __restrict__ E,
+                                                  const T* __restrict__ B,
+                                                  const T* __restrict__ J, T cflx, T cfly, T cflz,
+                                                  T delt)
 {
   long n   = (long)g.N[0] * g.N[1] * g.N[2];
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,9 +121,9 @@ __global__ void __launch_bounds__(256) k_push_efd(Geom g, double* __restrict__ E
   int  z  = (int)(r / g.N[1]) + g.pad[0];
   long i  = gidx(g, z, y, x);
   long sx = 1, sy = g.P[2], sz = g.plane, v = g.vol;
-  const double* Bx = B;
-  const double* By = B + v;
-  const double* Bz = B + 2 * v;
+  const T* Bx = B;
+  const T* By = B + v;
+  const T* Bz = B + 2 * v;
   if constexpr (Dim == 3) {
     E[i] += (+cfly) * (Bz[i + sy] - Bz[i]) + (-cflz) * (By[i + sz] - By[i]) - delt * J[v + i];
     E[v + i] +=
@@ -146,7 +147,8 @@ __device__ __forceinline__ int wrap(int i, int pad, int n)
 
 // Copy the periodic image into every padding cell (the EMF exchange,
 // nix/xtensor_halo3d.hpp:18-71, on a single periodic array).
-__global__ void __launch_bounds__(256) k_fill_ghost(Geom g, double* __restrict__ f, int ncomp)
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_ghost(Geom g, T* __restrict__ f, int ncomp)
 {
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= g.vol)
@@ -169,7 +171,8 @@ __global__ void __launch_bounds__(256) k_fill_ghost(Geom g, double* __restrict__
 // (the Cur exchange adds a neighbour's ghost into the interior edge,
 // nix/xtensor_halo3d.hpp:76-129).  One pass per dimension; each thread owns one
 // line, so the additions happen in a fixed order (x, then y, then z).
-__global__ void __launch_bounds__(256) k_fold_line(Geom g, double* __restrict__ J, int dir, int k0,
+template <typename T>
+__global__ void __launch_bounds__(256) k_fold_line(Geom g, T* __restrict__ J, int dir, int k0,
                                                    int k1)
 {
   // dir: 2 = x lines over (z, y); 1 = y lines over (z, x); 0 = z lines over (y, x)
@@ -192,7 +195,7 @@ __global__ void __launch_bounds__(256) k_fold_line(Geom g, double* __restrict__ 
   long stride = dir == 2 ? 1 : (dir == 1 ? g.P[2] : g.plane);
   long base   = dir == 2 ? gidx(g, i0, i1, 0) : (dir == 1 ? gidx(g, i0, 0, i1) : gidx(g, 0, i0, i1));
   for (int k = k0; k < k1; k++) {
-    double* f = J + k * g.vol + base;
+    T* f = J + k * g.vol + base;
     for (int p = 0; p < pad; p++) {
       // low padding p -> interior n + p ; high padding pad + n + p -> interior pad + p
       f[(long)(n + p) * stride] += f[(long)p * stride];
@@ -228,6 +231,11 @@ void launch_zero(Sim& s, double* p, long n)
   hipLaunchKernelGGL(k_fill<double>, dim3(fill_grid(n)), dim3(256), 0, s.stream, p, 0.0, n);
 }
 
+void launch_zero(Sim& s, float* p, long n)
+{
+  hipLaunchKernelGGL(k_fill<float>, dim3(fill_grid(n)), dim3(256), 0, s.stream, p, 0.0f, n);
+}
+
 void launch_zero(Sim& s, int32_t* p, long n)
 {
   hipLaunchKernelGGL(k_fill<int32_t>, dim3(fill_grid(n)), dim3(256), 0, s.stream, p, 0, n);
@@ -238,82 +246,129 @@ void launch_copy(Sim& s, int32_t* dst, const int32_t* src, long n)
   hipLaunchKernelGGL(k_copy_i32, dim3(fill_grid(n)), dim3(256), 0, s.stream, dst, src, n);
 }
 
-void launch_push_bfd(Sim& s, double delt)
+namespace {
+
+template <typename T>
+void push_bfd_impl(Sim& s, const T* E, T* B, T* ff, double delt)
 {
-  PhaseTimer  pt(s, 0);
   const Geom& g    = s.g;
-  double      cflx = g.cc * delt / g.del[2];
-  double      cfly = g.cc * delt / g.del[1];
-  double      cflz = g.cc * delt / g.del[0];
+  const T     cflx = (T)(g.cc * delt / g.del[2]);
+  const T     cfly = (T)(g.cc * delt / g.del[1]);
+  const T     cflz = (T)(g.cc * delt / g.del[0]);
   long        n    = (long)g.N[0] * g.N[1] * g.N[2];
-  const double* F  = s.d_E;
-  if (s.d_ff) {
+  const T*    F    = E;
+  if (ff) {
     const double th = s.cfg.friedman;
     const double A  = 1 + 0.5 * th;
     const double B  = -th * (1 - 0.5 * th);
     const double C  = 0.5 * th * (1 - th) * (1 - th);
-    hipLaunchKernelGGL(k_friedman_combine, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g,
-                       s.d_E, s.d_ff, A, B, C);
-    F = s.d_ff;
+    hipLaunchKernelGGL(k_friedman_combine<T>, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g,
+                       E, ff, (T)A, (T)B, (T)C);
+    F = ff;
   }
   if (g.dim == 3)
-    hipLaunchKernelGGL(k_push_bfd<3>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, s.d_B,
+    hipLaunchKernelGGL((k_push_bfd<3, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, B,
                        cflx, cfly, cflz);
   else
-    hipLaunchKernelGGL(k_push_bfd<2>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, s.d_B,
+    hipLaunchKernelGGL((k_push_bfd<2, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, B,
                        cflx, cfly, cflz);
-  launch_fill_ghost(s, s.d_B, 3);
+  hipLaunchKernelGGL(k_fill_ghost<T>, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g, B, 3);
 }
 
-void launch_push_efd(Sim& s, double delt)
+template <typename T>
+void push_efd_impl(Sim& s, T* E, const T* B, const T* J, T* ff, double delt)
 {
-  PhaseTimer  pt(s, 5);
   const Geom& g    = s.g;
-  double      cflx = g.cc * delt / g.del[2];
-  double      cfly = g.cc * delt / g.del[1];
-  double      cflz = g.cc * delt / g.del[0];
+  const T     cflx = (T)(g.cc * delt / g.del[2]);
+  const T     cfly = (T)(g.cc * delt / g.del[1]);
+  const T     cflz = (T)(g.cc * delt / g.del[0]);
   long        n    = (long)g.N[0] * g.N[1] * g.N[2];
-  if (s.d_ff)
-    hipLaunchKernelGGL(k_friedman_shift, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g,
-                       s.d_E, s.d_ff, s.cfg.friedman);
+  if (ff)
+    hipLaunchKernelGGL(k_friedman_shift<T>, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g, E,
+                       ff, (T)s.cfg.friedman);
   if (g.dim == 3)
-    hipLaunchKernelGGL(k_push_efd<3>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_E,
-                       s.d_B, s.d_J, cflx, cfly, cflz, delt);
+    hipLaunchKernelGGL((k_push_efd<3, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, E, B, J,
+                       cflx, cfly, cflz, (T)delt);
   else
-    hipLaunchKernelGGL(k_push_efd<2>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, s.d_E,
-                       s.d_B, s.d_J, cflx, cfly, cflz, delt);
-  launch_fill_ghost(s, s.d_E, 3);
+    hipLaunchKernelGGL((k_push_efd<2, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, E, B, J,
+                       cflx, cfly, cflz, (T)delt);
+  hipLaunchKernelGGL(k_fill_ghost<T>, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g, E, 3);
 }
 
-void launch_init_friedman(Sim& s)
+template <typename T>
+void fold_impl(Sim& s, T* F, int k0, int k1)
 {
-  if (s.d_ff)
-    hipLaunchKernelGGL(k_friedman_init, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream, s.g,
-                       s.d_E, s.d_ff);
-}
-
-void launch_fill_ghost(Sim& s, double* f, int ncomp)
-{
-  hipLaunchKernelGGL(k_fill_ghost, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream, s.g, f,
-                     ncomp);
-}
-
-void launch_fold(Sim& s, double* F, int k0, int k1, int phase)
-{
-  PhaseTimer  pt(s, phase);
   const Geom& g = s.g;
   for (int dir = 2; dir >= 0; dir--) {
     if (!g.has[dir] || g.pad[dir] == 0)
       continue;
     long n = dir == 2 ? (long)g.P[0] * g.P[1] : (dir == 1 ? (long)g.P[0] * g.P[2] : g.plane);
-    hipLaunchKernelGGL(k_fold_line, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, dir, k0,
+    hipLaunchKernelGGL(k_fold_line<T>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, dir, k0,
                        k1);
   }
 }
 
+} // namespace
+
+void launch_push_bfd(Sim& s, double delt)
+{
+  PhaseTimer pt(s, 0);
+  if (s.fp32)
+    push_bfd_impl<float>(s, s.f_E, s.f_B, s.f_ff, delt);
+  else
+    push_bfd_impl<double>(s, s.d_E, s.d_B, s.d_ff, delt);
+}
+
+void launch_push_efd(Sim& s, double delt)
+{
+  PhaseTimer pt(s, 5);
+  if (s.fp32)
+    push_efd_impl<float>(s, s.f_E, s.f_B, s.f_J, s.f_ff, delt);
+  else
+    push_efd_impl<double>(s, s.d_E, s.d_B, s.d_J, s.d_ff, delt);
+}
+
+void launch_init_friedman(Sim& s)
+{
+  if (s.d_ff)
+    hipLaunchKernelGGL(k_friedman_init<double>, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream,
+                       s.g, s.d_E, s.d_ff);
+  if (s.f_ff)
+    hipLaunchKernelGGL(k_friedman_init<float>, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream,
+                       s.g, s.f_E, s.f_ff);
+}
+
+void launch_fill_ghost(Sim& s, double* f, int ncomp)
+{
+  hipLaunchKernelGGL(k_fill_ghost<double>, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream, s.g,
+                     f, ncomp);
+}
+
+void launch_fill_ghost(Sim& s, float* f, int ncomp)
+{
+  hipLaunchKernelGGL(k_fill_ghost<float>, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream, s.g,
+                     f, ncomp);
+}
+
+void launch_fold(Sim& s, double* F, int k0, int k1, int phase)
+{
+  PhaseTimer pt(s, phase);
+  fold_impl<double>(s, F, k0, k1);
+}
+
+void launch_fold(Sim& s, float* F, int k0, int k1, int phase)
+{
+  PhaseTimer pt(s, phase);
+  fold_impl<float>(s, F, k0, k1);
+}
+
 void launch_fold_current(Sim& s, int k0, int k1)
 {
-  launch_fold(s, s.d_J, k0, k1, k0 == 0 && k1 == 1 ? 7 : 4);
+  const int phase = k0 == 0 && k1 == 1 ? 7 : 4;
+  if (s.fp32)
+    launch_fold(s, s.f_J, k0, k1, phase);
+  else
+    launch_fold(s, s.d_J, k0, k1, phase);
 }
 
 } // namespace picnix

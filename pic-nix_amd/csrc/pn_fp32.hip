@@ -1,0 +1,1125 @@
+// pn_fp32.hip -- the FP32 form of the step (precision = "fp32").
+//
+// Particles are stored cell-relative: the cell (the chunk-local sort bin) and
+// the offset inside it in cell units, r in [0, 1) per axis after every sort
+// (struct P32).  Fields E, B, J are FP32 on the same padded global grid as the
+// FP64 form.  The arithmetic follows the FP64 kernels operation by operation
+// (gather + pusher, position push, Esirkepov, rho), in FP32, with three
+// differences that the representation brings:
+//   * shape weights take delta = r - offset directly (no digitize of an
+//     absolute position: the particle's cell is known), so weights keep their
+//     full FP32 precision however large the domain is;
+//   * the position push adds u dt / (gamma dx) to r; a particle that leaves its
+//     cell changes its cell index, and one that leaves its chunk keeps r and
+//     gets the neighbour's cell index (the periodic wrap is exact);
+//   * per-cell current sums run in FP32 registers and the block's ring tile
+//     accumulates in FP64 LDS before the FP32 global add (compensated tile
+//     accumulation: each node's tile value is exact to FP64 rounding).
+// FP64 stays for the chunk origins (positions are rebuilt as
+// xmin + (cell + r) dx in FP64 at the C-ABI boundary) and for the energy and
+// charge reductions, which read FP64 mirrors of the FP32 fields.
+#include "../../include/picnix.h"
+#include "pn_sim.hpp"
+
+namespace picnix {
+
+namespace {
+
+inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+struct Cell3 {
+  int x, y, z;
+};
+
+__device__ __forceinline__ Cell3 cell_of(const Geom& g, int bin)
+{
+  Cell3 c;
+  c.z       = bin / g.bsz;
+  const int r = bin - c.z * g.bsz;
+  c.y       = r / g.bsy;
+  c.x       = r - c.y * g.bsy;
+  return c;
+}
+
+__device__ __forceinline__ int bin_of3(const Geom& g, int x, int y, int z)
+{
+  return z * g.bsz + y * g.bsy + x;
+}
+
+// Integer-grid (cell-centred) and half-grid weights of one axis from the
+// cell-relative offset r; `first` is the stencil's first cell relative to the
+// particle's cell.  The same deltas as digitize + shape_mc on the absolute
+// position (pic/engine/velocity.hpp:223-359): integer grid
+// delta = x/dx - 1/2 - floor(x/dx - odd/2), half grid delta = x/dx - floor(x/dx + 1/2 - odd/2).
+template <int Order>
+__device__ __forceinline__ float int_delta(float r, int& first)
+{
+  constexpr float ho = Order % 2 == 1 ? 0.5f : 0.0f;
+  const float     f  = floorf(r - ho);
+  first              = (int)f;
+  return r - 0.5f - f;
+}
+
+template <int Order>
+__device__ __forceinline__ float half_delta(float r, int& first)
+{
+  constexpr float ho = Order % 2 == 1 ? 0.5f : 0.0f;
+  const float     f  = floorf(r + 0.5f - ho);
+  first              = (int)f;
+  return r - f;
+}
+
+// ---------------------------------------------------------------------------
+// conversions at the C-ABI boundary (FP64 absolute <-> FP32 cell-relative)
+// ---------------------------------------------------------------------------
+
+// FP64 SoA records of segments -> FP32 cell-relative (every particle of every
+// segment below np).  A particle outside its chunk gets cell -1 (dropped by
+// the next count, as the reference's sort drops bin Ng).
+__global__ void __launch_bounds__(256)
+    k32_from_f64(Geom g, PArr p, P32 q, const int32_t* __restrict__ np,
+                 const double* __restrict__ lim, int seg0, int nseg)
+{
+  const int  seg = seg0 + blockIdx.y;
+  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.y >= (unsigned)nseg || ip >= np[seg])
+    return;
+  const int    c    = seg / g.Ns;
+  const long   i    = (long)seg * g.cap + ip;
+  const double x[3] = {p.x[i], p.y[i], p.z[i]};
+  float        r[3];
+  int          cl[3] = {0, 0, 0};
+  bool         in    = true;
+  for (int a = 0; a < 3; a++) {
+    const int ax = 2 - a; // Geom tables are (z, y, x)
+    if (!g.has[ax]) {
+      r[a] = (float)x[a]; // an absent axis keeps its absolute coordinate
+      continue;
+    }
+    const double u  = (x[a] - lim[c * 6 + 2 * ax]) / g.del[ax];
+    const double fl = floor(u);
+    float        rf = (float)(u - fl);
+    int          ci = (int)fl;
+    if (rf >= 1.0f) {
+      rf = 0.0f;
+      ci += 1;
+    }
+    cl[a] = ci;
+    r[a]  = rf;
+    in    = in && ci >= 0 && ci < g.dims[ax];
+  }
+  q.rx[i]   = r[0];
+  q.ry[i]   = r[1];
+  q.rz[i]   = r[2];
+  q.ux[i]   = (float)p.ux[i];
+  q.uy[i]   = (float)p.uy[i];
+  q.uz[i]   = (float)p.uz[i];
+  q.id[i]   = p.id[i];
+  q.cell[i] = in ? bin_of3(g, cl[0], cl[1], cl[2]) : -1;
+}
+
+// FP32 -> FP64 SoA (x = xmin + (cell + r) dx in FP64); r may be unwrapped.
+__global__ void __launch_bounds__(256)
+    k32_to_f64(Geom g, P32 q, PArr p, const float* __restrict__ ro0,
+               const float* __restrict__ ro1, const float* __restrict__ ro2,
+               double* __restrict__ xo0, double* __restrict__ xo1, double* __restrict__ xo2,
+               const int32_t* __restrict__ np, const double* __restrict__ lim, int seg0)
+{
+  const int  seg = seg0 + blockIdx.y;
+  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
+    return;
+  const int    c      = seg / g.Ns;
+  const long   i      = (long)seg * g.cap + ip;
+  const int    bin    = q.cell[i];
+  const Cell3  cl     = bin >= 0 ? cell_of(g, bin) : Cell3{0, 0, 0};
+  const int    ci[3]  = {cl.x, cl.y, cl.z};
+  const float  r[3]   = {q.rx[i], q.ry[i], q.rz[i]};
+  const float* ro[3]  = {ro0, ro1, ro2};
+  double*      out[3] = {p.x, p.y, p.z};
+  double*      old[3] = {xo0, xo1, xo2};
+  for (int a = 0; a < 3; a++) {
+    const int ax = 2 - a;
+    if (!g.has[ax]) {
+      out[a][i] = (double)r[a];
+      if (old[a])
+        old[a][i] = (double)ro[a][i];
+      continue;
+    }
+    const double x0 = lim[c * 6 + 2 * ax];
+    out[a][i]       = x0 + ((double)ci[a] + (double)r[a]) * g.del[ax];
+    if (old[a])
+      old[a][i] = x0 + ((double)ci[a] + (double)ro[a][i]) * g.del[ax];
+  }
+  p.ux[i] = (double)q.ux[i];
+  p.uy[i] = (double)q.uy[i];
+  p.uz[i] = (double)q.uz[i];
+  p.id[i] = q.id[i];
+}
+
+template <typename S, typename D>
+__global__ void __launch_bounds__(256) k32_convert(const S* __restrict__ a, D* __restrict__ b,
+                                                   long n)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    b[i] = (D)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// gather + pusher (pic/engine/velocity.hpp:101-135,223-450) in FP32
+// ---------------------------------------------------------------------------
+
+struct Push32 {
+  float qmdt[kMaxSpecies];
+  float cdt[3], rcdt[3]; // c dt / d(x, y, z), for WT
+  int   pusher;
+  int   wt;
+};
+
+template <int Order>
+__device__ __forceinline__ void int_w(const Push32& pc, int axis, float r, int& first, float* w)
+{
+  const float d = int_delta<Order>(r, first);
+  if (pc.wt)
+    shape_wt<Order, float>(d, 0.0f, 1.0f, pc.cdt[axis], pc.rcdt[axis], w);
+  else
+    shape_mc<Order, float>(d, 0.0f, 1.0f, w);
+}
+
+// One block per (bin row, segment) of sorted particles: the row's E and B
+// stencil region is staged in LDS, as in the FP64 row kernel; one thread per
+// particle, (Order+1)^Dim-point gathers with x innermost.
+template <int Dim, int Order>
+__global__ void __launch_bounds__(256)
+    k32_push_velocity_rows(Geom g, P32 p, const int32_t* __restrict__ cellstart,
+                           const int* __restrict__ off, const float* __restrict__ E,
+                           const float* __restrict__ B, Push32 pc)
+{
+  constexpr int is_odd = Order % 2;
+  constexpr int S      = Order + 1;
+  constexpr int T      = Order + 2;
+  extern __shared__ float ft32[]; // [6][TZ][T][TXv]
+  const int seg = blockIdx.y;
+  const int c   = seg / g.Ns;
+  const int sp  = seg - c * g.Ns;
+  const int nbx = g.bsy;
+  int       bz = 0, by, b0;
+  if constexpr (Dim == 3) {
+    const int nby = g.bsz / g.bsy;
+    bz            = blockIdx.x / nby;
+    by            = blockIdx.x - bz * nby;
+    b0            = bz * g.bsz + by * g.bsy;
+  } else {
+    by = blockIdx.x;
+    b0 = by * g.bsy;
+  }
+  const int32_t* cs = cellstart + (long)seg * (g.nbin + 1);
+  const int      p0 = cs[b0], p1 = cs[b0 + nbx];
+  if (p1 <= p0)
+    return;
+  const int  lb  = g.Nb;
+  const int  TZ  = Dim == 3 ? T : 1;
+  const int  tz0 = Dim == 3 ? bz - is_odd + lb - (Order / 2) : lb;
+  const int  ty0 = by - is_odd + lb - (Order / 2);
+  const int  tx0 = -is_odd + lb - (Order / 2);
+  const int  TXv = nbx + Order + 1;
+  const int  pc1 = TZ * T * TXv;
+  const long sy = g.P[2], sz = g.plane, v = g.vol;
+  const int  offz = off[c * 3 + 0], offy = off[c * 3 + 1], offx = off[c * 3 + 2];
+  for (int t = threadIdx.x; t < 6 * pc1; t += blockDim.x) {
+    int k  = t / pc1;
+    int r  = t - k * pc1;
+    int tz = r / (T * TXv);
+    int r2 = r - tz * T * TXv;
+    int ty = r2 / TXv;
+    int tx = r2 - ty * TXv;
+    int gz = Dim == 3 ? offz + tz0 + tz : 0;
+    const float* F = k < 3 ? E + k * v : B + (k - 3) * v;
+    ft32[t] = F[gz * sz + (offy + ty0 + ty) * sy + offx + tx0 + tx];
+  }
+  __syncthreads();
+
+  const float qmdt = pc.qmdt[sp];
+  auto interp = [&](const float* base, long bs_z, long bs_y, int z0, int y0, int x0,
+                    const float* wz, const float* wy, const float* wx, int nz) {
+    float rz = 0;
+    for (int jz = 0; jz < nz; jz++) {
+      float ry = 0;
+#pragma unroll
+      for (int jy = 0; jy < S; jy++) {
+        const float* row = base + (z0 + jz) * bs_z + (y0 + jy) * bs_y + x0;
+        float        rx  = 0;
+#pragma unroll
+        for (int jx = 0; jx < S; jx++)
+          rx += row[jx] * wx[jx];
+        ry += rx * wy[jy];
+      }
+      rz += ry * wz[jz];
+    }
+    return rz * qmdt;
+  };
+
+  for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+    const long  i    = (long)seg * g.cap + q;
+    const Cell3 cl   = cell_of(g, p.cell[i]);
+    const float r[3] = {p.rx[i], p.ry[i], p.rz[i]};
+    float       wix[S], wiy[S], wiz[S], whx[S], why[S], whz[S];
+    int         ix0, iy0, iz0 = 0, hx0, hy0, hz0 = 0;
+    int_w<Order>(pc, 0, r[0], ix0, wix);
+    int_w<Order>(pc, 1, r[1], iy0, wiy);
+    shape_mc<Order, float>(half_delta<Order>(r[0], hx0), 0.0f, 1.0f, whx);
+    shape_mc<Order, float>(half_delta<Order>(r[1], hy0), 0.0f, 1.0f, why);
+    // first stencil node relative to the staged tile
+    const int X  = cl.x + ix0 + lb - (Order / 2) - tx0;
+    const int HX = cl.x + hx0 + lb - (Order / 2) - tx0;
+    const int Y  = cl.y + iy0 + lb - (Order / 2) - ty0;
+    const int HY = cl.y + hy0 + lb - (Order / 2) - ty0;
+    int       Z = 0, HZ = 0;
+    if constexpr (Dim == 3) {
+      int_w<Order>(pc, 2, r[2], iz0, wiz);
+      shape_mc<Order, float>(half_delta<Order>(r[2], hz0), 0.0f, 1.0f, whz);
+      Z  = cl.z + iz0 + lb - (Order / 2) - tz0;
+      HZ = cl.z + hz0 + lb - (Order / 2) - tz0;
+    } else {
+      wiz[0] = whz[0] = 1.0f;
+    }
+    // sorted particles sit in the row's bins, so the stencils are inside the tile
+    const bool in = min(X, HX) >= 0 && max(X, HX) + S <= TXv && min(Y, HY) >= 0 &&
+                    max(Y, HY) + S <= T && min(Z, HZ) >= 0 && max(Z, HZ) + (Dim == 3 ? S : 1) <= TZ;
+    if (!in)
+      continue;
+    const long bsz = (long)T * TXv, bsy = TXv;
+    const int  nz  = Dim == 3 ? S : 1;
+    float      eb[6];
+    eb[0] = interp(ft32 + 0 * pc1, bsz, bsy, Z, Y, HX, wiz, wiy, whx, nz);
+    eb[1] = interp(ft32 + 1 * pc1, bsz, bsy, Z, HY, X, wiz, why, wix, nz);
+    eb[2] = interp(ft32 + 2 * pc1, bsz, bsy, HZ, Y, X, whz, wiy, wix, nz);
+    eb[3] = interp(ft32 + 3 * pc1, bsz, bsy, HZ, HY, X, whz, why, wix, nz);
+    eb[4] = interp(ft32 + 4 * pc1, bsz, bsy, HZ, Y, HX, whz, wiy, whx, nz);
+    eb[5] = interp(ft32 + 5 * pc1, bsz, bsy, Z, HY, HX, wiz, why, whx, nz);
+    float ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+    push_momentum<float>(pc.pusher, ux, uy, uz, eb, (float)g.cc);
+    p.ux[i] = ux;
+    p.uy[i] = uy;
+    p.uz[i] = uz;
+  }
+}
+
+// One thread per particle, straight from global memory (unsorted particles).
+template <int Dim, int Order>
+__global__ void __launch_bounds__(256)
+    k32_push_velocity(Geom g, P32 p, const int32_t* __restrict__ np, const int* __restrict__ off,
+                      const float* __restrict__ E, const float* __restrict__ B, Push32 pc)
+{
+  constexpr int S   = Order + 1;
+  const int     seg = blockIdx.y;
+  const long    ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
+    return;
+  const int  c   = seg / g.Ns;
+  const int  sp  = seg - c * g.Ns;
+  const long i   = (long)seg * g.cap + ip;
+  const int  bin = p.cell[i];
+  if (bin < 0)
+    return;
+  const Cell3 cl   = cell_of(g, bin);
+  const float r[3] = {p.rx[i], p.ry[i], p.rz[i]};
+  float       wix[S], wiy[S], wiz[S], whx[S], why[S], whz[S];
+  int         ix0, iy0, iz0 = 0, hx0, hy0, hz0 = 0;
+  int_w<Order>(pc, 0, r[0], ix0, wix);
+  int_w<Order>(pc, 1, r[1], iy0, wiy);
+  shape_mc<Order, float>(half_delta<Order>(r[0], hx0), 0.0f, 1.0f, whx);
+  shape_mc<Order, float>(half_delta<Order>(r[1], hy0), 0.0f, 1.0f, why);
+  const int lb = g.Nb;
+  const int X  = off[c * 3 + 2] + cl.x + ix0 + lb - (Order / 2);
+  const int HX = off[c * 3 + 2] + cl.x + hx0 + lb - (Order / 2);
+  const int Y  = off[c * 3 + 1] + cl.y + iy0 + lb - (Order / 2);
+  const int HY = off[c * 3 + 1] + cl.y + hy0 + lb - (Order / 2);
+  int       Z = 0, HZ = 0;
+  if constexpr (Dim == 3) {
+    int_w<Order>(pc, 2, r[2], iz0, wiz);
+    shape_mc<Order, float>(half_delta<Order>(r[2], hz0), 0.0f, 1.0f, whz);
+    Z  = off[c * 3 + 0] + cl.z + iz0 + lb - (Order / 2);
+    HZ = off[c * 3 + 0] + cl.z + hz0 + lb - (Order / 2);
+    if (min(Z, HZ) < 0 || max(Z, HZ) > g.P[0] - S)
+      return;
+  } else {
+    wiz[0] = whz[0] = 1.0f;
+  }
+  if (min(Y, HY) < 0 || max(Y, HY) > g.P[1] - S || min(X, HX) < 0 || max(X, HX) > g.P[2] - S)
+    return;
+  const long  sy = g.P[2], sz = g.plane, v = g.vol;
+  const int   nz   = Dim == 3 ? S : 1;
+  const float qmdt = pc.qmdt[sp];
+  auto interp = [&](const float* base, int z0, int y0, int x0, const float* wz, const float* wy,
+                    const float* wx) {
+    float rz = 0;
+    for (int jz = 0; jz < nz; jz++) {
+      float ry = 0;
+      for (int jy = 0; jy < S; jy++) {
+        const float* row = base + (z0 + jz) * sz + (y0 + jy) * sy + x0;
+        float        rx  = 0;
+        for (int jx = 0; jx < S; jx++)
+          rx += row[jx] * wx[jx];
+        ry += rx * wy[jy];
+      }
+      rz += ry * wz[jz];
+    }
+    return rz * qmdt;
+  };
+  float eb[6];
+  eb[0] = interp(E, Z, Y, HX, wiz, wiy, whx);
+  eb[1] = interp(E + v, Z, HY, X, wiz, why, wix);
+  eb[2] = interp(E + 2 * v, HZ, Y, X, whz, wiy, wix);
+  eb[3] = interp(B, HZ, HY, X, whz, why, wix);
+  eb[4] = interp(B + v, HZ, Y, HX, whz, wiy, whx);
+  eb[5] = interp(B + 2 * v, Z, HY, HX, wiz, why, whx);
+  float ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+  push_momentum<float>(pc.pusher, ux, uy, uz, eb, (float)g.cc);
+  p.ux[i] = ux;
+  p.uy[i] = uy;
+  p.uz[i] = uz;
+}
+
+// ---------------------------------------------------------------------------
+// position push + count (pic/engine/position.hpp:117-130,
+// nix/xtensor_particle.hpp:324-357) in FP32
+// ---------------------------------------------------------------------------
+
+// r_old -> ro; r += u dt / (gamma d) (unwrapped, relative to the old cell);
+// stayers take a slot in their new bin, leavers record their direction and
+// the tile's per-direction totals (the FP64 form's bookkeeping).
+__global__ void __launch_bounds__(256)
+    k32_push_position(Geom g, P32 p, float* __restrict__ ro0, float* __restrict__ ro1,
+                      float* __restrict__ ro2, const int32_t* __restrict__ np, float delt,
+                      float rc, float rdx, float rdy, float rdz, int32_t* __restrict__ cellcnt,
+                      int32_t* __restrict__ key, int32_t* __restrict__ slot,
+                      int32_t* __restrict__ tilecnt, int ntile)
+{
+  __shared__ int cnt[27];
+  if (threadIdx.x < 27)
+    cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int  seg    = blockIdx.y;
+  const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = ip < np[seg];
+  const long i      = (long)seg * g.cap + ip;
+  int        b      = -1;
+  if (active) {
+    float       rx = p.rx[i], ry = p.ry[i], rz = p.rz[i];
+    const float ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+    ro0[i]         = rx;
+    ro1[i]         = ry;
+    ro2[i]         = rz;
+    const float gm = lorentz_factor(ux, uy, uz, rc);
+    const float dt = delt / gm;
+    rx += g.has[2] ? ux * dt * rdx : ux * dt;
+    ry += g.has[1] ? uy * dt * rdy : uy * dt;
+    rz += g.has[0] ? uz * dt * rdz : uz * dt;
+    p.rx[i] = rx;
+    p.ry[i] = ry;
+    p.rz[i] = rz;
+    const int bin = p.cell[i];
+    if (bin >= 0) {
+      const Cell3 cl = cell_of(g, bin);
+      const int   cx = cl.x + (g.has[2] ? (int)floorf(rx) : 0);
+      const int   cy = cl.y + (g.has[1] ? (int)floorf(ry) : 0);
+      const int   cz = cl.z + (g.has[0] ? (int)floorf(rz) : 0);
+      const int   dx = (cx >= g.dims[2]) - (cx < 0) + 1;
+      const int   dy = (cy >= g.dims[1]) - (cy < 0) + 1;
+      const int   dz = (cz >= g.dims[0]) - (cz < 0) + 1;
+      const int   d  = 9 * dz + 3 * dy + dx;
+      if (d == 13) {
+        b = bin_of3(g, cx, cy, cz);
+      } else {
+        key[i] = -1 - d;
+        atomicAdd(&cnt[d], 1);
+      }
+    } else {
+      key[i] = -1 - 13; // never in the chunk: dropped by the migration
+      atomicAdd(&cnt[13], 1);
+    }
+  }
+  const bool stay = active && b >= 0;
+  int        sl   = wave_slot(cellcnt + (long)seg * g.nbin, b, stay);
+  if (stay) {
+    slot[i] = sl;
+    key[i]  = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < 27)
+    tilecnt[((long)seg * ntile + blockIdx.x) * 27 + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// Esirkepov deposit (nix/esirkepov.hpp:76-337) in FP32
+// ---------------------------------------------------------------------------
+
+struct Dep32 {
+  float q[kMaxSpecies];
+  float delt, dxdt, dydt, dzdt;
+};
+
+// Old and new weights of one axis in the old cell's frame (even orders:
+// i0 = the cell, shift = floor(r_new); odd orders: i0 = cell + floor(r_old - 1/2)).
+// s0, ds have Order+3 entries; returns the first node relative to the cell.
+template <int Order>
+__device__ __forceinline__ int esk_axis(float ro, float rn, float* s0, float* ds)
+{
+  constexpr int   N  = Order + 3;
+  constexpr float ho = Order % 2 == 1 ? 0.5f : 0.0f;
+  float           s1[N];
+#pragma unroll
+  for (int l = 0; l < N; l++)
+    s0[l] = s1[l] = 0.0f;
+  const float f0 = floorf(ro - ho);
+  const float f1 = floorf(rn - ho);
+  shape_mc<Order, float>(ro - 0.5f - f0, 0.0f, 1.0f, &s0[1]);
+  shape_mc<Order, float>(rn - 0.5f - f1, 0.0f, 1.0f, &s1[1]);
+  const int sh = (int)(f1 - f0);
+  if (sh < 0) {
+#pragma unroll
+    for (int ii = 0; ii < Order + 2; ii++)
+      s1[ii] = s1[ii + 1];
+  } else if (sh > 0) {
+#pragma unroll
+    for (int ii = Order + 2; ii > 0; ii--)
+      s1[ii] = s1[ii - 1];
+  }
+#pragma unroll
+  for (int l = 0; l < N; l++)
+    ds[l] = s1[l] - s0[l];
+  return (int)f0 - (Order / 2) - 1;
+}
+
+// One thread per particle (odd orders, 2D, unsorted particles): the full
+// (Order+3)^3 (3D) or (Order+3)^2 (2D) tile of J with global FP32 atomics.
+// rho is deposited separately (k32_rho_global / k32_rho_march).
+template <int Dim, int Order>
+__global__ void __launch_bounds__(256)
+    k32_deposit_global(Geom g, P32 p, const float* __restrict__ ro0,
+                       const float* __restrict__ ro1, const float* __restrict__ ro2,
+                       const int32_t* __restrict__ np, const int* __restrict__ off,
+                       float* __restrict__ J, Dep32 dc)
+{
+  constexpr int N   = Order + 3;
+  const int     seg = blockIdx.y;
+  const long    ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
+    return;
+  const int  c   = seg / g.Ns;
+  const int  sp  = seg - c * g.Ns;
+  const long i   = (long)seg * g.cap + ip;
+  const int  bin = p.cell[i];
+  if (bin < 0)
+    return;
+  const Cell3 cl = cell_of(g, bin);
+  float       s0x[N], s0y[N], s0z[N], dsx[N], dsy[N], dsz[N];
+  const int   fx = esk_axis<Order>(ro0[i], p.rx[i], s0x, dsx);
+  const int   fy = esk_axis<Order>(ro1[i], p.ry[i], s0y, dsy);
+  int         fz = 0;
+  if constexpr (Dim == 3)
+    fz = esk_axis<Order>(ro2[i], p.rz[i], s0z, dsz);
+  const int   lb = g.Nb;
+  const int   gx = off[c * 3 + 2] + cl.x + fx + lb;
+  const int   gy = off[c * 3 + 1] + cl.y + fy + lb;
+  const int   gz = Dim == 3 ? off[c * 3 + 0] + cl.z + fz + lb : 0;
+  const long  v  = g.vol;
+  const float qs = dc.q[sp];
+  const float A = 0.5f, B = 1.0f / 3.0f;
+  auto add = [&](int z, int y, int x, int k, float val) {
+    if ((unsigned)z >= (unsigned)g.P[0] || (unsigned)y >= (unsigned)g.P[1] ||
+        (unsigned)x >= (unsigned)g.P[2])
+      return;
+    atomicAdd(&J[k * v + ((long)z * g.P[1] + y) * g.P[2] + x], val);
+  };
+  if constexpr (Dim == 3) {
+    const float qx = qs * dc.dxdt, qy = qs * dc.dydt, qz = qs * dc.dzdt;
+    for (int jz = 0; jz < N; jz++)
+      for (int jy = 0; jy < N; jy++) {
+        const float w  = -((s0y[jy] + A * dsy[jy]) * s0z[jz] + (A * s0y[jy] + B * dsy[jy]) * dsz[jz]) * qx;
+        float       ww = 0;
+        for (int jx = 0; jx < N - 1; jx++) {
+          ww += dsx[jx] * w;
+          add(gz + jz, gy + jy, gx + jx + 1, 1, ww);
+        }
+      }
+    for (int jz = 0; jz < N; jz++)
+      for (int jx = 0; jx < N; jx++) {
+        const float w  = -((s0z[jz] + A * dsz[jz]) * s0x[jx] + (A * s0z[jz] + B * dsz[jz]) * dsx[jx]) * qy;
+        float       ww = 0;
+        for (int jy = 0; jy < N - 1; jy++) {
+          ww += dsy[jy] * w;
+          add(gz + jz, gy + jy + 1, gx + jx, 2, ww);
+        }
+      }
+    for (int jy = 0; jy < N; jy++)
+      for (int jx = 0; jx < N; jx++) {
+        const float w  = -((s0x[jx] + A * dsx[jx]) * s0y[jy] + (A * s0x[jx] + B * dsx[jx]) * dsy[jy]) * qz;
+        float       ww = 0;
+        for (int jz = 0; jz < N - 1; jz++) {
+          ww += dsz[jz] * w;
+          add(gz + jz + 1, gy + jy, gx + jx, 3, ww);
+        }
+      }
+  } else {
+    // 2D (esirkepov.hpp:76-143): Jz from the velocity (current.hpp:285)
+    const float vz = (p.rz[i] - ro2[i]) / dc.delt;
+    const float qx = qs * dc.dxdt, qy = qs * dc.dydt;
+    for (int jy = 0; jy < N; jy++) {
+      const float w  = -(s0y[jy] + A * dsy[jy]) * qx;
+      float       ww = 0;
+      for (int jx = 0; jx < N - 1; jx++) {
+        ww += dsx[jx] * w;
+        add(gz, gy + jy, gx + jx + 1, 1, ww);
+      }
+    }
+    for (int jx = 0; jx < N; jx++) {
+      const float w  = -(s0x[jx] + A * dsx[jx]) * qy;
+      float       ww = 0;
+      for (int jy = 0; jy < N - 1; jy++) {
+        ww += dsy[jy] * w;
+        add(gz, gy + jy + 1, gx + jx, 2, ww);
+      }
+    }
+    for (int jy = 0; jy < N; jy++)
+      for (int jx = 0; jx < N; jx++) {
+        const float w =
+            ((s0x[jx] + A * dsx[jx]) * s0y[jy] + (A * s0x[jx] + B * dsx[jx]) * dsy[jy]) * (qs * vz);
+        add(gz, gy + jy, gx + jx, 3, w);
+      }
+  }
+}
+
+// Plane-lane march kernel (3D, even orders): the FP64 k_deposit_march in FP32
+// registers, with the block's ring tile in FP64 LDS.  All particles of a bin
+// share the tile origin (even orders: the old cell), so no particle needs the
+// own-origin path.
+template <int Order>
+struct March32 {
+  static constexpr int N     = Order + 3;
+  static constexpr int NPL   = 3 * N;
+  static constexpr int PPS   = 64 / NPL >= 4 ? 4 : (64 / NPL >= 2 ? 2 : 1);
+  static constexpr int BATCH = 16;
+  static constexpr int NS    = N & ~1;
+  static constexpr int ROWS  = N + NS / 2;
+  static constexpr int PITCH = BATCH + 4;
+  static constexpr int WPAIR = 3 * ROWS * PITCH;
+  static constexpr int NWAVE = 4;
+  static constexpr size_t WLDS = sizeof(float) * 2 * WPAIR * NWAVE;
+};
+
+template <int Order>
+__global__ void __launch_bounds__(256)
+    k32_deposit_march(Geom g, const P32 p, const float* __restrict__ ro0,
+                      const float* __restrict__ ro1, const float* __restrict__ ro2,
+                      const int32_t* __restrict__ cellstart, const int* __restrict__ off,
+                      float* __restrict__ J, Dep32 dc, int zrows)
+{
+  using MP            = March32<Order>;
+  constexpr int N     = MP::N;
+  constexpr int NPL   = MP::NPL;
+  constexpr int PPS   = MP::PPS;
+  constexpr int BATCH = MP::BATCH;
+  constexpr int NS    = MP::NS;
+  constexpr int ROWS  = MP::ROWS;
+  constexpr int PITCH = MP::PITCH;
+  constexpr int NWAVE = MP::NWAVE;
+  static_assert(Order % 2 == 0, "the FP32 march kernel takes even orders");
+
+  __shared__ float2 wts[NWAVE][MP::WPAIR];
+  __shared__ int    next_cell;
+  extern __shared__ double tile[]; // [3][N z-ring][N][TX] FP64, then rcs
+
+  const int c   = blockIdx.y;
+  const int ns  = g.Ns;
+  const int nbx = g.bsy;
+  const int nby = g.bsz / g.bsy;
+  const int nbz = g.nbin / g.bsz;
+  const int by  = blockIdx.x % nby;
+  const int bz0 = (blockIdx.x / nby) * zrows;
+  const int bz1 = min(nbz, bz0 + zrows);
+
+  int any = 0;
+  for (int q = threadIdx.x; q < ns * (bz1 - bz0); q += blockDim.x) {
+    const int      sp = q % ns;
+    const int      bz = bz0 + q / ns;
+    const int32_t* cs = cellstart + (long)(c * ns + sp) * (g.nbin + 1);
+    const int      b0 = bz * g.bsz + by * g.bsy;
+    any |= cs[b0 + nbx] > cs[b0] ? 1 : 0;
+  }
+  if (!__syncthreads_or(any))
+    return;
+
+  const int lb  = g.Nb;
+  const int ylo = by + lb - (Order / 2) - 1; // tile origin (local node index)
+  const int xlo = lb - (Order / 2) - 1;
+  const int TX  = nbx + N - 1;
+  const int psz = N * TX;
+  const int tsz = N * psz;
+  int*      rcs = reinterpret_cast<int*>(tile + 3 * tsz);
+  for (int q = threadIdx.x; q < 3 * tsz; q += blockDim.x)
+    tile[q] = 0.0;
+  auto load_row = [&](int bz) {
+    for (int q = threadIdx.x; q < ns * (nbx + 1); q += blockDim.x) {
+      const int sp = q / (nbx + 1);
+      const int i  = q - sp * (nbx + 1);
+      rcs[q] = cellstart[(long)(c * ns + sp) * (g.nbin + 1) + bz * g.bsz + by * g.bsy + i];
+    }
+  };
+  load_row(bz0);
+  if (threadIdx.x == 0)
+    next_cell = 0;
+  __syncthreads();
+
+  const int   lane = threadIdx.x & 63;
+  const int   wv   = threadIdx.x >> 6;
+  const long  v    = g.vol;
+  const float A    = 0.5f;
+  const float B    = 1.0f / 3.0f;
+  const bool  act   = lane < NPL * PPS;
+  const int   plane = act ? lane / PPS : 0;
+  const int   k     = lane & (PPS - 1);
+  const int   comp  = plane / N;
+  const int   t     = plane - comp * N;
+  const int   axF   = comp == 0 ? 1 : (comp == 1 ? 2 : 0);
+  const int   axS   = comp == 0 ? 2 : (comp == 1 ? 0 : 1);
+  const int   axC   = comp;
+  const float ddt   = comp == 0 ? dc.dxdt : (comp == 1 ? dc.dydt : dc.dzdt);
+  const int   a1    = lane / BATCH;
+  const int   k1    = lane - a1 * BATCH;
+  const float* R0   = a1 == 0 ? ro0 : (a1 == 1 ? ro1 : ro2);
+  const float* R1   = a1 == 0 ? p.rx : (a1 == 1 ? p.ry : p.rz);
+  float2*      W    = &wts[wv][0];
+
+  auto claim = [&](int& sp, int& bx, int& p0, int& p1) {
+    for (;;) {
+      int q = 0;
+      if (lane == 0)
+        q = atomicAdd(&next_cell, 1);
+      q = __builtin_amdgcn_readfirstlane(q);
+      if (q >= ns * nbx)
+        return false;
+      sp = q / nbx;
+      bx = q - sp * nbx;
+      p0 = rcs[sp * (nbx + 1) + bx];
+      p1 = rcs[sp * (nbx + 1) + bx + 1];
+      if (p1 > p0)
+        return true;
+    }
+  };
+  auto fetch = [&](int seg, int pb, int p1, float& xo, float& xn) {
+    if (a1 < 3 && pb + k1 < p1) {
+      const long i = (long)seg * g.cap + pb + k1;
+      xo           = R0[i];
+      xn           = R1[i];
+    }
+  };
+
+  for (int bz = bz0; bz < bz1; bz++) {
+    const int zlo = bz + lb - (Order / 2) - 1;
+    const int zs  = ((zlo % N) + N) % N;
+    int       sp, bx, p0, p1;
+    bool      have = claim(sp, bx, p0, p1);
+    float     nxo = 0, nxn = 0;
+    if (have)
+      fetch(c * ns + sp, p0, p1, nxo, nxn);
+    while (have) {
+      int         spn, bxn, p0n, p1n;
+      const bool  more = claim(spn, bxn, p0n, p1n);
+      const int   seg  = c * ns + sp;
+      const float qd   = dc.q[sp] * ddt;
+      float       acc[N][N - 1];
+#pragma unroll
+      for (int r = 0; r < N; r++)
+#pragma unroll
+        for (int j = 0; j < N - 1; j++)
+          acc[r][j] = 0.0f;
+      for (int pb = p0; pb < p1; pb += BATCH) {
+        const int   nbt = min(BATCH, p1 - pb);
+        const float xo  = nxo, xn = nxn;
+        if (pb + BATCH < p1)
+          fetch(seg, pb + BATCH, p1, nxo, nxn);
+        else if (more)
+          fetch(c * ns + spn, p0n, p1n, nxo, nxn);
+        if (a1 < 3 && k1 < nbt) {
+          float s0[N], ds[N];
+          esk_axis<Order>(xo, xn, s0, ds);
+          float2* pr = W + a1 * ROWS * PITCH + k1;
+#pragma unroll
+          for (int l = 0; l < N; l++)
+            pr[l * PITCH] = make_float2(s0[l], ds[l]);
+          float pre[NS];
+          pre[0] = ds[0];
+#pragma unroll
+          for (int l = 1; l < NS; l++)
+            pre[l] = l < N - 1 ? pre[l - 1] + ds[l] : 0.0f;
+#pragma unroll
+          for (int l = 0; l < NS / 2; l++)
+            pr[(N + l) * PITCH] = make_float2(pre[2 * l], pre[2 * l + 1]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k0 = 0; k0 < nbt; k0 += PPS) {
+          const int kk = k0 + k;
+          if (act && kk < nbt) {
+            const float2* pp = W + kk;
+            const float2  pf = pp[(axF * ROWS + t) * PITCH];
+            const float   aF = fmaf(A, pf.y, pf.x);
+            const float   bF = fmaf(B, pf.y, A * pf.x);
+            float         P[NS];
+#pragma unroll
+            for (int j = 0; j < NS / 2; j++) {
+              const float2 d2 = pp[(axC * ROWS + N + j) * PITCH];
+              P[2 * j]        = d2.x;
+              P[2 * j + 1]    = d2.y;
+            }
+#pragma unroll
+            for (int r = 0; r < N; r++) {
+              const float2 ps = pp[(axS * ROWS + r) * PITCH];
+              const float  w  = -fmaf(aF, ps.x, bF * ps.y) * qd;
+#pragma unroll
+              for (int j = 0; j < N - 1; j++)
+                acc[r][j] = fmaf(P[j], w, acc[r][j]);
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+#pragma unroll
+      for (int r = 0; r < N; r++)
+#pragma unroll
+        for (int j = 0; j < N - 1; j++) {
+          float s = acc[r][j];
+          if (PPS >= 2)
+            s += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0xB1, 0xF, 0xF, true));
+          if (PPS >= 4)
+            s += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0x4E, 0xF, 0xF, true));
+          acc[r][j] = s;
+        }
+      if (act && k == 0) {
+#pragma unroll
+        for (int r = 0; r < N; r++)
+#pragma unroll
+          for (int j = 0; j < N - 1; j++) {
+            const int oz  = comp == 0 ? r : (comp == 1 ? t : j + 1);
+            const int oy  = comp == 0 ? t : (comp == 1 ? j + 1 : r);
+            const int oxx = comp == 0 ? j + 1 : (comp == 1 ? r : t);
+            const int sl  = zs + oz >= N ? zs + oz - N : zs + oz;
+            atomicAdd(&tile[comp * tsz + sl * psz + oy * TX + bx + oxx], (double)acc[r][j]);
+          }
+      }
+      have = more;
+      sp   = spn;
+      bx   = bxn;
+      p0   = p0n;
+      p1   = p1n;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < 3 * psz; q += blockDim.x) {
+      const int    kc  = q / psz;
+      const int    r2  = q - kc * psz;
+      const int    idx = kc * tsz + zs * psz + r2;
+      const double val = tile[idx];
+      if (val != 0.0) {
+        const int ty = r2 / TX;
+        const int tx = r2 - ty * TX;
+        const int gz = off[c * 3 + 0] + zlo, gy = off[c * 3 + 1] + ylo + ty,
+                  gx = off[c * 3 + 2] + xlo + tx;
+        if ((unsigned)gz < (unsigned)g.P[0] && (unsigned)gy < (unsigned)g.P[1] &&
+            (unsigned)gx < (unsigned)g.P[2])
+          atomicAdd(&J[(kc + 1) * v + ((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
+        tile[idx] = 0.0;
+      }
+    }
+    if (bz + 1 < bz1)
+      load_row(bz + 1);
+    if (threadIdx.x == 0)
+      next_cell = 0;
+    __syncthreads();
+  }
+  const int zl = (bz1 - 1) + lb - (Order / 2) - 1;
+  const int zs = ((zl % N) + N) % N;
+  for (int q = threadIdx.x; q < 3 * (N - 1) * psz; q += blockDim.x) {
+    const int    kc  = q / ((N - 1) * psz);
+    const int    r   = q - kc * (N - 1) * psz;
+    const int    dz  = 1 + r / psz;
+    const int    r2  = r - (dz - 1) * psz;
+    const int    sl  = zs + dz >= N ? zs + dz - N : zs + dz;
+    const double val = tile[kc * tsz + sl * psz + r2];
+    if (val == 0.0)
+      continue;
+    const int ty = r2 / TX;
+    const int tx = r2 - ty * TX;
+    const int gz = off[c * 3 + 0] + zl + dz, gy = off[c * 3 + 1] + ylo + ty,
+              gx = off[c * 3 + 2] + xlo + tx;
+    if ((unsigned)gz < (unsigned)g.P[0] && (unsigned)gy < (unsigned)g.P[1] &&
+        (unsigned)gx < (unsigned)g.P[2])
+      atomicAdd(&J[(kc + 1) * v + ((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
+  }
+}
+
+// rho from the current (sorted or not) positions, one thread per particle:
+// qs times the (Order+1)^Dim cell-centred weights (esirkepov.hpp:145-155).
+template <int Dim, int Order>
+__global__ void __launch_bounds__(256)
+    k32_rho_global(Geom g, P32 p, const int32_t* __restrict__ np, const int* __restrict__ off,
+                   float* __restrict__ J)
+{
+  constexpr int S   = Order + 1;
+  const int     seg = blockIdx.y;
+  const long    ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
+    return;
+  const int  c   = seg / g.Ns;
+  const int  sp  = seg - c * g.Ns;
+  const long i   = (long)seg * g.cap + ip;
+  const int  bin = p.cell[i];
+  if (bin < 0)
+    return;
+  const Cell3 cl   = cell_of(g, bin);
+  const float r[3] = {p.rx[i], p.ry[i], p.rz[i]};
+  const int   ci[3] = {cl.x, cl.y, cl.z};
+  float       w[3][S];
+  int         o[3];
+  for (int a = 0; a < 3; a++) {
+    if (a < Dim) {
+      // unwrapped r (after a position push) is allowed: split off its cell shift
+      const float fl  = floorf(r[a]);
+      const float rr  = r[a] - fl;
+      int         fst = 0;
+      shape_mc<Order, float>(int_delta<Order>(rr, fst), 0.0f, 1.0f, w[a]);
+      o[a] = ci[a] + (int)fl + fst + g.Nb - (Order / 2);
+    } else {
+      w[a][0] = 1.0f;
+      o[a]    = g.Nb;
+    }
+  }
+  const float qs = (float)g.q[sp];
+  for (int jz = 0; jz < (Dim == 3 ? S : 1); jz++)
+    for (int jy = 0; jy < S; jy++)
+      for (int jx = 0; jx < S; jx++) {
+        const float val = Dim == 3 ? qs * w[0][jx] * w[1][jy] * w[2][jz] : qs * w[0][jx] * w[1][jy];
+        const int   gz  = g.has[0] ? off[c * 3 + 0] + o[2] + jz : 0;
+        const int   gy  = off[c * 3 + 1] + o[1] + jy;
+        const int   gx  = off[c * 3 + 2] + o[0] + jx;
+        if ((unsigned)gz >= (unsigned)g.P[0] || (unsigned)gy >= (unsigned)g.P[1] ||
+            (unsigned)gx >= (unsigned)g.P[2])
+          continue;
+        atomicAdd(&J[((long)gz * g.P[1] + gy) * g.P[2] + gx], val);
+      }
+}
+
+void march32_split(const Geom& g, int fixed, int& zrows, int& nzr)
+{
+  const int  nby   = g.bsz / g.bsy;
+  const int  nbz   = g.nbin / g.bsz;
+  const long cols  = (long)nby * g.nchunk;
+  const int  split = (int)std::min<long>(nbz, std::max<long>(1, (2048 + cols - 1) / cols));
+  zrows            = fixed > 0 ? std::min(fixed, nbz) : (nbz + split - 1) / split;
+  nzr              = (nbz + zrows - 1) / zrows;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+void launch32_from_f64(Sim& s, int seg0, int nseg)
+{
+  const Geom& g = s.g;
+  hipLaunchKernelGGL(k32_from_f64, dim3(nblk(g.cap, 256), nseg), dim3(256), 0, s.stream, g, s.pa,
+                     s.qa, s.d_np, s.d_lim, seg0, nseg);
+}
+
+void launch32_to_f64(Sim& s, int seg0, int nseg, bool old)
+{
+  const Geom& g = s.g;
+  hipLaunchKernelGGL(k32_to_f64, dim3(nblk(g.cap, 256), nseg), dim3(256), 0, s.stream, g, s.qa,
+                     s.pa, s.f_ro[0], s.f_ro[1], s.f_ro[2], old ? s.d_xo[0] : nullptr,
+                     old ? s.d_xo[1] : nullptr, old ? s.d_xo[2] : nullptr, s.d_np, s.d_lim, seg0);
+}
+
+void launch32_fields_to_f64(Sim& s, int which)
+{
+  const long v    = s.g.vol;
+  auto       conv = [&](const float* a, double* b, long n) {
+    hipLaunchKernelGGL((k32_convert<float, double>), dim3(4096), dim3(256), 0, s.stream, a, b, n);
+  };
+  if (which == PN_FIELD_UF) {
+    conv(s.f_E, s.d_E, 3 * v);
+    conv(s.f_B, s.d_B, 3 * v);
+  } else {
+    conv(s.f_J, s.d_J, 4 * v);
+  }
+}
+
+void launch32_fields_from_f64(Sim& s, int which)
+{
+  const long v    = s.g.vol;
+  auto       conv = [&](const double* a, float* b, long n) {
+    hipLaunchKernelGGL((k32_convert<double, float>), dim3(4096), dim3(256), 0, s.stream, a, b, n);
+  };
+  if (which == PN_FIELD_UF) {
+    conv(s.d_E, s.f_E, 3 * v);
+    conv(s.d_B, s.f_B, 3 * v);
+  } else {
+    conv(s.d_J, s.f_J, 4 * v);
+  }
+}
+
+void launch32_push_velocity(Sim& s, double delt)
+{
+  PhaseTimer  pt(s, 1);
+  const Geom& g = s.g;
+  Push32      pc{};
+  for (int sp = 0; sp < g.Ns; sp++)
+    pc.qmdt[sp] = (float)(0.5 * g.q[sp] / g.m[sp] * delt);
+  pc.pusher = s.cfg.pusher;
+  pc.wt     = s.cfg.interp;
+  for (int a = 0; a < 3; a++) {
+    pc.cdt[a]  = (float)(g.cc * delt / g.del[2 - a]);
+    pc.rcdt[a] = 1.0f / pc.cdt[a];
+  }
+  const int    T    = g.order + 2;
+  const size_t lds  = sizeof(float) * 6 * (g.dim == 3 ? T : 1) * T * (g.bsy + g.order + 1);
+  const bool   rows = s.sorted && !s.generic_kernels && lds <= 65536;
+  const int    nrow = g.nbin / g.bsy;
+#define PN32_VEL(D, O)                                                                        \
+  do {                                                                                        \
+    if (rows)                                                                                 \
+      hipLaunchKernelGGL((k32_push_velocity_rows<D, O>), dim3(nrow, g.nseg), dim3(256), lds,   \
+                         s.stream, g, s.qa, s.ss.cellstart, s.d_off, s.f_E, s.f_B, pc);       \
+    else                                                                                      \
+      hipLaunchKernelGGL((k32_push_velocity<D, O>), dim3(nblk(g.cap, 256), g.nseg), dim3(256), \
+                         0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc);            \
+  } while (0)
+  if (g.dim == 3) {
+    switch (g.order) {
+    case 1: PN32_VEL(3, 1); break;
+    case 2: PN32_VEL(3, 2); break;
+    case 3: PN32_VEL(3, 3); break;
+    default: PN32_VEL(3, 4); break;
+    }
+  } else {
+    switch (g.order) {
+    case 1: PN32_VEL(2, 1); break;
+    case 2: PN32_VEL(2, 2); break;
+    case 3: PN32_VEL(2, 3); break;
+    default: PN32_VEL(2, 4); break;
+    }
+  }
+#undef PN32_VEL
+}
+
+void launch32_push_position(Sim& s, double delt)
+{
+  PhaseTimer  pt(s, 2);
+  const Geom& g = s.g;
+  launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
+  hipLaunchKernelGGL(k32_push_position, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa,
+                     s.f_ro[0], s.f_ro[1], s.f_ro[2], s.d_np, (float)delt, (float)(1 / g.cc),
+                     (float)(1 / g.del[2]), (float)(1 / g.del[1]), (float)(1 / g.del[0]),
+                     s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.tilecnt, s.ss.ntile);
+  s.counted = true;
+  s.sorted  = false;
+}
+
+bool launch32_deposit(Sim& s, double delt, bool defer_rho)
+{
+  const Geom& g = s.g;
+  Dep32       dc{};
+  for (int sp = 0; sp < g.Ns; sp++)
+    dc.q[sp] = (float)g.q[sp];
+  dc.delt = (float)delt;
+  dc.dxdt = (float)(g.del[2] / delt);
+  dc.dydt = (float)(g.del[1] / delt);
+  dc.dzdt = (float)(g.del[0] / delt);
+  {
+    PhaseTimer pz(s, 4);
+    launch_zero(s, s.f_J, 4 * g.vol);
+  }
+  bool march = false;
+  {
+    PhaseTimer pt(s, 3);
+    const int    N    = g.order + 3;
+    const size_t mlds = sizeof(double) * 3 * (size_t)N * N * (g.bsy + N - 1) +
+                        sizeof(int) * (size_t)g.Ns * (g.bsy + 1);
+    int zrows, nzr;
+    march32_split(g, s.march_rows, zrows, nzr);
+    const dim3 mgrid(nzr * (g.bsz / g.bsy), g.nchunk);
+    if (g.dim == 3 && s.sorted && !s.generic_kernels && g.order == 2 &&
+        mlds + March32<2>::WLDS + 64 <= 65536) {
+      hipLaunchKernelGGL(k32_deposit_march<2>, mgrid, dim3(256), mlds, s.stream, g, s.qa,
+                         s.f_ro[0], s.f_ro[1], s.f_ro[2], s.ss.cellstart, s.d_off, s.f_J, dc,
+                         zrows);
+      march = true;
+    } else if (g.dim == 3 && s.sorted && !s.generic_kernels && g.order == 4 &&
+               mlds + March32<4>::WLDS + 64 <= 65536) {
+      hipLaunchKernelGGL(k32_deposit_march<4>, mgrid, dim3(256), mlds, s.stream, g, s.qa,
+                         s.f_ro[0], s.f_ro[1], s.f_ro[2], s.ss.cellstart, s.d_off, s.f_J, dc,
+                         zrows);
+      march = true;
+    } else {
+      const dim3 grid(nblk(g.cap, 256), g.nseg);
+#define PN32_DEP(D, O)                                                                         \
+  hipLaunchKernelGGL((k32_deposit_global<D, O>), grid, dim3(256), 0, s.stream, g, s.qa,        \
+                     s.f_ro[0], s.f_ro[1], s.f_ro[2], s.d_np, s.d_off, s.f_J, dc)
+      if (g.dim == 3) {
+        switch (g.order) {
+        case 1: PN32_DEP(3, 1); break;
+        case 2: PN32_DEP(3, 2); break;
+        case 3: PN32_DEP(3, 3); break;
+        default: PN32_DEP(3, 4); break;
+        }
+      } else {
+        switch (g.order) {
+        case 1: PN32_DEP(2, 1); break;
+        case 2: PN32_DEP(2, 2); break;
+        case 3: PN32_DEP(2, 3); break;
+        default: PN32_DEP(2, 4); break;
+        }
+      }
+#undef PN32_DEP
+    }
+  }
+  (void)march;
+  if (!defer_rho) {
+    launch32_rho(s);
+    return false;
+  }
+  return true;
+}
+
+void launch32_rho(Sim& s)
+{
+  PhaseTimer  pt(s, 7);
+  const Geom& g = s.g;
+  const dim3  grid(nblk(g.cap, 256), g.nseg);
+#define PN32_RHO(D, O)                                                                        \
+  hipLaunchKernelGGL((k32_rho_global<D, O>), grid, dim3(256), 0, s.stream, g, s.qa, s.d_np,    \
+                     s.d_off, s.f_J)
+  if (g.dim == 3) {
+    switch (g.order) {
+    case 1: PN32_RHO(3, 1); break;
+    case 2: PN32_RHO(3, 2); break;
+    case 3: PN32_RHO(3, 3); break;
+    default: PN32_RHO(3, 4); break;
+    }
+  } else {
+    switch (g.order) {
+    case 1: PN32_RHO(2, 1); break;
+    case 2: PN32_RHO(2, 2); break;
+    case 3: PN32_RHO(2, 3); break;
+    default: PN32_RHO(2, 4); break;
+    }
+  }
+#undef PN32_RHO
+}
+
+} // namespace picnix

@@ -55,6 +55,17 @@ struct Sim {
   double* d_M = nullptr; // [Ns][14][vol] moments (PicChunk::um), allocated on first use
   double* d_ff = nullptr; // [3 levels][3][vol] Friedman history, only for theta != 0
 
+  // precision = "fp32": FP32 fields and cell-relative FP32 particles.  The FP64
+  // arrays above stay as the mirror used at the C-ABI boundary and for the FP64
+  // energy and charge reductions.
+  bool   fp32 = false;
+  float* f_E  = nullptr; // [3][vol]
+  float* f_B  = nullptr; // [3][vol]
+  float* f_J  = nullptr; // [4][vol]
+  float* f_ff = nullptr; // [3][3][vol], theta != 0 only
+  P32    qa{}, qb{};     // current / sort target
+  float* f_ro[3] = {nullptr, nullptr, nullptr}; // old cell-relative positions
+
   // particles: A = current, Bv = sort destination
   PArr     pa{}, pb{};
   double*  d_xo[3] = {nullptr, nullptr, nullptr}; // old positions (XtensorParticle::xv)
@@ -73,7 +84,7 @@ struct Sim {
   bool            step_graph  = false;
   hipGraphExec_t  gexec[2]    = {nullptr, nullptr};
   double          graph_dt[2] = {0, 0};
-  double*         buf0_x      = nullptr; // pa.x of parity 0
+  void*           buf0_x      = nullptr; // first position array of parity 0
 
   // staging for host <-> device particle transfers
   double* d_stage     = nullptr;
@@ -106,14 +117,17 @@ struct PhaseTimer {
 void launch_push_bfd(Sim& s, double delt);
 void launch_push_efd(Sim& s, double delt);
 void launch_fill_ghost(Sim& s, double* f, int ncomp);
+void launch_fill_ghost(Sim& s, float* f, int ncomp);
 // zero fills and copies as kernels (no memset/memcpy nodes under graph capture)
 void launch_zero(Sim& s, double* p, long n);
+void launch_zero(Sim& s, float* p, long n);
 void launch_zero(Sim& s, int32_t* p, long n);
 void launch_copy(Sim& s, int32_t* dst, const int32_t* src, long n);
 void launch_init_friedman(Sim& s); // every Friedman level = E (no-op for theta = 0)
 // fold the padding of components [k0, k1) of a [ncomp][vol] global array onto
 // the periodic interior (the Cur / Mom exchanges); phase: timing slot
 void launch_fold(Sim& s, double* F, int k0, int k1, int phase);
+void launch_fold(Sim& s, float* F, int k0, int k1, int phase);
 void launch_fold_current(Sim& s, int k0 = 0, int k1 = 4);
 // moment deposit into d_M (allocated on first use), without the Mom exchange
 void launch_deposit_moment(Sim& s);
@@ -128,6 +142,18 @@ bool launch_deposit(Sim& s, double delt, bool defer_rho = false);
 // rho from the current positions; sorted_by_new_cell: cellstart describes them
 void launch_rho(Sim& s, bool sorted_by_new_cell);
 void launch_energy(Sim& s, double* out /* host: efd, bfd, divE, divB, particle[Ns] */);
+
+// FP32 form (pn_fp32.hip, pn_sort.hip)
+void launch32_from_f64(Sim& s, int seg0, int nseg);        // pa (FP64 SoA) -> qa
+void launch32_to_f64(Sim& s, int seg0, int nseg, bool old); // qa -> pa (and f_ro -> d_xo)
+void launch32_fields_to_f64(Sim& s, int which);            // f_* -> d_* mirror
+void launch32_fields_from_f64(Sim& s, int which);          // d_* mirror -> f_*
+void launch32_push_velocity(Sim& s, double delt);
+void launch32_push_position(Sim& s, double delt);
+bool launch32_deposit(Sim& s, double delt, bool defer_rho);
+void launch32_rho(Sim& s);
+void launch32_migrate(Sim& s);
+void launch32_sort(Sim& s);
 void launch_aos_to_soa(Sim& s, int seg, int64_t np);
 void launch_soa_to_aos(Sim& s, int seg, int64_t np);
 // which: PN_FIELD_UF, PN_FIELD_UJ or PN_FIELD_UM

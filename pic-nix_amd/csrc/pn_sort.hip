@@ -337,6 +337,198 @@ void finish_sort(Sim& s)
   s.sorted        = true;
 }
 
+// ---- FP32 form (cell-relative particles, struct P32) ----
+
+__device__ __forceinline__ void cell3_of(const Geom& g, int bin, int& x, int& y, int& z)
+{
+  z           = bin / g.bsz;
+  const int r = bin - z * g.bsz;
+  y           = r / g.bsy;
+  x           = r - y * g.bsy;
+}
+
+// count(reset) for the FP32 form: the bin of cell + floor(r); outside the chunk
+// (or never placed in it) -> dropped by the sort.
+__global__ void __launch_bounds__(256)
+    k32_count(Geom g, P32 p, const int32_t* __restrict__ np, int32_t* __restrict__ cellcnt,
+              int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ dseg,
+              int32_t* __restrict__ didx)
+{
+  const int  seg    = blockIdx.y;
+  const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = ip < np[seg];
+  const long i      = (long)seg * g.cap + ip;
+  int        b      = -1;
+  if (active && p.cell[i] >= 0) {
+    int cx, cy, cz;
+    cell3_of(g, p.cell[i], cx, cy, cz);
+    cx += g.has[2] ? (int)floorf(p.rx[i]) : 0;
+    cy += g.has[1] ? (int)floorf(p.ry[i]) : 0;
+    cz += g.has[0] ? (int)floorf(p.rz[i]) : 0;
+    if (cx >= 0 && cx < g.dims[2] && cy >= 0 && cy < g.dims[1] && cz >= 0 && cz < g.dims[0])
+      b = cz * g.bsz + cy * g.bsy + cx;
+  }
+  const bool stay = active && b >= 0;
+  int        sl   = wave_slot(cellcnt + (long)seg * g.nbin, b, stay);
+  if (!active)
+    return;
+  if (stay) {
+    slot[i] = sl;
+    key[i]  = b;
+    dseg[i] = seg;
+    didx[i] = (int32_t)ip;
+  } else {
+    key[i]  = -1;
+    dseg[i] = -1;
+  }
+}
+
+// Destinations for the FP32 form (k_migrate's bookkeeping): a leaver keeps r and
+// takes the neighbour's cell index, cell + floor(r) -/+ the chunk extent.
+__global__ void __launch_bounds__(256)
+    k32_migrate(Geom g, P32 p, const int32_t* __restrict__ np, const int* __restrict__ nb,
+                const int32_t* __restrict__ tilecnt, const int32_t* __restrict__ recvoff,
+                int32_t* __restrict__ cellcnt, int32_t* __restrict__ key,
+                int32_t* __restrict__ slot, int32_t* __restrict__ dseg,
+                int32_t* __restrict__ didx, int ntile)
+{
+  __shared__ int dirs[256];
+  const int  seg    = blockIdx.y;
+  const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = ip < np[seg];
+  const long i      = (long)seg * g.cap + ip;
+  int        k      = active ? key[i] : 0;
+  int        d      = (active && k < 0) ? (-1 - k) : -1;
+  dirs[threadIdx.x] = d;
+  __syncthreads();
+  int sR = -1, b = -1, idx = 0;
+  if (d >= 0 && d != 13 && p.cell[i] >= 0) {
+    int rank = tilecnt[((long)seg * ntile + blockIdx.x) * 27 + d];
+    for (int t = 0; t < (int)threadIdx.x; t++)
+      rank += (dirs[t] == d);
+    const int c  = seg / g.Ns;
+    const int sp = seg - c * g.Ns;
+    const int R  = nb[c * 27 + d];
+    sR           = R * g.Ns + sp;
+    idx          = recvoff[sR * 27 + (26 - d)] + rank;
+    int cx, cy, cz;
+    cell3_of(g, p.cell[i], cx, cy, cz);
+    cx += g.has[2] ? (int)floorf(p.rx[i]) : 0;
+    cy += g.has[1] ? (int)floorf(p.ry[i]) : 0;
+    cz += g.has[0] ? (int)floorf(p.rz[i]) : 0;
+    cx += (cx < 0) * g.dims[2] - (cx >= g.dims[2]) * g.dims[2];
+    cy += (cy < 0) * g.dims[1] - (cy >= g.dims[1]) * g.dims[1];
+    cz += (cz < 0) * g.dims[0] - (cz >= g.dims[0]) * g.dims[0];
+    if (cx >= 0 && cx < g.dims[2] && cy >= 0 && cy < g.dims[1] && cz >= 0 && cz < g.dims[0])
+      b = cz * g.bsz + cy * g.bsy + cx;
+  }
+  const bool arrive = b >= 0;
+  int        sl     = wave_slot(cellcnt, arrive ? sR * g.nbin + b : 0, arrive);
+  if (!active)
+    return;
+  if (d < 0) {
+    dseg[i] = seg;
+    didx[i] = (int32_t)ip;
+    return;
+  }
+  if (!arrive) {
+    dseg[i] = -1;
+    return;
+  }
+  slot[i] = sl;
+  key[i]  = b;
+  dseg[i] = sR;
+  didx[i] = idx;
+}
+
+// rank_one_bin for the FP32 record; the record lands in bin b with its offset
+// folded back into [0, 1) and its cell set to b.
+__device__ __forceinline__ float frac01_s(float r)
+{
+  const float f = r - floorf(r);
+  return f < 1.0f ? f : 0.99999994f;
+}
+
+__global__ void __launch_bounds__(256)
+    k32_rank_bin(Geom g, P32 src, P32 dst, const int32_t* __restrict__ cellstart,
+                 const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc, long nblocks)
+{
+  const long blk = xcd_block(blockIdx.x, nblocks);
+  if (blk >= nblocks)
+    return;
+  const int  lane = threadIdx.x & 63;
+  const long w    = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= (long)g.nseg * g.nbin)
+    return;
+  const int      seg  = (int)(w / g.nbin);
+  const int      b    = (int)(w - (long)seg * g.nbin);
+  const int32_t* cs   = cellstart + (long)seg * (g.nbin + 1);
+  const int      base = cs[b];
+  const int      end  = min(cs[b + 1], (int)g.cap);
+  if (base >= end)
+    return;
+  const int      n  = end - base;
+  const int32_t* mk = mkey + (long)seg * g.cap + base;
+  const int32_t* ms = msrc + (long)seg * g.cap + base;
+  for (int c0 = 0; c0 < n; c0 += 64) {
+    const int  q     = c0 + lane;
+    const bool valid = q < n;
+    float      rx = 0, ry = 0, rz = 0, ux = 0, uy = 0, uz = 0;
+    int64_t    id = 0;
+    if (valid) {
+      const long si = ms[q];
+      rx            = src.rx[si];
+      ry            = src.ry[si];
+      rz            = src.rz[si];
+      ux            = src.ux[si];
+      uy            = src.uy[si];
+      uz            = src.uz[si];
+      id            = src.id[si];
+    }
+    const unsigned int mine = valid ? sort_key(mk[q]) : 0xFFFFFFFFu;
+    int                rank = 0;
+    for (int c1 = 0; c1 < n; c1 += 64) {
+      const int          q1    = c1 + lane;
+      const unsigned int other = q1 < n ? sort_key(mk[q1]) : 0xFFFFFFFFu;
+#pragma unroll
+      for (int j = 0; j < 64; j++) {
+        unsigned int o = (unsigned int)__builtin_amdgcn_readlane((int)other, j);
+        rank += (o < mine);
+      }
+    }
+    if (valid) {
+      const long di = (long)seg * g.cap + base + rank;
+      dst.rx[di]    = g.has[2] ? frac01_s(rx) : rx;
+      dst.ry[di]    = g.has[1] ? frac01_s(ry) : ry;
+      dst.rz[di]    = g.has[0] ? frac01_s(rz) : rz;
+      dst.ux[di]    = ux;
+      dst.uy[di]    = uy;
+      dst.uz[di]    = uz;
+      dst.id[di]    = id;
+      dst.cell[di]  = b;
+    }
+  }
+}
+
+void finish_sort32(Sim& s)
+{
+  const Geom& g = s.g;
+  hipLaunchKernelGGL(k_cell_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.ss.cellcnt,
+                     s.ss.cellstart, s.ss.npnew, s.ss.errflag);
+  dim3 grid(s.ss.ntile, g.nseg);
+  hipLaunchKernelGGL(k_members, grid, dim3(256), 0, s.stream, g, s.d_np, s.ss.cellstart,
+                     s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx, s.ss.mkey, s.ss.msrc);
+  const long nwave = (long)g.nseg * g.nbin;
+  const long nrb   = nblk(nwave, 4);
+  hipLaunchKernelGGL(k32_rank_bin, dim3(xcd_grid(nrb)), dim3(256), 0, s.stream, g, s.qa, s.qb,
+                     s.ss.cellstart, s.ss.mkey, s.ss.msrc, nrb);
+  launch_copy(s, s.d_np, s.ss.npnew, g.nseg);
+  std::swap(s.qa, s.qb);
+  s.np_host_valid = false;
+  s.counted       = false;
+  s.sorted        = true;
+}
+
 } // namespace
 
 void launch_migrate(Sim& s)
@@ -359,6 +551,31 @@ void launch_sort(Sim& s)
   PhaseTimer pt(s, 6);
   launch_count(s);
   finish_sort(s);
+}
+
+void launch32_migrate(Sim& s)
+{
+  PhaseTimer  pt(s, 6);
+  const Geom& g = s.g;
+  hipLaunchKernelGGL(k_tile_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.d_np,
+                     s.ss.tilecnt, s.ss.nout, s.ss.ntile);
+  hipLaunchKernelGGL(k_recv_offsets, dim3(nblk(g.nseg, 256)), dim3(256), 0, s.stream, g, s.d_np,
+                     s.d_nb, s.ss.nout, s.ss.recvoff);
+  dim3 grid(s.ss.ntile, g.nseg);
+  hipLaunchKernelGGL(k32_migrate, grid, dim3(256), 0, s.stream, g, s.qa, s.d_np, s.d_nb,
+                     s.ss.tilecnt, s.ss.recvoff, s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg,
+                     s.ss.didx, s.ss.ntile);
+  finish_sort32(s);
+}
+
+void launch32_sort(Sim& s)
+{
+  PhaseTimer  pt(s, 6);
+  const Geom& g = s.g;
+  launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
+  hipLaunchKernelGGL(k32_count, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa,
+                     s.d_np, s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx);
+  finish_sort32(s);
 }
 
 } // namespace picnix

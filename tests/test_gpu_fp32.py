@@ -1,0 +1,160 @@
+"""The FP32 form (precision = "fp32") against the FP64 oracle.
+
+FP32 cell-relative particles and FP32 fields are compared with the FP64 oracle
+on the same seeded inputs; the bars are FP32 error bounds, stated per quantity:
+  * after the loader and the initial sort: every particle present with the same
+    id; positions within 1e-6 of a cell, momenta within 1e-6 relative (FP32
+    rounding of the FP64 loader's values);
+  * after 1 and 10 steps: particle positions within 1e-5 of a cell and momenta
+    within 1e-4 relative; E, B and J within 1e-4 relative (max |diff| / max |ref|);
+  * field and kinetic energies within 1e-4 relative after 10 steps;
+  * div(E) - rho: the FP32 charge-conservation residual, summed over all cells,
+    below 1e-4 of the summed |rho| after 10 steps.
+Particles are matched by id (for odd orders the FP32 form bins by cell, the
+reference by the half-shifted grid, so the orders differ).
+"""
+import copy
+import sys
+
+import numpy as np
+import pytest
+
+import helpers
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _fp32(cfg):
+    cfg = copy.deepcopy(cfg)
+    cfg["application"]["option"]["precision"] = "fp32"
+    return cfg
+
+
+def _sim(cfg):
+    import picnix_amd
+    return picnix_amd.Simulation(helpers.dumps(cfg))
+
+
+def _by_id(xu):
+    ids = xu[:, 6].view(np.int64)
+    order = np.argsort(ids)
+    return ids[order], xu[order]
+
+
+def _particle_errors(o, g, dh):
+    dx_err, du_err = 0.0, 0.0
+    for c in range(o.nchunks):
+        for s in range(o.Ns):
+            ia, a = _by_id(o.particles(c, s))
+            ib, b = _by_id(g.particles(c, s))
+            assert np.array_equal(ia, ib), f"chunk {c} species {s}: particle sets differ"
+            if a.shape[0] == 0:
+                continue
+            dx_err = max(dx_err, float(np.max(np.abs(a[:, 0:3] - b[:, 0:3]))) / dh)
+            scale = max(float(np.max(np.abs(a[:, 3:6]))), 1e-30)
+            du_err = max(du_err, float(np.max(np.abs(a[:, 3:6] - b[:, 3:6]))) / scale)
+    return dx_err, du_err
+
+
+def _field_errors(o, g, dim):
+    worst, scale = np.zeros(10), np.zeros(10)
+    for c in range(o.nchunks):
+        info = o.chunk_info(c)
+        fa = helpers.interior(o.field(c, 0), info, dim)
+        fb = helpers.interior(g.field(c, 0), info, dim)
+        ja = helpers.interior(o.field(c, 1), info, dim)
+        jb = helpers.interior(g.field(c, 1), info, dim)
+        for k in range(6):
+            worst[k] = max(worst[k], np.max(np.abs(fa[..., k] - fb[..., k])))
+            scale[k] = max(scale[k], np.max(np.abs(fa[..., k])))
+        for k in range(4):
+            worst[6 + k] = max(worst[6 + k], np.max(np.abs(ja[..., k] - jb[..., k])))
+            scale[6 + k] = max(scale[6 + k], np.max(np.abs(ja[..., k])))
+    return np.where(scale > 0, worst / np.where(scale > 0, scale, 1), worst)
+
+
+CASES = {
+    "thermal3d_o2": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=2)),
+    "thermal3d_o1": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=1)),
+    "thermal3d_o4": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=4, order=4)),
+    "twostream2d_o1": (2, lambda: helpers.small_twostream(N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1)),
+}
+
+
+@pytest.mark.parametrize("kind", list(CASES))
+def test_fp32_against_fp64_oracle(kind):
+    dim, make = CASES[kind]
+    cfg = make()
+    dt = float(cfg["parameter"]["delt"])
+    dh = float(cfg["parameter"]["delh"])
+    _say(f"[fp32] {kind}: create")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(_fp32(cfg))
+    dx, du = _particle_errors(o, g, dh)
+    _say(f"[fp32] {kind}: loader position err {dx:.2e} cells, momentum {du:.2e}")
+    assert dx <= 1e-6 and du <= 1e-6
+    for nstep in (1, 9):
+        for _ in range(nstep):
+            o.step()
+            g.step(dt)
+        g.synchronize()
+        dx, du = _particle_errors(o, g, dh)
+        rel = _field_errors(o, g, dim)
+        _say(f"[fp32] {kind}: after {'1' if nstep == 1 else '10'} steps: position {dx:.2e} cells, "
+             f"momentum {du:.2e}, fields {np.array2string(rel, precision=2)}")
+        assert dx <= 1e-5 and du <= 1e-4
+        assert np.all(rel <= 1e-4)
+    ho, hg = o.history(), g.history()
+    _say(f"[fp32] {kind}: history oracle {ho} fp32 {hg}")
+    assert np.allclose(hg[2:], ho[2:], rtol=1e-4, atol=1e-12)
+    # charge-conservation residual, against the total charge density scale
+    rho_abs = 0.0
+    for c in range(o.nchunks):
+        rho_abs += np.abs(helpers.interior(o.field(c, 1), o.chunk_info(c), dim)[..., 0]).sum()
+    assert abs(hg[0] - ho[0]) <= 1e-4 * max(rho_abs, 1.0)
+    g.close()
+    o.close()
+
+
+def test_fp32_graph_and_operators():
+    """FP32 steps replayed from hipGraphs and an operator-by-operator step in
+    between agree with eager FP32 steps on the same inputs."""
+    dim, make = CASES["thermal3d_o2"]
+    cfg = _fp32(make())
+    dt = float(cfg["parameter"]["delt"])
+    a = _sim(cfg)
+    b = _sim(cfg)
+    b.step_graph(True)
+    for it in range(5):
+        a.step(dt)
+        if it == 2:
+            b.push_bfd(0.5 * dt)
+            b.push_velocity(dt)
+            b.push_position(dt)
+            b.deposit_current(dt)
+            b.push_bfd(0.5 * dt)
+            b.exchange(1)
+            b.push_efd(dt)
+            b.exchange(0)
+            b.exchange(3)
+        else:
+            b.step(dt)
+    a.synchronize()
+    b.synchronize()
+    ha, hb = a.history(), b.history()
+    _say(f"[fp32] graph: eager {ha} graph {hb}")
+    assert np.allclose(ha[2:], hb[2:], rtol=1e-5, atol=1e-12)
+    for c in range(a.nchunks):
+        for s in range(a.Ns):
+            ia, pa = _by_id(a.particles(c, s))
+            ib, pb = _by_id(b.particles(c, s))
+            assert np.array_equal(ia, ib)
+            assert np.max(np.abs(pa[:, :6] - pb[:, :6])) <= 1e-5
+    a.close()
+    b.close()
