@@ -914,6 +914,233 @@ __global__ void __launch_bounds__(256)
       }
 }
 
+// rho after the sort (3D, even orders): the FP64 k_rho_march walk in FP32
+// registers with an FP64 LDS ring tile.  Sorted FP32 particles carry
+// r in [0, 1) and sit in their cell's bin, so every particle of a bin has the
+// bin's stencil origin.
+template <int Order>
+struct Rho32 {
+  static constexpr int S     = Order + 1;
+  static constexpr int PPS   = 64 / S >= 32 ? 32 : (64 / S >= 16 ? 16 : 8);
+  static constexpr int BATCH = 64;
+  static constexpr int NWAVE = 4;
+  static constexpr size_t WLDS = sizeof(float) * NWAVE * BATCH * 3 * S;
+};
+
+template <int Order>
+__global__ void __launch_bounds__(256)
+    k32_rho_march(Geom g, P32 p, const int32_t* __restrict__ cellstart,
+                  const int* __restrict__ off, float* __restrict__ J, int zrows)
+{
+  using RP            = Rho32<Order>;
+  constexpr int S     = RP::S;
+  constexpr int PPS   = RP::PPS;
+  constexpr int BATCH = RP::BATCH;
+  constexpr int NWAVE = RP::NWAVE;
+  static_assert(Order % 2 == 0, "the FP32 rho march kernel takes even orders");
+
+  __shared__ float wts[NWAVE][BATCH][3 * S];
+  __shared__ int   next_cell;
+  extern __shared__ double tile[]; // [S z-ring][S][TX] FP64, then rcs
+
+  const int c   = blockIdx.y;
+  const int ns  = g.Ns;
+  const int nbx = g.bsy;
+  const int nby = g.bsz / g.bsy;
+  const int nbz = g.nbin / g.bsz;
+  const int by  = blockIdx.x % nby;
+  const int bz0 = (blockIdx.x / nby) * zrows;
+  const int bz1 = min(nbz, bz0 + zrows);
+
+  int any = 0;
+  for (int q = threadIdx.x; q < ns * (bz1 - bz0); q += blockDim.x) {
+    const int      sp = q % ns;
+    const int      bz = bz0 + q / ns;
+    const int32_t* cs = cellstart + (long)(c * ns + sp) * (g.nbin + 1);
+    const int      b0 = bz * g.bsz + by * g.bsy;
+    any |= cs[b0 + nbx] > cs[b0] ? 1 : 0;
+  }
+  if (!__syncthreads_or(any))
+    return;
+
+  const int lb  = g.Nb;
+  const int ylo = by + lb - (Order / 2);
+  const int xlo = lb - (Order / 2);
+  const int TX  = nbx + Order;
+  const int psz = S * TX;
+  const int tsz = S * psz;
+  int*      rcs = reinterpret_cast<int*>(tile + tsz);
+  for (int q = threadIdx.x; q < tsz; q += blockDim.x)
+    tile[q] = 0.0;
+  auto load_row = [&](int bz) {
+    for (int q = threadIdx.x; q < ns * (nbx + 1); q += blockDim.x) {
+      const int sp = q / (nbx + 1);
+      const int i  = q - sp * (nbx + 1);
+      rcs[q] = cellstart[(long)(c * ns + sp) * (g.nbin + 1) + bz * g.bsz + by * g.bsy + i];
+    }
+  };
+  load_row(bz0);
+  if (threadIdx.x == 0)
+    next_cell = 0;
+  __syncthreads();
+
+  const int  lane = threadIdx.x & 63;
+  const int  wv   = threadIdx.x >> 6;
+  const bool act  = lane < S * PPS;
+  const int  t    = act ? lane / PPS : 0; // z plane
+  const int  k    = lane & (PPS - 1);
+  float*     W    = &wts[wv][0][0];
+
+  auto claim = [&](int& sp, int& bx, int& p0, int& p1) {
+    for (;;) {
+      int q = 0;
+      if (lane == 0)
+        q = atomicAdd(&next_cell, 1);
+      q = __builtin_amdgcn_readfirstlane(q);
+      if (q >= ns * nbx)
+        return false;
+      sp = q / nbx;
+      bx = q - sp * nbx;
+      p0 = rcs[sp * (nbx + 1) + bx];
+      p1 = rcs[sp * (nbx + 1) + bx + 1];
+      if (p1 > p0)
+        return true;
+    }
+  };
+  auto fetch = [&](int seg, int pb, int p1, float* x3) {
+    if (pb + lane < p1) {
+      const long i = (long)seg * g.cap + pb + lane;
+      x3[0]        = p.rx[i];
+      x3[1]        = p.ry[i];
+      x3[2]        = p.rz[i];
+    }
+  };
+
+  for (int bz = bz0; bz < bz1; bz++) {
+    const int zlo = bz + lb - (Order / 2);
+    const int zs  = ((zlo % S) + S) % S;
+    int       sp, bx, p0, p1;
+    bool      have   = claim(sp, bx, p0, p1);
+    float     nx3[3] = {0, 0, 0};
+    if (have)
+      fetch(c * ns + sp, p0, p1, nx3);
+    while (have) {
+      int         spn, bxn, p0n, p1n;
+      const bool  more = claim(spn, bxn, p0n, p1n);
+      const int   seg  = c * ns + sp;
+      const float qs   = (float)g.q[sp];
+      float       acc[S][S];
+#pragma unroll
+      for (int a = 0; a < S; a++)
+#pragma unroll
+        for (int e = 0; e < S; e++)
+          acc[a][e] = 0.0f;
+      for (int pb = p0; pb < p1; pb += BATCH) {
+        const int   nbt   = min(BATCH, p1 - pb);
+        const float r3[3] = {nx3[0], nx3[1], nx3[2]};
+        if (pb + BATCH < p1)
+          fetch(seg, pb + BATCH, p1, nx3);
+        else if (more)
+          fetch(c * ns + spn, p0n, p1n, nx3);
+        if (lane < nbt) {
+          float* rec = W + lane * 3 * S;
+#pragma unroll
+          for (int a = 0; a < 3; a++) {
+            int fst;
+            shape_mc<Order, float>(int_delta<Order>(r3[a], fst), 0.0f, 1.0f, &rec[a * S]);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k0 = 0; k0 < nbt; k0 += PPS) {
+          const int kk = k0 + k;
+          if (act && kk < nbt) {
+            const float* rec = W + kk * 3 * S;
+            const float  sz  = rec[2 * S + t];
+            float        sx[S];
+#pragma unroll
+            for (int jx = 0; jx < S; jx++)
+              sx[jx] = rec[jx];
+#pragma unroll
+            for (int jy = 0; jy < S; jy++) {
+              const float sy = rec[S + jy];
+#pragma unroll
+              for (int jx = 0; jx < S; jx++)
+                acc[jy][jx] = fmaf(qs * sx[jx] * sy, sz, acc[jy][jx]);
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+#pragma unroll
+      for (int a = 0; a < S; a++)
+#pragma unroll
+        for (int e = 0; e < S; e++) {
+          float x = acc[a][e];
+          x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, true));
+          x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, true));
+#pragma unroll
+          for (int m = 4; m < PPS; m <<= 1)
+            x += __shfl_xor(x, m);
+          acc[a][e] = x;
+        }
+      if (act && k == 0) {
+        const int sl = zs + t >= S ? zs + t - S : zs + t;
+#pragma unroll
+        for (int jy = 0; jy < S; jy++)
+#pragma unroll
+          for (int jx = 0; jx < S; jx++)
+            atomicAdd(&tile[sl * psz + jy * TX + bx + jx], (double)acc[jy][jx]);
+      }
+      have = more;
+      sp   = spn;
+      bx   = bxn;
+      p0   = p0n;
+      p1   = p1n;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < psz; q += blockDim.x) {
+      const int    idx = zs * psz + q;
+      const double val = tile[idx];
+      if (val != 0.0) {
+        const int ty = q / TX;
+        const int tx = q - ty * TX;
+        const int gz = off[c * 3 + 0] + zlo, gy = off[c * 3 + 1] + ylo + ty,
+                  gx = off[c * 3 + 2] + xlo + tx;
+        if ((unsigned)gz < (unsigned)g.P[0] && (unsigned)gy < (unsigned)g.P[1] &&
+            (unsigned)gx < (unsigned)g.P[2])
+          atomicAdd(&J[((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
+        tile[idx] = 0.0;
+      }
+    }
+    if (bz + 1 < bz1)
+      load_row(bz + 1);
+    if (threadIdx.x == 0)
+      next_cell = 0;
+    __syncthreads();
+  }
+  const int zl = (bz1 - 1) + lb - (Order / 2);
+  const int zs = ((zl % S) + S) % S;
+  for (int q = threadIdx.x; q < (S - 1) * psz; q += blockDim.x) {
+    const int    dz  = 1 + q / psz;
+    const int    r2  = q - (dz - 1) * psz;
+    const int    sl  = zs + dz >= S ? zs + dz - S : zs + dz;
+    const double val = tile[sl * psz + r2];
+    if (val == 0.0)
+      continue;
+    const int ty = r2 / TX;
+    const int tx = r2 - ty * TX;
+    const int gz = off[c * 3 + 0] + zl + dz, gy = off[c * 3 + 1] + ylo + ty,
+              gx = off[c * 3 + 2] + xlo + tx;
+    if ((unsigned)gz < (unsigned)g.P[0] && (unsigned)gy < (unsigned)g.P[1] &&
+        (unsigned)gx < (unsigned)g.P[2])
+      atomicAdd(&J[((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
+  }
+}
+
 void march32_split(const Geom& g, int fixed, int& zrows, int& nzr)
 {
   const int  nby   = g.bsz / g.bsy;
@@ -1100,7 +1327,25 @@ void launch32_rho(Sim& s)
 {
   PhaseTimer  pt(s, 7);
   const Geom& g = s.g;
-  const dim3  grid(nblk(g.cap, 256), g.nseg);
+  if (g.dim == 3 && s.sorted && !s.generic_kernels && (g.order == 2 || g.order == 4)) {
+    const int    S    = g.order + 1;
+    const size_t mlds = sizeof(double) * (size_t)S * S * (g.bsy + g.order) +
+                        sizeof(int) * (size_t)g.Ns * (g.bsy + 1);
+    const size_t wlds = g.order == 2 ? Rho32<2>::WLDS : Rho32<4>::WLDS;
+    if (mlds + wlds + 64 <= 65536) {
+      int zrows, nzr;
+      march32_split(g, s.march_rows, zrows, nzr);
+      const dim3 mgrid(nzr * (g.bsz / g.bsy), g.nchunk);
+      if (g.order == 2)
+        hipLaunchKernelGGL(k32_rho_march<2>, mgrid, dim3(256), mlds, s.stream, g, s.qa,
+                           s.ss.cellstart, s.d_off, s.f_J, zrows);
+      else
+        hipLaunchKernelGGL(k32_rho_march<4>, mgrid, dim3(256), mlds, s.stream, g, s.qa,
+                           s.ss.cellstart, s.d_off, s.f_J, zrows);
+      return;
+    }
+  }
+  const dim3 grid(nblk(g.cap, 256), g.nseg);
 #define PN32_RHO(D, O)                                                                        \
   hipLaunchKernelGGL((k32_rho_global<D, O>), grid, dim3(256), 0, s.stream, g, s.qa, s.d_np,    \
                      s.d_off, s.f_J)
