@@ -240,7 +240,7 @@ void allocate(Sim& s)
   ss.slot         = dalloc<int32_t>(np);
   ss.dseg         = dalloc<int32_t>(np);
   ss.didx         = dalloc<int32_t>(np);
-  ss.mkey         = dalloc<int32_t>(np);
+  ss.mkey         = dalloc<int32_t>(np + kMemberPad);
   ss.msrc         = dalloc<int32_t>(np);
   ss.tilecnt      = dalloc<int32_t>((size_t)g.nseg * ss.ntile * 27);
   ss.nout         = dalloc<int32_t>((size_t)g.nseg * 27);

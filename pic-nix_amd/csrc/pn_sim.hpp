@@ -12,6 +12,10 @@
 namespace picnix {
 
 // Per-step scratch of the counting sort and particle migration.
+// spare entries after the member list: the rank kernels read up to 15 keys past
+// the end of a bin in 16-key groups
+constexpr int kMemberPad = 16;
+
 struct SortScratch {
   int32_t* cellcnt   = nullptr; // [nseg][nbin]   particles per bin (atomic slots)
   int32_t* cellstart = nullptr; // [nseg][nbin+1] first particle of every bin
@@ -19,7 +23,7 @@ struct SortScratch {
   int32_t* slot      = nullptr; // [nseg*cap]     arbitrary slot inside that bin
   int32_t* dseg      = nullptr; // [nseg*cap]     destination segment (-1 = dropped)
   int32_t* didx      = nullptr; // [nseg*cap]     index in the destination's pre-sort array
-  int32_t* mkey      = nullptr; // [nseg*cap]     member list of every bin (pre-sort indices)
+  int32_t* mkey      = nullptr; // [nseg*cap+pad] member list of every bin (pre-sort indices)
   int32_t* msrc      = nullptr; // [nseg*cap]     where each member's record currently is
   int32_t* tilecnt   = nullptr; // [nseg][ntile][27] leavers per 256-particle tile and direction
   int32_t* nout      = nullptr; // [nseg][27]     leavers per direction

@@ -211,8 +211,14 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Member lists: every bin receives the pre-sort index of each of its particles
-// (the sort key) and where the particle's record currently lives.
+// Total order of the members of one bin: (idx mod 8, idx).
+__device__ __forceinline__ unsigned int sort_key(int idx)
+{
+  return ((unsigned int)(idx & (kSimdWidth - 1)) << 28) | ((unsigned int)idx >> 3);
+}
+
+// Member lists: every bin receives the sort key of each of its particles (from
+// its pre-sort index) and where the particle's record currently lives.
 __global__ void __launch_bounds__(256)
     k_members(Geom g, const int32_t* __restrict__ np, const int32_t* __restrict__ cellstart,
               const int32_t* __restrict__ key, const int32_t* __restrict__ slot,
@@ -229,20 +235,95 @@ __global__ void __launch_bounds__(256)
     return;
   long pos = (long)cellstart[(long)ds * (g.nbin + 1) + key[i]] + slot[i];
   if (pos < g.cap) {
-    mkey[(long)ds * g.cap + pos] = didx[i];
+    mkey[(long)ds * g.cap + pos] = (int32_t)sort_key(didx[i]);
     msrc[(long)ds * g.cap + pos] = (int32_t)i;
   }
 }
 
-// Total order of the members of one bin: (idx mod 8, idx).
-__device__ __forceinline__ unsigned int sort_key(int idx)
+
+// The member list seen through the constant address space: its entries are
+// read-only while the rank kernels run, so wave-uniform reads of it become
+// scalar loads.
+typedef const __attribute__((address_space(4))) int32_t* member_ptr;
+
+// Ranks of the lane's members (keys m0, and m1 when two) among all n members
+// of a bin.  The other members' keys are wave-uniform: they are read 16 at a
+// time with scalar loads and compared with the lane's keys (one compare and one
+// add per key and member).  The last group reads up to 15 entries past n and
+// ignores them (the member list is allocated with kMemberPad spare entries).
+template <bool Two>
+__device__ __forceinline__ void bin_ranks(const int32_t* mk, int n, unsigned int m0,
+                                          unsigned int m1, int& r0, int& r1)
 {
-  return ((unsigned int)(idx & (kSimdWidth - 1)) << 28) | ((unsigned int)idx >> 3);
+  const member_ptr ck = (member_ptr)mk;
+  r0                  = 0;
+  r1                  = 0;
+  int c1              = 0;
+  for (; c1 + 16 <= n; c1 += 16) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const unsigned int o = (unsigned int)ck[c1 + j];
+      r0 += o < m0;
+      if constexpr (Two)
+        r1 += o < m1;
+    }
+  }
+  if (c1 < n) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const unsigned int o = c1 + j < n ? (unsigned int)ck[c1 + j] : 0xFFFFFFFFu;
+      r0 += o < m0;
+      if constexpr (Two)
+        r1 += o < m1;
+    }
+  }
+}
+
+// Ranks of members c0 + lane and c0 + 64 + lane (keys m0, m1; valid below n).
+__device__ __forceinline__ void pair_ranks(const int32_t* mk, int n, int c0, unsigned int m0,
+                                           unsigned int m1, int& r0, int& r1)
+{
+  if (c0 + 64 < n)
+    bin_ranks<true>(mk, n, m0, m1, r0, r1);
+  else
+    bin_ranks<false>(mk, n, m0, m1, r0, r1);
+}
+
+// The lane's two member keys, loaded before the records so that the ranking
+// waits for them only (vector loads complete in order).
+__device__ __forceinline__ void lane_keys(const int32_t* mk, int n, int c0, int lane,
+                                          unsigned int& m0, unsigned int& m1)
+{
+  const int q0 = c0 + lane, q1 = q0 + 64;
+  m0           = q0 < n ? (unsigned int)mk[q0] : 0xFFFFFFFFu;
+  m1           = q1 < n ? (unsigned int)mk[q1] : 0xFFFFFFFFu;
+}
+
+struct Rec64 {
+  double  x, y, z, ux, uy, uz;
+  int64_t id;
+};
+
+__device__ __forceinline__ Rec64 load_rec(const PArr& p, long i)
+{
+  return Rec64{p.x[i], p.y[i], p.z[i], p.ux[i], p.uy[i], p.uz[i], p.id[i]};
+}
+
+__device__ __forceinline__ void store_rec(const PArr& p, long i, const Rec64& r)
+{
+  p.x[i]  = r.x;
+  p.y[i]  = r.y;
+  p.z[i]  = r.z;
+  p.ux[i] = r.ux;
+  p.uy[i] = r.uy;
+  p.uz[i] = r.uz;
+  p.id[i] = r.id;
 }
 
 // Rank the members of bin w (segment-major flat index) by (idx mod 8, idx) --
-// the order of XtensorParticle::sort -- by comparing every pair in registers,
-// then move each record to start(bin) + rank.  One wavefront; all lanes call.
+// the order of XtensorParticle::sort -- and move each record to
+// start(bin) + rank.  One wavefront, two members per lane per pass; all lanes
+// call.
 __device__ __forceinline__ void rank_one_bin(const Geom& g, const PArr& src, const PArr& dst,
                                              const int32_t* __restrict__ cellstart,
                                              const int32_t* __restrict__ mkey,
@@ -258,43 +339,23 @@ __device__ __forceinline__ void rank_one_bin(const Geom& g, const PArr& src, con
   const int      n  = end - base;
   const int32_t* mk = mkey + (long)seg * g.cap + base;
   const int32_t* ms = msrc + (long)seg * g.cap + base;
-  for (int c0 = 0; c0 < n; c0 += 64) {
-    const int  q     = c0 + lane;
-    const bool valid = q < n;
-    // the record is loaded first, so that its latency overlaps the ranking
-    double  rx = 0, ry = 0, rz = 0, rux = 0, ruy = 0, ruz = 0;
-    int64_t rid = 0;
-    if (valid) {
-      const long si = ms[q];
-      rx            = src.x[si];
-      ry            = src.y[si];
-      rz            = src.z[si];
-      rux           = src.ux[si];
-      ruy           = src.uy[si];
-      ruz           = src.uz[si];
-      rid           = src.id[si];
-    }
-    const unsigned int mine = valid ? sort_key(mk[q]) : 0xFFFFFFFFu;
-    int                rank = 0;
-    for (int c1 = 0; c1 < n; c1 += 64) {
-      const int          q1    = c1 + lane;
-      const unsigned int other = q1 < n ? sort_key(mk[q1]) : 0xFFFFFFFFu;
-#pragma unroll
-      for (int j = 0; j < 64; j++) {
-        unsigned int o = (unsigned int)__builtin_amdgcn_readlane((int)other, j);
-        rank += (o < mine);
-      }
-    }
-    if (valid) {
-      const long di = (long)seg * g.cap + base + rank;
-      dst.x[di]     = rx;
-      dst.y[di]     = ry;
-      dst.z[di]     = rz;
-      dst.ux[di]    = rux;
-      dst.uy[di]    = ruy;
-      dst.uz[di]    = ruz;
-      dst.id[di]    = rid;
-    }
+  for (int c0 = 0; c0 < n; c0 += 128) {
+    const int    q0 = c0 + lane, q1 = q0 + 64;
+    unsigned int m0, m1;
+    lane_keys(mk, n, c0, lane, m0, m1);
+    // the records are loaded before the ranking, which overlaps their latency
+    Rec64 a{}, c{};
+    if (q0 < n)
+      a = load_rec(src, ms[q0]);
+    if (q1 < n)
+      c = load_rec(src, ms[q1]);
+    int r0, r1;
+    pair_ranks(mk, n, c0, m0, m1, r0, r1);
+    const long d0 = (long)seg * g.cap + base;
+    if (q0 < n)
+      store_rec(dst, d0 + r0, a);
+    if (q1 < n)
+      store_rec(dst, d0 + r1, c);
   }
 }
 
@@ -314,7 +375,10 @@ __global__ void __launch_bounds__(256)
   const int  lane  = threadIdx.x & 63;
   const long nbins = (long)g.nseg * g.nbin;
   const long wend  = min(nbins, (blk + 1) * kRankBins);
-  for (long w = blk * kRankBins + (threadIdx.x >> 6); w < wend; w += blockDim.x >> 6)
+  // the wavefront index, marked wave-uniform so that the bin's member list is
+  // read with scalar loads
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (long w = blk * kRankBins + wv; w < wend; w += blockDim.x >> 6)
     rank_one_bin(g, src, dst, cellstart, mkey, msrc, w, lane);
 }
 
@@ -449,6 +513,29 @@ __device__ __forceinline__ float frac01_s(float r)
   return f < 1.0f ? f : 0.99999994f;
 }
 
+struct Rec32 {
+  float   rx, ry, rz, ux, uy, uz;
+  int64_t id;
+};
+
+__device__ __forceinline__ Rec32 load_rec32(const P32& p, long i)
+{
+  return Rec32{p.rx[i], p.ry[i], p.rz[i], p.ux[i], p.uy[i], p.uz[i], p.id[i]};
+}
+
+__device__ __forceinline__ void store_rec32(const Geom& g, const P32& p, long i, const Rec32& r,
+                                            int b)
+{
+  p.rx[i]   = g.has[2] ? frac01_s(r.rx) : r.rx;
+  p.ry[i]   = g.has[1] ? frac01_s(r.ry) : r.ry;
+  p.rz[i]   = g.has[0] ? frac01_s(r.rz) : r.rz;
+  p.ux[i]   = r.ux;
+  p.uy[i]   = r.uy;
+  p.uz[i]   = r.uz;
+  p.id[i]   = r.id;
+  p.cell[i] = b;
+}
+
 __global__ void __launch_bounds__(256)
     k32_rank_bin(Geom g, P32 src, P32 dst, const int32_t* __restrict__ cellstart,
                  const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc, long nblocks)
@@ -457,7 +544,7 @@ __global__ void __launch_bounds__(256)
   if (blk >= nblocks)
     return;
   const int  lane = threadIdx.x & 63;
-  const long w    = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long w    = blk * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (w >= (long)g.nseg * g.nbin)
     return;
   const int      seg  = (int)(w / g.nbin);
@@ -470,43 +557,22 @@ __global__ void __launch_bounds__(256)
   const int      n  = end - base;
   const int32_t* mk = mkey + (long)seg * g.cap + base;
   const int32_t* ms = msrc + (long)seg * g.cap + base;
-  for (int c0 = 0; c0 < n; c0 += 64) {
-    const int  q     = c0 + lane;
-    const bool valid = q < n;
-    float      rx = 0, ry = 0, rz = 0, ux = 0, uy = 0, uz = 0;
-    int64_t    id = 0;
-    if (valid) {
-      const long si = ms[q];
-      rx            = src.rx[si];
-      ry            = src.ry[si];
-      rz            = src.rz[si];
-      ux            = src.ux[si];
-      uy            = src.uy[si];
-      uz            = src.uz[si];
-      id            = src.id[si];
-    }
-    const unsigned int mine = valid ? sort_key(mk[q]) : 0xFFFFFFFFu;
-    int                rank = 0;
-    for (int c1 = 0; c1 < n; c1 += 64) {
-      const int          q1    = c1 + lane;
-      const unsigned int other = q1 < n ? sort_key(mk[q1]) : 0xFFFFFFFFu;
-#pragma unroll
-      for (int j = 0; j < 64; j++) {
-        unsigned int o = (unsigned int)__builtin_amdgcn_readlane((int)other, j);
-        rank += (o < mine);
-      }
-    }
-    if (valid) {
-      const long di = (long)seg * g.cap + base + rank;
-      dst.rx[di]    = g.has[2] ? frac01_s(rx) : rx;
-      dst.ry[di]    = g.has[1] ? frac01_s(ry) : ry;
-      dst.rz[di]    = g.has[0] ? frac01_s(rz) : rz;
-      dst.ux[di]    = ux;
-      dst.uy[di]    = uy;
-      dst.uz[di]    = uz;
-      dst.id[di]    = id;
-      dst.cell[di]  = b;
-    }
+  for (int c0 = 0; c0 < n; c0 += 128) {
+    const int    q0 = c0 + lane, q1 = q0 + 64;
+    unsigned int m0, m1;
+    lane_keys(mk, n, c0, lane, m0, m1);
+    Rec32 a{}, c{};
+    if (q0 < n)
+      a = load_rec32(src, ms[q0]);
+    if (q1 < n)
+      c = load_rec32(src, ms[q1]);
+    int r0, r1;
+    pair_ranks(mk, n, c0, m0, m1, r0, r1);
+    const long d0 = (long)seg * g.cap + base;
+    if (q0 < n)
+      store_rec32(g, dst, d0 + r0, a, b);
+    if (q1 < n)
+      store_rec32(g, dst, d0 + r1, c, b);
   }
 }
 
