@@ -588,6 +588,7 @@ int pn_get_particles(pn_sim* h, int chunk, int species, double* xu7, int64_t cap
 int pn_set_particles(pn_sim* h, int chunk, int species, const double* xu7, int64_t np)
 {
   return guarded(h, [&](Sim& s) {
+    s.rho_valid = false; // particles move or J is written outside pn_step
     int rc = check_seg(s, chunk, species);
     if (rc)
       return rc;
@@ -708,6 +709,7 @@ int pn_get_fields(pn_sim* h, int chunk, int which, double* data)
 int pn_set_fields(pn_sim* h, int chunk, int which, const double* data)
 {
   return guarded(h, [&](Sim& s) {
+    s.rho_valid = false; // particles move or J is written outside pn_step
     if (chunk < 0 || chunk >= s.g.nchunk || !data || (which != PN_FIELD_UF && which != PN_FIELD_UJ))
       return fail(s, PN_ERR_ARG, "invalid chunk, field selector or buffer");
     const Geom& g  = s.g;
@@ -769,6 +771,7 @@ int pn_push_velocity(pn_sim* h, int chunk, double delt)
 int pn_push_position(pn_sim* h, int chunk, double delt)
 {
   return guarded(h, [&](Sim& s) {
+    s.rho_valid = false; // particles move or J is written outside pn_step
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
@@ -787,11 +790,12 @@ int pn_push_position(pn_sim* h, int chunk, double delt)
 int pn_deposit_current(pn_sim* h, int chunk, double delt)
 {
   return guarded(h, [&](Sim& s) {
+    s.rho_valid = false; // particles move or J is written outside pn_step
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
     if (s.fp32)
-      launch32_deposit(s, delt, false);
+      launch32_deposit(s, delt, RhoMode::Now);
     else
       launch_deposit(s, delt);
     return PN_OK;
@@ -824,6 +828,7 @@ int pn_deposit_moment(pn_sim* h, int chunk)
 int pn_sort_particle(pn_sim* h, int chunk)
 {
   return guarded(h, [&](Sim& s) {
+    s.rho_valid = false; // particles move or J is written outside pn_step
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
@@ -849,9 +854,11 @@ int pn_exchange(pn_sim* h, int mode)
       }
       return PN_OK;
     case PN_BOUNDARY_CUR:
+      s.rho_valid = false;
       launch_fold_current(s);
       return PN_OK;
     case PN_BOUNDARY_PARTICLE:
+      s.rho_valid = false;
       if (!s.counted)
         return fail(s, PN_ERR_STATE, "particle exchange needs the count of pn_push_position");
       if (s.fp32)
@@ -878,6 +885,11 @@ int pn_exchange(pn_sim* h, int mode)
 // that follows every field update).
 namespace {
 
+// rho: after a step, rho is the deposit of the current positions, so the next
+// step can carry it by the continuity equation the Esirkepov current satisfies
+// (launch_rho_continuity) instead of depositing it again; the first step after
+// anything else moved particles or wrote J deposits it (after the sort, where
+// the new positions are grouped by cell).
 void step_launches32(Sim& s, double delt)
 {
   launch_push_bfd(s, 0.5 * delt);
@@ -885,13 +897,18 @@ void step_launches32(Sim& s, double delt)
   bool was_sorted = s.sorted;
   launch32_push_position(s, delt);
   s.sorted = was_sorted;
-  launch32_deposit(s, delt, true);
+  const RhoDone rho = launch32_deposit(s, delt, s.rho_valid ? RhoMode::Keep : RhoMode::Defer);
   launch_push_bfd(s, 0.5 * delt);
   launch_fold_current(s, 1, 4);
+  if (rho == RhoDone::Kept)
+    launch_rho_continuity(s, delt);
   launch_push_efd(s, delt);
   launch32_migrate(s);
-  launch32_rho(s);
-  launch_fold_current(s, 0, 1);
+  if (rho == RhoDone::Deferred) {
+    launch32_rho(s);
+    launch_fold_current(s, 0, 1);
+  }
+  s.rho_valid = true;
 }
 
 void step_launches(Sim& s, double delt)
@@ -905,17 +922,18 @@ void step_launches(Sim& s, double delt)
   bool was_sorted = s.sorted;
   launch_push_position(s, delt);
   s.sorted = was_sorted;
-  // rho is deposited after the sort when the 3D march kernel runs: the new
-  // positions are then grouped by cell, and the result is the same sum
-  const bool rho_later = launch_deposit(s, delt, true);
+  const RhoDone rho = launch_deposit(s, delt, s.rho_valid ? RhoMode::Keep : RhoMode::Defer);
   launch_push_bfd(s, 0.5 * delt);
-  launch_fold_current(s, rho_later ? 1 : 0, 4);
+  launch_fold_current(s, rho == RhoDone::Deposited ? 0 : 1, 4);
+  if (rho == RhoDone::Kept)
+    launch_rho_continuity(s, delt);
   launch_push_efd(s, delt);
   launch_migrate(s);
-  if (rho_later) {
+  if (rho == RhoDone::Deferred) {
     launch_rho(s, true);
     launch_fold_current(s, 0, 1);
   }
+  s.rho_valid = true;
 }
 
 void drop_graphs(Sim& s)
@@ -934,7 +952,8 @@ int pn_step(pn_sim* h, double delt)
   return guarded(h, [&](Sim& s) {
     // a replayed graph needs the state the capture saw: sorted particles and
     // no phase timing (event records are host-side bookkeeping)
-    if (!s.step_graph || !s.sorted || s.timing) {
+    // (and rho carried from the previous step: the recording uses the continuity update)
+    if (!s.step_graph || !s.sorted || s.timing || !s.rho_valid) {
       step_launches(s, delt);
       return PN_OK;
     }

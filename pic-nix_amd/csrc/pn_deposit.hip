@@ -1075,6 +1075,26 @@ void march_split(const Geom& g, int fixed, int& zrows, int& nzr)
   nzr              = (nbz + zrows - 1) / zrows;
 }
 
+// The 3D march kernel runs (sorted particles, its LDS fits); it deposits J only.
+template <int Order>
+bool march_fits(const Sim& s)
+{
+  const Geom& g = s.g;
+  const int   N = Order + 3;
+  return g.dim == 3 && s.sorted && !s.generic_kernels &&
+         march_lds(g, 3 * N * N * (g.bsy + N - 1)) + MarchPlan<Order>::WLDS + 64 <= 65536;
+}
+
+bool deposit_marches(const Sim& s)
+{
+  switch (s.g.order) {
+  case 1: return march_fits<1>(s);
+  case 2: return march_fits<2>(s);
+  case 3: return march_fits<3>(s);
+  default: return march_fits<4>(s);
+  }
+}
+
 // Returns true when rho still has to be deposited (3D march path).
 template <int Dim, int Order>
 bool deposit_impl(Sim& s, const DepConsts& dc)
@@ -1091,8 +1111,8 @@ bool deposit_impl(Sim& s, const DepConsts& dc)
     lds   = sizeof(double) * 4 * (size_t)N * TX;
     nslab = g.dims[1] + 1;
   }
-  const size_t mlds = march_lds(g, 3 * N * N * (g.bsy + N - 1));
-  if (Dim == 3 && s.sorted && !s.generic_kernels && mlds + MarchPlan<Order>::WLDS + 64 <= 65536) {
+  if (Dim == 3 && march_fits<Order>(s)) {
+    const size_t mlds = march_lds(g, 3 * N * N * (g.bsy + N - 1));
     int zrows, nzr;
     march_split(g, s.march_rows, zrows, nzr);
     hipLaunchKernelGGL((k_deposit_march<Order>), dim3(nzr * (g.bsz / g.bsy), g.nchunk), dim3(256),
@@ -1132,7 +1152,7 @@ void rho_impl(Sim& s, bool sorted_by_new_cell)
 
 } // namespace
 
-bool launch_deposit(Sim& s, double delt, bool defer_rho)
+RhoDone launch_deposit(Sim& s, double delt, RhoMode mode)
 {
   const Geom& g = s.g;
   DepConsts   dc{};
@@ -1142,14 +1162,19 @@ bool launch_deposit(Sim& s, double delt, bool defer_rho)
   dc.dxdt = g.del[2] / delt;
   dc.dydt = g.del[1] / delt;
   dc.dzdt = g.del[0] / delt;
+  // the 3D march kernel deposits J only; the others deposit rho with J
+  const bool keep = mode == RhoMode::Keep && deposit_marches(s);
   {
     PhaseTimer pz(s, 4);
-    launch_zero(s, s.d_J, 4 * g.vol);
+    if (keep)
+      launch_zero(s, s.d_J + g.vol, 3 * g.vol);
+    else
+      launch_zero(s, s.d_J, 4 * g.vol);
   }
-  bool need_rho;
+  bool deferred;
   {
     PhaseTimer pt(s, 3);
-#define PN_DEP(D, O) need_rho = deposit_impl<D, O>(s, dc)
+#define PN_DEP(D, O) deferred = deposit_impl<D, O>(s, dc)
     if (g.dim == 3) {
       switch (g.order) {
       case 1: PN_DEP(3, 1); break;
@@ -1167,11 +1192,15 @@ bool launch_deposit(Sim& s, double delt, bool defer_rho)
     }
 #undef PN_DEP
   }
-  if (need_rho && !defer_rho) {
+  if (!deferred)
+    return RhoDone::Deposited;
+  if (keep)
+    return RhoDone::Kept;
+  if (mode == RhoMode::Now) {
     launch_rho(s, false);
-    need_rho = false;
+    return RhoDone::Deposited;
   }
-  return need_rho;
+  return RhoDone::Deferred;
 }
 
 void launch_rho(Sim& s, bool sorted_by_new_cell)

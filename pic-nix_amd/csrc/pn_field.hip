@@ -295,6 +295,51 @@ void push_efd_impl(Sim& s, T* E, const T* B, const T* J, T* ff, double delt)
   hipLaunchKernelGGL(k_fill_ghost<T>, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g, E, 3);
 }
 
+// rho^{n+1} from the continuity equation that the Esirkepov current satisfies
+// node by node (nix/esirkepov.hpp:76-337), written with the discrete
+// divergence of BaseMaxwell::get_diverror_{2,3}d (pic/engine/maxwell.hpp:
+// 203-229,350-381):
+//   rho -= dt [(Jx(x+1) - Jx(x))/dx + (Jy(y+1) - Jy(y))/dy + (Jz(z+1) - Jz(z))/dz]
+// on the interior of the periodic domain (upper neighbours wrap), after the
+// current fold.  div(E) - rho is then carried exactly as E is updated.
+template <int Dim, typename T>
+__global__ void __launch_bounds__(256) k_rho_continuity(Geom g, T* __restrict__ J, T rdx, T rdy,
+                                                        T rdz, T delt)
+{
+  long n   = (long)g.N[0] * g.N[1] * g.N[2];
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n)
+    return;
+  int       x  = (int)(tid % g.N[2]) + g.pad[2];
+  long      r  = tid / g.N[2];
+  int       y  = (int)(r % g.N[1]) + g.pad[1];
+  int       z  = (int)(r / g.N[1]) + g.pad[0];
+  const int xp = wrap(x + 1, g.pad[2], g.N[2]);
+  const int yp = wrap(y + 1, g.pad[1], g.N[1]);
+  const long v = g.vol;
+  const long i = gidx(g, z, y, x);
+  T div = (J[v + gidx(g, z, y, xp)] - J[v + i]) * rdx + (J[2 * v + gidx(g, z, yp, x)] - J[2 * v + i]) * rdy;
+  if constexpr (Dim == 3) {
+    const int zp = wrap(z + 1, g.pad[0], g.N[0]);
+    div += (J[3 * v + gidx(g, zp, y, x)] - J[3 * v + i]) * rdz;
+  }
+  J[i] -= delt * div;
+}
+
+template <typename T>
+void rho_continuity_impl(Sim& s, T* J, double delt)
+{
+  const Geom& g = s.g;
+  const long  n = (long)g.N[0] * g.N[1] * g.N[2];
+  const T     rdx = (T)(1 / g.del[2]), rdy = (T)(1 / g.del[1]), rdz = (T)(1 / g.del[0]);
+  if (g.dim == 3)
+    hipLaunchKernelGGL((k_rho_continuity<3, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, J,
+                       rdx, rdy, rdz, (T)delt);
+  else
+    hipLaunchKernelGGL((k_rho_continuity<2, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, J,
+                       rdx, rdy, rdz, (T)delt);
+}
+
 template <typename T>
 void fold_impl(Sim& s, T* F, int k0, int k1)
 {
@@ -360,6 +405,15 @@ void launch_fold(Sim& s, float* F, int k0, int k1, int phase)
 {
   PhaseTimer pt(s, phase);
   fold_impl<float>(s, F, k0, k1);
+}
+
+void launch_rho_continuity(Sim& s, double delt)
+{
+  PhaseTimer pt(s, 7);
+  if (s.fp32)
+    rho_continuity_impl<float>(s, s.f_J, delt);
+  else
+    rho_continuity_impl<double>(s, s.d_J, delt);
 }
 
 void launch_fold_current(Sim& s, int k0, int k1)

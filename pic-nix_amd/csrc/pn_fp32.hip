@@ -1257,7 +1257,8 @@ void launch32_push_position(Sim& s, double delt)
   s.sorted  = false;
 }
 
-bool launch32_deposit(Sim& s, double delt, bool defer_rho)
+// The FP32 kernels deposit J only; rho follows as the mode asks.
+RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
 {
   const Geom& g = s.g;
   Dep32       dc{};
@@ -1269,7 +1270,10 @@ bool launch32_deposit(Sim& s, double delt, bool defer_rho)
   dc.dzdt = (float)(g.del[0] / delt);
   {
     PhaseTimer pz(s, 4);
-    launch_zero(s, s.f_J, 4 * g.vol);
+    if (mode == RhoMode::Keep)
+      launch_zero(s, s.f_J + g.vol, 3 * g.vol);
+    else
+      launch_zero(s, s.f_J, 4 * g.vol);
   }
   bool march = false;
   {
@@ -1316,11 +1320,13 @@ bool launch32_deposit(Sim& s, double delt, bool defer_rho)
     }
   }
   (void)march;
-  if (!defer_rho) {
+  if (mode == RhoMode::Keep)
+    return RhoDone::Kept;
+  if (mode == RhoMode::Now) {
     launch32_rho(s);
-    return false;
+    return RhoDone::Deposited;
   }
-  return true;
+  return RhoDone::Deferred;
 }
 
 void launch32_rho(Sim& s)

@@ -80,6 +80,9 @@ struct Sim {
   SortScratch ss{};
   bool        counted = false; // keys hold a fresh count (push_position / count)
   bool        sorted  = false; // particles are in sorted order and cellstart is valid
+  // rho (component 0 of J) is the deposit of the current positions, left by
+  // pn_step; any other call that moves particles or writes J clears it
+  bool        rho_valid = false;
   bool        generic_kernels = false; // PICNIX_KERNELS=generic: one-thread-per-particle forms
   int         march_rows      = 0;     // PICNIX_MARCH_ROWS: bin rows per march block (0: auto)
 
@@ -140,9 +143,16 @@ void launch_push_position(Sim& s, double delt);
 void launch_count(Sim& s);
 void launch_migrate(Sim& s);
 void launch_sort(Sim& s);
-// Esirkepov current.  With defer_rho the 3D plane path leaves rho (component 0)
-// untouched and returns true: the caller deposits it later with launch_rho.
-bool launch_deposit(Sim& s, double delt, bool defer_rho = false);
+// What the Esirkepov deposit does with rho (component 0 of J):
+//   Now   -- deposits it from the new positions, as the reference does;
+//   Defer -- where the kernel allows, leaves it for launch_rho after the sort;
+//   Keep  -- where the kernel allows, keeps rho^n for launch_rho_continuity.
+// The result says what happened.
+enum class RhoMode { Now, Defer, Keep };
+enum class RhoDone { Deposited, Deferred, Kept };
+RhoDone launch_deposit(Sim& s, double delt, RhoMode mode = RhoMode::Now);
+// rho^{n+1} = rho^n - dt div J, after the current fold
+void launch_rho_continuity(Sim& s, double delt);
 // rho from the current positions; sorted_by_new_cell: cellstart describes them
 void launch_rho(Sim& s, bool sorted_by_new_cell);
 void launch_energy(Sim& s, double* out /* host: efd, bfd, divE, divB, particle[Ns] */);
@@ -154,7 +164,7 @@ void launch32_fields_to_f64(Sim& s, int which);            // f_* -> d_* mirror
 void launch32_fields_from_f64(Sim& s, int which);          // d_* mirror -> f_*
 void launch32_push_velocity(Sim& s, double delt);
 void launch32_push_position(Sim& s, double delt);
-bool launch32_deposit(Sim& s, double delt, bool defer_rho);
+RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode);
 void launch32_rho(Sim& s);
 void launch32_migrate(Sim& s);
 void launch32_sort(Sim& s);

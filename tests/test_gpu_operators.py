@@ -299,3 +299,36 @@ def test_moment_deposit(kind):
     assert rel <= TOL
     g.close()
     o.close()
+
+
+def _rho_interior(g, dim):
+    return [helpers.interior(g.field(c, 1), g.chunk_info(c), dim)[..., 0].copy()
+            for c in range(g.nchunks)]
+
+
+@pytest.mark.parametrize("kind", ["thermal3d_o2", "thermal3d_o1", "twostream2d_o1"])
+def test_rho_carried_by_continuity(kind):
+    """After its first step pn_step carries rho by rho -= dt div J (the
+    continuity equation the Esirkepov current satisfies node by node) instead of
+    depositing it again.  After 20 steps a direct deposit from the same
+    positions (position push with dt = 0, then deposit_current + Cur exchange)
+    gives the same rho to round-off."""
+    import picnix_amd
+    dim, make = CASES[kind]
+    cfg = make()
+    dt = float(cfg["parameter"]["delt"])
+    _say(f"[ops] {kind} rho continuity: create")
+    g = _sim(cfg)
+    for _ in range(20):
+        g.step(dt)
+    g.synchronize()
+    carried = _rho_interior(g, dim)
+    g.push_position(0.0)
+    g.deposit_current(dt)
+    g.exchange(picnix_amd.BOUNDARY_CUR)
+    direct = _rho_interior(g, dim)
+    worst = max(float(np.max(np.abs(a - b))) for a, b in zip(carried, direct))
+    scale = max(float(np.max(np.abs(b))) for b in direct)
+    _say(f"[ops] {kind} rho continuity: max |carried - direct| {worst:.3e}, max |rho| {scale:.3e}")
+    assert worst <= TOL * scale
+    g.close()
