@@ -1146,7 +1146,7 @@ void march32_split(const Geom& g, int fixed, int& zrows, int& nzr)
   const int  nby   = g.bsz / g.bsy;
   const int  nbz   = g.nbin / g.bsz;
   const long cols  = (long)nby * g.nchunk;
-  const int  split = (int)std::min<long>(nbz, std::max<long>(1, (2048 + cols - 1) / cols));
+  const int  split = (int)std::min<long>(nbz, std::max<long>(1, (4096 + cols - 1) / cols));
   zrows            = fixed > 0 ? std::min(fixed, nbz) : (nbz + split - 1) / split;
   nzr              = (nbz + zrows - 1) / zrows;
 }
