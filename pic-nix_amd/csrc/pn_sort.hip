@@ -505,8 +505,8 @@ __global__ void __launch_bounds__(256)
   didx[i] = idx;
 }
 
-// rank_one_bin for the FP32 record; the record lands in bin b with its offset
-// folded back into [0, 1) and its cell set to b.
+// An FP32 offset folded back into [0, 1) (the largest float below 1 when the
+// subtraction rounds up to 1).
 __device__ __forceinline__ float frac01_s(float r)
 {
   const float f = r - floorf(r);
@@ -536,44 +536,29 @@ __device__ __forceinline__ void store_rec32(const Geom& g, const P32& p, long i,
   p.cell[i] = b;
 }
 
+// The FP32 form's sort: every record goes straight to start(bin) + the slot
+// its count handed out, with its offset folded back into [0, 1) and its cell
+// set to the bin.  One pass, no member lists and no ranking: inside a bin the
+// order is the order of the slot atomics, not the reference's (idx mod 8, idx)
+// (the FP32 form is compared with the reference particle by particle, by id).
 __global__ void __launch_bounds__(256)
-    k32_rank_bin(Geom g, P32 src, P32 dst, const int32_t* __restrict__ cellstart,
-                 const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc, long nblocks)
+    k32_scatter(Geom g, P32 src, P32 dst, const int32_t* __restrict__ np,
+                const int32_t* __restrict__ cellstart, const int32_t* __restrict__ key,
+                const int32_t* __restrict__ slot, const int32_t* __restrict__ dseg)
 {
-  const long blk = xcd_block(blockIdx.x, nblocks);
-  if (blk >= nblocks)
+  const int  seg = blockIdx.y;
+  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
     return;
-  const int  lane = threadIdx.x & 63;
-  const long w    = blk * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (w >= (long)g.nseg * g.nbin)
+  const long i  = (long)seg * g.cap + ip;
+  const int  ds = dseg[i];
+  if (ds < 0)
     return;
-  const int      seg  = (int)(w / g.nbin);
-  const int      b    = (int)(w - (long)seg * g.nbin);
-  const int32_t* cs   = cellstart + (long)seg * (g.nbin + 1);
-  const int      base = cs[b];
-  const int      end  = min(cs[b + 1], (int)g.cap);
-  if (base >= end)
-    return;
-  const int      n  = end - base;
-  const int32_t* mk = mkey + (long)seg * g.cap + base;
-  const int32_t* ms = msrc + (long)seg * g.cap + base;
-  for (int c0 = 0; c0 < n; c0 += 128) {
-    const int    q0 = c0 + lane, q1 = q0 + 64;
-    unsigned int m0, m1;
-    lane_keys(mk, n, c0, lane, m0, m1);
-    Rec32 a{}, c{};
-    if (q0 < n)
-      a = load_rec32(src, ms[q0]);
-    if (q1 < n)
-      c = load_rec32(src, ms[q1]);
-    int r0, r1;
-    pair_ranks(mk, n, c0, m0, m1, r0, r1);
-    const long d0 = (long)seg * g.cap + base;
-    if (q0 < n)
-      store_rec32(g, dst, d0 + r0, a, b);
-    if (q1 < n)
-      store_rec32(g, dst, d0 + r1, c, b);
-  }
+  const int  b   = key[i];
+  const long pos = (long)cellstart[(long)ds * (g.nbin + 1) + b] + slot[i];
+  if (pos >= g.cap)
+    return; // overflow, flagged by k_cell_scan
+  store_rec32(g, dst, (long)ds * g.cap + pos, load_rec32(src, i), b);
 }
 
 void finish_sort32(Sim& s)
@@ -581,13 +566,8 @@ void finish_sort32(Sim& s)
   const Geom& g = s.g;
   hipLaunchKernelGGL(k_cell_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.ss.cellcnt,
                      s.ss.cellstart, s.ss.npnew, s.ss.errflag);
-  dim3 grid(s.ss.ntile, g.nseg);
-  hipLaunchKernelGGL(k_members, grid, dim3(256), 0, s.stream, g, s.d_np, s.ss.cellstart,
-                     s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx, s.ss.mkey, s.ss.msrc);
-  const long nwave = (long)g.nseg * g.nbin;
-  const long nrb   = nblk(nwave, 4);
-  hipLaunchKernelGGL(k32_rank_bin, dim3(xcd_grid(nrb)), dim3(256), 0, s.stream, g, s.qa, s.qb,
-                     s.ss.cellstart, s.ss.mkey, s.ss.msrc, nrb);
+  hipLaunchKernelGGL(k32_scatter, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa, s.qb,
+                     s.d_np, s.ss.cellstart, s.ss.key, s.ss.slot, s.ss.dseg);
   launch_copy(s, s.d_np, s.ss.npnew, g.nseg);
   std::swap(s.qa, s.qb);
   s.np_host_valid = false;
