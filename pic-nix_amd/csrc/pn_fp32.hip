@@ -387,25 +387,25 @@ __global__ void __launch_bounds__(256)
 // nix/xtensor_particle.hpp:324-357) in FP32
 // ---------------------------------------------------------------------------
 
-// r_old -> ro; r += u dt / (gamma d) (unwrapped, relative to the old cell);
-// stayers take a slot in their new bin, leavers record their direction and
-// the tile's per-direction totals (the FP64 form's bookkeeping).
+// r_old -> ro; r += u dt / (gamma d) (unwrapped, relative to the old cell).
+// Then the particle's destination for the sort: its own segment and new bin, or
+// for a leaver the neighbour chunk in its direction (chunk table nb) and the bin
+// there -- the periodic wrap is a shift of the cell index, so r stays as it is
+// (set_boundary_periodic, nix/xtensor_particle.hpp:359-376, and the migration,
+// nix/xtensor_halo3d.hpp:192-499).  Every particle takes a slot in its
+// destination bin here, so the sort needs no migration pass.
 __global__ void __launch_bounds__(256)
     k32_push_position(Geom g, P32 p, float* __restrict__ ro0, float* __restrict__ ro1,
                       float* __restrict__ ro2, const int32_t* __restrict__ np, float delt,
-                      float rc, float rdx, float rdy, float rdz, int32_t* __restrict__ cellcnt,
-                      int32_t* __restrict__ key, int32_t* __restrict__ slot,
-                      int32_t* __restrict__ tilecnt, int ntile)
+                      float rc, float rdx, float rdy, float rdz, const int* __restrict__ nb,
+                      int32_t* __restrict__ cellcnt, int32_t* __restrict__ key,
+                      int32_t* __restrict__ slot, int32_t* __restrict__ dseg)
 {
-  __shared__ int cnt[27];
-  if (threadIdx.x < 27)
-    cnt[threadIdx.x] = 0;
-  __syncthreads();
   const int  seg    = blockIdx.y;
   const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = ip < np[seg];
   const long i      = (long)seg * g.cap + ip;
-  int        b      = -1;
+  int        ds = -1, b = -1;
   if (active) {
     float       rx = p.rx[i], ry = p.ry[i], rz = p.rz[i];
     const float ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
@@ -423,33 +423,36 @@ __global__ void __launch_bounds__(256)
     const int bin = p.cell[i];
     if (bin >= 0) {
       const Cell3 cl = cell_of(g, bin);
-      const int   cx = cl.x + (g.has[2] ? (int)floorf(rx) : 0);
-      const int   cy = cl.y + (g.has[1] ? (int)floorf(ry) : 0);
-      const int   cz = cl.z + (g.has[0] ? (int)floorf(rz) : 0);
+      int         cx = cl.x + (g.has[2] ? (int)floorf(rx) : 0);
+      int         cy = cl.y + (g.has[1] ? (int)floorf(ry) : 0);
+      int         cz = cl.z + (g.has[0] ? (int)floorf(rz) : 0);
       const int   dx = (cx >= g.dims[2]) - (cx < 0) + 1;
       const int   dy = (cy >= g.dims[1]) - (cy < 0) + 1;
       const int   dz = (cz >= g.dims[0]) - (cz < 0) + 1;
       const int   d  = 9 * dz + 3 * dy + dx;
-      if (d == 13) {
-        b = bin_of3(g, cx, cy, cz);
-      } else {
-        key[i] = -1 - d;
-        atomicAdd(&cnt[d], 1);
+      int         R  = seg / g.Ns;
+      if (d != 13) {
+        R = nb[R * 27 + d];
+        cx += (cx < 0) * g.dims[2] - (cx >= g.dims[2]) * g.dims[2];
+        cy += (cy < 0) * g.dims[1] - (cy >= g.dims[1]) * g.dims[1];
+        cz += (cz < 0) * g.dims[0] - (cz >= g.dims[0]) * g.dims[0];
       }
-    } else {
-      key[i] = -1 - 13; // never in the chunk: dropped by the migration
-      atomicAdd(&cnt[13], 1);
+      if (R >= 0 && cx >= 0 && cx < g.dims[2] && cy >= 0 && cy < g.dims[1] && cz >= 0 &&
+          cz < g.dims[0]) {
+        ds = R * g.Ns + (seg - (seg / g.Ns) * g.Ns);
+        b  = bin_of3(g, cx, cy, cz);
+      }
     }
   }
-  const bool stay = active && b >= 0;
-  int        sl   = wave_slot(cellcnt + (long)seg * g.nbin, b, stay);
-  if (stay) {
-    slot[i] = sl;
+  const bool want = ds >= 0;
+  const int  sl   = wave_slot(cellcnt, want ? ds * g.nbin + b : 0, want);
+  if (!active)
+    return;
+  dseg[i] = ds; // -1: dropped by the sort
+  if (want) {
     key[i]  = b;
+    slot[i] = sl;
   }
-  __syncthreads();
-  if (threadIdx.x < 27)
-    tilecnt[((long)seg * ntile + blockIdx.x) * 27 + threadIdx.x] = cnt[threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -1252,7 +1255,7 @@ void launch32_push_position(Sim& s, double delt)
   hipLaunchKernelGGL(k32_push_position, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa,
                      s.f_ro[0], s.f_ro[1], s.f_ro[2], s.d_np, (float)delt, (float)(1 / g.cc),
                      (float)(1 / g.del[2]), (float)(1 / g.del[1]), (float)(1 / g.del[0]),
-                     s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.tilecnt, s.ss.ntile);
+                     s.d_nb, s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg);
   s.counted = true;
   s.sorted  = false;
 }

@@ -415,8 +415,7 @@ __device__ __forceinline__ void cell3_of(const Geom& g, int bin, int& x, int& y,
 // (or never placed in it) -> dropped by the sort.
 __global__ void __launch_bounds__(256)
     k32_count(Geom g, P32 p, const int32_t* __restrict__ np, int32_t* __restrict__ cellcnt,
-              int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ dseg,
-              int32_t* __restrict__ didx)
+              int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ dseg)
 {
   const int  seg    = blockIdx.y;
   const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -440,69 +439,10 @@ __global__ void __launch_bounds__(256)
     slot[i] = sl;
     key[i]  = b;
     dseg[i] = seg;
-    didx[i] = (int32_t)ip;
   } else {
     key[i]  = -1;
     dseg[i] = -1;
   }
-}
-
-// Destinations for the FP32 form (k_migrate's bookkeeping): a leaver keeps r and
-// takes the neighbour's cell index, cell + floor(r) -/+ the chunk extent.
-__global__ void __launch_bounds__(256)
-    k32_migrate(Geom g, P32 p, const int32_t* __restrict__ np, const int* __restrict__ nb,
-                const int32_t* __restrict__ tilecnt, const int32_t* __restrict__ recvoff,
-                int32_t* __restrict__ cellcnt, int32_t* __restrict__ key,
-                int32_t* __restrict__ slot, int32_t* __restrict__ dseg,
-                int32_t* __restrict__ didx, int ntile)
-{
-  __shared__ int dirs[256];
-  const int  seg    = blockIdx.y;
-  const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = ip < np[seg];
-  const long i      = (long)seg * g.cap + ip;
-  int        k      = active ? key[i] : 0;
-  int        d      = (active && k < 0) ? (-1 - k) : -1;
-  dirs[threadIdx.x] = d;
-  __syncthreads();
-  int sR = -1, b = -1, idx = 0;
-  if (d >= 0 && d != 13 && p.cell[i] >= 0) {
-    int rank = tilecnt[((long)seg * ntile + blockIdx.x) * 27 + d];
-    for (int t = 0; t < (int)threadIdx.x; t++)
-      rank += (dirs[t] == d);
-    const int c  = seg / g.Ns;
-    const int sp = seg - c * g.Ns;
-    const int R  = nb[c * 27 + d];
-    sR           = R * g.Ns + sp;
-    idx          = recvoff[sR * 27 + (26 - d)] + rank;
-    int cx, cy, cz;
-    cell3_of(g, p.cell[i], cx, cy, cz);
-    cx += g.has[2] ? (int)floorf(p.rx[i]) : 0;
-    cy += g.has[1] ? (int)floorf(p.ry[i]) : 0;
-    cz += g.has[0] ? (int)floorf(p.rz[i]) : 0;
-    cx += (cx < 0) * g.dims[2] - (cx >= g.dims[2]) * g.dims[2];
-    cy += (cy < 0) * g.dims[1] - (cy >= g.dims[1]) * g.dims[1];
-    cz += (cz < 0) * g.dims[0] - (cz >= g.dims[0]) * g.dims[0];
-    if (cx >= 0 && cx < g.dims[2] && cy >= 0 && cy < g.dims[1] && cz >= 0 && cz < g.dims[0])
-      b = cz * g.bsz + cy * g.bsy + cx;
-  }
-  const bool arrive = b >= 0;
-  int        sl     = wave_slot(cellcnt, arrive ? sR * g.nbin + b : 0, arrive);
-  if (!active)
-    return;
-  if (d < 0) {
-    dseg[i] = seg;
-    didx[i] = (int32_t)ip;
-    return;
-  }
-  if (!arrive) {
-    dseg[i] = -1;
-    return;
-  }
-  slot[i] = sl;
-  key[i]  = b;
-  dseg[i] = sR;
-  didx[i] = idx;
 }
 
 // An FP32 offset folded back into [0, 1) (the largest float below 1 when the
@@ -599,18 +539,12 @@ void launch_sort(Sim& s)
   finish_sort(s);
 }
 
+// The FP32 position push already gave every particle its destination segment,
+// bin and slot, leavers included, so the migration + sort is the bin scan and
+// the record scatter.
 void launch32_migrate(Sim& s)
 {
-  PhaseTimer  pt(s, 6);
-  const Geom& g = s.g;
-  hipLaunchKernelGGL(k_tile_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.d_np,
-                     s.ss.tilecnt, s.ss.nout, s.ss.ntile);
-  hipLaunchKernelGGL(k_recv_offsets, dim3(nblk(g.nseg, 256)), dim3(256), 0, s.stream, g, s.d_np,
-                     s.d_nb, s.ss.nout, s.ss.recvoff);
-  dim3 grid(s.ss.ntile, g.nseg);
-  hipLaunchKernelGGL(k32_migrate, grid, dim3(256), 0, s.stream, g, s.qa, s.d_np, s.d_nb,
-                     s.ss.tilecnt, s.ss.recvoff, s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg,
-                     s.ss.didx, s.ss.ntile);
+  PhaseTimer pt(s, 6);
   finish_sort32(s);
 }
 
@@ -620,7 +554,7 @@ void launch32_sort(Sim& s)
   const Geom& g = s.g;
   launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
   hipLaunchKernelGGL(k32_count, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa,
-                     s.d_np, s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx);
+                     s.d_np, s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg);
   finish_sort32(s);
 }
 
