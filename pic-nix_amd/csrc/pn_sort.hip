@@ -223,10 +223,16 @@ __global__ void __launch_bounds__(256)
     k_members(Geom g, const int32_t* __restrict__ np, const int32_t* __restrict__ cellstart,
               const int32_t* __restrict__ key, const int32_t* __restrict__ slot,
               const int32_t* __restrict__ dseg, const int32_t* __restrict__ didx,
-              int32_t* __restrict__ mkey, int32_t* __restrict__ msrc)
+              int32_t* __restrict__ mkey, int32_t* __restrict__ msrc, int ntile)
 {
-  const int  seg = blockIdx.y;
-  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // consecutive tiles on one XCD: their members land in neighbouring bins, so
+  // the partially written lines of the member lists meet in one L2
+  const long nblocks = (long)g.nseg * ntile;
+  const long blk     = xcd_block(blockIdx.x, nblocks);
+  if (blk >= nblocks)
+    return;
+  const int  seg = (int)(blk / ntile);
+  const long ip  = (blk - (long)seg * ntile) * blockDim.x + threadIdx.x;
   if (ip >= np[seg])
     return;
   const long i  = (long)seg * g.cap + ip;
@@ -387,9 +393,9 @@ void finish_sort(Sim& s)
   const Geom& g = s.g;
   hipLaunchKernelGGL(k_cell_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.ss.cellcnt,
                      s.ss.cellstart, s.ss.npnew, s.ss.errflag);
-  dim3 grid(s.ss.ntile, g.nseg);
-  hipLaunchKernelGGL(k_members, grid, dim3(256), 0, s.stream, g, s.d_np, s.ss.cellstart,
-                     s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx, s.ss.mkey, s.ss.msrc);
+  hipLaunchKernelGGL(k_members, dim3(xcd_grid((long)g.nseg * s.ss.ntile)), dim3(256), 0, s.stream,
+                     g, s.d_np, s.ss.cellstart, s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx,
+                     s.ss.mkey, s.ss.msrc, s.ss.ntile);
   const long nwave = (long)g.nseg * g.nbin;
   const long nrb   = nblk(nwave, kRankBins);
   hipLaunchKernelGGL(k_rank_bin, dim3(xcd_grid(nrb)), dim3(256), 0, s.stream, g, s.pa, s.pb,
@@ -484,10 +490,16 @@ __device__ __forceinline__ void store_rec32(const Geom& g, const P32& p, long i,
 __global__ void __launch_bounds__(256)
     k32_scatter(Geom g, P32 src, P32 dst, const int32_t* __restrict__ np,
                 const int32_t* __restrict__ cellstart, const int32_t* __restrict__ key,
-                const int32_t* __restrict__ slot, const int32_t* __restrict__ dseg)
+                const int32_t* __restrict__ slot, const int32_t* __restrict__ dseg, int ntile)
 {
-  const int  seg = blockIdx.y;
-  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // consecutive tiles on one XCD (as in k_members): records moving to
+  // neighbouring bins meet in one L2
+  const long nblocks = (long)g.nseg * ntile;
+  const long blk     = xcd_block(blockIdx.x, nblocks);
+  if (blk >= nblocks)
+    return;
+  const int  seg = (int)(blk / ntile);
+  const long ip  = (blk - (long)seg * ntile) * blockDim.x + threadIdx.x;
   if (ip >= np[seg])
     return;
   const long i  = (long)seg * g.cap + ip;
@@ -506,8 +518,9 @@ void finish_sort32(Sim& s)
   const Geom& g = s.g;
   hipLaunchKernelGGL(k_cell_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.ss.cellcnt,
                      s.ss.cellstart, s.ss.npnew, s.ss.errflag);
-  hipLaunchKernelGGL(k32_scatter, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa, s.qb,
-                     s.d_np, s.ss.cellstart, s.ss.key, s.ss.slot, s.ss.dseg);
+  hipLaunchKernelGGL(k32_scatter, dim3(xcd_grid((long)g.nseg * s.ss.ntile)), dim3(256), 0,
+                     s.stream, g, s.qa, s.qb, s.d_np, s.ss.cellstart, s.ss.key, s.ss.slot, s.ss.dseg,
+                     s.ss.ntile);
   launch_copy(s, s.d_np, s.ss.npnew, g.nseg);
   std::swap(s.qa, s.qb);
   s.np_host_valid = false;
