@@ -127,6 +127,24 @@ int pn_get_diverror(pn_sim* sim, double* efd, double* bfd);
 /* Block until queued device work on the handle's stream is done. */
 int pn_synchronize(pn_sim* sim);
 
+/* Snapshot outputs in the reference's on-disk format (one process, "mpiio"
+ * layout): <dir>/<step %08d>.data (raw native-endian f8, chunk after chunk in
+ * chunk-id order) + <dir>/<step %08d>.json (nixio metadata per dataset, meta
+ * record).
+ * pn_write_field_diag: FieldDiag (pic/diag/field.hpp:66-185): "uf" colocated
+ * at cell centres and block-averaged by decimate (>= 1), or the raw interior
+ * (decimate < 0); then "um" after the moment deposit + Mom exchange.  FP64 only.
+ * pn_write_particle_diag: ParticleDiag (pic/diag/particle.hpp:56-130): per
+ * species "up%02d", the records of a random fraction of every chunk's particles
+ * (std::mt19937_64(seed) shuffle, kept in index order).
+ * pn_write_profile: the application profile (nix/application.cpp:11-26 saved by
+ * nix/statehandler.hpp:44-63, e.g. <basedir>/profile.msgpack): configuration,
+ * chunk map and step/time, as msgpack. */
+int pn_write_field_diag(pn_sim* sim, const char* dir, int step, double time, int decimate);
+int pn_write_particle_diag(pn_sim* sim, const char* dir, int step, double time, double fraction,
+                           int seed);
+int pn_write_profile(pn_sim* sim, const char* path, int step, double time);
+
 /* Phase timing with HIP events recorded on the handle's stream.  No counterpart
  * in the reference beyond its per-step wall-clock "push" time
  * (PicApplication::push, pic/pic_application.cpp:196-216); it exists so that a

@@ -77,6 +77,21 @@ public:
   void set_step_graph(bool enable) { check(pn_set_step_graph(sim_, enable ? 1 : 0)); }
   void synchronize() { check(pn_synchronize(sim_)); }
 
+  // FieldDiag / ParticleDiag files and the application profile
+  void write_field_diag(const std::string& dir, int step, double time, int decimate = 1)
+  {
+    check(pn_write_field_diag(sim_, dir.c_str(), step, time, decimate));
+  }
+  void write_particle_diag(const std::string& dir, int step, double time, double fraction = 0.01,
+                           int seed = 0)
+  {
+    check(pn_write_particle_diag(sim_, dir.c_str(), step, time, fraction, seed));
+  }
+  void write_profile(const std::string& path, int step = 0, double time = 0.0)
+  {
+    check(pn_write_profile(sim_, path.c_str(), step, time));
+  }
+
   // PicChunk::get_energy / get_diverror, summed over chunks (pic/diag/history.hpp:44-61)
   void get_energy(double& efd, double& bfd, std::vector<double>& particle)
   {

@@ -5,8 +5,12 @@
 // with the command-line options -c (config), -t (tmax), -e (emax, seconds) of
 // nix::ArgParser (nix/argparser.hpp:18-23).  The 'history' diagnostic writes
 // <basedir>/history.txt in the format of HistoryDiag (pic/diag/history.hpp:
-// 75-111); the per-step elapsed time of the push is printed like the
-// reference's "push" log record (pic/pic_application.cpp:196-216).
+// 75-111); 'field' and 'particle' write FieldDiag / ParticleDiag snapshots
+// under <basedir>/<prefix>/ (pic/diag/field.hpp, pic/diag/particle.hpp); the
+// application profile goes to <basedir>/profile.msgpack before the first step
+// (nix/application.cpp:50-60,409-414).  The per-step elapsed time of the push
+// is printed like the reference's "push" log record
+// (pic/pic_application.cpp:196-216).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -24,8 +28,10 @@
 namespace {
 
 struct Diag {
-  std::string name;
+  std::string name, prefix;
   int         interval = 1, begin = 0, end = std::numeric_limits<int>::max();
+  int         decimate = 1;    // FieldDiag default (pic/diag/field.hpp:72)
+  double      fraction = 0.01; // ParticleDiag default (pic/diag/particle.hpp:87)
   bool        due(int step) const
   {
     return step >= begin && step <= end && (step - begin) % interval == 0;
@@ -90,6 +96,11 @@ int main(int argc, char** argv)
       d.begin = std::atoi(kv[p + "begin"].c_str());
     if (kv.count(p + "end"))
       d.end = std::atoi(kv[p + "end"].c_str());
+    d.prefix = kv.count(p + "prefix") ? kv[p + "prefix"] : d.name;
+    if (kv.count(p + "decimate"))
+      d.decimate = std::atoi(kv[p + "decimate"].c_str());
+    if (kv.count(p + "fraction"))
+      d.fraction = std::atof(kv[p + "fraction"].c_str());
     diags.push_back(d);
   }
 
@@ -102,9 +113,17 @@ int main(int argc, char** argv)
     auto      t_start = std::chrono::steady_clock::now();
     int       step    = 0;
     double    time    = 0.0;
+    sim.write_profile((std::filesystem::path(basedir) / "profile.msgpack").string(), step, time);
     while (time < tmax + delt) {
       for (const Diag& d : diags) {
-        if (d.name != "history" || !d.due(step))
+        if (!d.due(step))
+          continue;
+        const std::string dir = (std::filesystem::path(basedir) / d.prefix).string();
+        if (d.name == "field")
+          sim.write_field_diag(dir, step, time, d.decimate);
+        if (d.name == "particle")
+          sim.write_particle_diag(dir, step, time, d.fraction, 0);
+        if (d.name != "history")
           continue;
         double              de, db, ee, eb;
         std::vector<double> ep;

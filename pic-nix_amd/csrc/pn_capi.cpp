@@ -23,6 +23,8 @@ struct pn_sim {
   Sim s;
 };
 
+Sim* picnix::sim_state(pn_sim* h) { return h ? &h->s : nullptr; }
+
 namespace {
 
 struct HipError {
@@ -458,6 +460,7 @@ int pn_create(const char* toml, int load, pn_sim** out)
   Sim& s = h->s;
   try {
     TomlMap kv;
+    s.toml = toml;
     if (!parse_toml(toml, kv, s.err) || !build_config(kv, s.cfg, s.err)) {
       // keep the handle so that pn_last_error can report the reason
       *out = h;

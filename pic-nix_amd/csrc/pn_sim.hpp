@@ -9,6 +9,8 @@
 #include "pn_config.hpp"
 #include "pn_device.hpp"
 
+struct pn_sim; // the C-ABI handle (include/picnix.h)
+
 namespace picnix {
 
 // Per-step scratch of the counting sort and particle migration.
@@ -33,9 +35,14 @@ struct SortScratch {
   int      ntile     = 0;
 };
 
+struct Sim;
+// the state behind a C-ABI handle (for host-side modules of the library)
+Sim* sim_state(::pn_sim* h);
+
 struct Sim {
-  Config cfg;
-  Geom   g{};
+  Config      cfg;
+  std::string toml; // the configuration text given to pn_create
+  Geom        g{};
 
   // chunk tables (indexed by chunk id)
   std::vector<int>    id_of_coord; // [Cz][Cy][Cx] -> id

@@ -33,6 +33,7 @@ EXPORTS = [
     "pn_sort_particle", "pn_exchange", "pn_step",
     "pn_get_energy",
     "pn_get_diverror", "pn_synchronize", "pn_timing", "pn_timing_report",
+    "pn_write_field_diag", "pn_write_particle_diag", "pn_write_profile",
 ]
 
 PHASES = ["bfd", "velocity", "position", "deposit", "fold", "efd", "sort", "rho"]
@@ -94,6 +95,9 @@ def load_library(path=LIB_PATH):
         "pn_synchronize": [P],
         "pn_timing": [P, i],
         "pn_timing_report": [P, dp, ip],
+        "pn_write_field_diag": [P, ctypes.c_char_p, i, d, i],
+        "pn_write_particle_diag": [P, ctypes.c_char_p, i, d, d, i],
+        "pn_write_profile": [P, ctypes.c_char_p, i, d],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -263,6 +267,20 @@ class Simulation:
         e, b = ctypes.c_double(), ctypes.c_double()
         self._chk(self.L.pn_get_diverror(self.h, ctypes.byref(e), ctypes.byref(b)))
         return e.value, b.value
+
+    def write_field_diag(self, directory, step, time, decimate=1):
+        """FieldDiag files <directory>/<step>.data/.json (pn_write_field_diag)."""
+        self._chk(self.L.pn_write_field_diag(self.h, str(directory).encode(), int(step),
+                                             float(time), int(decimate)))
+
+    def write_particle_diag(self, directory, step, time, fraction=0.01, seed=0):
+        """ParticleDiag files <directory>/<step>.data/.json (pn_write_particle_diag)."""
+        self._chk(self.L.pn_write_particle_diag(self.h, str(directory).encode(), int(step),
+                                                float(time), float(fraction), int(seed)))
+
+    def write_profile(self, path, step=0, time=0.0):
+        """The application profile as msgpack (pn_write_profile)."""
+        self._chk(self.L.pn_write_profile(self.h, str(path).encode(), int(step), float(time)))
 
     def history(self):
         """[div(E)-rho, div(B), E^2/2, B^2/2, particle energy per species] (history.txt columns)."""
