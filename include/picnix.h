@@ -145,6 +145,18 @@ int pn_write_particle_diag(pn_sim* sim, const char* dir, int step, double time, 
                            int seed);
 int pn_write_profile(pn_sim* sim, const char* path, int step, double time);
 
+/* Checkpoint and restart (StateHandler::save / load, nix/statehandler.hpp:36-147,
+ * with PicChunk::pack / unpack, pic/pic_chunk.cpp:59-104): <prefix>.msgpack holds
+ * the application record (as pn_write_profile, plus the library's state flags
+ * and the block table) and <prefix>.data the raw device state: E, B, J, the
+ * Friedman history, the bin starts and every segment's particle records.  Load
+ * into a handle created from the same configuration (pn_create with load = 0
+ * or 1); the grid, chunks, [parameter] and [application.option] tables and the
+ * precision must match (PN_ERR_CONFIG otherwise, as Application::from_json,
+ * nix/application.cpp:28-48).  step/time return the saved values. */
+int pn_save_state(pn_sim* sim, const char* prefix, int step, double time);
+int pn_load_state(pn_sim* sim, const char* prefix, int* step, double* time);
+
 /* Phase timing with HIP events recorded on the handle's stream.  No counterpart
  * in the reference beyond its per-step wall-clock "push" time
  * (PicApplication::push, pic/pic_application.cpp:196-216); it exists so that a

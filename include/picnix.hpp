@@ -91,6 +91,15 @@ public:
   {
     check(pn_write_profile(sim_, path.c_str(), step, time));
   }
+  // StateHandler::save / load
+  void save_state(const std::string& prefix, int step, double time)
+  {
+    check(pn_save_state(sim_, prefix.c_str(), step, time));
+  }
+  void load_state(const std::string& prefix, int& step, double& time)
+  {
+    check(pn_load_state(sim_, prefix.c_str(), &step, &time));
+  }
 
   // PicChunk::get_energy / get_diverror, summed over chunks (pic/diag/history.hpp:44-61)
   void get_energy(double& efd, double& bfd, std::vector<double>& particle)

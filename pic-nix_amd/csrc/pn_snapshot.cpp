@@ -34,6 +34,9 @@
 
 namespace {
 
+using picnix::Geom;
+using picnix::P32;
+using picnix::PArr;
 using picnix::Sim;
 
 // ---------------------------------------------------------------------------
@@ -579,6 +582,231 @@ void pack_interior(const ChunkShape& c, const double* x, int nc, int decimate,
   out.insert(out.end(), z.begin(), z.end());
 }
 
+// ---------------------------------------------------------------------------
+// checkpoint: the device state as raw blocks (our own layout; the content of
+// PicChunk::pack, pic/pic_chunk.cpp:59-79: fields, current, Friedman history,
+// particles per species) plus the application record of StateHandler
+// (nix/statehandler.hpp:36-63, nix/application.cpp:11-48)
+// ---------------------------------------------------------------------------
+
+struct Block {
+  std::string name;
+  const void* dev; // device pointer (save) or destination (load)
+  size_t      bytes;
+};
+
+std::vector<Block> state_blocks(Sim& s, const std::vector<int32_t>& np)
+{
+  const Geom&        g   = s.g;
+  const size_t       vol = (size_t)g.vol;
+  std::vector<Block> b;
+  if (s.fp32) {
+    b.push_back({"E", s.f_E, 3 * vol * sizeof(float)});
+    b.push_back({"B", s.f_B, 3 * vol * sizeof(float)});
+    b.push_back({"J", s.f_J, 4 * vol * sizeof(float)});
+    if (s.f_ff)
+      b.push_back({"ff", s.f_ff, 9 * vol * sizeof(float)});
+  } else {
+    b.push_back({"E", s.d_E, 3 * vol * sizeof(double)});
+    b.push_back({"B", s.d_B, 3 * vol * sizeof(double)});
+    b.push_back({"J", s.d_J, 4 * vol * sizeof(double)});
+    if (s.d_ff)
+      b.push_back({"ff", s.d_ff, 9 * vol * sizeof(double)});
+  }
+  b.push_back({"cellstart", s.ss.cellstart, (size_t)g.nseg * (g.nbin + 1) * sizeof(int32_t)});
+  for (int seg = 0; seg < g.nseg; seg++) {
+    const size_t off = (size_t)seg * g.cap, n = (size_t)np[seg];
+    const std::string t = "p" + std::to_string(seg) + ".";
+    if (s.fp32) {
+      const P32& q = s.qa;
+      b.push_back({t + "rx", q.rx + off, n * sizeof(float)});
+      b.push_back({t + "ry", q.ry + off, n * sizeof(float)});
+      b.push_back({t + "rz", q.rz + off, n * sizeof(float)});
+      b.push_back({t + "ux", q.ux + off, n * sizeof(float)});
+      b.push_back({t + "uy", q.uy + off, n * sizeof(float)});
+      b.push_back({t + "uz", q.uz + off, n * sizeof(float)});
+      b.push_back({t + "id", q.id + off, n * sizeof(int64_t)});
+      b.push_back({t + "cell", q.cell + off, n * sizeof(int32_t)});
+    } else {
+      const PArr& q = s.pa;
+      b.push_back({t + "x", q.x + off, n * sizeof(double)});
+      b.push_back({t + "y", q.y + off, n * sizeof(double)});
+      b.push_back({t + "z", q.z + off, n * sizeof(double)});
+      b.push_back({t + "ux", q.ux + off, n * sizeof(double)});
+      b.push_back({t + "uy", q.uy + off, n * sizeof(double)});
+      b.push_back({t + "uz", q.uz + off, n * sizeof(double)});
+      b.push_back({t + "id", q.id + off, n * sizeof(int64_t)});
+    }
+  }
+  return b;
+}
+
+// the application record (Application::to_json) with the library's state flags
+Json app_state(Sim& s, int step, double time)
+{
+  const auto&  cfg = s.cfg;
+  const double now =
+      std::chrono::duration<double>(std::chrono::system_clock::now().time_since_epoch()).count();
+  Json state;
+  state["timestamp"]     = Json::real(now);
+  state["wclock"]        = Json::real(now);
+  state["ndims"]         = Json::ints(std::vector<int>{cfg.N[0], cfg.N[1], cfg.N[2]});
+  state["cdims"]         = Json::ints(std::vector<int>{cfg.C[0], cfg.C[1], cfg.C[2]});
+  state["curstep"]       = Json::integer(step);
+  state["curtime"]       = Json::real(time);
+  state["thisrank"]      = Json::integer(0);
+  state["nprocess"]      = Json::integer(1);
+  state["nthread"]       = Json::integer(1);
+  state["configuration"] = toml_to_json(s.toml);
+  // ChunkMap::to_json (nix/chunkmap.cpp:45-61): chunkid[z][y][x], coord[id] = (x, y, z)
+  Json& cm          = state["chunkmap"];
+  cm["size"]        = Json::integer(s.g.nchunk);
+  cm["ndim"]        = Json::integer(3);
+  cm["shape"]       = Json::ints(std::vector<int>{cfg.C[0], cfg.C[1], cfg.C[2]});
+  cm["periodicity"] = Json::ints(std::vector<int>{1, 1, 1});
+  Json ids          = Json::array();
+  for (int cz = 0; cz < cfg.C[0]; cz++) {
+    Json plane = Json::array();
+    for (int cy = 0; cy < cfg.C[1]; cy++) {
+      std::vector<int> row(cfg.C[2]);
+      for (int cx = 0; cx < cfg.C[2]; cx++)
+        row[cx] = s.id_of_coord[((size_t)cz * cfg.C[1] + cy) * cfg.C[2] + cx];
+      plane.push(Json::ints(row));
+    }
+    ids.push(plane);
+  }
+  cm["chunkid"] = ids;
+  Json coord    = Json::array();
+  for (int id = 0; id < s.g.nchunk; id++)
+    coord.push(Json::ints(
+        std::vector<int>{s.coord[id * 3 + 2], s.coord[id * 3 + 1], s.coord[id * 3 + 0]}));
+  cm["coord"]    = coord;
+  cm["boundary"] = Json::ints(std::vector<int>{0, s.g.nchunk});
+  return state;
+}
+
+bool write_msgpack(const std::string& path, const Json& j)
+{
+  std::vector<uint8_t> bytes;
+  msgpack(j, bytes);
+  std::error_code             ec;
+  const std::filesystem::path p(path);
+  if (p.has_parent_path())
+    std::filesystem::create_directories(p.parent_path(), ec);
+  std::ofstream ofs(p, std::ios::binary);
+  ofs.write(reinterpret_cast<const char*>(bytes.data()), (std::streamsize)bytes.size());
+  return (bool)ofs;
+}
+
+// a msgpack reader for what write_msgpack writes
+struct Reader {
+  const std::vector<uint8_t>& b;
+  size_t                      p = 0;
+  uint64_t                    be(int n)
+  {
+    uint64_t v = 0;
+    for (int k = 0; k < n; k++)
+      v = (v << 8) | b.at(p++);
+    return v;
+  }
+  Json read()
+  {
+    const uint8_t t = b.at(p++);
+    if (t <= 0x7f)
+      return Json::integer(t);
+    if (t >= 0xe0)
+      return Json::integer((int8_t)t);
+    if ((t & 0xe0) == 0xa0)
+      return str(t & 0x1f);
+    if ((t & 0xf0) == 0x90)
+      return arr(t & 0x0f);
+    if ((t & 0xf0) == 0x80)
+      return obj(t & 0x0f);
+    switch (t) {
+    case 0xc0: return Json();
+    case 0xc2: return Json::boolean(false);
+    case 0xc3: return Json::boolean(true);
+    case 0xcc: return Json::integer((int64_t)be(1));
+    case 0xcd: return Json::integer((int64_t)be(2));
+    case 0xce: return Json::integer((int64_t)be(4));
+    case 0xcf: return Json::integer((int64_t)be(8));
+    case 0xd0: return Json::integer((int8_t)be(1));
+    case 0xd1: return Json::integer((int16_t)be(2));
+    case 0xd2: return Json::integer((int32_t)be(4));
+    case 0xd3: return Json::integer((int64_t)be(8));
+    case 0xcb: {
+      const uint64_t bits = be(8);
+      double         f;
+      std::memcpy(&f, &bits, 8);
+      return Json::real(f);
+    }
+    case 0xd9: return str((size_t)be(1));
+    case 0xda: return str((size_t)be(2));
+    case 0xdb: return str((size_t)be(4));
+    case 0xdc: return arr((size_t)be(2));
+    case 0xdd: return arr((size_t)be(4));
+    case 0xde: return obj((size_t)be(2));
+    case 0xdf: return obj((size_t)be(4));
+    default: throw std::runtime_error("unsupported msgpack type in state file");
+    }
+  }
+  Json str(size_t n)
+  {
+    if (p + n > b.size())
+      throw std::runtime_error("truncated state file");
+    Json j = Json::string(std::string(b.begin() + p, b.begin() + p + n));
+    p += n;
+    return j;
+  }
+  Json arr(size_t n)
+  {
+    Json j = Json::array();
+    for (size_t k = 0; k < n; k++)
+      j.push(read());
+    return j;
+  }
+  Json obj(size_t n)
+  {
+    Json j;
+    j.kind = Json::Obj;
+    for (size_t k = 0; k < n; k++) {
+      Json key = read();
+      j[key.s] = read();
+    }
+    return j;
+  }
+};
+
+bool same_json(const Json& a, const Json& b)
+{
+  if (a.kind != b.kind)
+    return false;
+  switch (a.kind) {
+  case Json::Null: return true;
+  case Json::Bool: return a.b == b.b;
+  case Json::Int: return a.i == b.i;
+  case Json::Float: return a.f == b.f;
+  case Json::Str: return a.s == b.s;
+  case Json::Arr:
+    if (a.a.size() != b.a.size())
+      return false;
+    for (size_t k = 0; k < a.a.size(); k++)
+      if (!same_json(a.a[k], b.a[k]))
+        return false;
+    return true;
+  case Json::Obj:
+    if (a.o.size() != b.o.size())
+      return false;
+    for (const auto& kv : a.o) {
+      auto it = b.o.find(kv.first);
+      if (it == b.o.end() || !same_json(kv.second, it->second))
+        return false;
+    }
+    return true;
+  }
+  return false;
+}
+
 } // namespace
 
 extern "C" {
@@ -732,55 +960,133 @@ int pn_write_profile(pn_sim* h, const char* path, int step, double time)
   Sim* sp = picnix::sim_state(h);
   if (!sp || !path)
     return PN_ERR_ARG;
-  Sim&       s  = *sp;
-  const auto& cfg = s.cfg;
-  const double now =
-      std::chrono::duration<double>(std::chrono::system_clock::now().time_since_epoch()).count();
-  Json state;
-  state["timestamp"]     = Json::real(now);
-  state["wclock"]        = Json::real(now);
-  state["ndims"]         = Json::ints(std::vector<int>{cfg.N[0], cfg.N[1], cfg.N[2]});
-  state["cdims"]         = Json::ints(std::vector<int>{cfg.C[0], cfg.C[1], cfg.C[2]});
-  state["curstep"]       = Json::integer(step);
-  state["curtime"]       = Json::real(time);
-  state["thisrank"]      = Json::integer(0);
-  state["nprocess"]      = Json::integer(1);
-  state["nthread"]       = Json::integer(1);
-  state["configuration"] = toml_to_json(s.toml);
-  // ChunkMap::to_json (nix/chunkmap.cpp:45-61): chunkid[z][y][x], coord[id] = (x, y, z)
-  Json& cm          = state["chunkmap"];
-  cm["size"]        = Json::integer(s.g.nchunk);
-  cm["ndim"]        = Json::integer(3);
-  cm["shape"]       = Json::ints(std::vector<int>{cfg.C[0], cfg.C[1], cfg.C[2]});
-  cm["periodicity"] = Json::ints(std::vector<int>{1, 1, 1});
-  Json ids          = Json::array();
-  for (int cz = 0; cz < cfg.C[0]; cz++) {
-    Json plane = Json::array();
-    for (int cy = 0; cy < cfg.C[1]; cy++) {
-      std::vector<int> row(cfg.C[2]);
-      for (int cx = 0; cx < cfg.C[2]; cx++)
-        row[cx] = s.id_of_coord[((size_t)cz * cfg.C[1] + cy) * cfg.C[2] + cx];
-      plane.push(Json::ints(row));
+  if (!write_msgpack(path, app_state(*sp, step, time)))
+    return fail(*sp, PN_ERR_ARG, std::string("cannot write ") + path);
+  return PN_OK;
+}
+
+int pn_save_state(pn_sim* h, const char* prefix, int step, double time)
+{
+  Sim* sp = picnix::sim_state(h);
+  if (!sp || !prefix)
+    return PN_ERR_ARG;
+  Sim& s = *sp;
+  try {
+    if (hipStreamSynchronize(s.stream) != hipSuccess)
+      return fail(s, PN_ERR_HIP, "stream synchronisation failed before the checkpoint");
+    std::vector<int32_t> np(s.g.nseg);
+    if (hipMemcpy(np.data(), s.d_np, np.size() * sizeof(int32_t), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      return fail(s, PN_ERR_HIP, "particle counts copy failed");
+    Json state          = app_state(s, step, time);
+    Json& lib           = state["picnix_amd"];
+    lib["precision"]    = Json::string(s.fp32 ? "fp32" : "fp64");
+    lib["sorted"]       = Json::boolean(s.sorted);
+    lib["rho_valid"]    = Json::boolean(s.rho_valid);
+    lib["np"]           = Json::ints(np);
+    lib["rawfile"]      = Json::string(std::filesystem::path(std::string(prefix) + ".data").filename().string());
+    Json&        blocks = lib["blocks"];
+    blocks.kind         = Json::Obj;
+    std::ofstream ofs(std::string(prefix) + ".data", std::ios::binary);
+    if (!ofs)
+      return fail(s, PN_ERR_ARG, std::string("cannot write ") + prefix + ".data");
+    std::vector<char> host;
+    size_t            disp = 0;
+    for (const Block& bl : state_blocks(s, np)) {
+      host.resize(bl.bytes);
+      if (bl.bytes &&
+          hipMemcpy(host.data(), bl.dev, bl.bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(s, PN_ERR_HIP, "device copy failed for block " + bl.name);
+      ofs.write(host.data(), (std::streamsize)bl.bytes);
+      Json& e     = blocks[bl.name];
+      e["offset"] = Json::integer((int64_t)disp);
+      e["size"]   = Json::integer((int64_t)bl.bytes);
+      disp += bl.bytes;
     }
-    ids.push(plane);
+    if (!ofs)
+      return fail(s, PN_ERR_ARG, std::string("write failed: ") + prefix + ".data");
+    if (!write_msgpack(std::string(prefix) + ".msgpack", state))
+      return fail(s, PN_ERR_ARG, std::string("cannot write ") + prefix + ".msgpack");
+  } catch (const std::exception& e) {
+    return fail(s, PN_ERR_ARG, e.what());
   }
-  cm["chunkid"] = ids;
-  Json coord    = Json::array();
-  for (int id = 0; id < s.g.nchunk; id++)
-    coord.push(Json::ints(
-        std::vector<int>{s.coord[id * 3 + 2], s.coord[id * 3 + 1], s.coord[id * 3 + 0]}));
-  cm["coord"]    = coord;
-  cm["boundary"] = Json::ints(std::vector<int>{0, s.g.nchunk});
-  std::vector<uint8_t> bytes;
-  msgpack(state, bytes);
-  std::error_code ec;
-  const std::filesystem::path p(path);
-  if (p.has_parent_path())
-    std::filesystem::create_directories(p.parent_path(), ec);
-  std::ofstream ofs(p, std::ios::binary);
-  ofs.write(reinterpret_cast<const char*>(bytes.data()), (std::streamsize)bytes.size());
-  if (!ofs)
-    return fail(s, PN_ERR_ARG, "cannot write " + p.string());
+  return PN_OK;
+}
+
+int pn_load_state(pn_sim* h, const char* prefix, int* step, double* time)
+{
+  Sim* sp = picnix::sim_state(h);
+  if (!sp || !prefix)
+    return PN_ERR_ARG;
+  Sim& s = *sp;
+  try {
+    std::ifstream in(std::string(prefix) + ".msgpack", std::ios::binary);
+    if (!in)
+      return fail(s, PN_ERR_ARG, std::string("cannot open ") + prefix + ".msgpack");
+    std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(in)),
+                               std::istreambuf_iterator<char>());
+    Reader rd{bytes};
+    Json   state = rd.read();
+    // Application::from_json (nix/application.cpp:28-48): the same grid, chunks,
+    // process count and [parameter] table
+    Json cur = app_state(s, 0, 0.0);
+    bool ok  = same_json(cur["ndims"], state["ndims"]) && same_json(cur["cdims"], state["cdims"]) &&
+              same_json(cur["nprocess"], state["nprocess"]) &&
+              same_json(cur["configuration"]["parameter"], state["configuration"]["parameter"]);
+    Json& lib = state["picnix_amd"];
+    ok        = ok && lib["precision"].s == (s.fp32 ? "fp32" : "fp64") &&
+         same_json(cur["configuration"]["application"]["option"],
+                   state["configuration"]["application"]["option"]);
+    if (!ok)
+      return fail(s, PN_ERR_CONFIG, "trying to load an inconsistent state");
+    std::vector<int32_t> np;
+    for (const Json& v : lib["np"].a)
+      np.push_back((int32_t)v.i);
+    if ((int)np.size() != s.g.nseg)
+      return fail(s, PN_ERR_CONFIG, "state file has another segment count");
+    for (int32_t n : np)
+      if (n < 0 || n > s.g.cap)
+        return fail(s, PN_ERR_CAPACITY, "state file holds more particles than a segment's capacity");
+    std::ifstream dat(std::string(prefix) + ".data", std::ios::binary);
+    if (!dat)
+      return fail(s, PN_ERR_ARG, std::string("cannot open ") + prefix + ".data");
+    if (hipStreamSynchronize(s.stream) != hipSuccess)
+      return fail(s, PN_ERR_HIP, "stream synchronisation failed before the restart");
+    std::vector<char> host;
+    for (const Block& bl : state_blocks(s, np)) {
+      auto it = lib["blocks"].o.find(bl.name);
+      if (it == lib["blocks"].o.end() || (size_t)it->second.o.at("size").i != bl.bytes)
+        return fail(s, PN_ERR_CONFIG, "state file lacks or mis-sizes block " + bl.name);
+      host.resize(bl.bytes);
+      dat.seekg(it->second.o.at("offset").i);
+      dat.read(host.data(), (std::streamsize)bl.bytes);
+      if (!dat)
+        return fail(s, PN_ERR_ARG, "truncated state data for block " + bl.name);
+      if (bl.bytes && hipMemcpy(const_cast<void*>(bl.dev), host.data(), bl.bytes,
+                                hipMemcpyHostToDevice) != hipSuccess)
+        return fail(s, PN_ERR_HIP, "device copy failed for block " + bl.name);
+    }
+    if (hipMemcpy(s.d_np, np.data(), np.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return fail(s, PN_ERR_HIP, "particle counts copy failed");
+    s.sorted        = lib["sorted"].b;
+    s.rho_valid     = lib["rho_valid"].b;
+    s.counted       = false;
+    s.np_host_valid = false;
+    // recorded step graphs refer to the buffer roles of before
+    for (auto& ge : s.gexec)
+      if (ge) {
+        (void)hipGraphExecDestroy(ge);
+        ge = nullptr;
+      }
+    s.buf0_x = nullptr;
+    if (step)
+      *step = (int)state["curstep"].i;
+    if (time)
+      *time = state["curtime"].f;
+  } catch (const std::exception& e) {
+    return fail(s, PN_ERR_CONFIG, std::string("bad state file: ") + e.what());
+  }
   return PN_OK;
 }
 

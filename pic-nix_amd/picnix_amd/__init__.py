@@ -34,6 +34,7 @@ EXPORTS = [
     "pn_get_energy",
     "pn_get_diverror", "pn_synchronize", "pn_timing", "pn_timing_report",
     "pn_write_field_diag", "pn_write_particle_diag", "pn_write_profile",
+    "pn_save_state", "pn_load_state",
 ]
 
 PHASES = ["bfd", "velocity", "position", "deposit", "fold", "efd", "sort", "rho"]
@@ -98,6 +99,8 @@ def load_library(path=LIB_PATH):
         "pn_write_field_diag": [P, ctypes.c_char_p, i, d, i],
         "pn_write_particle_diag": [P, ctypes.c_char_p, i, d, d, i],
         "pn_write_profile": [P, ctypes.c_char_p, i, d],
+        "pn_save_state": [P, ctypes.c_char_p, i, d],
+        "pn_load_state": [P, ctypes.c_char_p, ip, dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -281,6 +284,18 @@ class Simulation:
     def write_profile(self, path, step=0, time=0.0):
         """The application profile as msgpack (pn_write_profile)."""
         self._chk(self.L.pn_write_profile(self.h, str(path).encode(), int(step), float(time)))
+
+    def save_state(self, prefix, step=0, time=0.0):
+        """Checkpoint <prefix>.msgpack + <prefix>.data (pn_save_state)."""
+        self._chk(self.L.pn_save_state(self.h, str(prefix).encode(), int(step), float(time)))
+
+    def load_state(self, prefix):
+        """Restart from a checkpoint; returns (step, time) (pn_load_state)."""
+        st = ctypes.c_int(0)
+        tm = ctypes.c_double(0.0)
+        self._chk(self.L.pn_load_state(self.h, str(prefix).encode(), ctypes.byref(st),
+                                       ctypes.byref(tm)))
+        return st.value, tm.value
 
     def history(self):
         """[div(E)-rho, div(B), E^2/2, B^2/2, particle energy per species] (history.txt columns)."""
