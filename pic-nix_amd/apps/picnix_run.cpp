@@ -3,7 +3,10 @@
 // The driver loop of nix::Application::main (nix/application.cpp:50-87):
 // while curtime < tmax + delt { diagnostic; push; curstep++; curtime += delt },
 // with the command-line options -c (config), -t (tmax), -e (emax, seconds) of
-// nix::ArgParser (nix/argparser.hpp:18-23).  The 'history' diagnostic writes
+// nix::ArgParser (nix/argparser.hpp:18-23), plus -l (load) and -s (save): a
+// restart from <basedir>/<load> and a checkpoint to <basedir>/<save> at the
+// end (Application::setup_chunks / finalize, nix/application.cpp:113-120,
+// 311-318).  The 'history' diagnostic writes
 // <basedir>/history.txt in the format of HistoryDiag (pic/diag/history.hpp:
 // 75-111); 'field' and 'particle' write FieldDiag / ParticleDiag snapshots
 // under <basedir>/<prefix>/ (pic/diag/field.hpp, pic/diag/particle.hpp); the
@@ -52,6 +55,7 @@ int main(int argc, char** argv)
   std::string config = "config.toml";
   double      tmax   = std::numeric_limits<double>::max();
   double      emax   = 3600.0;
+  std::string load, save;
   for (int i = 1; i + 1 < argc; i += 2) {
     std::string k = argv[i];
     if (k == "-c" || k == "--config")
@@ -60,8 +64,12 @@ int main(int argc, char** argv)
       tmax = std::atof(argv[i + 1]);
     else if (k == "-e" || k == "--emax")
       emax = std::atof(argv[i + 1]);
+    else if (k == "-l" || k == "--load")
+      load = argv[i + 1];
+    else if (k == "-s" || k == "--save")
+      save = argv[i + 1];
     else {
-      std::cerr << "usage: picnix_run -c config.toml [-t tmax] [-e emax]\n";
+      std::cerr << "usage: picnix_run -c config.toml [-t tmax] [-e emax] [-l load] [-s save]\n";
       return 2;
     }
   }
@@ -105,7 +113,7 @@ int main(int argc, char** argv)
   }
 
   try {
-    picnix::Simulation sim(text);
+    picnix::Simulation sim(text, load.empty());
     std::filesystem::path hist = std::filesystem::path(basedir) / "history.txt";
     if (!basedir.empty())
       std::filesystem::create_directories(basedir);
@@ -113,7 +121,11 @@ int main(int argc, char** argv)
     auto      t_start = std::chrono::steady_clock::now();
     int       step    = 0;
     double    time    = 0.0;
-    sim.write_profile((std::filesystem::path(basedir) / "profile.msgpack").string(), step, time);
+    namespace fs      = std::filesystem;
+    if (!load.empty())
+      sim.load_state((fs::path(basedir) / load).string(), step, time);
+    else // Application::save_profile, initial runs only (nix/application.cpp:409-419)
+      sim.write_profile((fs::path(basedir) / "profile.msgpack").string(), step, time);
     while (time < tmax + delt) {
       for (const Diag& d : diags) {
         if (!d.due(step))
@@ -165,6 +177,8 @@ int main(int argc, char** argv)
       if (std::chrono::duration<double>(t1 - t_start).count() > emax)
         break;
     }
+    if (!save.empty())
+      sim.save_state((fs::path(basedir) / save).string(), step, time);
   } catch (const picnix::Error& e) {
     std::cerr << "picnix error " << e.code() << ": " << e.what() << "\n";
     return 1;

@@ -101,3 +101,32 @@ def test_restart_refuses_other_configuration(tmp_path):
     assert err.value.code == -2  # PN_ERR_CONFIG
     a.close()
     b.close()
+
+
+def test_driver_save_and_load(tmp_path):
+    """picnix_run -s saves at the end of a run and -l continues from it
+    (Application::finalize / setup_chunks): a run split at step 4 writes the
+    same history.txt as one uninterrupted run."""
+    import os
+    import subprocess
+    exe = os.path.join(helpers.ROOT, "pic-nix_amd", "picnix_run")
+    cfg = helpers.small_twostream(N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1)
+    dt = float(cfg["parameter"]["delt"])
+
+    def run(base, *args):
+        c = dict(cfg)
+        c["application"] = dict(cfg["application"], basedir=str(tmp_path / base))
+        conf = tmp_path / f"{base}.toml"
+        conf.write_text(helpers.dumps(c) + "[[diagnostic]]\n  name = 'history'\n  interval = 2\n")
+        r = subprocess.run([exe, "-c", str(conf), *args], capture_output=True, text=True, timeout=120)
+        _say(f"[ckpt] driver {base} {args}: rc {r.returncode} {r.stderr[-200:]}")
+        assert r.returncode == 0
+        return (tmp_path / base / "history.txt").read_text().splitlines()
+
+    whole = run("whole", "-t", str(7 * dt))
+    run("split", "-t", str(3 * dt), "-s", "ckpt")
+    split = run("split", "-t", str(7 * dt), "-l", "ckpt")
+    assert len(whole) == len(split) == 5
+    a = np.array([[float(v) for v in ln.split()] for ln in whole[1:]])
+    b = np.array([[float(v) for v in ln.split()] for ln in split[1:]])
+    assert np.allclose(a, b, rtol=1e-6, atol=1e-12)
