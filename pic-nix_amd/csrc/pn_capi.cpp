@@ -659,15 +659,6 @@ int pn_get_gindex(pn_sim* h, int chunk, int species, int32_t* gindex, int64_t ca
     HIPCHK(hipMemcpy(gindex, s.ss.key + (size_t)seg * s.g.cap, n * sizeof(int32_t),
                      hipMemcpyDeviceToHost));
     int ng = (s.g.dims[0] + 2 * s.g.Nb) * (s.g.dims[1] + 2 * s.g.Nb) * (s.g.dims[2] + 2 * s.g.Nb);
-    if (s.fp32) {
-      // the FP32 count also gives leavers their bin in the receiving chunk
-      std::vector<int32_t> ds(n);
-      HIPCHK(hipMemcpy(ds.data(), s.ss.dseg + (size_t)seg * s.g.cap, n * sizeof(int32_t),
-                       hipMemcpyDeviceToHost));
-      for (int64_t i = 0; i < n; i++)
-        if (ds[i] != seg)
-          gindex[i] = -1;
-    }
     for (int64_t i = 0; i < n; i++)
       if (gindex[i] < 0)
         gindex[i] = ng;

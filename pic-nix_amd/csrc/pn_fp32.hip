@@ -393,19 +393,21 @@ __global__ void __launch_bounds__(256)
 // there -- the periodic wrap is a shift of the cell index, so r stays as it is
 // (set_boundary_periodic, nix/xtensor_particle.hpp:359-376, and the migration,
 // nix/xtensor_halo3d.hpp:192-499).  Every particle takes a slot in its
-// destination bin here, so the sort needs no migration pass.
+// destination bin here, so the sort needs no migration pass.  key: the new bin
+// (>= 0) for a particle that stays in its chunk, -2 - (d * nbin + bin) for one
+// that moves to the neighbour in direction d, -1 for one that is dropped.
 __global__ void __launch_bounds__(256)
     k32_push_position(Geom g, P32 p, float* __restrict__ ro0, float* __restrict__ ro1,
                       float* __restrict__ ro2, const int32_t* __restrict__ np, float delt,
                       float rc, float rdx, float rdy, float rdz, const int* __restrict__ nb,
                       int32_t* __restrict__ cellcnt, int32_t* __restrict__ key,
-                      int32_t* __restrict__ slot, int32_t* __restrict__ dseg)
+                      int32_t* __restrict__ slot)
 {
   const int  seg    = blockIdx.y;
   const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = ip < np[seg];
   const long i      = (long)seg * g.cap + ip;
-  int        ds = -1, b = -1;
+  int        ds = -1, b = -1, k = -1;
   if (active) {
     float       rx = p.rx[i], ry = p.ry[i], rz = p.rz[i];
     const float ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
@@ -441,6 +443,7 @@ __global__ void __launch_bounds__(256)
           cz < g.dims[0]) {
         ds = R * g.Ns + (seg - (seg / g.Ns) * g.Ns);
         b  = bin_of3(g, cx, cy, cz);
+        k  = d == 13 ? b : -2 - (d * g.nbin + b);
       }
     }
   }
@@ -448,11 +451,9 @@ __global__ void __launch_bounds__(256)
   const int  sl   = wave_slot(cellcnt, want ? ds * g.nbin + b : 0, want);
   if (!active)
     return;
-  dseg[i] = ds; // -1: dropped by the sort
-  if (want) {
-    key[i]  = b;
+  key[i] = k;
+  if (want)
     slot[i] = sl;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1260,7 +1261,7 @@ void launch32_push_position(Sim& s, double delt)
   hipLaunchKernelGGL(k32_push_position, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa,
                      s.f_ro[0], s.f_ro[1], s.f_ro[2], s.d_np, (float)delt, (float)(1 / g.cc),
                      (float)(1 / g.del[2]), (float)(1 / g.del[1]), (float)(1 / g.del[0]),
-                     s.d_nb, s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg);
+                     s.d_nb, s.ss.cellcnt, s.ss.key, s.ss.slot);
   s.counted = true;
   s.sorted  = false;
 }

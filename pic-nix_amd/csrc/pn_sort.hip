@@ -421,7 +421,7 @@ __device__ __forceinline__ void cell3_of(const Geom& g, int bin, int& x, int& y,
 // (or never placed in it) -> dropped by the sort.
 __global__ void __launch_bounds__(256)
     k32_count(Geom g, P32 p, const int32_t* __restrict__ np, int32_t* __restrict__ cellcnt,
-              int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ dseg)
+              int32_t* __restrict__ key, int32_t* __restrict__ slot)
 {
   const int  seg    = blockIdx.y;
   const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -444,10 +444,8 @@ __global__ void __launch_bounds__(256)
   if (stay) {
     slot[i] = sl;
     key[i]  = b;
-    dseg[i] = seg;
   } else {
-    key[i]  = -1;
-    dseg[i] = -1;
+    key[i] = -1; // dropped
   }
 }
 
@@ -490,7 +488,7 @@ __device__ __forceinline__ void store_rec32(const Geom& g, const P32& p, long i,
 __global__ void __launch_bounds__(256)
     k32_scatter(Geom g, P32 src, P32 dst, const int32_t* __restrict__ np,
                 const int32_t* __restrict__ cellstart, const int32_t* __restrict__ key,
-                const int32_t* __restrict__ slot, const int32_t* __restrict__ dseg, int ntile)
+                const int32_t* __restrict__ slot, const int* __restrict__ nb, int ntile)
 {
   // consecutive tiles on one XCD (as in k_members): records moving to
   // neighbouring bins meet in one L2
@@ -502,11 +500,20 @@ __global__ void __launch_bounds__(256)
   const long ip  = (blk - (long)seg * ntile) * blockDim.x + threadIdx.x;
   if (ip >= np[seg])
     return;
-  const long i  = (long)seg * g.cap + ip;
-  const int  ds = dseg[i];
-  if (ds < 0)
-    return;
-  const int  b   = key[i];
+  const long i = (long)seg * g.cap + ip;
+  const int  k = key[i];
+  if (k == -1)
+    return; // dropped
+  // key >= 0: the bin in this segment; else -2 - (d * nbin + bin) in the
+  // neighbour chunk in direction d
+  int ds = seg, b = k;
+  if (k < -1) {
+    const int v  = -2 - k;
+    const int d  = v / g.nbin;
+    const int c  = seg / g.Ns;
+    b            = v - d * g.nbin;
+    ds           = nb[c * 27 + d] * g.Ns + (seg - c * g.Ns);
+  }
   const long pos = (long)cellstart[(long)ds * (g.nbin + 1) + b] + slot[i];
   if (pos >= g.cap)
     return; // overflow, flagged by k_cell_scan
@@ -519,7 +526,7 @@ void finish_sort32(Sim& s)
   hipLaunchKernelGGL(k_cell_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.ss.cellcnt,
                      s.ss.cellstart, s.ss.npnew, s.ss.errflag);
   hipLaunchKernelGGL(k32_scatter, dim3(xcd_grid((long)g.nseg * s.ss.ntile)), dim3(256), 0,
-                     s.stream, g, s.qa, s.qb, s.d_np, s.ss.cellstart, s.ss.key, s.ss.slot, s.ss.dseg,
+                     s.stream, g, s.qa, s.qb, s.d_np, s.ss.cellstart, s.ss.key, s.ss.slot, s.d_nb,
                      s.ss.ntile);
   launch_copy(s, s.d_np, s.ss.npnew, g.nseg);
   std::swap(s.qa, s.qb);
@@ -567,7 +574,7 @@ void launch32_sort(Sim& s)
   const Geom& g = s.g;
   launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
   hipLaunchKernelGGL(k32_count, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa,
-                     s.d_np, s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.dseg);
+                     s.d_np, s.ss.cellcnt, s.ss.key, s.ss.slot);
   finish_sort32(s);
 }
 
