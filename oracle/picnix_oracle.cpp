@@ -371,6 +371,37 @@ inline void deposit3d(double dxdt, double dydt, double dzdt, double qs, double s
 }
 
 // nix/esirkepov.hpp:76-143 (ro2d .. jz2d) + deposit2d :287-302
+// nix/esirkepov.hpp:18-67,277-291 (ro1d, ds1d, jx1d, jy1d, jz1d)
+template <int Order>
+inline void deposit1d(double dxdt, double vy, double vz, double qs, double ss[2][1][Order + 3],
+                      double* cur /* [N][4] */)
+{
+  constexpr int N = Order + 3;
+  for (int jx = 0; jx < N; jx++)
+    cur[jx * 4 + 0] += qs * ss[1][0][jx];
+  for (int l = 0; l < N; l++)
+    ss[1][0][l] -= ss[0][0][l];
+  const double A = 1.0 / 2;
+  {
+    double ww = 0;
+    double wx = -(qs * dxdt);
+    for (int jx = 0; jx < N - 1; jx++) {
+      ww += ss[1][0][jx] * wx;
+      cur[(jx + 1) * 4 + 1] += ww;
+    }
+  }
+  {
+    const double qvy = qs * vy;
+    for (int jx = 0; jx < N; jx++)
+      cur[jx * 4 + 2] += (ss[0][0][jx] + A * ss[1][0][jx]) * qvy;
+  }
+  {
+    const double qvz = qs * vz;
+    for (int jx = 0; jx < N; jx++)
+      cur[jx * 4 + 3] += (ss[0][0][jx] + A * ss[1][0][jx]) * qvz;
+  }
+}
+
 template <int Order>
 inline void deposit2d(double dxdt, double dydt, double vz, double qs, double ss[2][2][Order + 3],
                       double* cur /* [N][N][4] */)
@@ -888,7 +919,7 @@ void init_friedman(Chunk& c)
         }
 }
 
-// pic/engine/maxwell.hpp:292-348 (2D) and :442-496 (3D)
+// pic/engine/maxwell.hpp:148-201 (1D), :292-348 (2D) and :442-496 (3D)
 void push_bfd(Chunk& c, int dim, double cc, double theta, double delt)
 {
   const int    Nb   = c.Nb;
@@ -901,9 +932,10 @@ void push_bfd(Chunk& c, int dim, double cc, double theta, double delt)
   const int    lbx = c.Lb[2], ubx = c.Ub[2], lby = c.Lb[1], uby = c.Ub[1], lbz = c.Lb[0],
             ubz = c.Ub[0];
   const int zlo = dim == 3 ? lbz - Nb : lbz, zhi = dim == 3 ? ubz + Nb : lbz;
+  const int ylo = dim >= 2 ? lby - Nb : lby, yhi = dim >= 2 ? uby + Nb : lby;
 
   for (int iz = zlo; iz <= zhi; iz++)
-    for (int iy = lby - Nb; iy <= uby + Nb; iy++)
+    for (int iy = ylo; iy <= yhi; iy++)
       for (int ix = lbx - Nb; ix <= ubx + Nb; ix++)
         for (int k = 0; k < 3; k++)
           c.FF(iz, iy, ix, 0, k) =
@@ -925,6 +957,12 @@ void push_bfd(Chunk& c, int dim, double cc, double theta, double delt)
         for (int ix = lbx - Nb + 1; ix <= ubx + Nb; ix++)
           c.UF(iz, iy, ix, 5) += (-cflx) * (c.FF(iz, iy, ix, 0, 1) - c.FF(iz, iy, ix - 1, 0, 1)) +
                                  (+cfly) * (c.FF(iz, iy, ix, 0, 0) - c.FF(iz, iy - 1, ix, 0, 0));
+  } else if (dim == 1) {
+    const int iz = lbz, iy = lby;
+    for (int ix = lbx - Nb + 1; ix <= ubx + Nb; ix++)
+      c.UF(iz, iy, ix, 4) += (+cflx) * (c.FF(iz, iy, ix, 0, 2) - c.FF(iz, iy, ix - 1, 0, 2));
+    for (int ix = lbx - Nb + 1; ix <= ubx + Nb; ix++)
+      c.UF(iz, iy, ix, 5) += (-cflx) * (c.FF(iz, iy, ix, 0, 1) - c.FF(iz, iy, ix - 1, 0, 1));
   } else {
     const int iz = lbz;
     for (int iy = lby - Nb + 1; iy <= uby + Nb; iy++)
@@ -940,7 +978,7 @@ void push_bfd(Chunk& c, int dim, double cc, double theta, double delt)
   }
 }
 
-// pic/engine/maxwell.hpp:231-290 (2D) and :383-440 (3D)
+// pic/engine/maxwell.hpp:85-146 (1D), :231-290 (2D) and :383-440 (3D)
 void push_efd(Chunk& c, int dim, double cc, double theta, double delt)
 {
   const int    Nb   = c.Nb;
@@ -950,9 +988,10 @@ void push_efd(Chunk& c, int dim, double cc, double theta, double delt)
   const int    lbx = c.Lb[2], ubx = c.Ub[2], lby = c.Lb[1], uby = c.Ub[1], lbz = c.Lb[0],
             ubz = c.Ub[0];
   const int zlo = dim == 3 ? lbz - Nb : lbz, zhi = dim == 3 ? ubz + Nb : lbz;
+  const int ylo = dim >= 2 ? lby - Nb : lby, yhi = dim >= 2 ? uby + Nb : lby;
 
   for (int iz = zlo; iz <= zhi; iz++)
-    for (int iy = lby - Nb; iy <= uby + Nb; iy++)
+    for (int iy = ylo; iy <= yhi; iy++)
       for (int ix = lbx - Nb; ix <= ubx + Nb; ix++)
         for (int k = 0; k < 3; k++) {
           c.FF(iz, iy, ix, 2, k) = c.FF(iz, iy, ix, 1, k) + theta * c.FF(iz, iy, ix, 2, k);
@@ -978,6 +1017,16 @@ void push_efd(Chunk& c, int dim, double cc, double theta, double delt)
           c.UF(iz, iy, ix, 2) += (+cflx) * (c.UF(iz, iy, ix + 1, 4) - c.UF(iz, iy, ix, 4)) +
                                  (-cfly) * (c.UF(iz, iy + 1, ix, 3) - c.UF(iz, iy, ix, 3)) -
                                  delt * c.UJ(iz, iy, ix, 3);
+  } else if (dim == 1) {
+    const int iz = lbz, iy = lby;
+    for (int ix = lbx - Nb; ix <= ubx + Nb; ix++)
+      c.UF(iz, iy, ix, 0) += -delt * c.UJ(iz, iy, ix, 1);
+    for (int ix = lbx - Nb; ix <= ubx + Nb - 1; ix++)
+      c.UF(iz, iy, ix, 1) +=
+          (-cflx) * (c.UF(iz, iy, ix + 1, 5) - c.UF(iz, iy, ix, 5)) - delt * c.UJ(iz, iy, ix, 2);
+    for (int ix = lbx - Nb; ix <= ubx + Nb - 1; ix++)
+      c.UF(iz, iy, ix, 2) +=
+          (+cflx) * (c.UF(iz, iy, ix + 1, 4) - c.UF(iz, iy, ix, 4)) - delt * c.UJ(iz, iy, ix, 3);
   } else {
     const int iz = lbz;
     for (int iy = lby - Nb; iy <= uby + Nb - 1; iy++)
@@ -996,7 +1045,8 @@ void push_efd(Chunk& c, int dim, double cc, double theta, double delt)
   }
 }
 
-// pic/engine/maxwell.hpp:203-229 (2D), :350-381 (3D); buffers are zero for periodic runs
+// pic/engine/maxwell.hpp:63-83 (1D), :203-229 (2D), :350-381 (3D); buffers are zero for
+// periodic runs
 void get_diverror(Chunk& c, int dim, double& efd, double& bfd)
 {
   const double rdx = 1 / c.del[2], rdy = 1 / c.del[1], rdz = 1 / c.del[0];
@@ -1013,6 +1063,12 @@ void get_diverror(Chunk& c, int dim, double& efd, double& bfd)
                  (c.UF(iz, iy, ix, 4) - c.UF(iz, iy - 1, ix, 4)) * rdy +
                  (c.UF(iz, iy, ix, 5) - c.UF(iz - 1, iy, ix, 5)) * rdz;
         }
+  } else if (dim == 1) {
+    const int iz = c.Lb[0], iy = c.Lb[1];
+    for (int ix = c.Lb[2]; ix <= c.Ub[2]; ix++) {
+      efd += (c.UF(iz, iy, ix + 1, 0) - c.UF(iz, iy, ix, 0)) * rdx - c.UJ(iz, iy, ix, 0);
+      bfd += (c.UF(iz, iy, ix, 3) - c.UF(iz, iy, ix - 1, 3)) * rdx;
+    }
   } else {
     const int iz = c.Lb[0];
     for (int iy = c.Lb[1]; iy <= c.Ub[1]; iy++)
@@ -1151,6 +1207,21 @@ void push_velocity_chunk(Chunk& c, int Ns, double cc, int pusher, int interp, do
         eb[3] = interp(hz0, hy0, ix0, 3, whz, why, wix);
         eb[4] = interp(hz0, iy0, hx0, 4, whz, wiy, whx);
         eb[5] = interp(iz0, hy0, hx0, 5, wiz, why, whx);
+      } else if constexpr (Dim == 1) {
+        const int iz0 = g.lbz, iy1 = g.lby;
+        // nix/interp.hpp:14-26 (interp1d_impl_sorted), velocity.hpp:362-388 (push_scalar1d)
+        auto interp = [&](int x0, int k, const double* wx) {
+          double rx = 0;
+          for (int jx = 0; jx < S; jx++)
+            rx += c.UF(iz0, iy1, x0 + jx, k) * wx[jx];
+          return rx * qmdt;
+        };
+        eb[0] = interp(hx0, 0, whx);
+        eb[1] = interp(ix0, 1, wix);
+        eb[2] = interp(ix0, 2, wix);
+        eb[3] = interp(ix0, 3, wix);
+        eb[4] = interp(hx0, 4, whx);
+        eb[5] = interp(hx0, 5, whx);
       } else {
         const int iz0 = g.lbz;
         // nix/interp.hpp:50-63 (interp2d_impl_sorted)
@@ -1243,6 +1314,29 @@ void local_current(const Geo& g, const double* xv, const double* xu, double qs, 
     origin[0] = iz0 + g.lbz - (Order / 2) - 1;
     origin[1] = iy0 + g.lby - (Order / 2) - 1;
     origin[2] = ix0 + g.lbx - (Order / 2) - 1;
+  } else if constexpr (Dim == 1) {
+    // pic/engine/current.hpp:207-258 (local1d)
+    double ss[2][1][N] = {};
+    double vy          = (xu[1] - xv[1]) / delt;
+    double vz          = (xu[2] - xv[2]) / delt;
+    int    ix0         = digitize(xv[0], ximin, rdx);
+    shape_mc<Order>(xv[0], xgrid + static_cast<double>(ix0) * g.dx, rdx, &ss[0][0][1]);
+    int ix1 = digitize(xu[0], ximin, rdx);
+    shape_mc<Order>(xu[0], xgrid + static_cast<double>(ix1) * g.dx, rdx, &ss[1][0][1]);
+    int shift[1] = {ix1 - ix0};
+    shift_weights<1, Order>(shift, ss[1]);
+    deposit1d<Order>(dxdt, vy, vz, qs, ss, cur);
+    origin[0] = g.lbz;
+    origin[1] = g.lby;
+    origin[2] = ix0 + g.lbx - (Order / 2) - 1;
+    (void)yimin;
+    (void)zimin;
+    (void)ygrid;
+    (void)zgrid;
+    (void)dydt;
+    (void)dzdt;
+    (void)rdy;
+    (void)rdz;
   } else {
     double ss[2][2][N] = {};
     double vz          = (xu[2] - xv[2]) / delt;
@@ -1268,7 +1362,7 @@ template <int Dim, int Order>
 void deposit_current_chunk(Chunk& c, int Ns, double delt)
 {
   constexpr int N  = Order + 3;
-  constexpr int NT = (Dim == 3 ? N * N * N : N * N) * 4;
+  constexpr int NT = (Dim == 3 ? N * N * N : (Dim == 2 ? N * N : N)) * 4;
   Geo           g(c);
   std::fill(c.uj.begin(), c.uj.end(), 0.0);
   for (int is = 0; is < Ns; is++) {
@@ -1283,11 +1377,15 @@ void deposit_current_chunk(Chunk& c, int Ns, double delt)
             for (int jx = 0; jx < N; jx++)
               for (int k = 0; k < 4; k++)
                 c.UJ(o[0] + jz, o[1] + jy, o[2] + jx, k) += cur[((jz * N + jy) * N + jx) * 4 + k];
-      } else {
+      } else if constexpr (Dim == 2) {
         for (int jy = 0; jy < N; jy++)
           for (int jx = 0; jx < N; jx++)
             for (int k = 0; k < 4; k++)
               c.UJ(o[0], o[1] + jy, o[2] + jx, k) += cur[(jy * N + jx) * 4 + k];
+      } else {
+        for (int jx = 0; jx < N; jx++)
+          for (int k = 0; k < 4; k++)
+            c.UJ(o[0], o[1], o[2] + jx, k) += cur[jx * 4 + k];
       }
     }
   }
@@ -1310,9 +1408,16 @@ void deposit_moment_chunk(Chunk& c, int Ns, double cc)
       double  rc    = 1 / cc;
       double  rdx = 1 / g.dx, rdy = 1 / g.dy, rdz = 1 / g.dz;
       int     ix = digitize(xu[0], g.xmin + 0.5 * g.dx * is_odd, rdx);
-      int     iy = digitize(xu[1], g.ymin + 0.5 * g.dy * is_odd, rdy);
       shape_mc<Order>(xu[0], (g.xmin + 0.5 * g.dx) + static_cast<double>(ix) * g.dx, rdx, wx);
-      shape_mc<Order>(xu[1], (g.ymin + 0.5 * g.dy) + static_cast<double>(iy) * g.dy, rdy, wy);
+      int iy = g.lby, Sy = 1;
+      if constexpr (Dim >= 2) {
+        iy = digitize(xu[1], g.ymin + 0.5 * g.dy * is_odd, rdy);
+        shape_mc<Order>(xu[1], (g.ymin + 0.5 * g.dy) + static_cast<double>(iy) * g.dy, rdy, wy);
+        iy += g.lby - (Order / 2);
+        Sy = S;
+      } else {
+        wy[0] = 1.0;
+      }
       int iz = g.lbz, Sz = 1;
       if constexpr (Dim == 3) {
         iz = digitize(xu[2], g.zmin + 0.5 * g.dz * is_odd, rdz);
@@ -1323,11 +1428,12 @@ void deposit_moment_chunk(Chunk& c, int Ns, double cc)
         wz[0] = 1.0;
       }
       ix += g.lbx - (Order / 2);
-      iy += g.lby - (Order / 2);
       for (int jz = 0; jz < Sz; jz++)
-        for (int jy = 0; jy < S; jy++)
+        for (int jy = 0; jy < Sy; jy++)
           for (int jx = 0; jx < S; jx++) {
-            double ww = (Dim == 3) ? ms * wx[jx] * wy[jy] * wz[jz] : ms * wx[jx] * wy[jy];
+            // moment.hpp local1d/local2d/local3d: the product of the present axes' weights
+            double ww = (Dim == 3) ? ms * wx[jx] * wy[jy] * wz[jz]
+                                   : (Dim == 2 ? ms * wx[jx] * wy[jy] : ms * wx[jx]);
             double gm = lorentz_factor(xu[3], xu[4], xu[5], rc);
             double m[14];
             m[0]  = ww;
@@ -1485,12 +1591,19 @@ void dispatch(int dim, int order, Args&&... args)
     case 3: F<3, 3>::run(args...); break;
     default: F<3, 4>::run(args...); break;
     }
-  } else {
+  } else if (dim == 2) {
     switch (order) {
     case 1: F<2, 1>::run(args...); break;
     case 2: F<2, 2>::run(args...); break;
     case 3: F<2, 3>::run(args...); break;
     default: F<2, 4>::run(args...); break;
+    }
+  } else {
+    switch (order) {
+    case 1: F<1, 1>::run(args...); break;
+    case 2: F<1, 2>::run(args...); break;
+    case 3: F<1, 3>::run(args...); break;
+    default: F<1, 4>::run(args...); break;
     }
   }
 }
@@ -1524,7 +1637,7 @@ void setup(Sim& s)
   bool      has_dim[3];
   for (int d = 0; d < 3; d++)
     has_dim[d] = !(N[d] == 1 && C[d] == 1); // nix/application.cpp:263-267
-  s.dim = has_dim[0] ? 3 : 2;
+  s.dim = has_dim[0] ? 3 : (has_dim[1] ? 2 : 1);
 
   s.id_of_coord = chunk_map(C[0], C[1], C[2]);
   const int nc  = C[0] * C[1] * C[2];
@@ -1913,6 +2026,28 @@ void pno_esirkepov2d(int order, const double* xv, const double* xu, double q, do
   case 2: local_current<2, 2>(g, xv, xu, q, delt, tile, origin); break;
   case 3: local_current<2, 3>(g, xv, xu, q, delt, tile, origin); break;
   default: local_current<2, 4>(g, xv, xu, q, delt, tile, origin); break;
+  }
+}
+
+void pno_esirkepov1d(int order, const double* xv, const double* xu, double q, double delt,
+                     double dx, double* tile, int* origin)
+{
+  Chunk c{};
+  for (int d = 0; d < 3; d++) {
+    c.Lb[d]     = 0;
+    c.lim[d][0] = 0.0;
+  }
+  c.del[0] = 1.0;
+  c.del[1] = 1.0;
+  c.del[2] = dx;
+  Geo g(c);
+  int N    = order + 3;
+  std::fill(tile, tile + N * 4, 0.0);
+  switch (order) {
+  case 1: local_current<1, 1>(g, xv, xu, q, delt, tile, origin); break;
+  case 2: local_current<1, 2>(g, xv, xu, q, delt, tile, origin); break;
+  case 3: local_current<1, 3>(g, xv, xu, q, delt, tile, origin); break;
+  default: local_current<1, 4>(g, xv, xu, q, delt, tile, origin); break;
   }
 }
 

@@ -92,6 +92,9 @@ void   pno_esirkepov3d(int order, const double* xv, const double* xu, double q, 
                        double dx, double dy, double dz, double* tile, int* origin);
 void   pno_esirkepov2d(int order, const double* xv, const double* xu, double q, double delt,
                        double dx, double dy, double* tile, int* origin);
+/* one particle's 1D Esirkepov tile [order+3][4] (local1d, pic/engine/current.hpp:207-258) */
+void pno_esirkepov1d(int order, const double* xv, const double* xu, double q, double delt,
+                     double dx, double* tile, int* origin);
 
 /* ---- random number generation restated from libstdc++ (GCC 11.4) ---- */
 uint64_t pno_mt19937_64(uint64_t seed, int64_t skip);

@@ -80,6 +80,7 @@ def lib():
         L.pno_push_higuera_cary.argtypes = [dp, dp, d]
         L.pno_esirkepov3d.argtypes = [i, dp, dp, d, d, d, d, d, dp, ip]
         L.pno_esirkepov2d.argtypes = [i, dp, dp, d, d, d, d, dp, ip]
+        L.pno_esirkepov1d.argtypes = [i, dp, dp, d, d, d, dp, ip]
         L.pno_mt19937_64.restype = ctypes.c_uint64
         L.pno_mt19937_64.argtypes = [ctypes.c_uint64, i64]
         L.pno_uniform.argtypes = [ctypes.c_uint64, i64, dp]

@@ -23,7 +23,7 @@ def vacuum_cfg(N, C, order=2):
 
 
 @pytest.mark.parametrize("N,C,axis", [((4, 4, 32), (1, 1, 2), "x"), ((32, 4, 4), (2, 1, 1), "z"),
-                                      ((1, 32, 4), (1, 4, 1), "y")])
+                                      ((1, 32, 4), (1, 4, 1), "y"), ((1, 1, 32), (1, 1, 2), "x")])
 def test_plane_wave_dispersion(N, C, axis):
     cfg = vacuum_cfg(N, C)
     o = ol.Oracle(cfg, load=False)
@@ -95,14 +95,19 @@ def test_divb_preserved():
 
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
 @pytest.mark.parametrize("field", ["E", "B"])
-def test_gather_linear_fields(order, field):
-    cfg = vacuum_cfg((8, 8, 8), (1, 1, 1), order=order)
+@pytest.mark.parametrize("dim", [3, 1])
+def test_gather_linear_fields(order, field, dim):
+    """dim 1: fields linear in x only (the 1D gather reads the x line,
+    pic/engine/velocity.hpp:362-388)."""
+    cfg = vacuum_cfg((8, 8, 8) if dim == 3 else (1, 1, 16), (1, 1, 1), order=order)
     o = ol.Oracle(cfg, load=False)
     info = o.chunk_info(0)
     nb = int(info[9])
     n = [int(v) for v in info[6:9]]
     rng = np.random.default_rng(order)
     coef = rng.normal(size=(6, 4)) * 0.1
+    if dim == 1:
+        coef[:, 2:] = 0.0
     # staggered locations (z, y, x) per component: 0 face, 0.5 centre
     stag = {0: (0.5, 0.5, 0.0), 1: (0.5, 0.0, 0.5), 2: (0.0, 0.5, 0.5),
             3: (0.0, 0.0, 0.5), 4: (0.0, 0.5, 0.0), 5: (0.5, 0.0, 0.0)}
@@ -115,6 +120,9 @@ def test_gather_linear_fields(order, field):
                       coef[kk, 3] * (zz + sz)
     o.set_field(0, 0, uf)
     pos = rng.uniform(1.0, 7.0, size=(64, 3))
+    if dim == 1:
+        pos[:, 0] = rng.uniform(1.0, 15.0, size=64)
+        pos[:, 1:] = rng.uniform(0.0, 1.0, size=(64, 2))
     u0 = rng.normal(size=(64, 3)) * 0.1 if field == "B" else np.zeros((64, 3))
     xu = np.zeros((64, 7))
     xu[:, :3] = pos
@@ -184,9 +192,9 @@ def _yee_friedman_numpy(E, B, theta, nstep, c, dt, dh):
 
 
 @pytest.mark.parametrize("theta", [0.0, 0.3])
-@pytest.mark.parametrize("C", [(1, 1, 1), (2, 2, 2)])
-def test_friedman_filter_matches_published_recursion(theta, C):
-    N = (8, 8, 8)
+@pytest.mark.parametrize("N,C", [((8, 8, 8), (1, 1, 1)), ((8, 8, 8), (2, 2, 2)),
+                                 ((1, 1, 16), (1, 1, 2))])
+def test_friedman_filter_matches_published_recursion(theta, N, C):
     cfg = helpers.small_thermal(N=N, C=C, np_=1, order=2, friedman=theta)
     o = ol.Oracle(cfg, load=False)
     dt, dh, c = 0.5, 1.0, 1.0

@@ -53,10 +53,10 @@ def _chunk_box(o, c, dh):
 
 
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
-@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("dim", [1, 2, 3])
 def test_moment_sums(order, dim):
-    N = (8, 8, 8) if dim == 3 else (1, 8, 16)
-    C = (2, 2, 2) if dim == 3 else (1, 2, 2)
+    N = {3: (8, 8, 8), 2: (1, 8, 16), 1: (1, 1, 16)}[dim]
+    C = {3: (2, 2, 2), 2: (1, 2, 2), 1: (1, 1, 2)}[dim]
     cfg = helpers.small_thermal(N=N, C=C, np_=1, order=order)
     dh = float(cfg["parameter"]["delh"])
     cc = float(cfg["parameter"]["cc"])

@@ -42,10 +42,10 @@ def brute_sort(xu, lim, dh, order, dims, has):
 
 
 @pytest.mark.parametrize("order", [1, 2])
-@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("dim", [1, 2, 3])
 def test_sort_matches_brute_force(order, dim):
-    N = (8, 8, 8) if dim == 3 else (1, 8, 8)
-    C = (2, 2, 2) if dim == 3 else (1, 2, 2)
+    N = {3: (8, 8, 8), 2: (1, 8, 8), 1: (1, 1, 16)}[dim]
+    C = {3: (2, 2, 2), 2: (1, 2, 2), 1: (1, 1, 2)}[dim]
     cfg = helpers.small_thermal(N=N, C=C, np_=1, order=order)
     o = ol.Oracle(cfg, load=False)
     rng = np.random.default_rng(order * 10 + dim)
@@ -60,14 +60,16 @@ def test_sort_matches_brute_force(order, dim):
         for col, a in ((0, 2), (1, 1), (2, 0)):
             xu[:, col] = rng.uniform(lim[a][0] - 0.6, lim[a][1] + 0.6, size=n)
         xu[:50, 0] = np.floor(xu[:50, 0])
-        if dim == 2:
+        if dim <= 2:
             xu[:, 2] = rng.uniform(-3, 3, size=n)  # z is ignored in 2D
+        if dim == 1:
+            xu[:, 1] = rng.uniform(-3, 3, size=n)  # and y in 1D
         xu[:, 3:6] = rng.normal(size=(n, 3))
         xu[:, 6] = np.arange(n).astype(np.int64).view(np.float64)
         o.set_particles(c, 0, xu)
         o.sort_particle()
         got = o.particles(c, 0)
-        has = [dim == 3, True, True]
+        has = [dim == 3, dim >= 2, True]
         ref = brute_sort(xu, lim, 1.0, order, dims, has)
         assert np.array_equal(got.view(np.int64), ref.view(np.int64))
         # pindex: every bin holds exactly the particles that digitize to it
@@ -82,12 +84,14 @@ def _all_particles(o):
     return recs, sp
 
 
-@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("dim", [1, 2, 3])
 def test_chunked_equals_single_chunk(dim):
     if dim == 3:
         N, C = (8, 8, 8), (2, 2, 2)
-    else:
+    elif dim == 2:
         N, C = (1, 8, 16), (1, 2, 4)
+    else:
+        N, C = (1, 1, 32), (1, 1, 4)
     cfg = helpers.small_thermal(N=N, C=C, np_=4, order=2)
     for sp in cfg["parameter"]["particle"]:
         sp["vt"] = 0.3  # many chunk crossings
