@@ -87,7 +87,7 @@ void setup_geometry(Sim& s)
   Geom&         g   = s.g;
   for (int d = 0; d < 3; d++)
     g.has[d] = !(cfg.N[d] == 1 && cfg.C[d] == 1);
-  g.dim   = g.has[0] ? 3 : 2;
+  g.dim   = g.has[0] ? 3 : (g.has[1] ? 2 : 1);
   g.order = cfg.order;
   g.Nb    = (cfg.order + 3) / 2;
   g.Ns    = cfg.Ns;
@@ -473,6 +473,11 @@ int pn_create(const char* toml, int load, pn_sim** out)
       return PN_ERR_UNSUPPORTED;
     }
     s.fp32 = s.cfg.precision == "fp32";
+    if (s.fp32 && s.cfg.N[0] == 1 && s.cfg.C[0] == 1 && s.cfg.N[1] == 1 && s.cfg.C[1] == 1) {
+      s.err = "precision = fp32 is built for 2D and 3D domains; 1D runs use fp64";
+      *out  = h;
+      return PN_ERR_UNSUPPORTED;
+    }
     setup_geometry(s);
     if (const char* k = std::getenv("PICNIX_KERNELS"))
       s.generic_kernels = std::string(k) == "generic";

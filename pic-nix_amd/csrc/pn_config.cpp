@@ -229,10 +229,11 @@ bool build_config(const TomlMap& kv, Config& cfg, std::string& err)
   bool has[3];
   for (int d = 0; d < 3; d++)
     has[d] = !(cfg.N[d] == 1 && cfg.C[d] == 1);
+  bool is_1d = !has[0] && !has[1] && has[2];
   bool is_2d = !has[0] && has[1] && has[2];
   bool is_3d = has[0] && has[1] && has[2];
-  if (!is_2d && !is_3d) {
-    err = "config: only 2D (Nz = 1) and 3D domains are supported";
+  if (!is_1d && !is_2d && !is_3d) {
+    err = "config: the domain must be 1D (Nz = Ny = 1), 2D (Nz = 1) or 3D";
     return false;
   }
   // minimum chunk width (pic/pic_chunk.cpp:192-210)

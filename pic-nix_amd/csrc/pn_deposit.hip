@@ -49,6 +49,61 @@ __device__ __forceinline__ void esirkepov(const Geom& g, double xmin, double ymi
   const double  A     = 1.0 / 2;
   const double  B     = 1.0 / 3;
 
+  if constexpr (Dim == 1) {
+    // local1d (pic/engine/current.hpp:207-258) + deposit1d (nix/esirkepov.hpp:
+    // 18-67,277-291): rho, Jx by the prefix sum, Jy and Jz from the velocities
+    // (x_new - x_old) / dt times the time-averaged weight
+    double s0[N], s1[N];
+#pragma unroll
+    for (int l = 0; l < N; l++)
+      s0[l] = s1[l] = 0.0;
+    const int i0 = digitize(xv[0], ximin, rdx);
+    const int i1 = digitize(xu[0], ximin, rdx);
+    shape_mc<Order>(xv[0], xgrid + static_cast<double>(i0) * dx, rdx, &s0[1]);
+    shape_mc<Order>(xu[0], xgrid + static_cast<double>(i1) * dx, rdx, &s1[1]);
+    if (i1 - i0 < 0) {
+#pragma unroll
+      for (int ii = 0; ii < Order + 2; ii++)
+        s1[ii] = s1[ii + 1];
+    } else if (i1 - i0 > 0) {
+#pragma unroll
+      for (int ii = Order + 2; ii > 0; ii--)
+        s1[ii] = s1[ii - 1];
+    }
+    const int    ox = i0 + lb - (Order / 2) - 1;
+    const double vy = (xu[1] - xv[1]) / dc.delt;
+    const double vz = (xu[2] - xv[2]) / dc.delt;
+#pragma unroll
+    for (int jx = 0; jx < N; jx++)
+      add(lb, lb, ox + jx, 0, qs * s1[jx]);
+#pragma unroll
+    for (int l = 0; l < N; l++)
+      s1[l] -= s0[l];
+    {
+      double       ww = 0;
+      const double wx = -(qs * dc.dxdt);
+#pragma unroll
+      for (int jx = 0; jx < N - 1; jx++) {
+        ww += s1[jx] * wx;
+        add(lb, lb, ox + jx + 1, 1, ww);
+      }
+    }
+    const double qvy = qs * vy;
+    const double qvz = qs * vz;
+#pragma unroll
+    for (int jx = 0; jx < N; jx++) {
+      add(lb, lb, ox + jx, 2, (s0[jx] + A * s1[jx]) * qvy);
+      add(lb, lb, ox + jx, 3, (s0[jx] + A * s1[jx]) * qvz);
+    }
+    (void)yimin;
+    (void)zimin;
+    (void)ygrid;
+    (void)zgrid;
+    (void)rdy;
+    (void)rdz;
+    (void)B;
+    return;
+  }
   double s0x[N], s0y[N], s0z[N], s1x[N], s1y[N], s1z[N];
 #pragma unroll
   for (int l = 0; l < N; l++) {
@@ -219,9 +274,9 @@ __device__ __forceinline__ void esirkepov(const Geom& g, double xmin, double ymi
 __device__ __forceinline__ long gnode(const Geom& g, const int* __restrict__ off, int c, int lz,
                                       int ly, int lx)
 {
-  // chunk-local ghost-inclusive node -> global padded index (absent z maps to 0)
+  // chunk-local ghost-inclusive node -> global padded index (absent y, z map to 0)
   int gz = g.has[0] ? off[c * 3 + 0] + lz : 0;
-  int gy = off[c * 3 + 1] + ly;
+  int gy = g.has[1] ? off[c * 3 + 1] + ly : 0;
   int gx = off[c * 3 + 2] + lx;
   return ((long)gz * g.P[1] + gy) * g.P[2] + gx;
 }
@@ -232,7 +287,7 @@ __device__ __forceinline__ long gnode_checked(const Geom& g, const int* __restri
                                               int lz, int ly, int lx)
 {
   int gz = g.has[0] ? off[c * 3 + 0] + lz : 0;
-  int gy = off[c * 3 + 1] + ly;
+  int gy = g.has[1] ? off[c * 3 + 1] + ly : 0;
   int gx = off[c * 3 + 2] + lx;
   if ((unsigned)gz >= (unsigned)g.P[0] || (unsigned)gy >= (unsigned)g.P[1] ||
       (unsigned)gx >= (unsigned)g.P[2])
@@ -275,7 +330,7 @@ __global__ void __launch_bounds__(256)
                           // drop nodes outside the padded grid (particles placed
                           // outside their chunk through pn_set_particles)
                           int gz = g.has[0] ? off[c * 3 + 0] + lz : 0;
-                          int gy = off[c * 3 + 1] + ly;
+                          int gy = g.has[1] ? off[c * 3 + 1] + ly : 0;
                           int gx = off[c * 3 + 2] + lx;
                           if ((unsigned)gz >= (unsigned)g.P[0] || (unsigned)gy >= (unsigned)g.P[1] ||
                               (unsigned)gx >= (unsigned)g.P[2])
@@ -1052,11 +1107,14 @@ __global__ void __launch_bounds__(256)
   }
   const double qs = g.q[sp];
   for (int jz = 0; jz < (Dim == 3 ? S : 1); jz++)
-    for (int jy = 0; jy < S; jy++)
+    for (int jy = 0; jy < (Dim >= 2 ? S : 1); jy++)
       for (int jx = 0; jx < S; jx++) {
-        double    val = Dim == 3 ? qs * s[0][jx] * s[1][jy] * s[2][jz] : qs * s[0][jx] * s[1][jy];
+        // ro1d / ro2d / ro3d: the product of the present axes' new weights
+        double val = Dim == 3 ? qs * s[0][jx] * s[1][jy] * s[2][jz]
+                              : (Dim == 2 ? qs * s[0][jx] * s[1][jy] : qs * s[0][jx]);
         const int  lz = Dim == 3 ? o[2] + jz : g.Nb;
-        const long gi = gnode_checked(g, off, c, lz, o[1] + jy, o[0] + jx);
+        const int  ly = Dim >= 2 ? o[1] + jy : g.Nb;
+        const long gi = gnode_checked(g, off, c, lz, ly, o[0] + jx);
         if (gi >= 0)
           unsafeAtomicAdd(&J[gi], val);
       }
@@ -1122,7 +1180,7 @@ bool deposit_impl(Sim& s, const DepConsts& dc)
                        s.d_off, s.d_lim, s.d_J, dc, zrows);
     return true;
   }
-  if (s.sorted && lds <= 65536 && !s.generic_kernels) {
+  if (Dim >= 2 && s.sorted && lds <= 65536 && !s.generic_kernels) {
     hipLaunchKernelGGL((k_deposit_slab<Dim, Order>), dim3(nslab, g.nseg), dim3(256), lds,
                        s.stream, g, s.pa, s.d_xo[0], s.d_xo[1], s.d_xo[2], s.ss.cellstart,
                        s.d_off, s.d_lim, s.d_J, dc);
@@ -1184,6 +1242,13 @@ RhoDone launch_deposit(Sim& s, double delt, RhoMode mode)
       case 3: PN_DEP(3, 3); break;
       default: PN_DEP(3, 4); break;
       }
+    } else if (g.dim == 1) {
+      switch (g.order) {
+      case 1: PN_DEP(1, 1); break;
+      case 2: PN_DEP(1, 2); break;
+      case 3: PN_DEP(1, 3); break;
+      default: PN_DEP(1, 4); break;
+      }
     } else {
       switch (g.order) {
       case 1: PN_DEP(2, 1); break;
@@ -1216,6 +1281,13 @@ void launch_rho(Sim& s, bool sorted_by_new_cell)
     case 2: PN_RHO(3, 2); break;
     case 3: PN_RHO(3, 3); break;
     default: PN_RHO(3, 4); break;
+    }
+  } else if (g.dim == 1) {
+    switch (g.order) {
+    case 1: PN_RHO(1, 1); break;
+    case 2: PN_RHO(1, 2); break;
+    case 3: PN_RHO(1, 3); break;
+    default: PN_RHO(1, 4); break;
     }
   } else {
     switch (g.order) {

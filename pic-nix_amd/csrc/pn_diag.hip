@@ -58,7 +58,14 @@ __global__ void __launch_bounds__(256) k_field_energy(Geom g, const double* __re
     double bx = B[i], by = B[v + i], bz = B[2 * v + i];
     acc[0] = 0.5 * (ex * ex + ey * ey + ez * ez);
     acc[1] = 0.5 * (bx * bx + by * by + bz * bz);
-    if constexpr (Dim == 3) {
+    if constexpr (Dim == 1) {
+      // get_diverror_1d (pic/engine/maxwell.hpp:63-83)
+      acc[2] = (E[i + 1] - ex) * rdx - J[i];
+      acc[3] = (bx - B[i - 1]) * rdx;
+      (void)rdy;
+      (void)rdz;
+      (void)sz;
+    } else if constexpr (Dim == 3) {
       acc[2] = (E[i + 1] - ex) * rdx + (E[v + i + sy] - ey) * rdy + (E[2 * v + i + sz] - ez) * rdz -
                J[i];
       acc[3] = (bx - B[i - 1]) * rdx + (by - B[v + i - sy]) * rdy + (bz - B[2 * v + i - sz]) * rdz;
@@ -114,11 +121,14 @@ __global__ void __launch_bounds__(256) k_chunk_get(Geom g, const double* __restr
   long r  = t / nx;
   int  ly = (int)(r % ny);
   int  lz = (int)(r / ny);
-  bool zok = g.has[0] || lz == g.Nb;
-  int  gz  = g.has[0] ? wrapg(lz, oz, g.pad[0], g.N[0]) : 0;
-  long i   = gidx(g, gz, wrapg(ly, oy, g.pad[1], g.N[1]), wrapg(lx, ox, g.pad[2], g.N[2]));
+  // an absent dimension (z in 2D, y and z in 1D) has one global plane: the
+  // chunk's ghost planes along it read as zero
+  bool ok = (g.has[0] || lz == g.Nb) && (g.has[1] || ly == g.Nb);
+  int  gz = g.has[0] ? wrapg(lz, oz, g.pad[0], g.N[0]) : 0;
+  int  gy = g.has[1] ? wrapg(ly, oy, g.pad[1], g.N[1]) : 0;
+  long i  = gidx(g, gz, gy, wrapg(lx, ox, g.pad[2], g.N[2]));
   for (int k = 0; k < ncomp; k++)
-    out[t * stride + coff + k] = zok ? F[k * g.vol + i] : 0.0;
+    out[t * stride + coff + k] = ok ? F[k * g.vol + i] : 0.0;
 }
 
 // Interior cells of a per-chunk array -> global array (ghost cells are ignored).
@@ -135,12 +145,13 @@ __global__ void __launch_bounds__(256) k_chunk_set(Geom g, double* __restrict__ 
   int  ly = (int)(r % ny);
   int  lz = (int)(r / ny);
   bool in_x = lx >= g.Nb && lx < g.Nb + g.dims[2];
-  bool in_y = ly >= g.Nb && ly < g.Nb + g.dims[1];
+  bool in_y = g.has[1] ? (ly >= g.Nb && ly < g.Nb + g.dims[1]) : ly == g.Nb;
   bool in_z = g.has[0] ? (lz >= g.Nb && lz < g.Nb + g.dims[0]) : lz == g.Nb;
   if (!(in_x && in_y && in_z))
     return;
   int  gz = g.has[0] ? oz + lz : 0;
-  long i  = gidx(g, gz, oy + ly, ox + lx);
+  int  gy = g.has[1] ? oy + ly : 0;
+  long i  = gidx(g, gz, gy, ox + lx);
   for (int k = 0; k < ncomp; k++)
     F[k * g.vol + i] = in[t * stride + coff + k];
 }
@@ -165,6 +176,9 @@ void launch_energy(Sim& s, double* out)
   hipMemsetAsync(d_part, 0, sizeof(double) * (np1 + np2), s.stream);
   if (g.dim == 3)
     hipLaunchKernelGGL(k_field_energy<3>, dim3(nb1), dim3(256), 0, s.stream, g, s.d_E, s.d_B,
+                       s.d_J, d_part);
+  else if (g.dim == 1)
+    hipLaunchKernelGGL(k_field_energy<1>, dim3(nb1), dim3(256), 0, s.stream, g, s.d_E, s.d_B,
                        s.d_J, d_part);
   else
     hipLaunchKernelGGL(k_field_energy<2>, dim3(nb1), dim3(256), 0, s.stream, g, s.d_E, s.d_B,

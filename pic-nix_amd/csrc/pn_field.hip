@@ -45,6 +45,13 @@ __global__ void __launch_bounds__(256) k_push_bfd(Geom g, const T* __restrict__ 
     B[i] += (-cfly) * (Ez[i] - Ez[i - sy]) + (+cflz) * (Ey[i] - Ey[i - sz]);
     B[v + i] += (-cflz) * (Ex[i] - Ex[i - sz]) + (+cflx) * (Ez[i] - Ez[i - sx]);
     B[2 * v + i] += (-cflx) * (Ey[i] - Ey[i - sx]) + (+cfly) * (Ex[i] - Ex[i - sy]);
+  } else if constexpr (Dim == 1) {
+    // push_bfd_1d (pic/engine/maxwell.hpp:148-201): Bx unchanged
+    B[v + i] += (+cflx) * (Ez[i] - Ez[i - sx]);
+    B[2 * v + i] += (-cflx) * (Ey[i] - Ey[i - sx]);
+    (void)Ex;
+    (void)sy;
+    (void)sz;
   } else {
     B[i] += (-cfly) * (Ez[i] - Ez[i - sy]);
     B[v + i] += (+cflx) * (Ez[i] - Ez[i - sx]);
@@ -124,7 +131,15 @@ __global__ void __launch_bounds__(256) k_push_efd(Geom g, T* __restrict__ E,
   const T* Bx = B;
   const T* By = B + v;
   const T* Bz = B + 2 * v;
-  if constexpr (Dim == 3) {
+  if constexpr (Dim == 1) {
+    // push_efd_1d (pic/engine/maxwell.hpp:85-146)
+    E[i] += -delt * J[v + i];
+    E[v + i] += (-cflx) * (Bz[i + sx] - Bz[i]) - delt * J[2 * v + i];
+    E[2 * v + i] += (+cflx) * (By[i + sx] - By[i]) - delt * J[3 * v + i];
+    (void)Bx;
+    (void)sy;
+    (void)sz;
+  } else if constexpr (Dim == 3) {
     E[i] += (+cfly) * (Bz[i + sy] - Bz[i]) + (-cflz) * (By[i + sz] - By[i]) - delt * J[v + i];
     E[v + i] +=
         (+cflz) * (Bx[i + sz] - Bx[i]) + (-cflx) * (Bz[i + sx] - Bz[i]) - delt * J[2 * v + i];
@@ -269,6 +284,9 @@ void push_bfd_impl(Sim& s, const T* E, T* B, T* ff, double delt)
   if (g.dim == 3)
     hipLaunchKernelGGL((k_push_bfd<3, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, B,
                        cflx, cfly, cflz);
+  else if (g.dim == 1)
+    hipLaunchKernelGGL((k_push_bfd<1, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, B,
+                       cflx, cfly, cflz);
   else
     hipLaunchKernelGGL((k_push_bfd<2, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, B,
                        cflx, cfly, cflz);
@@ -288,6 +306,9 @@ void push_efd_impl(Sim& s, T* E, const T* B, const T* J, T* ff, double delt)
                        ff, (T)s.cfg.friedman);
   if (g.dim == 3)
     hipLaunchKernelGGL((k_push_efd<3, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, E, B, J,
+                       cflx, cfly, cflz, (T)delt);
+  else if (g.dim == 1)
+    hipLaunchKernelGGL((k_push_efd<1, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, E, B, J,
                        cflx, cfly, cflz, (T)delt);
   else
     hipLaunchKernelGGL((k_push_efd<2, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, E, B, J,

@@ -71,6 +71,7 @@ __global__ void __launch_bounds__(256)
 {
   constexpr int S   = Order + 1;
   constexpr int Sz  = Dim == 3 ? S : 1;
+  constexpr int Sy  = Dim >= 2 ? S : 1;
   const int     seg = blockIdx.y;
   const long    ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= np[seg])
@@ -87,11 +88,13 @@ __global__ void __launch_bounds__(256)
   const double ms   = g.m[sp];
   double*      Msp  = M + (long)sp * 14 * g.vol;
   for (int jz = 0; jz < Sz; jz++)
-    for (int jy = 0; jy < S; jy++)
+    for (int jy = 0; jy < Sy; jy++)
       for (int jx = 0; jx < S; jx++) {
-        const double ww = Dim == 3 ? ms * w[0][jx] * w[1][jy] * w[2][jz] : ms * w[0][jx] * w[1][jy];
+        // moment.hpp local1d/2d/3d: the product of the present axes' weights
+        const double ww = Dim == 3 ? ms * w[0][jx] * w[1][jy] * w[2][jz]
+                                   : (Dim == 2 ? ms * w[0][jx] * w[1][jy] : ms * w[0][jx]);
         const int    gz = g.has[0] ? off[c * 3 + 0] + o[2] + jz : 0;
-        const int    gy = off[c * 3 + 1] + o[1] + jy;
+        const int    gy = g.has[1] ? off[c * 3 + 1] + o[1] + jy : 0;
         const int    gx = off[c * 3 + 2] + o[0] + jx;
         if ((unsigned)gz >= (unsigned)g.P[0] || (unsigned)gy >= (unsigned)g.P[1] ||
             (unsigned)gx >= (unsigned)g.P[2])
@@ -212,7 +215,7 @@ void moment_impl(Sim& s)
   const int    S    = Order + 1;
   const int    Sz   = Dim == 3 ? S : 1;
   const size_t tlds = sizeof(double) * 14 * (size_t)Sz * S * (g.bsy + Order);
-  if (s.sorted && !s.generic_kernels && tlds <= 65536) {
+  if (Dim >= 2 && s.sorted && !s.generic_kernels && tlds <= 65536) {
     const int nrow = g.nbin / g.bsy;
     hipLaunchKernelGGL((k_moment_rows<Dim, Order>), dim3(nrow, g.nseg), dim3(256), tlds, s.stream,
                        g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_M);
@@ -237,6 +240,13 @@ void launch_deposit_moment(Sim& s)
     case 2: PN_MOM(3, 2); break;
     case 3: PN_MOM(3, 3); break;
     default: PN_MOM(3, 4); break;
+    }
+  } else if (g.dim == 1) {
+    switch (g.order) {
+    case 1: PN_MOM(1, 1); break;
+    case 2: PN_MOM(1, 2); break;
+    case 3: PN_MOM(1, 3); break;
+    default: PN_MOM(1, 4); break;
     }
   } else {
     switch (g.order) {

@@ -56,6 +56,44 @@ __global__ void __launch_bounds__(256)
 
   double xu[6] = {p.x[i], p.y[i], p.z[i], p.ux[i], p.uy[i], p.uz[i]};
 
+  if constexpr (Dim == 1) {
+    // weights1d + push_scalar1d (pic/engine/velocity.hpp:223-251,362-388): the
+    // x line of the grid (no y, z padding)
+    const double xim = xmin + 0.5 * dx * is_odd;
+    const double xhm = xmin + 0.5 * dx * is_odd - 0.5 * dx;
+    const double rd  = 1 / dx;
+    double       wi[S], wh[S];
+    int          i0 = digitize(xu[0], xim, rd);
+    int          h0 = digitize(xu[0], xhm, rd);
+    int_weights<Order>(pc, 0, xu[0], (xmin + 0.5 * dx) + static_cast<double>(i0) * dx, rd, wi);
+    shape_mc<Order>(xu[0], xmin + static_cast<double>(h0) * dx, rd, wh);
+    i0 += lb - (Order / 2) + off[c * 3 + 2];
+    h0 += lb - (Order / 2) + off[c * 3 + 2];
+    if (min(i0, h0) < 0 || max(i0, h0) > g.P[2] - S)
+      return;
+    const long v      = g.vol;
+    auto       interp = [&](const double* F, int x0, const double* w) {
+      double r = 0;
+#pragma unroll
+      for (int jx = 0; jx < S; jx++)
+        r += F[x0 + jx] * w[jx];
+      return r * qmdt;
+    };
+    double eb[6];
+    eb[0] = interp(E, h0, wh);
+    eb[1] = interp(E + v, i0, wi);
+    eb[2] = interp(E + 2 * v, i0, wi);
+    eb[3] = interp(B, i0, wi);
+    eb[4] = interp(B + v, h0, wh);
+    eb[5] = interp(B + 2 * v, h0, wh);
+    double ux = xu[3], uy = xu[4], uz = xu[5];
+    push_momentum(pc.pusher, ux, uy, uz, eb, g.cc);
+    p.ux[i] = ux;
+    p.uy[i] = uy;
+    p.uz[i] = uz;
+    return;
+  }
+
   double ximin  = xmin + 0.5 * dx * is_odd;
   double xhmin  = xmin + 0.5 * dx * is_odd - 0.5 * dx;
   double yimin  = ymin + 0.5 * dy * is_odd;
@@ -523,7 +561,7 @@ void launch_push_velocity(Sim& s, double delt)
   dim3         grid(nblk(g.cap, 256), g.nseg);
   const int    T     = g.order + 2;
   const size_t lds   = sizeof(double) * 6 * (g.dim == 3 ? T : 1) * T * (g.bsy + g.order + 1);
-  const bool   rows  = s.sorted && !s.generic_kernels && lds <= 65536;
+  const bool   rows  = g.dim >= 2 && s.sorted && !s.generic_kernels && lds <= 65536;
   const int    nrow  = g.nbin / g.bsy;
   const bool   plain = pc.pusher == 0 && pc.wt == 0; // Boris + MC: no option branches
 #define PN_VEL_ROWS(D, O, PL)                                                                 \
@@ -545,6 +583,13 @@ void launch_push_velocity(Sim& s, double delt)
     case 2: PN_VEL(3, 2); break;
     case 3: PN_VEL(3, 3); break;
     default: PN_VEL(3, 4); break;
+    }
+  } else if (g.dim == 1) {
+    switch (g.order) {
+    case 1: hipLaunchKernelGGL((k_push_velocity<1, 1>), grid, dim3(256), 0, s.stream, g, s.pa, s.d_np, s.d_off, s.d_lim, s.d_E, s.d_B, pc); break;
+    case 2: hipLaunchKernelGGL((k_push_velocity<1, 2>), grid, dim3(256), 0, s.stream, g, s.pa, s.d_np, s.d_off, s.d_lim, s.d_E, s.d_B, pc); break;
+    case 3: hipLaunchKernelGGL((k_push_velocity<1, 3>), grid, dim3(256), 0, s.stream, g, s.pa, s.d_np, s.d_off, s.d_lim, s.d_E, s.d_B, pc); break;
+    default: hipLaunchKernelGGL((k_push_velocity<1, 4>), grid, dim3(256), 0, s.stream, g, s.pa, s.d_np, s.d_off, s.d_lim, s.d_E, s.d_B, pc); break;
     }
   } else {
     switch (g.order) {

@@ -77,6 +77,11 @@ CASES = {
     "thermal3d_o4": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=4, order=4)),
     "twostream2d_o1": (2, lambda: helpers.small_twostream(N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1)),
     "twostream2d_o2": (2, lambda: helpers.small_twostream(N=(1, 16, 64), C=(1, 2, 4), np_=16, order=2)),
+    # 1D (Nz = Ny = 1): Yee 1D, 1D gather, Esirkepov 1D
+    "twostream1d_o1": (1, lambda: helpers.small_twostream(N=(1, 1, 128), C=(1, 1, 8), np_=64, order=1)),
+    "thermal1d_o2": (1, lambda: helpers.small_thermal(N=(1, 1, 64), C=(1, 1, 4), np_=32, order=2)),
+    "thermal1d_o3": (1, lambda: helpers.small_thermal(N=(1, 1, 64), C=(1, 1, 4), np_=32, order=3)),
+    "thermal1d_o4": (1, lambda: helpers.small_thermal(N=(1, 1, 64), C=(1, 1, 4), np_=32, order=4)),
 }
 
 
@@ -178,6 +183,9 @@ OPTION_CASES = {
     "wt_hc_2d_o3": (2, lambda: _with_fields(helpers.small_twostream(
         N=(1, 16, 64), C=(1, 1, 4), np_=16, order=3, interpolation="WT", pusher="HigueraCary"),
         E=(0.0, 0.001, 0.0), B=(0.0, 0.1, 0.2))),
+    "friedman_wt_vay_1d_o2": (1, lambda: _with_fields(helpers.small_thermal(
+        N=(1, 1, 64), C=(1, 1, 4), np_=32, order=2, friedman=0.2, interpolation="WT",
+        pusher="Vay"), E=(0.001, 0.0, 0.0), B=(0.1, 0.2, 0.3))),
 }
 
 
