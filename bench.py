@@ -69,10 +69,10 @@ PHASE_KERNEL = {"bfd": "k_push_bfd", "velocity": "k_push_velocity_rows",
 
 def profile_traffic(config, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    profiles/hbm_traffic_<config>_fp64.txt: FETCH_SIZE + WRITE_SIZE per dispatch,
+    profiles/hbm_traffic_<config>.txt: FETCH_SIZE + WRITE_SIZE per dispatch,
     each scaled by the copy-probe calibration recorded in the same file.
     None when the summary or the kernel is missing."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}_fp64.txt")
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}.txt")
     if not os.path.exists(path):
         return None
     total, seen = 0.0, 0
