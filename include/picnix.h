@@ -127,6 +127,13 @@ int pn_get_diverror(pn_sim* sim, double* efd, double* bfd);
 /* Block until queued device work on the handle's stream is done. */
 int pn_synchronize(pn_sim* sim);
 
+/* Upstream boundary state of one chunk for [application] problem = 'shock' (the
+ * values MainChunk keeps in option["random_seed"] and option["boundary"],
+ * example/shock/main.cpp:19-24,212-226,440-442,528-534): out6 = {random_seed,
+ * lastid, influx[0], influx[1], efflux[0], efflux[1]}.  PN_ERR_STATE for other
+ * problems. */
+int pn_get_boundary_state(pn_sim* sim, int chunk, int64_t* out6);
+
 /* Snapshot outputs in the reference's on-disk format (one process, "mpiio"
  * layout): <dir>/<step %08d>.data (raw native-endian f8, chunk after chunk in
  * chunk-id order) + <dir>/<step %08d>.json (nixio metadata per dataset, meta

@@ -534,6 +534,108 @@ struct Normal {
   }
 };
 
+// libstdc++ gamma_distribution<double> (Marsaglia and Tsang, ACM TOMS 26 (2000)
+// 363-372, as written in bits/random.tcc): the normal member keeps its cached
+// second value between calls.
+struct Gamma {
+  double alpha, beta, malpha, a2;
+  Normal nd;
+  Gamma(double a, double b) : alpha(a), beta(b)
+  {
+    malpha          = alpha < 1.0 ? alpha + 1.0 : alpha;
+    const double a1 = malpha - 1.0 / 3.0;
+    a2              = 1.0 / std::sqrt(9.0 * a1);
+  }
+  double operator()(MT64& g)
+  {
+    double       u, v, n;
+    const double a1 = malpha - 1.0 / 3.0;
+    do {
+      do {
+        n = nd(g);
+        v = 1.0 + a2 * n;
+      } while (v <= 0.0);
+      v = v * v * v;
+      u = canonical(g);
+    } while (u > 1.0 - 0.0331 * n * n * n * n &&
+             (std::log(u) > (0.5 * n * n + a1 * (1.0 - v + std::log(v)))));
+    if (alpha == malpha)
+      return a1 * v * beta;
+    do
+      u = canonical(g);
+    while (u == 0.0);
+    return std::pow(u, 1.0 / alpha) * a1 * v * beta;
+  }
+};
+
+// libstdc++ poisson_distribution<int> for mean < 12: multiply uniforms until the
+// product drops to exp(-mean) (bits/random.tcc; Devroye 1986, Ch. X).  Means of
+// 12 and above take libstdc++'s rejection branch, which is not restated here.
+inline int poisson_small(MT64& g, double lm_thr)
+{
+  int    x    = 0;
+  double prod = 1.0;
+  do {
+    prod *= canonical(g);
+    x += 1;
+  } while (prod > lm_thr);
+  return x - 1;
+}
+
+// nix::MaxwellJuttner (nix/random.hpp:27-163; Zenitani, Phys. Plasmas 22, 042116
+// (2015)): Lorentz factor by the modified Canfield method, isotropic direction,
+// then the flipping-method boost along x.
+struct MaxwellJuttner {
+  static constexpr double a = 0.56;
+  static constexpr double b = 0.35;
+  double                  temperature, drift;
+  double                  table[5];
+  std::vector<Gamma>      gamma;
+  MaxwellJuttner(double T, double u0) : temperature(T), drift(u0)
+  {
+    table[0] = 0;
+    table[1] = std::sqrt(M_PI);
+    table[2] = a * std::sqrt(2 * T);
+    table[3] = b * 1.5 * std::sqrt(M_PI) * T;
+    table[4] = std::sqrt(2 * T) * (2 * T);
+    for (int i = 1; i < 5; i++) // std::partial_sum
+      table[i] = table[i - 1] + table[i];
+    const double last = table[4];
+    for (int i = 0; i < 5; i++) // normalised by the total (the lambda reads back())
+      table[i] = table[i] / last;
+    gamma = {Gamma(1.5, T), Gamma(2.0, T), Gamma(2.5, T), Gamma(3.0, T)};
+  }
+  void set_drift(double u0) { drift = u0; }
+  void operator()(MT64& g, double& ux, double& uy, double& uz)
+  {
+    double xx     = -1;
+    bool   status = false;
+    do {
+      double r1    = uniform01(g);
+      double r2    = uniform01(g);
+      int    index = (int)(std::upper_bound(table, table + 5, r1) - table) - 1;
+      xx           = gamma[index](g);
+      double R1    = (xx + 1) * std::sqrt(xx + 2);
+      double R2 = std::sqrt(2) + a * std::sqrt(xx) + b * std::sqrt(2) * xx + std::pow(xx, 1.5);
+      status    = (r2 > R1 / R2);
+    } while (status);
+    double uu = std::sqrt(xx * (xx + 2));
+    double r3 = uniform01(g);
+    double r4 = uniform01(g);
+    ux        = uu * (2 * r3 - 1);
+    uy        = uu * 2 * std::sqrt(r3 * (1 - r3)) * std::cos(2 * M_PI * r4);
+    uz        = uu * 2 * std::sqrt(r3 * (1 - r3)) * std::sin(2 * M_PI * r4);
+    // lorentz_boost (nix/random.hpp:144-156)
+    double rr = uniform01(g);
+    double Gm = std::sqrt(1 + drift * drift);
+    double Vx = drift / Gm;
+    double gm = std::sqrt(1 + ux * ux + uy * uy + uz * uz);
+    double vx = ux / gm;
+    ux        = (-Vx * vx > rr) ? -ux : +ux;
+    ux        = Gm * (ux + Vx * gm);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // chunk map: generalized Hilbert ("gilbert") curve, nix/sfc.cpp:31-141,200-509
 // (published algorithm of J. Cerveny, https://github.com/jakubcerveny/gilbert)
@@ -761,7 +863,15 @@ struct Chunk {
   double lim[3][2];    // physical range (z, y, x), nix/chunk.cpp:210-237
   double glim[3][2];   // global physical range
   double rng[3][2];    // get_{z,y,x}range: +-max for absent dimensions (nix/chunk.hpp:295-335)
-  int    nb_id[27];
+  int    nb_id[27]; // -1: no neighbour (a physical boundary, MPI_PROC_NULL)
+
+  // example/shock: physical walls in x and the state the example keeps in
+  // option["random_seed"] / option["boundary"] (example/shock/main.cpp:19-24,212-226)
+  bool    wall_lo = false, wall_hi = false;
+  int     random_seed = 0;
+  int64_t lastid      = 0;
+  int     influx[2] = {0, 0}, efflux[2] = {0, 0};
+  MT64    mt;
 
   std::vector<double>    uf, uj, ff, um;
   std::vector<Particles> up;
@@ -790,7 +900,16 @@ struct Chunk {
   int ihi(int d) const { return has_dim[d] ? 2 : 1; }
 };
 
+// example/shock/main.cpp:56-92 derived constants
+struct ShockParams {
+  int    nppc = 0;
+  double delt = 0, u0 = 0, vte = 0, vti = 0, taper = 0;
+  double E0[3] = {0, 0, 0}, B0[3] = {0, 0, 0};
+  double me = 0, qe = 0, mi = 0, qi = 0;
+};
+
 struct Sim {
+  ShockParams        sh;
   pno_params         p;
   int                dim;
   int                order;
@@ -1045,17 +1164,20 @@ void push_efd(Chunk& c, int dim, double cc, double theta, double delt)
   }
 }
 
-// pic/engine/maxwell.hpp:63-83 (1D), :203-229 (2D), :350-381 (3D); buffers are zero for
-// periodic runs
+// pic/engine/maxwell.hpp:63-83 (1D), :203-229 (2D), :350-381 (3D); the x buffers are
+// zero for periodic runs
 void get_diverror(Chunk& c, int dim, double& efd, double& bfd)
 {
   const double rdx = 1 / c.del[2], rdy = 1 / c.del[1], rdz = 1 / c.del[0];
   efd = 0;
   bfd = 0;
+  // cells next to a physical boundary are skipped (pic/pic_engine.hpp:42-71)
+  const int lbx = c.Lb[2] + (c.wall_lo ? c.Nb : 0);
+  const int ubx = c.Ub[2] - (c.wall_hi ? c.Nb : 0);
   if (dim == 3) {
     for (int iz = c.Lb[0]; iz <= c.Ub[0]; iz++)
       for (int iy = c.Lb[1]; iy <= c.Ub[1]; iy++)
-        for (int ix = c.Lb[2]; ix <= c.Ub[2]; ix++) {
+        for (int ix = lbx; ix <= ubx; ix++) {
           efd += (c.UF(iz, iy, ix + 1, 0) - c.UF(iz, iy, ix, 0)) * rdx +
                  (c.UF(iz, iy + 1, ix, 1) - c.UF(iz, iy, ix, 1)) * rdy +
                  (c.UF(iz + 1, iy, ix, 2) - c.UF(iz, iy, ix, 2)) * rdz - c.UJ(iz, iy, ix, 0);
@@ -1065,14 +1187,14 @@ void get_diverror(Chunk& c, int dim, double& efd, double& bfd)
         }
   } else if (dim == 1) {
     const int iz = c.Lb[0], iy = c.Lb[1];
-    for (int ix = c.Lb[2]; ix <= c.Ub[2]; ix++) {
+    for (int ix = lbx; ix <= ubx; ix++) {
       efd += (c.UF(iz, iy, ix + 1, 0) - c.UF(iz, iy, ix, 0)) * rdx - c.UJ(iz, iy, ix, 0);
       bfd += (c.UF(iz, iy, ix, 3) - c.UF(iz, iy, ix - 1, 3)) * rdx;
     }
   } else {
     const int iz = c.Lb[0];
     for (int iy = c.Lb[1]; iy <= c.Ub[1]; iy++)
-      for (int ix = c.Lb[2]; ix <= c.Ub[2]; ix++) {
+      for (int ix = lbx; ix <= ubx; ix++) {
         efd += (c.UF(iz, iy, ix + 1, 0) - c.UF(iz, iy, ix, 0)) * rdx +
                (c.UF(iz, iy + 1, ix, 1) - c.UF(iz, iy, ix, 1)) * rdy - c.UJ(iz, iy, ix, 0);
         bfd += (c.UF(iz, iy, ix, 3) - c.UF(iz, iy, ix - 1, 3)) * rdx +
@@ -1254,7 +1376,9 @@ void push_velocity_chunk(Chunk& c, int Ns, double cc, int pusher, int interp, do
 }
 
 // pic/engine/position.hpp:55-68,117-130 then Position::set_boundary (pic/pic_engine.hpp:292-303)
-void push_position_chunk(Chunk& c, int Ns, double cc, int order, double delt)
+void shock_boundary_particle(Sim& s, Chunk& c);
+
+void push_position_chunk(Sim& s, Chunk& c, int Ns, double cc, int order, double delt)
 {
   double rc = 1 / cc;
   for (int is = 0; is < Ns; is++) {
@@ -1271,7 +1395,10 @@ void push_position_chunk(Chunk& c, int Ns, double cc, int order, double delt)
       xu[2] += xu[5] * dt;
     }
   }
-  // set_boundary_particle is a no-op for periodic runs (pic/pic_chunk.cpp:483-485)
+  // set_boundary_particle: a no-op for periodic runs (pic/pic_chunk.cpp:483-485),
+  // the wall and upstream boundary for the shock
+  if (s.p.problem == 1)
+    shock_boundary_particle(s, c);
   for (int is = 0; is < Ns; is++)
     count(c, c.up[is], 0, c.up[is].Np - 1, true, order);
 }
@@ -1481,15 +1608,23 @@ void box_op(const Chunk& src, const std::vector<double>& sa, const int slo[3], C
 // Cur / Mom (XtensorHaloCurrent3D / Moment3D: send ghost -> neighbour interior edge, add).
 // Receiver R unpacks slot (iz,iy,ix) from neighbour S = nb_R(slot), whose message was
 // packed for direction -slot (MPI matching of nix/chunk.hpp:465-524).
+void set_boundary_emf(const Sim& s, Chunk& c);
+void set_boundary_mom(const Sim& s, Chunk& c);
+
 void exchange_field(Sim& s, int mode)
 {
   const bool add = (mode != 0);
+  // the split-independent order resets the walls before the neighbours copy them
+  if (mode == 0 && s.p.problem == 1 && s.p.emf_order == 1)
+    for_chunks(s, [&](Chunk& c) { set_boundary_emf(s, c); });
   for_chunks(s, [&](Chunk& R) {
     for (int iz = R.ilo(0); iz <= R.ihi(0); iz++)
       for (int iy = R.ilo(1); iy <= R.ihi(1); iy++)
         for (int ix = R.ilo(2); ix <= R.ihi(2); ix++) {
           if (iz == 1 && iy == 1 && ix == 1)
             continue;
+          if (R.nb_id[dir_index(iz, iy, ix)] < 0)
+            continue; // MPI_PROC_NULL: nothing is sent or received
           const Chunk& S   = s.chunks[R.nb_id[dir_index(iz, iy, ix)]];
           const int    o[3] = {2 - iz, 2 - iy, 2 - ix}; // sender's slot
           const int    r[3] = {iz, iy, ix};
@@ -1516,13 +1651,25 @@ void exchange_field(Sim& s, int mode)
                    [](double& a, double b) { a += b; });
         }
   });
+  // physical boundary after the unpack (pic/pic_chunk.cpp:339-340 -> set_boundary_field)
+  if (s.p.problem == 1) {
+    if (mode == 0)
+      for_chunks(s, [&](Chunk& c) { set_boundary_emf(s, c); });
+    else if (mode == 2)
+      for_chunks(s, [&](Chunk& c) { set_boundary_mom(s, c); });
+  }
 }
 
 // particle migration: XtensorHaloParticle3D::{pre_pack, pack, pre_unpack, unpack,
 // post_unpack} (nix/xtensor_halo3d.hpp:192-499)
+void shock_inject_particle(Sim& s, Chunk& c);
+
 void exchange_particle(Sim& s)
 {
   const int nc = (int)s.chunks.size();
+  // set_boundary_pack(BoundaryParticle) injects first (pic/pic_chunk.cpp:305-309)
+  if (s.p.problem == 1)
+    for_chunks(s, [&](Chunk& c) { shock_inject_particle(s, c); });
   // packed leavers: [chunk][species][27 directions] -> flat records
   std::vector<std::vector<std::vector<double>>> sendbuf(nc);
   for_chunks(s, [&](Chunk& S) {
@@ -1553,6 +1700,8 @@ void exchange_particle(Sim& s)
             if (iz == 1 && iy == 1 && ix == 1)
               continue;
             int   sid = R.nb_id[dir_index(iz, iy, ix)];
+            if (sid < 0)
+              continue;
             auto& b   = sendbuf[sid][is * 27 + dir_index(2 - iz, 2 - iy, 2 - ix)];
             nrecv += (int)(b.size() / NC);
           }
@@ -1564,6 +1713,8 @@ void exchange_particle(Sim& s)
             if (iz == 1 && iy == 1 && ix == 1)
               continue;
             int   sid = R.nb_id[dir_index(iz, iy, ix)];
+            if (sid < 0)
+              continue;
             auto& b   = sendbuf[sid][is * 27 + dir_index(2 - iz, 2 - iy, 2 - ix)];
             if (!b.empty())
               std::memcpy(P.XU(pos), b.data(), b.size() * sizeof(double));
@@ -1625,13 +1776,393 @@ struct MomentOp {
 };
 
 // --------------------------------------------------------------------------
+// example/shock: reflecting wall at x = 0 and upstream injection at x = Lx on the
+// chunks whose x neighbour is MPI_PROC_NULL (set_periodicity(1, 1, 0),
+// example/shock/main.cpp:550-558)
+// --------------------------------------------------------------------------
+// example/shock/main.cpp:50-54
+double initial_flow_profile(const Chunk& c, double x, double L)
+{
+  double xmin = c.glim[2][0] + c.Nb * c.del[2];
+  return -(x - xmin < L ? std::sin(0.5 * M_PI * (x - xmin) / L) : 1.0);
+}
+
+// density extension n(x) (1 everywhere for density = 1, the reference's loader)
+double shock_density(const pno_params& p, double x, double delh)
+{
+  if (p.density == 1.0)
+    return 1.0;
+  double xc = x / delh;
+  if (xc < p.ramp_x)
+    return 1.0;
+  if (xc < p.ramp_x + p.ramp_w)
+    return 1.0 + (p.density - 1.0) * (xc - p.ramp_x) / p.ramp_w;
+  return p.density;
+}
+
+int shock_npc(const pno_params& p, double x, double delh)
+{
+  return (int)std::floor(p.nppc * shock_density(p, x, delh) + 0.5);
+}
+
+// MainChunk::reset_random_number (example/shock/main.cpp:19-24)
+void reset_random_number(Chunk& c)
+{
+  c.random_seed += 1;
+  c.mt.seed((uint64_t)(int64_t)c.random_seed);
+}
+
+// MainChunk::generate_injection_particle (example/shock/main.cpp:26-48), c = 1
+void generate_injection_particle(Chunk& c, MaxwellJuttner& mj, double delt, double& x,
+                                 double& ux, double& uy, double& uz)
+{
+  const double cc = 1.0;
+  const double rc = 1.0 / cc;
+  mj(c.mt, ux, uy, uz);
+  double xx = c.lim[2][1] + cc * delt * uniform01(c.mt);
+  double dx = ux * delt / lorentz_factor(ux, uy, uz, rc);
+  while (xx + dx >= c.lim[2][1]) {
+    mj(c.mt, ux, uy, uz);
+    xx = c.lim[2][1] + cc * delt * uniform01(c.mt);
+    dx = ux * delt / lorentz_factor(ux, uy, uz, rc);
+  }
+  x = xx + dx;
+}
+
+// MainChunk::set_boundary_emf (example/shock/main.cpp:232-340): conducting wall
+// over the first Nb cells and the lower ghosts; fixed upstream fields over the
+// last Nb cells and the upper ghosts.
+void set_boundary_emf(const Sim& s, Chunk& c)
+{
+  const int Nb  = c.Nb;
+  const int Lbx = c.Lb[2], Ubx = c.Ub[2], Lby = c.Lb[1], Uby = c.Ub[1], Lbz = c.Lb[0],
+            Ubz = c.Ub[0];
+  if (c.wall_lo) {
+    for (int ix = 0; ix < 2 * Nb; ix++) {
+      int ix1 = Lbx - ix + Nb - 1;
+      for (int iz = Lbz - Nb; iz <= Ubz + Nb; iz++)
+        for (int iy = Lby - Nb; iy <= Uby + Nb; iy++) {
+          c.UF(iz, iy, ix1, 1) = 0;
+          c.UF(iz, iy, ix1, 2) = 0;
+        }
+    }
+    for (int ix = 0; ix < 2 * Nb; ix++) {
+      int ix1 = Lbx - ix + Nb - 1;
+      for (int iz = Lbz - Nb; iz <= Ubz + Nb - 1; iz++)
+        for (int iy = Lby - Nb; iy <= Uby + Nb - 1; iy++)
+          c.UF(iz, iy, ix1, 0) = 0;
+    }
+    for (int ix = 0; ix < 2 * Nb; ix++) {
+      int ix1 = Lbx - ix + Nb - 1;
+      int ix2 = Lbx + Nb;
+      for (int iz = Lbz - Nb; iz <= Ubz + Nb; iz++)
+        for (int iy = Lby - Nb; iy <= Uby + Nb; iy++) {
+          c.UF(iz, iy, ix1, 4) = c.UF(iz, iy, ix2, 4);
+          c.UF(iz, iy, ix1, 5) = c.UF(iz, iy, ix2, 5);
+        }
+    }
+    for (int ix = 0; ix < 2 * Nb; ix++) {
+      int ix1 = Lbx - ix + Nb - 1;
+      int ix2 = Lbx + Nb;
+      for (int iz = Lbz - Nb + 1; iz <= Ubz + Nb; iz++)
+        for (int iy = Lby - Nb + 1; iy <= Uby + Nb; iy++)
+          c.UF(iz, iy, ix1, 3) = c.UF(iz, iy, ix2, 3);
+    }
+  }
+  if (c.wall_hi) {
+    const double* E0 = s.sh.E0;
+    const double* B0 = s.sh.B0;
+    for (int ix = 0; ix < 2 * Nb; ix++) {
+      int ix1 = Ubx + ix - Nb + 1;
+      for (int iz = Lbz - Nb; iz <= Ubz + Nb; iz++)
+        for (int iy = Lby - Nb; iy <= Uby + Nb; iy++) {
+          c.UF(iz, iy, ix1, 1) = E0[1];
+          c.UF(iz, iy, ix1, 2) = E0[2];
+        }
+    }
+    for (int ix = 0; ix < 2 * Nb - 1; ix++) {
+      int ix1 = Ubx + ix - Nb + 2;
+      for (int iz = Lbz - Nb; iz <= Ubz + Nb - 1; iz++)
+        for (int iy = Lby - Nb; iy <= Uby + Nb - 1; iy++)
+          c.UF(iz, iy, ix1, 0) = E0[0];
+    }
+    for (int ix = 0; ix < 2 * Nb - 1; ix++) {
+      int ix1 = Ubx + ix - Nb + 2;
+      for (int iz = Lbz - Nb; iz <= Ubz + Nb; iz++)
+        for (int iy = Lby - Nb; iy <= Uby + Nb; iy++) {
+          c.UF(iz, iy, ix1, 4) = B0[1];
+          c.UF(iz, iy, ix1, 5) = B0[2];
+        }
+    }
+    for (int ix = 0; ix < 2 * Nb; ix++) {
+      int ix1 = Ubx + ix - Nb + 1;
+      for (int iz = Lbz - Nb + 1; iz <= Ubz + Nb; iz++)
+        for (int iy = Lby - Nb + 1; iy <= Uby + Nb; iy++)
+          c.UF(iz, iy, ix1, 3) = B0[0];
+    }
+  }
+}
+
+// MainChunk::set_boundary_mom (example/shock/main.cpp:342-383).  It indexes the
+// 5-D moment array um(z, y, x, species, k) with four indices; xtensor matches
+// too few indices to the trailing axes (nix/thirdparty/xtensor/xstrides.hpp:
+// 471-476), so um(iz, iy, ix, ik) is the element at flat offset
+// iz*s_y + iy*s_x + ix*s_species + ik of the z = 0 plane.  Restated literally.
+void set_boundary_mom(const Sim& s, Chunk& c)
+{
+  const int Nb  = c.Nb;
+  const int Lbx = c.Lb[2], Ubx = c.Ub[2], Lby = c.Lb[1], Uby = c.Ub[1], Lbz = c.Lb[0],
+            Ubz = c.Ub[0];
+  const size_t sk = 1, ss = 14, sx = 14 * (size_t)s.Ns, sy = sx * c.n[2];
+  auto at = [&](int a, int b, int d, int k) -> double& {
+    return c.um[(size_t)a * sy + (size_t)b * sx + (size_t)d * ss + (size_t)k * sk];
+  };
+  if (c.wall_lo)
+    for (int ix = 0; ix < Nb; ix++) {
+      int ix1 = Lbx - ix + Nb - 1;
+      int ix2 = Lbx + Nb;
+      for (int iz = Lbz - Nb; iz <= Ubz + Nb; iz++)
+        for (int iy = Lby - Nb; iy <= Uby + Nb; iy++)
+          for (int ik = 0; ik < 14; ik++)
+            at(iz, iy, ix1, ik) = at(iz, iy, ix2, ik);
+    }
+  if (c.wall_hi)
+    for (int ix = 0; ix < Nb; ix++) {
+      int ix1 = Ubx + ix - Nb + 1;
+      int ix2 = Ubx - Nb;
+      for (int iz = Lbz - Nb; iz <= Ubz + Nb; iz++)
+        for (int iy = Lby - Nb; iy <= Uby + Nb; iy++)
+          for (int ik = 0; ik < 14; ik++)
+            at(iz, iy, ix1, ik) = at(iz, iy, ix2, ik);
+    }
+}
+
+// MainChunk::set_boundary_particle (example/shock/main.cpp:398-443), called by
+// Position before the count (pic/pic_engine.hpp:292-303)
+void shock_boundary_particle(Sim& s, Chunk& c)
+{
+  if (c.wall_lo)
+    for (int is = 0; is < s.Ns; is++) {
+      Particles& P = c.up[is];
+      for (int ip = 0; ip < P.Np; ip++) {
+        double* xu = P.XU(ip);
+        if (xu[0] < c.glim[2][0]) {
+          xu[0] = -xu[0] + 2 * c.glim[2][0];
+          xu[3] = -xu[3];
+          xu[4] = -xu[4];
+          xu[5] = -xu[5];
+        }
+      }
+    }
+  if (c.wall_hi) {
+    reset_random_number(c);
+    MaxwellJuttner mj[2] = {MaxwellJuttner(s.sh.vte * s.sh.vte, -s.sh.u0),
+                            MaxwellJuttner(s.sh.vti * s.sh.vti, -s.sh.u0)};
+    for (int is = 0; is < s.Ns; is++) {
+      int        efflux = 0;
+      Particles& P      = c.up[is];
+      for (int ip = 0; ip < P.Np; ip++) {
+        double* xu = P.XU(ip);
+        if (xu[0] >= c.glim[2][1]) {
+          double x, ux, uy, uz;
+          generate_injection_particle(c, mj[is], s.sh.delt, x, ux, uy, uz);
+          xu[0] = x;
+          xu[3] = ux;
+          xu[4] = uy;
+          xu[5] = uz;
+          efflux += 1;
+        }
+      }
+      c.efflux[is] = efflux;
+    }
+  }
+}
+
+// MainChunk::inject_particle (example/shock/main.cpp:445-535), called from
+// set_boundary_pack(BoundaryParticle) (pic/pic_chunk.cpp:305-309)
+void shock_inject_particle(Sim& s, Chunk& c)
+{
+  if (!c.wall_hi)
+    return;
+  const double rc   = 1.0 / 1.0;
+  const double u0   = s.sh.u0;
+  const double npu  = s.p.density == 1.0 ? (double)s.sh.nppc
+                                          : (double)shock_npc(s.p, c.glim[2][1], c.del[2]);
+  const double flux = npu * std::abs(u0) / std::sqrt(1 + u0 * u0 * rc * rc) * s.sh.delt / c.del[2];
+  const int    Lbx = c.Lb[2], Lby = c.Lb[1], Uby = c.Ub[1], Lbz = c.Lb[0], Ubz = c.Ub[0];
+  (void)Lbx;
+  std::vector<int>     np_inj((size_t)c.n[0] * c.n[1], 0);
+  std::vector<int64_t> id64(s.Ns, c.lastid);
+  int                  np_inj_total = 0;
+  reset_random_number(c);
+  if (!(flux < 12))
+    return; // libstdc++'s rejection branch (mean >= 12) is not restated
+  const double   lm_thr = std::exp(-flux);
+  MaxwellJuttner mj[2]  = {MaxwellJuttner(s.sh.vte * s.sh.vte, -u0),
+                          MaxwellJuttner(s.sh.vti * s.sh.vti, -u0)};
+  for (int iz = Lbz; iz <= Ubz; iz++)
+    for (int iy = Lby; iy <= Uby; iy++) {
+      np_inj[(size_t)iz * c.n[1] + iy] = poisson_small(c.mt, lm_thr);
+      np_inj_total += np_inj[(size_t)iz * c.n[1] + iy];
+    }
+  for (int is = 0; is < s.Ns; is++)
+    c.up[is].reserve(c.up[is].Np + np_inj_total + 1);
+  for (int iz = Lbz; iz <= Ubz; iz++)
+    for (int iy = Lby; iy <= Uby; iy++)
+      for (int is = 0; is < s.Ns; is++) {
+        Particles& P       = c.up[is];
+        int        np_prev = P.Np;
+        int        np_next = P.Np + np_inj[(size_t)iz * c.n[1] + iy];
+        for (int ip = np_prev; ip < np_next; ip++) {
+          double x, ux, uy, uz;
+          generate_injection_particle(c, mj[is], s.sh.delt, x, ux, uy, uz);
+          double  y  = c.lim[1][0] + (iy - Lby + uniform01(c.mt)) * c.del[1];
+          double  z  = c.lim[0][0] + (iz - Lbz + uniform01(c.mt)) * c.del[0];
+          double* xu = P.XU(ip);
+          xu[0]      = x;
+          xu[1]      = y;
+          xu[2]      = z;
+          xu[3]      = ux;
+          xu[4]      = uy;
+          xu[5]      = uz;
+          std::memcpy(&xu[6], &id64[is], sizeof(int64_t));
+          id64[is]++;
+        }
+        if (np_next > np_prev)
+          count(c, P, np_prev, np_next - 1, false, s.order);
+        P.Np = np_next;
+      }
+  c.lastid += np_inj_total;
+  for (int is = 0; is < s.Ns; is++)
+    c.influx[is] = np_inj_total;
+}
+
+// --------------------------------------------------------------------------
 // set-up: PicApplication/Application chunk set-up + example loaders
 // --------------------------------------------------------------------------
+// example/shock/main.cpp:56-92: cc = 1 and Ns = 2 are fixed by the example
+void setup_shock_params(Sim& s)
+{
+  pno_params& p = s.p;
+  p.cc          = 1.0;
+  p.Ns          = 2;
+  ShockParams& h = s.sh;
+  const double cc = 1.0;
+  const double u0 = p.u0;
+  double gamma = std::sqrt(1.0 + (u0 * u0) / (cc * cc));
+  double me    = 1.0 / p.nppc;
+  double qe    = -p.wp / p.nppc * std::sqrt(gamma);
+  double mi    = me * p.mime;
+  double qi    = -qe;
+  double b0    = cc * std::sqrt(p.sigma) / std::abs(qe / me);
+  double vae   = cc * std::sqrt(p.sigma);
+  double vai   = cc * std::sqrt(p.sigma / p.mime);
+  h.nppc       = p.nppc;
+  h.delt       = p.delt;
+  h.u0         = u0;
+  h.taper      = p.taper;
+  h.vte        = vae * std::sqrt(0.5 * p.betae);
+  h.vti        = vai * std::sqrt(0.5 * p.betai);
+  h.B0[0]      = b0 * std::cos(p.theta / 180 * M_PI);
+  h.B0[1]      = b0 * std::sin(p.theta / 180 * M_PI) * std::cos(p.phi / 180 * M_PI);
+  h.B0[2]      = b0 * std::sin(p.theta / 180 * M_PI) * std::sin(p.phi / 180 * M_PI);
+  h.E0[0]      = 0;
+  h.E0[1]      = +(-u0) * h.B0[2] / (gamma * cc);
+  h.E0[2]      = -(-u0) * h.B0[1] / (gamma * cc);
+  h.me         = me;
+  h.qe         = qe;
+  h.mi         = mi;
+  h.qi         = qi;
+}
+
+// example/shock/main.cpp:94-137: fields over the ghost-inclusive block (the
+// position uses the array index, xi = (ix + 0.5) dx + xmin)
+void init_shock_fields(Sim& s, Chunk& c)
+{
+  const ShockParams& h  = s.sh;
+  const double       cc = 1.0;
+  const double       xmin = c.lim[2][0];
+  for (int iz = 0; iz < c.n[0]; iz++)
+    for (int iy = 0; iy < c.n[1]; iy++)
+      for (int ix = 0; ix < c.n[2]; ix++) {
+        double xi = (ix + 0.5) * c.del[2] + xmin;
+        double Ux = h.u0 * initial_flow_profile(c, xi, h.taper);
+        double gm = std::sqrt(1 + Ux * Ux / (cc * cc));
+        c.UF(iz, iy, ix, 0) = h.E0[0];
+        c.UF(iz, iy, ix, 1) = +Ux * h.B0[2] / (gm * cc);
+        c.UF(iz, iy, ix, 2) = -Ux * h.B0[1] / (gm * cc);
+        c.UF(iz, iy, ix, 3) = h.B0[0];
+        c.UF(iz, iy, ix, 4) = h.B0[1];
+        c.UF(iz, iy, ix, 5) = h.B0[2];
+      }
+}
+
+// example/shock/main.cpp:139-210: nppc particles per cell with a drifting
+// Maxwell-Juttner velocity, electrons and ions at the same positions, one
+// mt19937_64 per chunk
+void load_shock_particles(Sim& s, Chunk& c, const int gdims[3])
+{
+  const pno_params&  p = s.p;
+  const ShockParams& h = s.sh;
+  c.mt.seed((uint64_t)(int64_t)c.random_seed);
+  MaxwellJuttner mj[2] = {MaxwellJuttner(h.vte * h.vte, 0.0), MaxwellJuttner(h.vti * h.vti, 0.0)};
+  const int npc_max = p.density == 1.0 ? p.nppc
+                                       : (int)std::floor(p.nppc * std::max(1.0, p.density) + 0.5);
+  int       mp    = npc_max * c.dims[0] * c.dims[1] * c.dims[2];
+  int       cz    = gdims[0] / c.dims[0];
+  int       cy    = gdims[1] / c.dims[1];
+  int64_t   maxid = std::numeric_limits<int64_t>::max() / (cy * cz);
+  int64_t   ciz   = c.offset[0] / c.dims[0];
+  int64_t   ciy   = c.offset[1] / c.dims[1];
+  int64_t   cix   = c.offset[2] / c.dims[2];
+  c.lastid        = (ciy + ciz * cy) * maxid + cix * mp;
+  for (int is = 0; is < 2; is++) {
+    Particles& P = c.up[is];
+    P.m          = is == 0 ? h.me : h.mi;
+    P.q          = is == 0 ? h.qe : h.qi;
+    P.reserve((int)(mp * (1 + p.buffer_ratio)) + 1);
+  }
+  int ip = 0;
+  for (int iz = c.Lb[0]; iz <= c.Ub[0]; iz++)
+    for (int iy = c.Lb[1]; iy <= c.Ub[1]; iy++)
+      for (int ix = c.Lb[2]; ix <= c.Ub[2]; ix++) {
+        int npc = p.nppc;
+        if (p.density != 1.0)
+          npc = shock_npc(p, c.lim[2][0] + (ix - c.Lb[2] + 0.5) * c.del[2], c.del[2]);
+        for (int jp = 0; jp < npc; ip++, jp++) {
+          double x = c.lim[2][0] + (ix - c.Lb[2] + uniform01(c.mt)) * c.del[2];
+          double y = c.lim[1][0] + (iy - c.Lb[1] + uniform01(c.mt)) * c.del[1];
+          double z = c.lim[0][0] + (iz - c.Lb[0] + uniform01(c.mt)) * c.del[0];
+          double U = h.u0 * initial_flow_profile(c, x, h.taper);
+          for (int is = 0; is < 2; is++) {
+            mj[is].set_drift(U);
+            double ux, uy, uz;
+            mj[is](c.mt, ux, uy, uz);
+            double* xu = c.up[is].XU(ip);
+            xu[0]      = x;
+            xu[1]      = y;
+            xu[2]      = z;
+            xu[3]      = ux;
+            xu[4]      = uy;
+            xu[5]      = uz;
+            int64_t id = c.lastid + ip;
+            std::memcpy(&xu[6], &id, sizeof(id));
+          }
+        }
+      }
+  for (int is = 0; is < 2; is++)
+    c.up[is].Np = ip;
+  c.lastid += mp;
+}
+
 void setup(Sim& s)
 {
+  if (s.p.problem == 1)
+    setup_shock_params(s);
   const pno_params& p = s.p;
   s.Ns                = p.Ns;
   s.order             = p.order;
+  const bool xper     = p.problem != 1; // set_periodicity(1, 1, 0) for the shock
   const int N[3]      = {p.Nz, p.Ny, p.Nx};
   const int C[3]      = {p.Cz, p.Cy, p.Cx};
   bool      has_dim[3];
@@ -1677,6 +2208,10 @@ void setup(Sim& s)
       for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++) {
           int cc3[3] = {c.coord[0] + dz, c.coord[1] + dy, c.coord[2] + dx};
+          if (!xper && (cc3[2] < 0 || cc3[2] >= C[2])) {
+            c.nb_id[dir_index(dz + 1, dy + 1, dx + 1)] = -1; // MPI_PROC_NULL
+            continue;
+          }
           for (int d = 0; d < 3; d++) {
             cc3[d] = cc3[d] >= 0 ? cc3[d] : C[d] - 1;
             cc3[d] = cc3[d] < C[d] ? cc3[d] : 0;
@@ -1690,6 +2225,14 @@ void setup(Sim& s)
     c.uj.assign(ncell * 4, 0.0);
     c.ff.assign(ncell * 9, 0.0);
     c.um.assign(ncell * s.Ns * 14, 0.0);
+    c.wall_lo     = !xper && c.coord[2] == 0;
+    c.wall_hi     = !xper && c.coord[2] == C[2] - 1;
+    c.random_seed = c.id + p.seed_offset; // seed_type = 'fixed' (pic/pic_chunk.cpp:243-254)
+    if (p.problem == 1) {
+      init_shock_fields(s, c);
+      init_friedman(c);
+      continue;
+    }
     for (int iz = c.Lb[0]; iz <= c.Ub[0]; iz++)
       for (int iy = c.Lb[1]; iy <= c.Ub[1]; iy++)
         for (int ix = c.Lb[2]; ix <= c.Ub[2]; ix++) {
@@ -1703,6 +2246,30 @@ void setup(Sim& s)
     init_friedman(c);
   }
 
+  if (p.problem == 1) {
+    for_chunks(s, [&](Chunk& c) {
+      int Ng = 1;
+      for (int d = 0; d < 3; d++)
+        Ng *= (c.Ub[d] - c.Lb[d] + 2 * c.Nb + 1);
+      c.up.assign(s.Ns, Particles());
+      for (int is = 0; is < s.Ns; is++) {
+        Particles& P = c.up[is];
+        P.m          = is == 0 ? s.sh.me : s.sh.mi;
+        P.q          = is == 0 ? s.sh.qe : s.sh.qi;
+        P.Ng         = Ng;
+        P.pindex.assign(Ng + 1, 0);
+        P.pcount.assign((size_t)(Ng + 1) * SIMD_WIDTH, 0);
+      }
+      if (p.load)
+        load_shock_particles(s, c, N);
+      for (int is = 0; is < s.Ns; is++) {
+        count(c, c.up[is], 0, c.up[is].Np - 1, true, s.order);
+        sort(c.up[is]);
+      }
+    });
+    exchange_field(s, 0);
+    return;
+  }
   // particles: example/beam/main.cpp:67-116 (thermal is the vx=vy=vz=0 case)
   for_chunks(s, [&](Chunk& c) {
     int  nb = c.Nb;
@@ -1861,7 +2428,7 @@ void pno_push_velocity(pno_sim* h, double delt)
 void pno_push_position(pno_sim* h, double delt)
 {
   Sim& s = h->s;
-  for_chunks(s, [&](Chunk& c) { push_position_chunk(c, s.Ns, s.p.cc, s.order, delt); });
+  for_chunks(s, [&](Chunk& c) { push_position_chunk(s, c, s.Ns, s.p.cc, s.order, delt); });
 }
 
 void pno_deposit_current(pno_sim* h, double delt)
@@ -1904,7 +2471,7 @@ void pno_step(pno_sim* h)
   for_chunks(s, [&](Chunk& c) {
     push_bfd(c, s.dim, s.p.cc, s.p.friedman, 0.5 * delt);
     dispatch<VelocityOp>(s.dim, s.order, c, s.Ns, s.p.cc, s.p.pusher, s.p.interp, delt);
-    push_position_chunk(c, s.Ns, s.p.cc, s.order, delt);
+    push_position_chunk(s, c, s.Ns, s.p.cc, s.order, delt);
     dispatch<CurrentOp>(s.dim, s.order, c, s.Ns, delt);
     push_bfd(c, s.dim, s.p.cc, s.p.friedman, 0.5 * delt);
   });
@@ -2073,6 +2640,45 @@ void pno_normal(uint64_t seed, int64_t n, double* out)
   Normal nd;
   for (int64_t i = 0; i < n; i++)
     out[i] = nd(g);
+}
+
+void pno_gamma(double alpha, double beta, uint64_t seed, int64_t n, double* out)
+{
+  MT64  g(seed);
+  Gamma gd(alpha, beta);
+  for (int64_t i = 0; i < n; i++)
+    out[i] = gd(g);
+}
+
+int pno_poisson(double mean, uint64_t seed, int64_t n, int* out)
+{
+  if (!(mean < 12))
+    return -1;
+  MT64         g(seed);
+  const double thr = std::exp(-mean);
+  for (int64_t i = 0; i < n; i++)
+    out[i] = poisson_small(g, thr);
+  return 0;
+}
+
+void pno_maxwell_juttner(double temperature, double drift, uint64_t seed, int64_t n,
+                         double* out3)
+{
+  MT64           g(seed);
+  MaxwellJuttner mj(temperature, drift);
+  for (int64_t i = 0; i < n; i++)
+    mj(g, out3[3 * i], out3[3 * i + 1], out3[3 * i + 2]);
+}
+
+void pno_shock_state(const pno_sim* h, int chunk, int64_t* out6)
+{
+  const Chunk& c = h->s.chunks[chunk];
+  out6[0]        = c.random_seed;
+  out6[1]        = c.lastid;
+  out6[2]        = c.influx[0];
+  out6[3]        = c.influx[1];
+  out6[4]        = c.efflux[0];
+  out6[5]        = c.efflux[1];
 }
 
 void pno_chunk_map(int Cz, int Cy, int Cx, int* id_of_coord)

@@ -40,6 +40,15 @@ typedef struct pno_params {
   double qm[PNO_MAX_SPECIES], ro[PNO_MAX_SPECIES], vt[PNO_MAX_SPECIES];
   double vx[PNO_MAX_SPECIES], vy[PNO_MAX_SPECIES], vz[PNO_MAX_SPECIES];
   int    interp;         /* 0 MC, 1 WT ([application.option] interpolation) */
+  /* problem = 1: example/shock (non-periodic x, reflecting wall at x = 0, upstream
+     injection at x = Lx); the [parameter] keys of example/shock/config.toml */
+  int    problem;
+  int    emf_order;      /* shock: 0 the reference's order (set_boundary_field after the
+                            EMF unpack), 1 also before the pack (split-independent) */
+  int    nppc;
+  double wp, mime, u0, theta, phi, sigma, betae, betai, taper;
+  double density, ramp_x, ramp_w; /* density n(x): 1 below ramp_x, linear ramp over
+                                     ramp_w cells, `density` above (1 = the reference) */
 } pno_params;
 
 typedef struct pno_sim pno_sim;
@@ -100,6 +109,15 @@ void pno_esirkepov1d(int order, const double* xv, const double* xu, double q, do
 uint64_t pno_mt19937_64(uint64_t seed, int64_t skip);
 void     pno_uniform(uint64_t seed, int64_t n, double* out);
 void     pno_normal(uint64_t seed, int64_t n, double* out);
+
+/* libstdc++ gamma_distribution<double>(alpha, beta) and poisson_distribution<int>(mean < 12)
+   driven by mt19937_64(seed); nix::MaxwellJuttner(temperature, drift) -> n x (ux, uy, uz) */
+void pno_gamma(double alpha, double beta, uint64_t seed, int64_t n, double* out);
+int  pno_poisson(double mean, uint64_t seed, int64_t n, int* out);
+void pno_maxwell_juttner(double temperature, double drift, uint64_t seed, int64_t n,
+                         double* out3);
+/* shock problem state of a chunk: [random_seed, lastid, influx[0..1], efflux[0..1]] */
+void pno_shock_state(const pno_sim* s, int chunk, int64_t* out6);
 
 /* ---- chunk map (generalized Hilbert curve) ---- */
 void pno_chunk_map(int Cz, int Cy, int Cx, int* id_of_coord);

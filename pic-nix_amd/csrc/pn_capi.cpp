@@ -15,6 +15,7 @@
 
 namespace picnix {
 std::vector<int> make_chunk_map(int Cz, int Cy, int Cx);
+void             shock_constants(Sim& s);
 }
 
 using namespace picnix;
@@ -27,17 +28,6 @@ Sim* picnix::sim_state(pn_sim* h) { return h ? &h->s : nullptr; }
 
 namespace {
 
-struct HipError {
-  std::string msg;
-};
-
-void hip_check(hipError_t e, const char* what)
-{
-  if (e != hipSuccess)
-    throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
-}
-
-#define HIPCHK(call) hip_check((call), #call)
 
 // Run `f` and convert failures into an error code + message on the handle.
 template <typename F>
@@ -158,7 +148,12 @@ void setup_geometry(Sim& s)
   }
   // species: example/thermal/main.cpp:86-88
   int mp_max = 0;
-  for (int sp = 0; sp < g.Ns; sp++) {
+  g.xwall    = cfg.problem == 1; // set_periodicity(1, 1, 0), example/shock/main.cpp:550-558
+  if (cfg.problem == 1) {
+    shock_constants(s);
+    mp_max = s.shk.npc_max * g.dims[0] * g.dims[1] * g.dims[2];
+  }
+  for (int sp = 0; sp < g.Ns && cfg.problem == 0; sp++) {
     const SpeciesConfig& sc = cfg.species[sp];
     g.m[sp]                 = sc.ro / sc.np;
     g.q[sp]                 = sc.qm * g.m[sp];
@@ -305,6 +300,7 @@ void release(Sim& s)
                   (void*)ss.nout, (void*)ss.recvoff, (void*)ss.npnew, (void*)ss.errflag})
     f(p);
   f(s.d_stage);
+  shock_release(s);
   if (s.stream)
     (void)hipStreamDestroy(s.stream);
 }
@@ -473,6 +469,11 @@ int pn_create(const char* toml, int load, pn_sim** out)
       return PN_ERR_UNSUPPORTED;
     }
     s.fp32 = s.cfg.precision == "fp32";
+    if (s.cfg.problem == 1 && (s.fp32 || s.cfg.friedman != 0.0)) {
+      s.err = "the shock problem is built for precision = fp64 without the Friedman filter";
+      *out  = h;
+      return PN_ERR_UNSUPPORTED;
+    }
     if (s.fp32 && s.cfg.N[0] == 1 && s.cfg.C[0] == 1 && s.cfg.N[1] == 1 && s.cfg.C[1] == 1) {
       s.err = "precision = fp32 is built for 2D and 3D domains; 1D runs use fp64";
       *out  = h;
@@ -484,10 +485,18 @@ int pn_create(const char* toml, int load, pn_sim** out)
     if (const char* k = std::getenv("PICNIX_MARCH_ROWS"))
       s.march_rows = std::max(0, std::atoi(k));
     allocate(s);
-    init_fields(s);
+    if (s.cfg.problem == 1) {
+      shock_setup(s);
+      shock_init_fields(s);
+    } else {
+      init_fields(s);
+    }
     launch_init_friedman(s); // PicChunk::init_friedman in the examples' setup
     if (load) {
-      load_particles(s);
+      if (s.cfg.problem == 1)
+        shock_load_particles(s);
+      else
+        load_particles(s);
       // initial sort, pic/pic_chunk.cpp:447-453 (count + sort per species)
       if (s.fp32)
         launch32_sort(s);
@@ -788,6 +797,8 @@ int pn_push_position(pn_sim* h, int chunk, double delt)
       launch32_push_position(s, delt);
     else
       launch_push_position(s, delt);
+    if (s.g.xwall)
+      shock_position_boundary(s);
     // the particle order is unchanged, so cellstart still describes the old
     // positions the deposit walks
     s.sorted = was_sorted;
@@ -860,6 +871,7 @@ int pn_exchange(pn_sim* h, int mode)
         launch_fill_ghost(s, s.d_E, 3);
         launch_fill_ghost(s, s.d_B, 3);
       }
+      launch_wall_emf(s); // set_boundary_field(BoundaryEmf) after the unpack
       return PN_OK;
     case PN_BOUNDARY_CUR:
       s.rho_valid = false;
@@ -869,6 +881,8 @@ int pn_exchange(pn_sim* h, int mode)
       s.rho_valid = false;
       if (!s.counted)
         return fail(s, PN_ERR_STATE, "particle exchange needs the count of pn_push_position");
+      if (s.g.xwall)
+        shock_inject(s); // set_boundary_pack(BoundaryParticle), pic/pic_chunk.cpp:305-309
       if (s.fp32)
         launch32_migrate(s);
       else
@@ -879,7 +893,7 @@ int pn_exchange(pn_sim* h, int mode)
         return fail(s, PN_ERR_UNSUPPORTED, "moments are not built for precision = fp32");
       if (!s.d_M)
         return fail(s, PN_ERR_STATE, "moment exchange needs pn_deposit_moment first");
-      launch_fold(s, s.d_M, 0, 14 * s.g.Ns, PN_PHASE_RHO);
+      launch_moment_exchange(s, false);
       return PN_OK;
     default:
       return fail(s, PN_ERR_ARG, "invalid boundary mode");
@@ -919,8 +933,35 @@ void step_launches32(Sim& s, double delt)
   s.rho_valid = true;
 }
 
+// The shock problem: the same operator order with the physical x boundaries --
+// the wall reflection and the upstream redraw in the position push, the EMF
+// boundary after the field update, the injection before the migration.  The
+// injected particles carry charge that no current brought, so rho is deposited
+// every step rather than carried by continuity.
+void step_launches_wall(Sim& s, double delt)
+{
+  launch_push_bfd(s, 0.5 * delt);
+  launch_push_velocity(s, delt);
+  bool was_sorted = s.sorted;
+  launch_push_position(s, delt);
+  shock_position_boundary(s);
+  s.sorted = was_sorted;
+  launch_deposit(s, delt, RhoMode::Now);
+  launch_push_bfd(s, 0.5 * delt);
+  launch_fold_current(s, 0, 4);
+  launch_push_efd(s, delt);
+  launch_wall_emf(s);
+  shock_inject(s);
+  launch_migrate(s);
+  s.rho_valid = false;
+}
+
 void step_launches(Sim& s, double delt)
 {
+  if (s.g.xwall) {
+    step_launches_wall(s, delt);
+    return;
+  }
   if (s.fp32) {
     step_launches32(s, delt);
     return;
@@ -961,7 +1002,8 @@ int pn_step(pn_sim* h, double delt)
     // a replayed graph needs the state the capture saw: sorted particles and
     // no phase timing (event records are host-side bookkeeping)
     // (and rho carried from the previous step: the recording uses the continuity update)
-    if (!s.step_graph || !s.sorted || s.timing || !s.rho_valid) {
+    // (the shock problem's boundary draws run on the host between launches)
+    if (!s.step_graph || !s.sorted || s.timing || !s.rho_valid || s.g.xwall) {
       step_launches(s, delt);
       return PN_OK;
     }
@@ -1034,6 +1076,23 @@ int pn_get_diverror(pn_sim* h, double* efd, double* bfd)
       *efd = out[2];
     if (bfd)
       *bfd = out[3];
+    return PN_OK;
+  });
+}
+
+int pn_get_boundary_state(pn_sim* h, int chunk, int64_t* out6)
+{
+  return guarded(h, [&](Sim& s) {
+    if (!out6 || chunk < 0 || chunk >= s.g.nchunk)
+      return fail(s, PN_ERR_ARG, "invalid chunk or buffer");
+    if (!s.g.xwall)
+      return fail(s, PN_ERR_STATE, "no physical boundary state: the problem is periodic");
+    out6[0] = s.shk.random_seed[chunk];
+    out6[1] = s.shk.lastid[chunk];
+    out6[2] = s.shk.influx[2 * chunk];
+    out6[3] = s.shk.influx[2 * chunk + 1];
+    out6[4] = s.shk.efflux[2 * chunk];
+    out6[5] = s.shk.efflux[2 * chunk + 1];
     return PN_OK;
   });
 }

@@ -147,10 +147,42 @@ bool build_config(const TomlMap& kv, Config& cfg, std::string& err)
       return false;
     }
   }
-  if (!get_num(kv, "parameter.delt", cfg.delt, true, err) ||
-      !get_num(kv, "parameter.delh", cfg.delh, true, err) ||
-      !get_num(kv, "parameter.cc", cfg.cc, true, err) ||
-      !get_int(kv, "parameter.Ns", cfg.Ns, true, err))
+  const std::string problem = get_str(kv, "application.problem", "");
+  if (problem == "shock") {
+    cfg.problem = 1;
+  } else if (!problem.empty()) {
+    err = "config: unknown [application] problem: " + problem;
+    return false;
+  }
+  if (cfg.problem == 1) {
+    // MainChunk::setup fixes cc = 1 and Ns = 2 (example/shock/main.cpp:60-61)
+    cfg.cc = 1.0;
+    cfg.Ns = 2;
+    ShockConfig& sh = cfg.shock;
+    if (!get_num(kv, "parameter.delt", cfg.delt, true, err) ||
+        !get_num(kv, "parameter.delh", cfg.delh, true, err) ||
+        !get_int(kv, "parameter.nppc", sh.nppc, true, err) ||
+        !get_num(kv, "parameter.wp", sh.wp, true, err) ||
+        !get_num(kv, "parameter.mime", sh.mime, true, err) ||
+        !get_num(kv, "parameter.u0", sh.u0, true, err) ||
+        !get_num(kv, "parameter.theta", sh.theta, true, err) ||
+        !get_num(kv, "parameter.phi", sh.phi, true, err) ||
+        !get_num(kv, "parameter.sigma", sh.sigma, true, err) ||
+        !get_num(kv, "parameter.betae", sh.betae, true, err) ||
+        !get_num(kv, "parameter.betai", sh.betai, true, err) ||
+        !get_num(kv, "parameter.taper", sh.taper, true, err) ||
+        !get_num(kv, "parameter.density", sh.density, false, err) ||
+        !get_num(kv, "parameter.ramp_x", sh.ramp_x, false, err) ||
+        !get_num(kv, "parameter.ramp_w", sh.ramp_w, false, err))
+      return false;
+    if (sh.nppc <= 0 || sh.density <= 0 || sh.ramp_w <= 0) {
+      err = "config: shock needs nppc > 0, density > 0 and ramp_w > 0";
+      return false;
+    }
+  } else if (!get_num(kv, "parameter.delt", cfg.delt, true, err) ||
+             !get_num(kv, "parameter.delh", cfg.delh, true, err) ||
+             !get_num(kv, "parameter.cc", cfg.cc, true, err) ||
+             !get_int(kv, "parameter.Ns", cfg.Ns, true, err))
     return false;
   const char* eb[6] = {"Ex", "Ey", "Ez", "Bx", "By", "Bz"};
   for (int i = 0; i < 6; i++) {
@@ -168,12 +200,12 @@ bool build_config(const TomlMap& kv, Config& cfg, std::string& err)
     if (it != kv.end())
       npart = std::atoi(it->second.c_str());
   }
-  if (npart < cfg.Ns) {
+  if (cfg.problem == 0 && npart < cfg.Ns) {
     err = "config: fewer [[parameter.particle]] tables than Ns";
     return false;
   }
   cfg.species.resize(cfg.Ns);
-  for (int s = 0; s < cfg.Ns; s++) {
+  for (int s = 0; s < cfg.Ns && cfg.problem == 0; s++) {
     std::string    p  = "parameter.particle[" + std::to_string(s) + "].";
     SpeciesConfig& sp = cfg.species[s];
     if (!get_int(kv, p + "np", sp.np, true, err) || !get_num(kv, p + "qm", sp.qm, true, err) ||

@@ -13,6 +13,17 @@ struct SpeciesConfig {
   double qm = 0, ro = 0, vt = 0, vx = 0, vy = 0, vz = 0;
 };
 
+// example/shock ([application] problem = 'shock'): the [parameter] keys of
+// example/shock/config.toml read by MainChunk::setup (example/shock/main.cpp:56-92),
+// plus an optional density profile n(x): 1 below ramp_x, a linear ramp over ramp_w
+// cells, `density` above (density = 1 is the example's uniform plasma).
+struct ShockConfig {
+  int    nppc = 0;
+  double wp = 0, mime = 0, u0 = 0, theta = 0, phi = 0, sigma = 0, betae = 0, betai = 0,
+         taper = 0;
+  double density = 1.0, ramp_x = 0.0, ramp_w = 1.0;
+};
+
 // Parsed [parameter] and [application.option] tables.  Key names, defaults and
 // validation follow nix/cfgparser.hpp:145-217 and PicChunk::setup
 // (pic/pic_chunk.cpp:135-262).
@@ -23,6 +34,8 @@ struct Config {
   double cc = 1, delt = 1, delh = 1;
   double E0[3] = {0, 0, 0}, B0[3] = {0, 0, 0};
   std::vector<SpeciesConfig> species;
+  int         problem = 0; // 0: species tables (example/thermal, example/beam); 1: shock
+  ShockConfig shock;
 
   int         order        = 2;
   int         pusher       = 0; // Boris

@@ -319,7 +319,7 @@ __device__ __forceinline__ long xcd_block(long hw, long nblocks)
 
 // Device view of the domain: global padded periodic grid + chunk tables.
 struct Geom {
-  int    dim;      // 2 or 3
+  int    dim;      // 1, 2 or 3
   int    order;    // shape order
   int    Nb;       // ghost margin (order + 3) / 2
   int    Ns;       // species
@@ -341,7 +341,48 @@ struct Geom {
   double q[kMaxSpecies];
   double m[kMaxSpecies];
   bool   has[3];
+  bool   xwall;    // x is not periodic: a wall at x = 0 and an open end at x = Lx
 };
+
+// key of a particle that left through the open end x = Lx and waits for its
+// replacement (the shock problem's upstream boundary)
+constexpr int kKeyReplace = -1000;
+
+// Bin index of a position inside chunk c, or -1 when the particle lies outside
+// the chunk.  XtensorParticle::count (nix/xtensor_particle.hpp:324-357).
+__device__ __forceinline__ int bin_of(const Geom& g, const double* __restrict__ lim,
+                                      const double* __restrict__ rng, int c, double x, double y,
+                                      double z)
+{
+  const int    is_odd  = (g.order % 2 == 1) ? 1 : 0;
+  const double xoffset = lim[c * 6 + 4] - 0.5 * g.del[2] * is_odd;
+  const double yoffset = lim[c * 6 + 2] - 0.5 * g.del[1] * is_odd;
+  const double zoffset = lim[c * 6 + 0] - 0.5 * g.del[0] * is_odd;
+  const double rdx = 1 / g.del[2], rdy = 1 / g.del[1], rdz = 1 / g.del[0];
+  if (g.has[2] && (x < rng[c * 6 + 4] || x >= rng[c * 6 + 5]))
+    return -1;
+  if (g.has[1] && (y < rng[c * 6 + 2] || y >= rng[c * 6 + 3]))
+    return -1;
+  if (g.has[0] && (z < rng[c * 6 + 0] || z >= rng[c * 6 + 1]))
+    return -1;
+  int ix = g.has[2] ? digitize(x, xoffset, rdx) : 0;
+  int iy = g.has[1] ? digitize(y, yoffset, rdy) : 0;
+  int iz = g.has[0] ? digitize(z, zoffset, rdz) : 0;
+  return iz * g.bsz + iy * g.bsy + ix;
+}
+
+// Exchange direction slot (0..26, 13 = stays) of XtensorHaloParticle3D::pre_pack
+// (nix/xtensor_halo3d.hpp:230-244).
+__device__ __forceinline__ int dir_of(const double* __restrict__ rng, int c, double x, double y,
+                                      double z)
+{
+  int iz = (z >= rng[c * 6 + 1]) - (z < rng[c * 6 + 0]) + 1;
+  int iy = (y >= rng[c * 6 + 3]) - (y < rng[c * 6 + 2]) + 1;
+  int ix = (x >= rng[c * 6 + 5]) - (x < rng[c * 6 + 4]) + 1;
+  return 9 * iz + 3 * iy + ix;
+}
+
+
 
 // FP32 particle storage (precision = "fp32"): position as the cell index (the
 // chunk-local bin, packed like the sort bins) plus the offset inside the cell

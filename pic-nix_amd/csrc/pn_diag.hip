@@ -73,6 +73,12 @@ __global__ void __launch_bounds__(256) k_field_energy(Geom g, const double* __re
       acc[2] = (E[i + 1] - ex) * rdx + (E[v + i + sy] - ey) * rdy - J[i];
       acc[3] = (bx - B[i - 1]) * rdx + (by - B[v + i - sy]) * rdy;
     }
+    // cells next to a physical boundary are left out of the divergence errors
+    // (pic/pic_engine.hpp:42-71)
+    if (g.xwall && (x < g.pad[2] + g.Nb || x >= g.pad[2] + g.N[2] - g.Nb)) {
+      acc[2] = 0;
+      acc[3] = 0;
+    }
   }
   block_sum<4>(acc, partial + 4 * blockIdx.x);
 }
@@ -126,7 +132,8 @@ __global__ void __launch_bounds__(256) k_chunk_get(Geom g, const double* __restr
   bool ok = (g.has[0] || lz == g.Nb) && (g.has[1] || ly == g.Nb);
   int  gz = g.has[0] ? wrapg(lz, oz, g.pad[0], g.N[0]) : 0;
   int  gy = g.has[1] ? wrapg(ly, oy, g.pad[1], g.N[1]) : 0;
-  long i  = gidx(g, gz, gy, wrapg(lx, ox, g.pad[2], g.N[2]));
+  int  gx = g.xwall ? ox + lx : wrapg(lx, ox, g.pad[2], g.N[2]);
+  long i  = gidx(g, gz, gy, gx);
   for (int k = 0; k < ncomp; k++)
     out[t * stride + coff + k] = ok ? F[k * g.vol + i] : 0.0;
 }
@@ -183,8 +190,9 @@ void launch_energy(Sim& s, double* out)
   else
     hipLaunchKernelGGL(k_field_energy<2>, dim3(nb1), dim3(256), 0, s.stream, g, s.d_E, s.d_B,
                        s.d_J, d_part);
-  hipLaunchKernelGGL(k_particle_energy, dim3(nt, g.nseg), dim3(256), 0, s.stream, g, s.pa,
-                     s.d_np, d_part + np1, nt);
+  if (!g.xwall)
+    hipLaunchKernelGGL(k_particle_energy, dim3(nt, g.nseg), dim3(256), 0, s.stream, g, s.pa,
+                       s.d_np, d_part + np1, nt);
   std::vector<double> h(np1 + np2);
   hipMemcpyAsync(h.data(), d_part, sizeof(double) * (np1 + np2), hipMemcpyDeviceToHost, s.stream);
   hipFreeAsync(d_part, s.stream);
@@ -197,6 +205,12 @@ void launch_energy(Sim& s, double* out)
   for (int seg = 0; seg < g.nseg; seg++)
     for (int b = 0; b < nt; b++)
       out[4 + seg % g.Ns] += h[np1 + (size_t)seg * nt + b];
+  if (g.xwall) {
+    // with physical boundaries the moment sum differs from the particle sum (the
+    // weight beyond a wall is not folded back): form it as the reference does
+    launch_moment_exchange(s, true);
+    launch_moment_energy(s, out + 4);
+  }
 }
 
 void launch_chunk_fields_get(Sim& s, int chunk, int which, double* d_out)

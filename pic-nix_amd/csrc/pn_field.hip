@@ -176,8 +176,12 @@ __global__ void __launch_bounds__(256) k_fill_ghost(Geom g, T* __restrict__ f, i
             z >= g.pad[0] && z < g.pad[0] + g.N[0];
   if (in)
     return;
-  long src = gidx(g, wrap(z, g.pad[0], g.N[0]), wrap(y, g.pad[1], g.N[1]),
-                  wrap(x, g.pad[2], g.N[2]));
+  // a non-periodic x keeps its padding (the wall values, k_wall_emf); the y and
+  // z padding of those columns still takes the periodic image
+  const int xs = g.xwall ? x : wrap(x, g.pad[2], g.N[2]);
+  long src = gidx(g, wrap(z, g.pad[0], g.N[0]), wrap(y, g.pad[1], g.N[1]), xs);
+  if (src == tid)
+    return;
   for (int k = 0; k < ncomp; k++)
     f[k * g.vol + tid] = f[k * g.vol + src];
 }
@@ -368,6 +372,8 @@ void fold_impl(Sim& s, T* F, int k0, int k1)
   for (int dir = 2; dir >= 0; dir--) {
     if (!g.has[dir] || g.pad[dir] == 0)
       continue;
+    if (dir == 2 && g.xwall)
+      continue; // no neighbour beyond a physical boundary: nothing is folded
     long n = dir == 2 ? (long)g.P[0] * g.P[1] : (dir == 1 ? (long)g.P[0] * g.P[2] : g.plane);
     hipLaunchKernelGGL(k_fold_line<T>, dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, dir, k0,
                        k1);

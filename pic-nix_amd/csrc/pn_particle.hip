@@ -392,40 +392,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Bin index of a position inside chunk c, or -1 when the particle lies outside
-// the chunk.  XtensorParticle::count (nix/xtensor_particle.hpp:324-357).
-__device__ __forceinline__ int bin_of(const Geom& g, const double* __restrict__ lim,
-                                      const double* __restrict__ rng, int c, double x, double y,
-                                      double z)
-{
-  const int    is_odd  = (g.order % 2 == 1) ? 1 : 0;
-  const double xoffset = lim[c * 6 + 4] - 0.5 * g.del[2] * is_odd;
-  const double yoffset = lim[c * 6 + 2] - 0.5 * g.del[1] * is_odd;
-  const double zoffset = lim[c * 6 + 0] - 0.5 * g.del[0] * is_odd;
-  const double rdx = 1 / g.del[2], rdy = 1 / g.del[1], rdz = 1 / g.del[0];
-  if (g.has[2] && (x < rng[c * 6 + 4] || x >= rng[c * 6 + 5]))
-    return -1;
-  if (g.has[1] && (y < rng[c * 6 + 2] || y >= rng[c * 6 + 3]))
-    return -1;
-  if (g.has[0] && (z < rng[c * 6 + 0] || z >= rng[c * 6 + 1]))
-    return -1;
-  int ix = g.has[2] ? digitize(x, xoffset, rdx) : 0;
-  int iy = g.has[1] ? digitize(y, yoffset, rdy) : 0;
-  int iz = g.has[0] ? digitize(z, zoffset, rdz) : 0;
-  return iz * g.bsz + iy * g.bsy + ix;
-}
-
-// Exchange direction slot (0..26, 13 = stays) of XtensorHaloParticle3D::pre_pack
-// (nix/xtensor_halo3d.hpp:230-244).
-__device__ __forceinline__ int dir_of(const double* __restrict__ rng, int c, double x, double y,
-                                      double z)
-{
-  int iz = (z >= rng[c * 6 + 1]) - (z < rng[c * 6 + 0]) + 1;
-  int iy = (y >= rng[c * 6 + 3]) - (y < rng[c * 6 + 2]) + 1;
-  int ix = (x >= rng[c * 6 + 5]) - (x < rng[c * 6 + 4]) + 1;
-  return 9 * iz + 3 * iy + ix;
-}
-
 // BasePosition::push (pic/engine/position.hpp:117-130) followed by count(reset)
 // (Position::set_boundary, pic/pic_engine.hpp:292-303).  Stayers take a slot in
 // their bin (one atomic per distinct bin in the wave); leavers record their
@@ -459,14 +425,26 @@ __global__ void __launch_bounds__(256)
     x += ux * dt;
     y += uy * dt;
     z += uz * dt;
+    if (g.xwall && x < g.glim[2][0]) {
+      // reflecting wall, MainChunk::set_boundary_particle (example/shock/main.cpp:402-416)
+      x       = -x + 2 * g.glim[2][0];
+      p.ux[i] = -ux;
+      p.uy[i] = -uy;
+      p.uz[i] = -uz;
+    }
     p.x[i] = x;
     p.y[i] = y;
     p.z[i] = z;
-    b      = bin_of(g, lim, rng, c, x, y, z);
-    if (b < 0) {
-      int d  = dir_of(rng, c, x, y, z);
-      key[i] = -1 - d;
-      atomicAdd(&cnt[d], 1);
+    if (g.xwall && x >= g.glim[2][1]) {
+      // past the open end: replaced by an injected particle (k_wall_apply)
+      key[i] = kKeyReplace;
+    } else {
+      b = bin_of(g, lim, rng, c, x, y, z);
+      if (b < 0) {
+        int d  = dir_of(rng, c, x, y, z);
+        key[i] = -1 - d;
+        atomicAdd(&cnt[d], 1);
+      }
     }
   }
   const bool stay = active && b >= 0;

@@ -13,6 +13,19 @@ struct pn_sim; // the C-ABI handle (include/picnix.h)
 
 namespace picnix {
 
+// A failed HIP runtime call inside a C-ABI entry point (reported as PN_ERR_HIP).
+struct HipError {
+  std::string msg;
+};
+
+inline void hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess)
+    throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+#define HIPCHK(call) ::picnix::hip_check((call), #call)
+
 // Per-step scratch of the counting sort and particle migration.
 // spare entries after the member list: the rank kernels read up to 15 keys past
 // the end of a bin in 16-key groups
@@ -33,6 +46,28 @@ struct SortScratch {
   int32_t* npnew     = nullptr; // [nseg]         particle count after the sort
   int32_t* errflag   = nullptr; // [4]            device-side error flags
   int      ntile     = 0;
+};
+
+// example/shock (Config::problem == 1): the constants of MainChunk::setup
+// (example/shock/main.cpp:56-92), the per-chunk state the example keeps in
+// option["random_seed"] and option["boundary"], and the device buffers of the
+// upstream boundary.
+struct ShockState {
+  double delt = 0, u0 = 0, vte = 0, vti = 0, taper = 0;
+  double E0[3] = {0, 0, 0}, B0[3] = {0, 0, 0};
+  double flux = 0;                 // Poisson mean of injected particles per (z, y) column
+  int    npc_max = 0;              // particles per cell at the highest density
+  std::vector<int>     random_seed; // [chunk]
+  std::vector<int64_t> lastid;      // [chunk]
+  std::vector<int>     influx;      // [chunk][2]
+  std::vector<int>     efflux;      // [chunk][2]
+  std::vector<int>     useg;        // segments of the chunks at x = Lx, chunk-id order
+  int32_t* d_useg  = nullptr;       // [nk]
+  int32_t* d_list  = nullptr;       // [nk][cap] particles past x = Lx, in particle order
+  int32_t* d_count = nullptr;       // [nk] list lengths, then injected counts
+  int64_t* d_voff  = nullptr;       // [nk] offsets into d_vals
+  double*  d_vals  = nullptr;       // replacement (4 per particle) / injected (7) values
+  size_t   vals_cap = 0;
 };
 
 struct Sim;
@@ -85,6 +120,7 @@ struct Sim {
   bool     np_host_valid = false;
 
   SortScratch ss{};
+  ShockState  shk{};
   bool        counted = false; // keys hold a fresh count (push_position / count)
   bool        sorted  = false; // particles are in sorted order and cellstart is valid
   // rho (component 0 of J) is the deposit of the current positions, left by
@@ -163,6 +199,21 @@ void launch_rho_continuity(Sim& s, double delt);
 // rho from the current positions; sorted_by_new_cell: cellstart describes them
 void launch_rho(Sim& s, bool sorted_by_new_cell);
 void launch_energy(Sim& s, double* out /* host: efd, bfd, divE, divB, particle[Ns] */);
+// moments (PicChunk::deposit_moment, when `deposit`) and the Mom exchange
+void launch_moment_exchange(Sim& s, bool deposit);
+
+// shock problem (pn_boundary.hip device side, pn_shock.cpp host side)
+void launch_wall_emf(Sim& s);     // MainChunk::set_boundary_emf after an EMF exchange
+void launch_wall_list(Sim& s);    // particles past x = Lx -> shk.d_list / d_count
+void launch_wall_apply(Sim& s);   // their replacement values -> particles, bins, slots
+void launch_inject_append(Sim& s);
+void launch_moment_energy(Sim& s, double* out /* host: particle[Ns] */);
+void shock_setup(Sim& s);         // constants, per-chunk state, device buffers
+void shock_release(Sim& s);
+void shock_init_fields(Sim& s);   // MainChunk::setup field profile (host -> device)
+void shock_load_particles(Sim& s);
+void shock_position_boundary(Sim& s); // set_boundary_particle after the position push
+void shock_inject(Sim& s);            // inject_particle (before the migration)
 
 // FP32 form (pn_fp32.hip, pn_sort.hip)
 void launch32_from_f64(Sim& s, int seg0, int nseg);        // pa (FP64 SoA) -> qa

@@ -663,7 +663,7 @@ Json app_state(Sim& s, int step, double time)
   cm["size"]        = Json::integer(s.g.nchunk);
   cm["ndim"]        = Json::integer(3);
   cm["shape"]       = Json::ints(std::vector<int>{cfg.C[0], cfg.C[1], cfg.C[2]});
-  cm["periodicity"] = Json::ints(std::vector<int>{1, 1, 1});
+  cm["periodicity"] = Json::ints(std::vector<int>{1, 1, s.g.xwall ? 0 : 1});
   Json ids          = Json::array();
   for (int cz = 0; cz < cfg.C[0]; cz++) {
     Json plane = Json::array();
@@ -984,6 +984,18 @@ int pn_save_state(pn_sim* h, const char* prefix, int step, double time)
     lib["sorted"]       = Json::boolean(s.sorted);
     lib["rho_valid"]    = Json::boolean(s.rho_valid);
     lib["np"]           = Json::ints(np);
+    if (s.g.xwall) {
+      // MainChunk's option["random_seed"] and option["boundary"] state per chunk
+      Json& sh = lib["shock"];
+      sh.kind  = Json::Obj;
+      Json ids = Json::array();
+      for (int64_t v : s.shk.lastid)
+        ids.push(Json::integer(v));
+      sh["random_seed"] = Json::ints(s.shk.random_seed);
+      sh["lastid"]      = ids;
+      sh["influx"]      = Json::ints(s.shk.influx);
+      sh["efflux"]      = Json::ints(s.shk.efflux);
+    }
     lib["rawfile"]      = Json::string(std::filesystem::path(std::string(prefix) + ".data").filename().string());
     Json&        blocks = lib["blocks"];
     blocks.kind         = Json::Obj;
@@ -1069,6 +1081,20 @@ int pn_load_state(pn_sim* h, const char* prefix, int* step, double* time)
     if (hipMemcpy(s.d_np, np.data(), np.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
         hipSuccess)
       return fail(s, PN_ERR_HIP, "particle counts copy failed");
+    if (s.g.xwall) {
+      auto it = lib.o.find("shock");
+      if (it == lib.o.end())
+        return fail(s, PN_ERR_CONFIG, "state file lacks the shock boundary state");
+      Json& sh = it->second;
+      for (int c = 0; c < s.g.nchunk; c++) {
+        s.shk.random_seed[c] = (int)sh["random_seed"].a.at(c).i;
+        s.shk.lastid[c]      = sh["lastid"].a.at(c).i;
+      }
+      for (size_t k = 0; k < s.shk.influx.size(); k++) {
+        s.shk.influx[k] = (int)sh["influx"].a.at(k).i;
+        s.shk.efflux[k] = (int)sh["efflux"].a.at(k).i;
+      }
+    }
     s.sorted        = lib["sorted"].b;
     s.rho_valid     = lib["rho_valid"].b;
     s.counted       = false;

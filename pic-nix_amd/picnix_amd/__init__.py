@@ -34,7 +34,7 @@ EXPORTS = [
     "pn_get_energy",
     "pn_get_diverror", "pn_synchronize", "pn_timing", "pn_timing_report",
     "pn_write_field_diag", "pn_write_particle_diag", "pn_write_profile",
-    "pn_save_state", "pn_load_state",
+    "pn_save_state", "pn_load_state", "pn_get_boundary_state",
 ]
 
 PHASES = ["bfd", "velocity", "position", "deposit", "fold", "efd", "sort", "rho"]
@@ -101,6 +101,7 @@ def load_library(path=LIB_PATH):
         "pn_write_profile": [P, ctypes.c_char_p, i, d],
         "pn_save_state": [P, ctypes.c_char_p, i, d],
         "pn_load_state": [P, ctypes.c_char_p, ip, dp],
+        "pn_get_boundary_state": [P, i, ctypes.POINTER(i64)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -296,6 +297,12 @@ class Simulation:
         self._chk(self.L.pn_load_state(self.h, str(prefix).encode(), ctypes.byref(st),
                                        ctypes.byref(tm)))
         return st.value, tm.value
+
+    def boundary_state(self, c):
+        """{random_seed, lastid, influx[2], efflux[2]} of chunk c (shock problem)."""
+        out = (ctypes.c_int64 * 6)()
+        self._chk(self.L.pn_get_boundary_state(self.h, c, out))
+        return list(out)
 
     def history(self):
         """[div(E)-rho, div(B), E^2/2, B^2/2, particle energy per species] (history.txt columns)."""

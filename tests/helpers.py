@@ -70,6 +70,20 @@ def small_twostream(N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1, **opt):
     return cfg
 
 
+def small_shock(N=(1, 1, 64), C=(1, 1, 4), nppc=8, order=2, **opt):
+    """example/shock on a small domain (reflecting wall at x = 0, injection at x = Lx)."""
+    cfg = copy.deepcopy(load_config("shock1d"))
+    p = cfg["parameter"]
+    p.update(Nz=N[0], Ny=N[1], Nx=N[2], Cz=C[0], Cy=C[1], Cx=C[2], nppc=nppc)
+    for k in list(opt):
+        if k in ("density", "ramp_x", "ramp_w", "u0", "taper", "mime", "delt", "delh", "theta",
+                 "phi", "sigma"):
+            p[k] = opt.pop(k)
+    cfg["application"]["option"]["order"] = order
+    cfg["application"]["option"].update(opt)
+    return cfg
+
+
 def rel_err(a, b):
     """max |a - b| / max |b| (0 when both vanish)."""
     a = np.asarray(a, dtype=np.float64)

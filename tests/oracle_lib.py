@@ -29,6 +29,11 @@ class Params(ctypes.Structure):
         ("vt", ctypes.c_double * MAXS), ("vx", ctypes.c_double * MAXS),
         ("vy", ctypes.c_double * MAXS), ("vz", ctypes.c_double * MAXS),
         ("interp", ctypes.c_int),
+        ("problem", ctypes.c_int), ("emf_order", ctypes.c_int), ("nppc", ctypes.c_int),
+        ("wp", ctypes.c_double), ("mime", ctypes.c_double), ("u0", ctypes.c_double),
+        ("theta", ctypes.c_double), ("phi", ctypes.c_double), ("sigma", ctypes.c_double),
+        ("betae", ctypes.c_double), ("betai", ctypes.c_double), ("taper", ctypes.c_double),
+        ("density", ctypes.c_double), ("ramp_x", ctypes.c_double), ("ramp_w", ctypes.c_double),
     ]
 
 
@@ -86,6 +91,11 @@ def lib():
         L.pno_uniform.argtypes = [ctypes.c_uint64, i64, dp]
         L.pno_normal.argtypes = [ctypes.c_uint64, i64, dp]
         L.pno_chunk_map.argtypes = [i, i, i, ip]
+        L.pno_gamma.argtypes = [d, d, ctypes.c_uint64, i64, dp]
+        L.pno_poisson.restype = i
+        L.pno_poisson.argtypes = [d, ctypes.c_uint64, i64, ip]
+        L.pno_maxwell_juttner.argtypes = [d, d, ctypes.c_uint64, i64, dp]
+        L.pno_shock_state.argtypes = [P, i, ctypes.POINTER(ctypes.c_int64)]
         _lib = L
     return _lib
 
@@ -102,15 +112,28 @@ def make_params(cfg, load=True):
     """Build the oracle parameter block from a parsed config dict (see config.py)."""
     p = Params()
     par = cfg["parameter"]
-    opt = cfg.get("application", {}).get("option", {})
-    for k in ("Nx", "Ny", "Nz", "Cx", "Cy", "Cz", "Ns"):
+    app = cfg.get("application", {})
+    opt = app.get("option", {})
+    shock = app.get("problem", "") == "shock"
+    for k in ("Nx", "Ny", "Nz", "Cx", "Cy", "Cz"):
         setattr(p, k, int(par[k]))
+    p.Ns = 2 if shock else int(par["Ns"])
+    p.density = 1.0
+    if shock:
+        p.problem = 1
+        p.emf_order = int(opt.get("emf_order", 0))
+        p.nppc = int(par["nppc"])
+        for k in ("wp", "mime", "u0", "theta", "phi", "sigma", "betae", "betai", "taper"):
+            setattr(p, k, float(par[k]))
+        p.density = float(par.get("density", 1.0))
+        p.ramp_x = float(par.get("ramp_x", 0.0))
+        p.ramp_w = float(par.get("ramp_w", 1.0))
     p.order = int(opt.get("order", 2))
     p.pusher = {"Boris": 0, "Vay": 1, "HigueraCary": 2}[opt.get("pusher", "Boris")]
     p.load = 1 if load else 0
     p.loader = 0
     p.seed_offset = int(opt.get("seed_offset", 0))
-    p.cc = float(par["cc"])
+    p.cc = 1.0 if shock else float(par["cc"])
     p.delt = float(par["delt"])
     p.delh = float(par["delh"])
     p.friedman = float(opt.get("friedman", 0.0))
@@ -118,7 +141,7 @@ def make_params(cfg, load=True):
     p.buffer_ratio = float(opt.get("buffer_ratio", 0.2))
     for k in ("Ex", "Ey", "Ez", "Bx", "By", "Bz"):
         setattr(p, k, float(par.get(k, 0.0)))
-    for s, sp in enumerate(par["particle"]):
+    for s, sp in enumerate([] if shock else par["particle"]):
         p.np[s] = int(sp["np"])
         p.qm[s] = float(sp["qm"])
         p.ro[s] = float(sp["ro"])
@@ -184,6 +207,11 @@ class Oracle:
         out = np.zeros(ng + 1, dtype=np.int32)
         self.L.pno_get_pindex(self.h, c, s, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
         return out
+
+    def shock_state(self, c):
+        out = (ctypes.c_int64 * 6)()
+        self.L.pno_shock_state(self.h, c, out)
+        return list(out)
 
     def field(self, c, which):
         info = self.chunk_info(c)
