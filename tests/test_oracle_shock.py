@@ -213,3 +213,24 @@ def test_moment_boundary_leaves_interior():
         cells = um[nb, nb, nb:-nb]
         assert not np.any(cells[:2]), "wall cells received moments"
         assert np.all(cells[2:4, :, 0] > 0) and not np.any(cells[4:8])
+
+
+def test_emf_order_effect_in_2d_is_small():
+    # In 2D the reference's order (wall reset after the EMF unpack) lets a wall
+    # chunk's private ghost row along y keep a neighbour's pre-reset Ex, which
+    # enters its Bz and hence its particles: the run differs from the
+    # split-independent order (also reset before the pack) at a level far below
+    # the fields themselves.
+    a = _shock(N=(1, 16, 64), C=(1, 2, 4), nppc=4, order=2, emf_order=0)
+    b = _shock(N=(1, 16, 64), C=(1, 2, 4), nppc=4, order=2, emf_order=1)
+    for _ in range(10):
+        a.step()
+        b.step()
+    diff = scale = 0.0
+    for c in range(a.nchunks):
+        info = a.chunk_info(c)
+        fa = helpers.interior(a.field(c, 0), info, 2)
+        fb = helpers.interior(b.field(c, 0), info, 2)
+        diff = max(diff, float(np.max(np.abs(fa - fb))))
+        scale = max(scale, float(np.max(np.abs(fb))))
+    assert 0.0 < diff < 1e-5 * scale, (diff, scale)
