@@ -937,7 +937,9 @@ void step_launches32(Sim& s, double delt)
 // the wall reflection and the upstream redraw in the position push, the EMF
 // boundary after the field update, the injection before the migration.  The
 // injected particles carry charge that no current brought, so rho is deposited
-// every step rather than carried by continuity.
+// every step rather than carried by continuity: where the current kernel leaves
+// it, after the sort (grouped by the new cells) minus the particles the
+// injection has just appended, which the reference's deposit does not see.
 void step_launches_wall(Sim& s, double delt)
 {
   launch_push_bfd(s, 0.5 * delt);
@@ -946,13 +948,18 @@ void step_launches_wall(Sim& s, double delt)
   launch_push_position(s, delt);
   shock_position_boundary(s);
   s.sorted = was_sorted;
-  launch_deposit(s, delt, RhoMode::Now);
+  const RhoDone rho = launch_deposit(s, delt, RhoMode::Defer);
   launch_push_bfd(s, 0.5 * delt);
-  launch_fold_current(s, 0, 4);
+  launch_fold_current(s, rho == RhoDone::Deposited ? 0 : 1, 4);
   launch_push_efd(s, delt);
   launch_wall_emf(s);
   shock_inject(s);
   launch_migrate(s);
+  if (rho == RhoDone::Deferred) {
+    launch_rho(s, true);
+    launch_rho_unject(s, s.shk.last_ninj);
+    launch_fold_current(s, 0, 1);
+  }
   s.rho_valid = false;
 }
 

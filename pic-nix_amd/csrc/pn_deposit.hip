@@ -1120,6 +1120,53 @@ __global__ void __launch_bounds__(256)
       }
 }
 
+// Remove from rho the particles appended by the injection of this step (the
+// reference deposits rho before it injects, example/shock/main.cpp:445-535 runs
+// from set_boundary_pack(BoundaryParticle)): minus k_rho_global's deposit of
+// every injected particle, read from the injection staging buffer (7 doubles per
+// particle) of every upper segment k.
+template <int Dim, int Order>
+__global__ void __launch_bounds__(256)
+    k_rho_unject(Geom g, const int32_t* __restrict__ useg, const int32_t* __restrict__ ninj,
+                 const int64_t* __restrict__ voff, const double* __restrict__ vals,
+                 const int* __restrict__ off, const double* __restrict__ lim,
+                 double* __restrict__ J)
+{
+  constexpr int is_odd = Order % 2;
+  constexpr int S      = Order + 1;
+  const int     k      = blockIdx.y;
+  const long    t      = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ninj[k])
+    return;
+  const int     seg   = useg[k];
+  const int     c     = seg / g.Ns;
+  const int     sp    = seg - c * g.Ns;
+  const double* v     = vals + voff[k] + 7 * t;
+  const double  xu[3] = {v[0], v[1], v[2]};
+  const double  d3[3] = {g.del[2], g.del[1], g.del[0]};
+  const double  mn[3] = {lim[c * 6 + 4], lim[c * 6 + 2], lim[c * 6 + 0]};
+  double        s[3][S];
+  int           o[3];
+  for (int a = 0; a < Dim; a++) {
+    const double rd = 1 / d3[a];
+    int cell = digitize(xu[a], mn[a] + 0.5 * d3[a] * is_odd, rd);
+    shape_mc<Order>(xu[a], mn[a] + 0.5 * d3[a] + static_cast<double>(cell) * d3[a], rd, s[a]);
+    o[a] = cell + g.Nb - (Order / 2);
+  }
+  const double qs = g.q[sp];
+  for (int jz = 0; jz < (Dim == 3 ? S : 1); jz++)
+    for (int jy = 0; jy < (Dim >= 2 ? S : 1); jy++)
+      for (int jx = 0; jx < S; jx++) {
+        double val = Dim == 3 ? qs * s[0][jx] * s[1][jy] * s[2][jz]
+                              : (Dim == 2 ? qs * s[0][jx] * s[1][jy] : qs * s[0][jx]);
+        const int  lz = Dim == 3 ? o[2] + jz : g.Nb;
+        const int  ly = Dim >= 2 ? o[1] + jy : g.Nb;
+        const long gi = gnode_checked(g, off, c, lz, ly, o[0] + jx);
+        if (gi >= 0)
+          unsafeAtomicAdd(&J[gi], -val);
+      }
+}
+
 // Rows of bins per block of the march kernels: whole bin columns unless the
 // grid would then hold fewer than ~4096 blocks (16 per CU), in which case each
 // column is cut into nzr ranges of zrows rows (at 64^3 in 16^3 chunks: 4 ranges
@@ -1268,6 +1315,42 @@ RhoDone launch_deposit(Sim& s, double delt, RhoMode mode)
     return RhoDone::Deposited;
   }
   return RhoDone::Deferred;
+}
+
+void launch_rho_unject(Sim& s, int nmax)
+{
+  const Geom& g  = s.g;
+  const int   nk = (int)s.shk.useg.size();
+  if (nk == 0 || nmax <= 0)
+    return;
+  PhaseTimer pt(s, 7);
+  dim3       grid(nblk(nmax, 256), nk);
+#define PN_UNJ(D, O)                                                                        \
+  hipLaunchKernelGGL((k_rho_unject<D, O>), grid, dim3(256), 0, s.stream, g, s.shk.d_useg,  \
+                     s.shk.d_count, s.shk.d_voff, s.shk.d_vals, s.d_off, s.d_lim, s.d_J)
+  if (g.dim == 3) {
+    switch (g.order) {
+    case 1: PN_UNJ(3, 1); break;
+    case 2: PN_UNJ(3, 2); break;
+    case 3: PN_UNJ(3, 3); break;
+    default: PN_UNJ(3, 4); break;
+    }
+  } else if (g.dim == 1) {
+    switch (g.order) {
+    case 1: PN_UNJ(1, 1); break;
+    case 2: PN_UNJ(1, 2); break;
+    case 3: PN_UNJ(1, 3); break;
+    default: PN_UNJ(1, 4); break;
+    }
+  } else {
+    switch (g.order) {
+    case 1: PN_UNJ(2, 1); break;
+    case 2: PN_UNJ(2, 2); break;
+    case 3: PN_UNJ(2, 3); break;
+    default: PN_UNJ(2, 4); break;
+    }
+  }
+#undef PN_UNJ
 }
 
 void launch_rho(Sim& s, bool sorted_by_new_cell)

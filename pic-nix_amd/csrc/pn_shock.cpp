@@ -505,6 +505,7 @@ void shock_inject(Sim& s)
     }
   });
   upload_vals(s, vals, &ninj);
+  h.last_ninj = ninj.empty() ? 0 : *std::max_element(ninj.begin(), ninj.end());
   launch_inject_append(s);
   s.np_host_valid = false;
 }

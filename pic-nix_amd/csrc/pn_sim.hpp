@@ -57,6 +57,7 @@ struct ShockState {
   double E0[3] = {0, 0, 0}, B0[3] = {0, 0, 0};
   double flux = 0;                 // Poisson mean of injected particles per (z, y) column
   int    npc_max = 0;              // particles per cell at the highest density
+  int    last_ninj = 0;            // most particles the last injection appended to a segment
   std::vector<int>     random_seed; // [chunk]
   std::vector<int64_t> lastid;      // [chunk]
   std::vector<int>     influx;      // [chunk][2]
@@ -208,6 +209,9 @@ void launch_wall_list(Sim& s);    // particles past x = Lx -> shk.d_list / d_cou
 void launch_wall_apply(Sim& s);   // their replacement values -> particles, bins, slots
 void launch_inject_append(Sim& s);
 void launch_moment_energy(Sim& s, double* out /* host: particle[Ns] */);
+// minus the rho of the particles the last shock_inject appended (nmax: the most
+// injected into one segment)
+void launch_rho_unject(Sim& s, int nmax);
 void shock_setup(Sim& s);         // constants, per-chunk state, device buffers
 void shock_release(Sim& s);
 void shock_init_fields(Sim& s);   // MainChunk::setup field profile (host -> device)

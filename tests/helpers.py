@@ -77,7 +77,7 @@ def small_shock(N=(1, 1, 64), C=(1, 1, 4), nppc=8, order=2, **opt):
     p.update(Nz=N[0], Ny=N[1], Nx=N[2], Cz=C[0], Cy=C[1], Cx=C[2], nppc=nppc)
     for k in list(opt):
         if k in ("density", "ramp_x", "ramp_w", "u0", "taper", "mime", "delt", "delh", "theta",
-                 "phi", "sigma"):
+                 "phi", "sigma", "betae", "betai"):
             p[k] = opt.pop(k)
     cfg["application"]["option"]["order"] = order
     cfg["application"]["option"].update(opt)

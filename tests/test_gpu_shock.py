@@ -27,9 +27,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 CASES = {
-    "shock1d_o1": (1, dict(N=(1, 1, 128), C=(1, 1, 4), nppc=16, order=1, emf_order=0)),
-    "shock1d_o2": (1, dict(N=(1, 1, 128), C=(1, 1, 4), nppc=16, order=2, emf_order=0)),
-    "shock1d_o3": (1, dict(N=(1, 1, 96), C=(1, 1, 4), nppc=16, order=3, emf_order=0)),
+    # hot electrons (betae = 8, vth_e = 0.4c) so that some cross x = Lx in 10 steps
+    "shock1d_o1": (1, dict(N=(1, 1, 128), C=(1, 1, 4), nppc=64, order=1, emf_order=0, betae=8.0)),
+    "shock1d_o2": (1, dict(N=(1, 1, 128), C=(1, 1, 4), nppc=64, order=2, emf_order=0, betae=8.0)),
+    "shock1d_o3": (1, dict(N=(1, 1, 96), C=(1, 1, 4), nppc=64, order=3, emf_order=0, betae=8.0)),
     "shock2d_o2": (2, dict(N=(1, 16, 64), C=(1, 2, 4), nppc=4, order=2, emf_order=1)),
     "shock3d_o2": (3, dict(N=(8, 8, 32), C=(2, 2, 2), nppc=2, order=2, emf_order=1)),
     "shock3d_ramp": (3, dict(N=(8, 8, 32), C=(2, 2, 4), nppc=2, order=1, emf_order=1,
