@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--autotune", default="on", choices=["on", "off"],
+                    help="run an autotuning session (pn_autotune) before the warm-up steps")
     ap.add_argument("--fp32", default="on", choices=["on", "off"],
                     help="also time the FP32 form on the same workload")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -176,6 +178,12 @@ def main():
             text = text.replace("[application.option]",
                                 "[application.option]\n    precision = 'fp32'", 1)
         sim = picnix_amd.Simulation(text)
+        tuning = None
+        if args.autotune == "on":
+            sim.autotune(True)
+            nt = sim.tuning()["steps_left"]
+            sim.step(dt, nt)
+            tuning = dict(sim.tuning(), session_steps=nt)
         npart = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
         sim.step(dt, args.warmup)
         sim.synchronize()
@@ -189,7 +197,9 @@ def main():
         npart_end = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
         hist = sim.history()
         sim.close()
+        run_form.tuning[precision] = tuning
         return t1 - t0, npart, npart_end, rep, hist
+    run_form.tuning = {}
 
     elapsed, npart, npart_end, rep, hist64 = run_form("fp64")
     fp32 = None
@@ -259,6 +269,7 @@ def main():
         "step_roofline": {"bytes_per_step": step_bytes(npart, ncell), "achieved": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "phases": phases,
+        "tuning": run_form.tuning,
         "fp32_form": fp32,
     }
     if args.cpu_baseline == "on":

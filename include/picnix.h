@@ -127,6 +127,21 @@ int pn_get_diverror(pn_sim* sim, double* efd, double* bfd);
 /* Block until queued device work on the handle's stream is done. */
 int pn_synchronize(pn_sim* sim);
 
+/* Autotuner of the particle kernels' launch shapes (the bin rows per block of the
+ * current and rho march kernels, the bin-row or one-thread-per-particle gather).
+ * No reference counterpart (the reference picks its loop forms by the
+ * [application.option] vectorization table, pic/pic_chunk.cpp:140-189).
+ * enable = 1 starts a session: the next pn_get_tuning()[1] pn_step calls run
+ * eagerly, each timing candidate shapes on that step, then the fastest shape
+ * of every kernel stays for the rest of the run (step graphs are re-captured).
+ * The tuning steps are ordinary steps of the run: every candidate computes the
+ * same step, up to the order of the current sums.  enable = 0 returns to the
+ * built-in heuristics.  pn_get_tuning: out5 = {state (0 off, 1 measuring,
+ * 2 done), steps left, bin rows of the current march (0 = heuristic), bin rows of
+ * the rho march, 1 if the gather runs one thread per particle}. */
+int pn_autotune(pn_sim* sim, int enable);
+int pn_get_tuning(pn_sim* sim, int* out5);
+
 /* Upstream boundary state of one chunk for [application] problem = 'shock' (the
  * values MainChunk keeps in option["random_seed"] and option["boundary"],
  * example/shock/main.cpp:19-24,212-226,440-442,528-534): out6 = {random_seed,

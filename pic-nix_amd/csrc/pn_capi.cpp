@@ -1003,9 +1003,17 @@ void drop_graphs(Sim& s)
 
 } // namespace
 
+namespace picnix {
+void tune_step(Sim& s, double delt);
+}
+
 int pn_step(pn_sim* h, double delt)
 {
   return guarded(h, [&](Sim& s) {
+    if (s.tune.state == 1) {
+      tune_step(s, delt); // a measured step of an autotuning session
+      return PN_OK;
+    }
     // a replayed graph needs the state the capture saw: sorted particles and
     // no phase timing (event records are host-side bookkeeping)
     // (and rho carried from the previous step: the recording uses the continuity update)
@@ -1159,15 +1167,145 @@ static void drain_timing(Sim& s)
   s.ev_pending.clear();
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner (pn_autotune): the launch shapes of the particle kernels are picked
+// on the run's own steps.  A session of 2 x max(#candidates, 2) steps runs them
+// eagerly with per-phase HIP-event timing; step k tries candidate k mod n of
+// every knob (bin rows per march block for the current and rho march kernels;
+// bin-row or one-thread-per-particle gather), each candidate is measured twice
+// and keeps its faster time, and the fastest candidate of every knob stays.
+// Every candidate computes the same step (the current is summed in another
+// order), so the tuning steps are ordinary steps of the run.
+// ---------------------------------------------------------------------------
+static int tune_cycle(const Sim& s) { return std::max<int>((int)s.tune.cand.size(), 2); }
+
+void tune_begin(Sim& s)
+{
+  Sim::AutoTune& t = s.tune;
+  t                = Sim::AutoTune{};
+  const Geom& g    = s.g;
+  s.dep_rows = s.rho_rows = s.vel_generic = 0;
+  if (g.dim == 3 && !s.generic_kernels) {
+    const int nbz = g.nbin / g.bsz;
+    int       hz, nzr;
+    march_split(g, 0, hz, nzr);
+    for (int v : {1, 2, 3, 4, 5, 6, 9, nbz, hz})
+      if (v >= 1 && v <= nbz && std::find(t.cand.begin(), t.cand.end(), v) == t.cand.end())
+        t.cand.push_back(v);
+    std::sort(t.cand.begin(), t.cand.end());
+  }
+  t.tune_vel = !s.fp32 && g.dim >= 2 && !s.generic_kernels;
+  if (t.cand.empty() && !t.tune_vel) {
+    t.state = 2;
+    return;
+  }
+  const double inf = std::numeric_limits<double>::infinity();
+  t.t_dep.assign(t.cand.size(), inf);
+  t.t_rho.assign(t.cand.size(), inf);
+  t.t_vel[0] = t.t_vel[1] = inf;
+  t.state                 = 1;
+  drop_graphs(s);
+}
+
+static void tune_finish(Sim& s)
+{
+  Sim::AutoTune& t    = s.tune;
+  // a knob keeps the heuristic unless at least two of its candidates ran (rho is
+  // deposited only on steps that cannot carry it by continuity)
+  auto           best = [&](const std::vector<double>& v) {
+    const long ran = std::count_if(v.begin(), v.end(), [](double x) { return std::isfinite(x); });
+    size_t     k   = std::min_element(v.begin(), v.end()) - v.begin();
+    return (ran >= 2 && k < v.size()) ? t.cand[k] : 0;
+  };
+  s.dep_rows    = best(t.t_dep);
+  s.rho_rows    = best(t.t_rho);
+  s.vel_generic = (t.tune_vel && t.t_vel[1] < t.t_vel[0]) ? 1 : 0;
+  t.state       = 2;
+  drop_graphs(s);
+}
+
+void tune_step(Sim& s, double delt)
+{
+  Sim::AutoTune& t = s.tune;
+  const int      n = tune_cycle(s);
+  const int      c = t.step % n;
+  if (!t.cand.empty())
+    s.dep_rows = s.rho_rows = t.cand[c % t.cand.size()];
+  if (t.tune_vel)
+    s.vel_generic = c % 2;
+  // time this step's phases alone; the caller's accumulators are set aside
+  drain_timing(s);
+  double saved_ms[PN_NPHASE + 1];
+  int    saved_n[PN_NPHASE + 1];
+  for (int i = 0; i <= PN_NPHASE; i++) {
+    saved_ms[i]   = s.ph_ms[i];
+    saved_n[i]    = s.ph_count[i];
+    s.ph_ms[i]    = 0;
+    s.ph_count[i] = 0;
+  }
+  const bool saved_timing = s.timing;
+  s.timing                = true;
+  step_launches(s, delt);
+  drain_timing(s);
+  const double dep = s.ph_count[3] ? s.ph_ms[3] : -1.0;
+  const double rho = s.ph_count[kPhaseRhoDeposit] ? s.ph_ms[kPhaseRhoDeposit] : -1.0;
+  const double vel = s.ph_count[1] ? s.ph_ms[1] : -1.0;
+  for (int i = 0; i <= PN_NPHASE; i++) {
+    s.ph_ms[i]    = saved_ms[i];
+    s.ph_count[i] = saved_n[i];
+  }
+  s.timing = saved_timing;
+  if (!t.cand.empty()) {
+    const size_t k = c % t.cand.size();
+    if (dep >= 0)
+      t.t_dep[k] = std::min(t.t_dep[k], dep);
+    if (rho >= 0)
+      t.t_rho[k] = std::min(t.t_rho[k], rho);
+  }
+  if (t.tune_vel && vel >= 0)
+    t.t_vel[c % 2] = std::min(t.t_vel[c % 2], vel);
+  t.step++;
+  if (t.step >= 2 * n)
+    tune_finish(s);
+}
+
 } // namespace picnix
 
 extern "C" {
+
+int pn_autotune(pn_sim* h, int enable)
+{
+  return guarded(h, [&](Sim& s) {
+    if (enable) {
+      tune_begin(s);
+    } else {
+      s.tune     = Sim::AutoTune{};
+      s.dep_rows = s.rho_rows = s.vel_generic = 0;
+      drop_graphs(s);
+    }
+    return PN_OK;
+  });
+}
+
+int pn_get_tuning(pn_sim* h, int* out5)
+{
+  return guarded(h, [&](Sim& s) {
+    if (!out5)
+      return PN_ERR_ARG;
+    out5[0] = s.tune.state;
+    out5[1] = s.tune.state == 1 ? 2 * tune_cycle(s) - s.tune.step : 0;
+    out5[2] = s.dep_rows;
+    out5[3] = s.rho_rows;
+    out5[4] = s.vel_generic;
+    return PN_OK;
+  });
+}
 
 int pn_timing(pn_sim* h, int enable)
 {
   return guarded(h, [&](Sim& s) {
     drain_timing(s);
-    for (int i = 0; i < PN_NPHASE; i++) {
+    for (int i = 0; i <= PN_NPHASE; i++) {
       s.ph_ms[i]    = 0;
       s.ph_count[i] = 0;
     }

@@ -1172,6 +1172,8 @@ __global__ void __launch_bounds__(256)
 // column is cut into nzr ranges of zrows rows (at 64^3 in 16^3 chunks: 4 ranges
 // of 5 rows; measured 2.32 -> 2.26 ms FP64 and 1.44 -> 1.28 ms FP32 against 2
 // ranges of 9 rows).  `fixed` > 0 sets zrows.
+} // namespace
+
 void march_split(const Geom& g, int fixed, int& zrows, int& nzr)
 {
   const int  nby  = g.bsz / g.bsy;
@@ -1181,6 +1183,8 @@ void march_split(const Geom& g, int fixed, int& zrows, int& nzr)
   zrows            = fixed > 0 ? std::min(fixed, nbz) : (nbz + split - 1) / split;
   nzr              = (nbz + zrows - 1) / zrows;
 }
+
+namespace {
 
 // The 3D march kernel runs (sorted particles, its LDS fits); it deposits J only.
 template <int Order>
@@ -1221,7 +1225,7 @@ bool deposit_impl(Sim& s, const DepConsts& dc)
   if (Dim == 3 && march_fits<Order>(s)) {
     const size_t mlds = march_lds(g, 3 * N * N * (g.bsy + N - 1));
     int zrows, nzr;
-    march_split(g, s.march_rows, zrows, nzr);
+    march_split(g, s.dep_rows ? s.dep_rows : s.march_rows, zrows, nzr);
     hipLaunchKernelGGL((k_deposit_march<Order>), dim3(nzr * (g.bsz / g.bsy), g.nchunk), dim3(256),
                        mlds, s.stream, g, s.pa, s.d_xo[0], s.d_xo[1], s.d_xo[2], s.ss.cellstart,
                        s.d_off, s.d_lim, s.d_J, dc, zrows);
@@ -1248,7 +1252,7 @@ void rho_impl(Sim& s, bool sorted_by_new_cell)
   if (Dim == 3 && sorted_by_new_cell && !s.generic_kernels &&
       mlds + RhoPlan<Order>::WLDS + 64 <= 65536) {
     int zrows, nzr;
-    march_split(g, s.march_rows, zrows, nzr);
+    march_split(g, s.rho_rows ? s.rho_rows : s.march_rows, zrows, nzr);
     hipLaunchKernelGGL((k_rho_march<Order>), dim3(nzr * (g.bsz / g.bsy), g.nchunk), dim3(256),
                        mlds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_J, zrows);
   } else {
@@ -1356,6 +1360,7 @@ void launch_rho_unject(Sim& s, int nmax)
 void launch_rho(Sim& s, bool sorted_by_new_cell)
 {
   PhaseTimer  pt(s, 7);
+  PhaseTimer  pr(s, kPhaseRhoDeposit);
   const Geom& g = s.g;
 #define PN_RHO(D, O) rho_impl<D, O>(s, sorted_by_new_cell)
   if (g.dim == 3) {

@@ -1291,7 +1291,7 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
     const size_t mlds = sizeof(double) * 3 * (size_t)N * N * (g.bsy + N - 1) +
                         sizeof(int) * (size_t)g.Ns * (g.bsy + 1);
     int zrows, nzr;
-    march32_split(g, s.march_rows, zrows, nzr);
+    march32_split(g, s.dep_rows ? s.dep_rows : s.march_rows, zrows, nzr);
     const dim3 mgrid(nzr * (g.bsz / g.bsy), g.nchunk);
     if (g.dim == 3 && s.sorted && !s.generic_kernels && g.order == 2 &&
         mlds + March32<2>::WLDS + 64 <= 65536) {
@@ -1341,6 +1341,7 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
 void launch32_rho(Sim& s)
 {
   PhaseTimer  pt(s, 7);
+  PhaseTimer  pr(s, kPhaseRhoDeposit);
   const Geom& g = s.g;
   if (g.dim == 3 && s.sorted && !s.generic_kernels && (g.order == 2 || g.order == 4)) {
     const int    S    = g.order + 1;
@@ -1349,7 +1350,7 @@ void launch32_rho(Sim& s)
     const size_t wlds = g.order == 2 ? Rho32<2>::WLDS : Rho32<4>::WLDS;
     if (mlds + wlds + 64 <= 65536) {
       int zrows, nzr;
-      march32_split(g, s.march_rows, zrows, nzr);
+      march32_split(g, s.rho_rows ? s.rho_rows : s.march_rows, zrows, nzr);
       const dim3 mgrid(nzr * (g.bsz / g.bsy), g.nchunk);
       if (g.order == 2)
         hipLaunchKernelGGL(k32_rho_march<2>, mgrid, dim3(256), mlds, s.stream, g, s.qa,

@@ -539,7 +539,8 @@ void launch_push_velocity(Sim& s, double delt)
   dim3         grid(nblk(g.cap, 256), g.nseg);
   const int    T     = g.order + 2;
   const size_t lds   = sizeof(double) * 6 * (g.dim == 3 ? T : 1) * T * (g.bsy + g.order + 1);
-  const bool   rows  = g.dim >= 2 && s.sorted && !s.generic_kernels && lds <= 65536;
+  const bool   rows  = g.dim >= 2 && s.sorted && !s.generic_kernels && !s.vel_generic &&
+                     lds <= 65536;
   const int    nrow  = g.nbin / g.bsy;
   const bool   plain = pc.pusher == 0 && pc.wt == 0; // Boris + MC: no option branches
 #define PN_VEL_ROWS(D, O, PL)                                                                 \

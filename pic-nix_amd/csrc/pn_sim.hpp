@@ -129,6 +129,18 @@ struct Sim {
   bool        rho_valid = false;
   bool        generic_kernels = false; // PICNIX_KERNELS=generic: one-thread-per-particle forms
   int         march_rows      = 0;     // PICNIX_MARCH_ROWS: bin rows per march block (0: auto)
+  // launch shapes picked by the autotuner (pn_autotune); 0 = the heuristic
+  int         dep_rows    = 0; // bin rows per block of the current march kernels
+  int         rho_rows    = 0; // bin rows per block of the rho march kernels
+  int         vel_generic = 0; // 1: one thread per particle instead of the bin-row gather
+  struct AutoTune {
+    int                 state = 0; // 0 off, 1 measuring, 2 done
+    int                 step  = 0; // tuning steps taken
+    std::vector<int>    cand;      // candidate bin rows per march block
+    std::vector<double> t_dep, t_rho;
+    double              t_vel[2] = {0, 0};
+    bool                tune_vel = false;
+  } tune;
 
   // pn_step as a replayed hipGraph (pn_set_step_graph).  The particle arrays
   // swap roles every step, so there is one graph per buffer parity.
@@ -149,13 +161,17 @@ struct Sim {
     hipEvent_t start, stop;
   };
   std::vector<TimedPhase> ev_pending;
-  double                  ph_ms[8]    = {0};
-  int                     ph_count[8] = {0};
+  // phases 0..7 as pn_timing_report; slot 8 (internal) times the rho march
+  // deposit alone, for the autotuner
+  double                  ph_ms[9]    = {0};
+  int                     ph_count[9] = {0};
 
   std::string err;
 };
 
 // Records HIP events around one phase when timing is enabled.
+constexpr int kPhaseRhoDeposit = 8;
+
 struct PhaseTimer {
   Sim&       s;
   int        phase;
@@ -163,6 +179,9 @@ struct PhaseTimer {
   PhaseTimer(Sim& s_, int ph);
   ~PhaseTimer();
 };
+
+// bin rows per block of the FP64 march kernels (fixed > 0) or the heuristic
+void march_split(const Geom& g, int fixed, int& zrows, int& nzr);
 
 // kernels / launchers (pn_field.hip, pn_particle.hip, pn_sort.hip, pn_deposit.hip, pn_diag.hip)
 void launch_push_bfd(Sim& s, double delt);

@@ -34,7 +34,8 @@ EXPORTS = [
     "pn_get_energy",
     "pn_get_diverror", "pn_synchronize", "pn_timing", "pn_timing_report",
     "pn_write_field_diag", "pn_write_particle_diag", "pn_write_profile",
-    "pn_save_state", "pn_load_state", "pn_get_boundary_state",
+    "pn_save_state", "pn_load_state", "pn_get_boundary_state", "pn_autotune",
+    "pn_get_tuning",
 ]
 
 PHASES = ["bfd", "velocity", "position", "deposit", "fold", "efd", "sort", "rho"]
@@ -102,6 +103,8 @@ def load_library(path=LIB_PATH):
         "pn_save_state": [P, ctypes.c_char_p, i, d],
         "pn_load_state": [P, ctypes.c_char_p, ip, dp],
         "pn_get_boundary_state": [P, i, ctypes.POINTER(i64)],
+        "pn_autotune": [P, i],
+        "pn_get_tuning": [P, ip],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -297,6 +300,17 @@ class Simulation:
         self._chk(self.L.pn_load_state(self.h, str(prefix).encode(), ctypes.byref(st),
                                        ctypes.byref(tm)))
         return st.value, tm.value
+
+    def autotune(self, enable=True):
+        """Start (or cancel) an autotuning session over the next steps."""
+        self._chk(self.L.pn_autotune(self.h, 1 if enable else 0))
+
+    def tuning(self):
+        """{state, steps_left, deposit_rows, rho_rows, gather_per_particle}."""
+        out = (ctypes.c_int * 5)()
+        self._chk(self.L.pn_get_tuning(self.h, out))
+        keys = ("state", "steps_left", "deposit_rows", "rho_rows", "gather_per_particle")
+        return dict(zip(keys, list(out)))
 
     def boundary_state(self, c):
         """{random_seed, lastid, influx[2], efflux[2]} of chunk c (shock problem)."""
