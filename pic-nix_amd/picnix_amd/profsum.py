@@ -4,7 +4,8 @@
     python -m picnix_amd.profsum pmc [--probe-mb=M] FETCH_SIZE <fetch.db> [WRITE_SIZE <write.db>]
 
 `trace` prints calls, total and average duration per kernel (from a
-`--kernel-trace --stats` run).  `pmc` prints the per-dispatch average of each
+`--kernel-trace --stats` run), then, for kernels launched with several grids
+(an autotuning session tries candidate shapes), the same per launch shape.  `pmc` prints the per-dispatch average of each
 counter per kernel (KiB per dispatch for FETCH_SIZE / WRITE_SIZE) and, when the
 run held the bench's copy probe (`bench.py --copy-probe-mb M`: one device copy
 of M MiB by ROCm's copy kernel), the counter's calibration factor, i.e. the
@@ -32,6 +33,26 @@ def trace(db):
     out = [f"{'kernel':<48} {'calls':>6} {'total_us':>11} {'avg_us':>10} {'share':>6}"]
     for name, n, t, avg in rows:
         out.append(f"{_short(name, 48):<48} {n:>6} {t:>11.1f} {avg:>10.1f} {100 * t / tot:>5.1f}%")
+    return "\n".join(out)
+
+
+def trace_shapes(db):
+    """Kernels launched with more than one grid (the autotuner's candidates): calls
+    and average duration per launch shape, the chosen shape being the one with the
+    most calls."""
+    c = sqlite3.connect(db)
+    rows = c.execute(
+        "select name, grid_x, grid_y, grid_z, count(*), avg(duration) from kernels "
+        "group by name, grid_x, grid_y, grid_z order by name, count(*) desc").fetchall()
+    by = {}
+    for name, gx, gy, gz, n, avg in rows:
+        by.setdefault(name, []).append((gx, gy, gz, n, avg))
+    out = [f"{'kernel (launch shapes)':<48} {'grid (threads)':>22} {'calls':>6} {'avg_us':>10}"]
+    for name, shapes in by.items():
+        if len(shapes) < 2:
+            continue
+        for gx, gy, gz, n, avg in shapes:
+            out.append(f"{_short(name, 48):<48} {f'{gx}x{gy}x{gz}':>22} {n:>6} {avg / 1e3:>10.1f}")
     return "\n".join(out)
 
 
@@ -66,6 +87,10 @@ def pmc(pairs, probe_mb=None):
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(trace(argv[1]))
+        shapes = trace_shapes(argv[1])
+        if shapes.count("\n"):
+            print()
+            print(shapes)
         return 0
     if len(argv) >= 3 and argv[0] == "pmc":
         probe = None
