@@ -1189,7 +1189,7 @@ void tune_begin(Sim& s)
     const int nbz = g.nbin / g.bsz;
     int       hz, nzr;
     march_split(g, 0, hz, nzr);
-    for (int v : {1, 2, 3, 4, 5, 6, 9, nbz, hz})
+    for (int v : {1, 2, 3, 4, 5, 6, 7, 8, 9, 12, nbz, hz})
       if (v >= 1 && v <= nbz && std::find(t.cand.begin(), t.cand.end(), v) == t.cand.end())
         t.cand.push_back(v);
     std::sort(t.cand.begin(), t.cand.end());

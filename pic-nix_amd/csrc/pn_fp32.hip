@@ -200,9 +200,7 @@ __global__ void __launch_bounds__(256)
   constexpr int S      = Order + 1;
   constexpr int T      = Order + 2;
   extern __shared__ float ft32[]; // [6][TZ][T][TXv]
-  const int seg = blockIdx.y;
-  const int c   = seg / g.Ns;
-  const int sp  = seg - c * g.Ns;
+  const int c   = blockIdx.y; // chunk: every species' particles of the row share the tile
   const int nbx = g.bsy;
   int       bz = 0, by, b0;
   if constexpr (Dim == 3) {
@@ -214,9 +212,12 @@ __global__ void __launch_bounds__(256)
     by = blockIdx.x;
     b0 = by * g.bsy;
   }
-  const int32_t* cs = cellstart + (long)seg * (g.nbin + 1);
-  const int      p0 = cs[b0], p1 = cs[b0 + nbx];
-  if (p1 <= p0)
+  bool any = false;
+  for (int sp = 0; sp < g.Ns; sp++) {
+    const int32_t* cs = cellstart + (long)(c * g.Ns + sp) * (g.nbin + 1);
+    any               = any || cs[b0 + nbx] > cs[b0];
+  }
+  if (!any)
     return;
   const int  lb  = g.Nb;
   const int  TZ  = Dim == 3 ? T : 1;
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
 
-  const float qmdt = pc.qmdt[sp];
+  float qmdt = 0; // the species' (q/m) dt/2, set per species below
   auto interp = [&](const float* base, long bs_z, long bs_y, int z0, int y0, int x0,
                     const float* wz, const float* wy, const float* wx, int nz) {
     float rz = 0;
@@ -260,6 +261,11 @@ __global__ void __launch_bounds__(256)
     return rz * qmdt;
   };
 
+  for (int sp = 0; sp < g.Ns; sp++) {
+  const int      seg = c * g.Ns + sp;
+  const int32_t* cs  = cellstart + (long)seg * (g.nbin + 1);
+  const int      p0  = cs[b0], p1 = cs[b0 + nbx];
+  qmdt               = pc.qmdt[sp];
   for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
     const long  i    = (long)seg * g.cap + q;
     const Cell3 cl   = cell_of(g, p.cell[i]);
@@ -304,7 +310,9 @@ __global__ void __launch_bounds__(256)
     p.uy[i] = uy;
     p.uz[i] = uz;
   }
+  } // species
 }
+
 
 // One thread per particle, straight from global memory (unsorted particles).
 template <int Dim, int Order>
@@ -1229,7 +1237,7 @@ void launch32_push_velocity(Sim& s, double delt)
 #define PN32_VEL(D, O)                                                                        \
   do {                                                                                        \
     if (rows)                                                                                 \
-      hipLaunchKernelGGL((k32_push_velocity_rows<D, O>), dim3(nrow, g.nseg), dim3(256), lds,   \
+      hipLaunchKernelGGL((k32_push_velocity_rows<D, O>), dim3(nrow, g.nchunk), dim3(256), lds, \
                          s.stream, g, s.qa, s.ss.cellstart, s.d_off, s.f_E, s.f_B, pc);       \
     else                                                                                      \
       hipLaunchKernelGGL((k32_push_velocity<D, O>), dim3(nblk(g.cap, 256), g.nseg), dim3(256), \

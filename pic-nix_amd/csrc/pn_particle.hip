@@ -209,9 +209,7 @@ __global__ void __launch_bounds__(256)
   constexpr int S      = Order + 1;
   constexpr int T      = Order + 2;
   extern __shared__ double ft[]; // [6][TZ][T][TXv]
-  const int seg = blockIdx.y;
-  const int c   = seg / g.Ns;
-  const int sp  = seg - c * g.Ns;
+  const int c   = blockIdx.y; // chunk: every species' particles of the row share the tile
   const int nbx = g.bsy;
   int       bz = 0, by, b0;
   if constexpr (Dim == 3) {
@@ -223,9 +221,12 @@ __global__ void __launch_bounds__(256)
     by = blockIdx.x;
     b0 = by * g.bsy;
   }
-  const int32_t* cs = cellstart + (long)seg * (g.nbin + 1);
-  const int      p0 = cs[b0], p1 = cs[b0 + nbx];
-  if (p1 <= p0)
+  bool any = false;
+  for (int sp = 0; sp < g.Ns; sp++) {
+    const int32_t* cs = cellstart + (long)(c * g.Ns + sp) * (g.nbin + 1);
+    any               = any || cs[b0 + nbx] > cs[b0];
+  }
+  if (!any)
     return;
   const int  lb  = g.Nb;
   const int  TZ  = Dim == 3 ? T : 1;
@@ -249,7 +250,7 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
 
-  const double qmdt = pc.qmdt[sp];
+  double qmdt = 0; // the species' (q/m) dt/2, set per species below
   const double dx = g.del[2], dy = g.del[1], dz = g.del[0];
   const double xmin = lim[c * 6 + 4], ymin = lim[c * 6 + 2], zmin = lim[c * 6 + 0];
   const double ximin  = xmin + 0.5 * dx * is_odd;
@@ -295,6 +296,11 @@ __global__ void __launch_bounds__(256)
     return ry * qmdt;
   };
 
+  for (int sp = 0; sp < g.Ns; sp++) {
+  const int      seg = c * g.Ns + sp;
+  const int32_t* cs  = cellstart + (long)seg * (g.nbin + 1);
+  const int      p0  = cs[b0], p1 = cs[b0 + nbx];
+  qmdt               = pc.qmdt[sp];
   for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
     const long i     = (long)seg * g.cap + q;
     double     xu[6] = {p.x[i], p.y[i], p.z[i], p.ux[i], p.uy[i], p.uz[i]};
@@ -390,6 +396,7 @@ __global__ void __launch_bounds__(256)
     p.uy[i] = xu[4];
     p.uz[i] = xu[5];
   }
+  } // species
 }
 
 // BasePosition::push (pic/engine/position.hpp:117-130) followed by count(reset)
@@ -544,7 +551,7 @@ void launch_push_velocity(Sim& s, double delt)
   const int    nrow  = g.nbin / g.bsy;
   const bool   plain = pc.pusher == 0 && pc.wt == 0; // Boris + MC: no option branches
 #define PN_VEL_ROWS(D, O, PL)                                                                 \
-  hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL>), dim3(nrow, g.nseg), dim3(256), lds,     \
+  hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL>), dim3(nrow, g.nchunk), dim3(256), lds, \
                      s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc)
 #define PN_VEL(D, O)                                                                          \
   do {                                                                                        \
