@@ -36,6 +36,10 @@ EXPORTS = [
     "pn_write_field_diag", "pn_write_particle_diag", "pn_write_profile",
     "pn_save_state", "pn_load_state", "pn_get_boundary_state", "pn_autotune",
     "pn_get_tuning",
+    # include/picnix_adjoint.h (signatures in picnix_amd.adjoint)
+    "pn_adj_create", "pn_adj_destroy", "pn_adj_last_error", "pn_adj_modes", "pn_adj_forward",
+    "pn_adj_backward", "pn_adj_set_state", "pn_adj_get_state", "pn_adj_step",
+    "pn_adj_num_particles",
 ]
 
 PHASES = ["bfd", "velocity", "position", "deposit", "fold", "efd", "sort", "rho"]
