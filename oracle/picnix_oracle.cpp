@@ -2319,6 +2319,252 @@ void setup(Sim& s)
   exchange_field(s, 0);
 }
 
+
+// --------------------------------------------------------------------------
+// Tangent-linear reference of the adjoint form: the 2D periodic order-1 step of
+// the hot path (push_bfd_2d / push_efd_2d, pic/engine/maxwell.hpp:231-348; MC
+// gather + Boris, pic/engine/velocity.hpp:390-418, nix/primitives.hpp:163-189;
+// position push, pic/engine/position.hpp:117-130; Esirkepov 2D,
+// nix/esirkepov.hpp:76-143) evaluated in forward-mode dual numbers: every
+// state variable carries its derivative along one direction of the initial
+// state, and L = 1/2 sum B^2 carries dL along it.
+// --------------------------------------------------------------------------
+struct TD {
+  double v, d;
+  TD() : v(0), d(0) {}
+  TD(double x) : v(x), d(0) {}
+  TD(double x, double dx) : v(x), d(dx) {}
+};
+inline TD operator+(TD a, TD b) { return TD(a.v + b.v, a.d + b.d); }
+inline TD operator-(TD a, TD b) { return TD(a.v - b.v, a.d - b.d); }
+inline TD operator-(TD a) { return TD(-a.v, -a.d); }
+inline TD operator*(TD a, TD b) { return TD(a.v * b.v, a.d * b.v + a.v * b.d); }
+inline TD operator/(TD a, TD b)
+{
+  double r = a.v / b.v;
+  return TD(r, (a.d - r * b.d) / b.v);
+}
+inline TD& operator+=(TD& a, TD b) { return a = a + b; }
+inline TD& operator-=(TD& a, TD b) { return a = a - b; }
+inline TD& operator*=(TD& a, TD b) { return a = a * b; }
+inline TD tsqrt(TD a)
+{
+  double r = std::sqrt(a.v);
+  return TD(r, a.d * 0.5 / r);
+}
+
+struct TLGeom {
+  int    nx, ny;
+  double dx, dt, cc, q, qmdt, lx, ly;
+};
+
+inline size_t tl_cell(const TLGeom& g, int iy, int ix)
+{
+  iy %= g.ny;
+  iy = iy < 0 ? iy + g.ny : iy;
+  ix %= g.nx;
+  ix = ix < 0 ? ix + g.nx : ix;
+  return (size_t)iy * g.nx + ix;
+}
+
+// shape_mc1 (nix/primitives.hpp:256-266) in dual numbers
+inline void tl_shape1(TD x, double X, double rdx, TD* s)
+{
+  TD delta = (x - TD(X)) * TD(rdx);
+  s[0]     = TD(1.0) - delta;
+  s[1]     = delta;
+}
+
+// weights2d at order 1: integer grid (centres) and half grid (faces)
+inline void tl_weights(const TLGeom& g, TD x, TD wi[2], TD wh[2], int& i0, int& h0)
+{
+  const double rdx = 1 / g.dx;
+  i0               = digitize(x.v, 0.5 * g.dx, rdx);
+  h0               = digitize(x.v, 0.0, rdx);
+  tl_shape1(x, 0.5 * g.dx + static_cast<double>(i0) * g.dx, rdx, wi);
+  tl_shape1(x, 0.0 + static_cast<double>(h0) * g.dx, rdx, wh);
+}
+
+inline TD tl_interp(const TLGeom& g, const std::vector<TD>& F, size_t off, int y0, int x0,
+                    const TD wy[2], const TD wx[2])
+{
+  TD ry(0.0);
+  for (int jy = 0; jy < 2; jy++) {
+    TD rx(0.0);
+    for (int jx = 0; jx < 2; jx++)
+      rx += F[off + tl_cell(g, y0 + jy, x0 + jx)] * wx[jx];
+    ry += rx * wy[jy];
+  }
+  return ry * TD(g.qmdt);
+}
+
+// push_boris (nix/primitives.hpp:163-189)
+inline void tl_boris(TD& ux, TD& uy, TD& uz, TD ex, TD ey, TD ez, TD bx, TD by, TD bz, double cc)
+{
+  ux += ex;
+  uy += ey;
+  uz += ez;
+  TD gm = TD(1.0) / tsqrt(TD(cc * cc) + ux * ux + uy * uy + uz * uz);
+  bx *= gm;
+  by *= gm;
+  bz *= gm;
+  TD bb = TD(2.0) / (TD(1.0) + bx * bx + by * by + bz * bz);
+  TD vx = ux + (uy * bz - uz * by);
+  TD vy = uy + (uz * bx - ux * bz);
+  TD vz = uz + (ux * by - uy * bx);
+  ux += (vy * bz - vz * by) * bb + ex;
+  uy += (vz * bx - vx * bz) * bb + ey;
+  uz += (vx * by - vy * bx) * bb + ez;
+}
+
+// Esirkepov 2D, order 1, with shift_weights; J[3][ny][nx] accumulated
+inline void tl_deposit(const TLGeom& g, TD x0, TD y0, TD z0, TD x1, TD y1, TD z1,
+                       std::vector<TD>& J)
+{
+  const int    N = 4;
+  const double A = 1.0 / 2, B = 1.0 / 3, rdx = 1 / g.dx;
+  TD           s0x[N], s0y[N], s1x[N], s1y[N];
+  const int    ix0 = digitize(x0.v, 0.5 * g.dx, rdx), iy0 = digitize(y0.v, 0.5 * g.dx, rdx);
+  const int    ix1 = digitize(x1.v, 0.5 * g.dx, rdx), iy1 = digitize(y1.v, 0.5 * g.dx, rdx);
+  tl_shape1(x0, 0.5 * g.dx + static_cast<double>(ix0) * g.dx, rdx, &s0x[1]);
+  tl_shape1(y0, 0.5 * g.dx + static_cast<double>(iy0) * g.dx, rdx, &s0y[1]);
+  tl_shape1(x1, 0.5 * g.dx + static_cast<double>(ix1) * g.dx, rdx, &s1x[1]);
+  tl_shape1(y1, 0.5 * g.dx + static_cast<double>(iy1) * g.dx, rdx, &s1y[1]);
+  if (ix1 - ix0 < 0)
+    for (int ii = 0; ii < 3; ii++)
+      s1x[ii] = s1x[ii + 1];
+  else if (ix1 - ix0 > 0)
+    for (int ii = 3; ii > 0; ii--)
+      s1x[ii] = s1x[ii - 1];
+  if (iy1 - iy0 < 0)
+    for (int ii = 0; ii < 3; ii++)
+      s1y[ii] = s1y[ii + 1];
+  else if (iy1 - iy0 > 0)
+    for (int ii = 3; ii > 0; ii--)
+      s1y[ii] = s1y[ii - 1];
+  const int    ox = ix0 - 1, oy = iy0 - 1;
+  const TD     vz = (z1 - z0) / TD(g.dt);
+  const size_t v  = (size_t)g.nx * g.ny;
+  for (int l = 0; l < N; l++) {
+    s1x[l] -= s0x[l];
+    s1y[l] -= s0y[l];
+  }
+  const TD qdxdt(g.q * (g.dx / g.dt)), qvz = TD(g.q) * vz;
+  for (int jy = 0; jy < N; jy++) {
+    TD ww(0.0), wx = -(s0y[jy] + TD(A) * s1y[jy]) * qdxdt;
+    for (int jx = 0; jx < N - 1; jx++) {
+      ww += s1x[jx] * wx;
+      J[tl_cell(g, oy + jy, ox + jx + 1)] += ww;
+    }
+  }
+  for (int jx = 0; jx < N; jx++) {
+    TD ww(0.0), wy = -(s0x[jx] + TD(A) * s1x[jx]) * qdxdt;
+    for (int jy = 0; jy < N - 1; jy++) {
+      ww += s1y[jy] * wy;
+      J[v + tl_cell(g, oy + jy + 1, ox + jx)] += ww;
+    }
+  }
+  for (int jy = 0; jy < N; jy++)
+    for (int jx = 0; jx < N; jx++)
+      J[2 * v + tl_cell(g, oy + jy, ox + jx)] +=
+          ((TD(1.0) * s0x[jx] + TD(A) * s1x[jx]) * s0y[jy] +
+           (TD(A) * s0x[jx] + TD(B) * s1x[jx]) * s1y[jy]) *
+          qvz;
+}
+
+void tl_bfd(const TLGeom& g, const std::vector<TD>& E, std::vector<TD>& Bf, double half)
+{
+  const double cfl = g.cc * (half * g.dt) / g.dx;
+  const size_t v   = (size_t)g.nx * g.ny;
+  std::vector<TD> B = Bf;
+  for (int iy = 0; iy < g.ny; iy++)
+    for (int ix = 0; ix < g.nx; ix++) {
+      size_t i = tl_cell(g, iy, ix), jx = tl_cell(g, iy, ix - 1), jy = tl_cell(g, iy - 1, ix);
+      B[i] += TD(-cfl) * (E[2 * v + i] - E[2 * v + jy]);
+      B[v + i] += TD(+cfl) * (E[2 * v + i] - E[2 * v + jx]);
+      B[2 * v + i] += TD(-cfl) * (E[v + i] - E[v + jx]) + TD(+cfl) * (E[i] - E[jy]);
+    }
+  Bf = B;
+}
+
+void tl_efd(const TLGeom& g, std::vector<TD>& E, const std::vector<TD>& B, const std::vector<TD>& J)
+{
+  const double cfl = g.cc * g.dt / g.dx;
+  const size_t v   = (size_t)g.nx * g.ny;
+  for (int iy = 0; iy < g.ny; iy++)
+    for (int ix = 0; ix < g.nx; ix++) {
+      size_t i = tl_cell(g, iy, ix), px = tl_cell(g, iy, ix + 1), py = tl_cell(g, iy + 1, ix);
+      E[i] += TD(+cfl) * (B[2 * v + py] - B[2 * v + i]) - TD(g.dt) * J[i];
+      E[v + i] += TD(-cfl) * (B[2 * v + px] - B[2 * v + i]) - TD(g.dt) * J[v + i];
+      E[2 * v + i] += TD(+cfl) * (B[v + px] - B[v + i]) + TD(-cfl) * (B[py] - B[i]) -
+                      TD(g.dt) * J[2 * v + i];
+    }
+}
+
+} // namespace
+
+extern "C" void pno_adj_tangent(int nx, int ny, double delh, double delt, double cc, double q,
+                                double m, int64_t np, const double* xu6, const double* dxu6,
+                                int nsteps, double* loss, double* dloss)
+{
+  TLGeom g{nx, ny, delh, delt, cc, q, 0.5 * q / m * delt, nx * delh, ny * delh};
+  const size_t    v = (size_t)nx * ny;
+  std::vector<TD> P(6 * (size_t)np), E(3 * v), B(3 * v);
+  for (int64_t p = 0; p < np; p++)
+    for (int c = 0; c < 6; c++)
+      P[6 * p + c] = TD(xu6[6 * p + c], dxu6[6 * p + c]);
+  const int nthr = g_threads > 0 ? g_threads : omp_get_max_threads();
+  for (int st = 0; st < nsteps; st++) {
+    std::vector<TD> B1 = B;
+    tl_bfd(g, E, B1, 0.5);
+    std::vector<std::vector<TD>> Jt(nthr, std::vector<TD>(3 * v));
+#pragma omp parallel num_threads(nthr)
+    {
+      std::vector<TD>& J = Jt[omp_get_thread_num()];
+#pragma omp for schedule(static)
+      for (int64_t p = 0; p < np; p++) {
+        TD* s = &P[6 * p];
+        TD  wix[2], whx[2], wiy[2], why[2];
+        int ix0, hx0, iy0, hy0;
+        tl_weights(g, s[0], wix, whx, ix0, hx0);
+        tl_weights(g, s[1], wiy, why, iy0, hy0);
+        TD eb[6] = {tl_interp(g, E, 0, iy0, hx0, wiy, whx), tl_interp(g, E, v, hy0, ix0, why, wix),
+                    tl_interp(g, E, 2 * v, iy0, ix0, wiy, wix),
+                    tl_interp(g, B1, 0, hy0, ix0, why, wix),
+                    tl_interp(g, B1, v, iy0, hx0, wiy, whx),
+                    tl_interp(g, B1, 2 * v, hy0, hx0, why, whx)};
+        tl_boris(s[3], s[4], s[5], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], g.cc);
+        TD gm = tsqrt(TD(1.0) + (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]) * TD(1 / g.cc) *
+                                    TD(1 / g.cc));
+        TD dt = TD(g.dt) / gm;
+        TD x1 = s[0] + s[3] * dt, y1 = s[1] + s[4] * dt, z1 = s[2] + s[5] * dt;
+        tl_deposit(g, s[0], s[1], s[2], x1, y1, z1, J);
+        // set_boundary_periodic: the shift does not change the derivative
+        x1.v = x1.v + (x1.v < 0) * g.lx - (x1.v >= g.lx) * g.lx;
+        y1.v = y1.v + (y1.v < 0) * g.ly - (y1.v >= g.ly) * g.ly;
+        s[0] = x1;
+        s[1] = y1;
+        s[2] = z1;
+      }
+    }
+    std::vector<TD> J(3 * v);
+    for (int t = 0; t < nthr; t++)
+      for (size_t i = 0; i < 3 * v; i++)
+        J[i] += Jt[t][i];
+    B = B1;
+    tl_bfd(g, E, B, 0.5);
+    tl_efd(g, E, B, J);
+  }
+  double L = 0, dL = 0;
+  for (size_t i = 0; i < v; i++) {
+    L += 0.5 * (B[i].v * B[i].v + B[v + i].v * B[v + i].v + B[2 * v + i].v * B[2 * v + i].v);
+    dL += B[i].v * B[i].d + B[v + i].v * B[v + i].d + B[2 * v + i].v * B[2 * v + i].d;
+  }
+  *loss  = L;
+  *dloss = dL;
+}
+
+namespace {
 } // namespace
 
 // ===========================================================================

@@ -119,6 +119,14 @@ void pno_maxwell_juttner(double temperature, double drift, uint64_t seed, int64_
 /* shock problem state of a chunk: [random_seed, lastid, influx[0..1], efflux[0..1]] */
 void pno_shock_state(const pno_sim* s, int chunk, int64_t* out6);
 
+/* Tangent-linear reference of the adjoint form (include/picnix_adjoint.h): nsteps
+   2D periodic order-1 steps from particles xu6 [np][6] with E = B = 0, in
+   forward-mode dual numbers along the initial-state direction dxu6; returns
+   L = 1/2 sum B^2 and its directional derivative dL. */
+void pno_adj_tangent(int nx, int ny, double delh, double delt, double cc, double q, double m,
+                     int64_t np, const double* xu6, const double* dxu6, int nsteps, double* loss,
+                     double* dloss);
+
 /* ---- chunk map (generalized Hilbert curve) ---- */
 void pno_chunk_map(int Cz, int Cy, int Cx, int* id_of_coord);
 

@@ -96,6 +96,7 @@ def lib():
         L.pno_poisson.argtypes = [d, ctypes.c_uint64, i64, ip]
         L.pno_maxwell_juttner.argtypes = [d, d, ctypes.c_uint64, i64, dp]
         L.pno_shock_state.argtypes = [P, i, ctypes.POINTER(ctypes.c_int64)]
+        L.pno_adj_tangent.argtypes = [i, i, d, d, d, d, d, i64, dp, dp, i, dp, dp]
         _lib = L
     return _lib
 
@@ -265,3 +266,15 @@ class Oracle:
 
 def set_threads(n):
     lib().pno_set_threads(int(n))
+
+
+def adj_tangent(nx, ny, delh, delt, cc, q, m, xu6, dxu6, nsteps):
+    """L and dL of the adjoint form's forward along dxu6 (tangent-linear, CPU)."""
+    L = lib()
+    xu6 = np.ascontiguousarray(xu6, dtype=np.float64)
+    dxu6 = np.ascontiguousarray(dxu6, dtype=np.float64)
+    loss = ctypes.c_double()
+    dloss = ctypes.c_double()
+    L.pno_adj_tangent(nx, ny, delh, delt, cc, q, m, xu6.shape[0], _dp(xu6), _dp(dxu6), nsteps,
+                      ctypes.byref(loss), ctypes.byref(dloss))
+    return loss.value, dloss.value

@@ -98,3 +98,58 @@ def test_adjoint_gradient_independent_of_checkpoint_interval():
         assert helpers.rel_err(g2, g1) <= 1e-9 and helpers.rel_err(g3, g1) <= 1e-9, (g1, g2, g3)
     finally:
         form.close()
+
+
+def _tangent_check(nx, nppc, nmodes, steps, every, seed, theta0):
+    """Adjoint gradient (GPU) against the CPU tangent-linear reference along every
+    parameter direction; returns [(adjoint, tangent)] per parameter."""
+    import oracle_lib
+    from picnix_amd.adjoint import AdjointForm
+    form = AdjointForm(nx=nx, ny=nx, nppc=nppc, nmodes=nmodes, kmax=3, seed=seed)
+    try:
+        theta = np.array(theta0[:1 + nmodes], dtype=np.float64)
+        flat = theta.copy()
+        flat[1:] = 0.0
+        form.forward(flat, 0, 1)
+        base, _, _ = form.get_state()
+        form.forward(theta, 0, 1)
+        xu0, _, _ = form.get_state()
+        form.forward(theta, steps, every)
+        grad = form.backward()
+        modes = form.modes()
+        L = nx * 1.0
+        q, m = -1.0 / nppc, 1.0 / nppc
+        out = []
+        for j in range(1 + nmodes):
+            d = np.zeros_like(xu0)
+            if j == 0:
+                d[:, 3] = xu0[:, 3] / theta[0]
+            else:
+                mx, my, ph = modes[j - 1]
+                kx, ky = 2 * np.pi * mx / L, 2 * np.pi * my / L
+                s = np.sin(kx * base[:, 0] + ky * base[:, 1] + ph) / (kx * kx + ky * ky)
+                d[:, 0] = -kx * s
+                d[:, 1] = -ky * s
+            _, dl = oracle_lib.adj_tangent(nx, nx, 1.0, 0.5, 1.0, q, m, xu0, d, steps)
+            out.append((grad[j], dl))
+            _say(f"[adjoint] {nx}^2 x {nppc} ppc, {steps} steps, theta[{j}]: adjoint "
+                 f"{grad[j]:+.10e} tangent-linear {dl:+.10e} rel {abs(grad[j] - dl) / abs(dl):.2e}")
+        return out
+    finally:
+        form.close()
+
+
+def test_adjoint_matches_cpu_tangent_linear():
+    # beyond the horizon where finite differences of the order-1 loss converge
+    rows = _tangent_check(32, 16, 4, 100, 10, 11, [6.0, 1e-3, -1e-3, 2e-3, 1e-3])
+    for ga, gt in rows:
+        assert abs(ga - gt) <= 1e-6 * abs(gt), (ga, gt)
+
+
+@pytest.mark.skipif(not __import__("os").environ.get("PICNIX_FULL"),
+                    reason="the full 256^2 x 32 ppc x 200-step Weibel check takes minutes: "
+                           "set PICNIX_FULL=1")
+def test_adjoint_matches_cpu_tangent_linear_weibel_200():
+    rows = _tangent_check(256, 32, 8, 200, 20, 12345, [6.0] + [1e-3] * 8)
+    for ga, gt in rows:
+        assert abs(ga - gt) <= 1e-3 * abs(gt), (ga, gt)
