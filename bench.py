@@ -40,7 +40,7 @@ WORKLOADS = {
     "shock3d": "shock-like 3D 512x64x64 cells, 512 chunks of 16^3, reflecting wall at x=0 and "
                "Poisson injection at x=512, density 1 -> 4 (16 -> 64 particles/cell/species) "
                "with a 32-cell ramp at x=256, drift 0.2c, vth_e=0.05c, m_i/m_e=64, "
-               "2nd-order shape, FP64 plain (FP32 is not built for this problem)",
+               "2nd-order shape, FP64 plain (and the FP32 form)",
 }
 
 

@@ -469,8 +469,8 @@ int pn_create(const char* toml, int load, pn_sim** out)
       return PN_ERR_UNSUPPORTED;
     }
     s.fp32 = s.cfg.precision == "fp32";
-    if (s.cfg.problem == 1 && (s.fp32 || s.cfg.friedman != 0.0)) {
-      s.err = "the shock problem is built for precision = fp64 without the Friedman filter";
+    if (s.cfg.problem == 1 && s.cfg.friedman != 0.0) {
+      s.err = "the shock problem is built without the Friedman filter";
       *out  = h;
       return PN_ERR_UNSUPPORTED;
     }
@@ -943,20 +943,33 @@ void step_launches32(Sim& s, double delt)
 void step_launches_wall(Sim& s, double delt)
 {
   launch_push_bfd(s, 0.5 * delt);
-  launch_push_velocity(s, delt);
+  if (s.fp32)
+    launch32_push_velocity(s, delt);
+  else
+    launch_push_velocity(s, delt);
   bool was_sorted = s.sorted;
-  launch_push_position(s, delt);
+  if (s.fp32)
+    launch32_push_position(s, delt);
+  else
+    launch_push_position(s, delt);
   shock_position_boundary(s);
   s.sorted = was_sorted;
-  const RhoDone rho = launch_deposit(s, delt, RhoMode::Defer);
+  const RhoDone rho =
+      s.fp32 ? launch32_deposit(s, delt, RhoMode::Defer) : launch_deposit(s, delt, RhoMode::Defer);
   launch_push_bfd(s, 0.5 * delt);
   launch_fold_current(s, rho == RhoDone::Deposited ? 0 : 1, 4);
   launch_push_efd(s, delt);
   launch_wall_emf(s);
   shock_inject(s);
-  launch_migrate(s);
+  if (s.fp32)
+    launch32_migrate(s);
+  else
+    launch_migrate(s);
   if (rho == RhoDone::Deferred) {
-    launch_rho(s, true);
+    if (s.fp32)
+      launch32_rho(s);
+    else
+      launch_rho(s, true);
     launch_rho_unject(s, s.shk.last_ninj);
     launch_fold_current(s, 0, 1);
   }

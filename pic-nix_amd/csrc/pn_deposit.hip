@@ -1125,12 +1125,12 @@ __global__ void __launch_bounds__(256)
 // from set_boundary_pack(BoundaryParticle)): minus k_rho_global's deposit of
 // every injected particle, read from the injection staging buffer (7 doubles per
 // particle) of every upper segment k.
-template <int Dim, int Order>
+template <int Dim, int Order, typename TJ>
 __global__ void __launch_bounds__(256)
     k_rho_unject(Geom g, const int32_t* __restrict__ useg, const int32_t* __restrict__ ninj,
                  const int64_t* __restrict__ voff, const double* __restrict__ vals,
                  const int* __restrict__ off, const double* __restrict__ lim,
-                 double* __restrict__ J)
+                 TJ* __restrict__ J)
 {
   constexpr int is_odd = Order % 2;
   constexpr int S      = Order + 1;
@@ -1163,7 +1163,7 @@ __global__ void __launch_bounds__(256)
         const int  ly = Dim >= 2 ? o[1] + jy : g.Nb;
         const long gi = gnode_checked(g, off, c, lz, ly, o[0] + jx);
         if (gi >= 0)
-          unsafeAtomicAdd(&J[gi], -val);
+          unsafeAtomicAdd(&J[gi], static_cast<TJ>(-val));
       }
 }
 
@@ -1330,8 +1330,16 @@ void launch_rho_unject(Sim& s, int nmax)
   PhaseTimer pt(s, 7);
   dim3       grid(nblk(nmax, 256), nk);
 #define PN_UNJ(D, O)                                                                        \
-  hipLaunchKernelGGL((k_rho_unject<D, O>), grid, dim3(256), 0, s.stream, g, s.shk.d_useg,  \
-                     s.shk.d_count, s.shk.d_voff, s.shk.d_vals, s.d_off, s.d_lim, s.d_J)
+  do {                                                                                        \
+    if (s.fp32)                                                                               \
+      hipLaunchKernelGGL((k_rho_unject<D, O, float>), grid, dim3(256), 0, s.stream, g,        \
+                         s.shk.d_useg, s.shk.d_count, s.shk.d_voff, s.shk.d_vals, s.d_off,    \
+                         s.d_lim, s.f_J);                                                     \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_rho_unject<D, O, double>), grid, dim3(256), 0, s.stream, g,       \
+                         s.shk.d_useg, s.shk.d_count, s.shk.d_voff, s.shk.d_vals, s.d_off,    \
+                         s.d_lim, s.d_J);                                                     \
+  } while (0)
   if (g.dim == 3) {
     switch (g.order) {
     case 1: PN_UNJ(3, 1); break;

@@ -345,8 +345,9 @@ struct Geom {
 };
 
 // key of a particle that left through the open end x = Lx and waits for its
-// replacement (the shock problem's upstream boundary)
-constexpr int kKeyReplace = -1000;
+// replacement (the shock problem's upstream boundary); below every leaver code
+// of both forms (FP64: -1 - direction, FP32: -2 - (direction * nbin + bin))
+constexpr int kKeyReplace = -2147483647 - 1;
 
 // Bin index of a position inside chunk c, or -1 when the particle lies outside
 // the chunk.  XtensorParticle::count (nix/xtensor_particle.hpp:324-357).

@@ -404,12 +404,58 @@ __global__ void __launch_bounds__(256)
 // destination bin here, so the sort needs no migration pass.  key: the new bin
 // (>= 0) for a particle that stays in its chunk, -2 - (d * nbin + bin) for one
 // that moves to the neighbour in direction d, -1 for one that is dropped.
+// Destination of a particle of segment seg whose cell-relative position is r in
+// its old cell `bin` (see k32_push_position): segment ds, bin b, key k; ds < 0
+// when it is dropped.
+__device__ __forceinline__ void dest32(const Geom& g, int seg, int bin, float rx, float ry, float rz,
+                                       const int* __restrict__ nb, int& ds, int& b, int& k)
+{
+  ds = -1;
+  b  = -1;
+  k  = -1;
+  if (bin < 0)
+    return;
+  const Cell3 cl = cell_of(g, bin);
+  int         cx = cl.x + (g.has[2] ? (int)floorf(rx) : 0);
+  int         cy = cl.y + (g.has[1] ? (int)floorf(ry) : 0);
+  int         cz = cl.z + (g.has[0] ? (int)floorf(rz) : 0);
+  const int   dx = (cx >= g.dims[2]) - (cx < 0) + 1;
+  const int   dy = (cy >= g.dims[1]) - (cy < 0) + 1;
+  const int   dz = (cz >= g.dims[0]) - (cz < 0) + 1;
+  const int   d  = 9 * dz + 3 * dy + dx;
+  int         R  = seg / g.Ns;
+  if (d != 13) {
+    R = nb[R * 27 + d];
+    cx += (cx < 0) * g.dims[2] - (cx >= g.dims[2]) * g.dims[2];
+    cy += (cy < 0) * g.dims[1] - (cy >= g.dims[1]) * g.dims[1];
+    cz += (cz < 0) * g.dims[0] - (cz >= g.dims[0]) * g.dims[0];
+  }
+  if (R >= 0 && cx >= 0 && cx < g.dims[2] && cy >= 0 && cy < g.dims[1] && cz >= 0 &&
+      cz < g.dims[0]) {
+    ds = R * g.Ns + (seg - (seg / g.Ns) * g.Ns);
+    b  = bin_of3(g, cx, cy, cz);
+    k  = d == 13 ? b : -2 - (d * g.nbin + b);
+  }
+}
+
+// r_old -> ro; r += u dt / (gamma d) (unwrapped, relative to the old cell).
+// Then the particle's destination for the sort: its own segment and new bin, or
+// for a leaver the neighbour chunk in its direction (chunk table nb) and the bin
+// there -- the periodic wrap is a shift of the cell index, so r stays as it is
+// (set_boundary_periodic, nix/xtensor_particle.hpp:359-376, and the migration,
+// nix/xtensor_halo3d.hpp:192-499).  Every particle takes a slot in its
+// destination bin here, so the sort needs no migration pass.  key: the new bin
+// (>= 0) for a particle that stays in its chunk, -2 - (d * nbin + bin) for one
+// that moves to the neighbour in direction d, -1 for one that is dropped.
+// With a non-periodic x (the shock problem) a particle that crossed x = 0 is
+// reflected (MainChunk::set_boundary_particle, example/shock/main.cpp:402-416)
+// and one past x = Lx is marked kKeyReplace for its redraw.
 __global__ void __launch_bounds__(256)
     k32_push_position(Geom g, P32 p, float* __restrict__ ro0, float* __restrict__ ro1,
                       float* __restrict__ ro2, const int32_t* __restrict__ np, float delt,
                       float rc, float rdx, float rdy, float rdz, const int* __restrict__ nb,
-                      int32_t* __restrict__ cellcnt, int32_t* __restrict__ key,
-                      int32_t* __restrict__ slot)
+                      const int* __restrict__ off, int32_t* __restrict__ cellcnt,
+                      int32_t* __restrict__ key, int32_t* __restrict__ slot)
 {
   const int  seg    = blockIdx.y;
   const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,33 +473,27 @@ __global__ void __launch_bounds__(256)
     rx += g.has[2] ? ux * dt * rdx : ux * dt;
     ry += g.has[1] ? uy * dt * rdy : uy * dt;
     rz += g.has[0] ? uz * dt * rdz : uz * dt;
+    const int bin = p.cell[i];
+    bool      replace = false;
+    if (g.xwall && bin >= 0) {
+      // global x cell of the old cell; |dx| < 1 cell, so only the end cells can cross
+      const int gx = off[(seg / g.Ns) * 3 + 2] + cell_of(g, bin).x;
+      if (gx == 0 && rx < 0.0f) {
+        rx      = -rx;
+        p.ux[i] = -ux;
+        p.uy[i] = -uy;
+        p.uz[i] = -uz;
+      } else if (gx == g.N[2] - 1 && rx >= 1.0f) {
+        replace = true;
+      }
+    }
     p.rx[i] = rx;
     p.ry[i] = ry;
     p.rz[i] = rz;
-    const int bin = p.cell[i];
-    if (bin >= 0) {
-      const Cell3 cl = cell_of(g, bin);
-      int         cx = cl.x + (g.has[2] ? (int)floorf(rx) : 0);
-      int         cy = cl.y + (g.has[1] ? (int)floorf(ry) : 0);
-      int         cz = cl.z + (g.has[0] ? (int)floorf(rz) : 0);
-      const int   dx = (cx >= g.dims[2]) - (cx < 0) + 1;
-      const int   dy = (cy >= g.dims[1]) - (cy < 0) + 1;
-      const int   dz = (cz >= g.dims[0]) - (cz < 0) + 1;
-      const int   d  = 9 * dz + 3 * dy + dx;
-      int         R  = seg / g.Ns;
-      if (d != 13) {
-        R = nb[R * 27 + d];
-        cx += (cx < 0) * g.dims[2] - (cx >= g.dims[2]) * g.dims[2];
-        cy += (cy < 0) * g.dims[1] - (cy >= g.dims[1]) * g.dims[1];
-        cz += (cz < 0) * g.dims[0] - (cz >= g.dims[0]) * g.dims[0];
-      }
-      if (R >= 0 && cx >= 0 && cx < g.dims[2] && cy >= 0 && cy < g.dims[1] && cz >= 0 &&
-          cz < g.dims[0]) {
-        ds = R * g.Ns + (seg - (seg / g.Ns) * g.Ns);
-        b  = bin_of3(g, cx, cy, cz);
-        k  = d == 13 ? b : -2 - (d * g.nbin + b);
-      }
-    }
+    if (replace)
+      k = kKeyReplace;
+    else
+      dest32(g, seg, bin, rx, ry, rz, nb, ds, b, k);
   }
   const bool want = ds >= 0;
   const int  sl   = wave_slot(cellcnt, want ? ds * g.nbin + b : 0, want);
@@ -462,6 +502,111 @@ __global__ void __launch_bounds__(256)
   key[i] = k;
   if (want)
     slot[i] = sl;
+}
+
+// The shock problem's replacements in the FP32 form: x, ux, uy, uz of every
+// particle listed by k_wall_list, from the host's draws (absolute x), written
+// relative to the particle's old cell; then its destination and slot.
+__global__ void __launch_bounds__(256) k32_wall_apply(Geom g, P32 p, const int32_t* __restrict__ useg,
+                                                      const int32_t* __restrict__ list,
+                                                      const int32_t* __restrict__ count,
+                                                      const int64_t* __restrict__ voff,
+                                                      const double* __restrict__ vals,
+                                                      const double* __restrict__ lim,
+                                                      const int* __restrict__ nb,
+                                                      int32_t* __restrict__ cellcnt,
+                                                      int32_t* __restrict__ key,
+                                                      int32_t* __restrict__ slot)
+{
+  const int k   = blockIdx.x;
+  const int seg = useg[k];
+  const int c   = seg / g.Ns;
+  const int n   = count[k];
+  for (int t = (int)threadIdx.x; t < n; t += blockDim.x) {
+    const long    i   = (long)seg * g.cap + list[(long)k * g.cap + t];
+    const double* v   = vals + voff[k] + 4 * (long)t;
+    const int     bin = p.cell[i];
+    const Cell3   cl  = cell_of(g, bin);
+    const float   rx  = (float)((v[0] - lim[c * 6 + 4]) / g.del[2] - cl.x);
+    p.rx[i]           = rx;
+    p.ux[i]           = (float)v[1];
+    p.uy[i]           = (float)v[2];
+    p.uz[i]           = (float)v[3];
+    int ds, b, kk;
+    dest32(g, seg, bin, rx, p.ry[i], p.rz[i], nb, ds, b, kk);
+    key[i] = kk;
+    if (ds >= 0)
+      slot[i] = atomicAdd(&cellcnt[(long)ds * g.nbin + b], 1);
+  }
+}
+
+// The shock problem's injected particles in the FP32 form: appended after each
+// upper segment's particles as cell + offset in [0, 1) (old = new), each with its
+// destination and slot; the count grows, or the capacity flag is raised.
+__global__ void __launch_bounds__(256) k32_inject_append(Geom g, P32 p, float* __restrict__ ro0,
+                                                         float* __restrict__ ro1,
+                                                         float* __restrict__ ro2,
+                                                         const int32_t* __restrict__ useg,
+                                                         const int32_t* __restrict__ ninj,
+                                                         const int64_t* __restrict__ voff,
+                                                         const double* __restrict__ vals,
+                                                         const double* __restrict__ lim,
+                                                         const int* __restrict__ nb,
+                                                         int32_t* __restrict__ np,
+                                                         int32_t* __restrict__ cellcnt,
+                                                         int32_t* __restrict__ key,
+                                                         int32_t* __restrict__ slot,
+                                                         int32_t* __restrict__ errflag)
+{
+  __shared__ int n0s;
+  const int k   = blockIdx.x;
+  const int seg = useg[k];
+  const int c   = seg / g.Ns;
+  const int n   = ninj[k];
+  if (threadIdx.x == 0)
+    n0s = np[seg];
+  __syncthreads();
+  const int n0 = n0s;
+  if ((long)n0 + n > g.cap) {
+    if (threadIdx.x == 0)
+      atomicOr(&errflag[0], 1);
+    return;
+  }
+  for (int t = (int)threadIdx.x; t < n; t += blockDim.x) {
+    const long    i = (long)seg * g.cap + n0 + t;
+    const double* v = vals + voff[k] + 7 * (long)t;
+    const double  X = (v[0] - lim[c * 6 + 4]) / g.del[2];
+    const double  Y = g.has[1] ? (v[1] - lim[c * 6 + 2]) / g.del[1] : 0.0;
+    const double  Z = g.has[0] ? (v[2] - lim[c * 6 + 0]) / g.del[0] : 0.0;
+    int cx = g.has[2] ? (int)floor(X) : 0, cy = g.has[1] ? (int)floor(Y) : 0,
+        cz = g.has[0] ? (int)floor(Z) : 0;
+    cx = min(max(cx, 0), g.dims[2] - 1);
+    cy = min(max(cy, 0), g.dims[1] - 1);
+    cz = min(max(cz, 0), g.dims[0] - 1);
+    const float rx = (float)(X - cx);
+    const float ry = g.has[1] ? (float)(Y - cy) : (float)v[1];
+    const float rz = g.has[0] ? (float)(Z - cz) : (float)v[2];
+    const int   bin = bin_of3(g, cx, cy, cz);
+    p.rx[i]   = rx;
+    p.ry[i]   = ry;
+    p.rz[i]   = rz;
+    ro0[i]    = rx;
+    ro1[i]    = ry;
+    ro2[i]    = rz;
+    p.ux[i]   = (float)v[3];
+    p.uy[i]   = (float)v[4];
+    p.uz[i]   = (float)v[5];
+    p.id[i]   = __double_as_longlong(v[6]);
+    p.cell[i] = bin;
+    int ds, b, kk;
+    dest32(g, seg, bin, rx, ry, rz, nb, ds, b, kk);
+    key[i] = kk;
+    if (ds >= 0)
+      slot[i] = atomicAdd(&cellcnt[(long)ds * g.nbin + b], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    np[seg] = n0 + n;
 }
 
 // ---------------------------------------------------------------------------
@@ -1269,9 +1414,30 @@ void launch32_push_position(Sim& s, double delt)
   hipLaunchKernelGGL(k32_push_position, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa,
                      s.f_ro[0], s.f_ro[1], s.f_ro[2], s.d_np, (float)delt, (float)(1 / g.cc),
                      (float)(1 / g.del[2]), (float)(1 / g.del[1]), (float)(1 / g.del[0]),
-                     s.d_nb, s.ss.cellcnt, s.ss.key, s.ss.slot);
+                     s.d_nb, s.d_off, s.ss.cellcnt, s.ss.key, s.ss.slot);
   s.counted = true;
   s.sorted  = false;
+}
+
+void launch32_wall_apply(Sim& s)
+{
+  const int nk = (int)s.shk.useg.size();
+  if (nk == 0)
+    return;
+  hipLaunchKernelGGL(k32_wall_apply, dim3(nk), dim3(256), 0, s.stream, s.g, s.qa, s.shk.d_useg,
+                     s.shk.d_list, s.shk.d_count, s.shk.d_voff, s.shk.d_vals, s.d_lim, s.d_nb,
+                     s.ss.cellcnt, s.ss.key, s.ss.slot);
+}
+
+void launch32_inject_append(Sim& s)
+{
+  const int nk = (int)s.shk.useg.size();
+  if (nk == 0)
+    return;
+  hipLaunchKernelGGL(k32_inject_append, dim3(nk), dim3(256), 0, s.stream, s.g, s.qa, s.f_ro[0],
+                     s.f_ro[1], s.f_ro[2], s.shk.d_useg, s.shk.d_count, s.shk.d_voff,
+                     s.shk.d_vals, s.d_lim, s.d_nb, s.d_np, s.ss.cellcnt, s.ss.key, s.ss.slot,
+                     s.ss.errflag);
 }
 
 // The FP32 kernels deposit J only; rho follows as the mode asks.

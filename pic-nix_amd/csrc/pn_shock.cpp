@@ -18,6 +18,7 @@
 #include <thread>
 #include <vector>
 
+#include "../../include/picnix.h"
 #include "pn_sim.hpp"
 
 namespace picnix {
@@ -324,6 +325,8 @@ void shock_init_fields(Sim& s)
   HIPCHK(hipMemcpy(s.d_B, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice));
   launch_fill_ghost(s, s.d_E, 3);
   launch_fill_ghost(s, s.d_B, 3);
+  if (s.fp32)
+    launch32_fields_from_f64(s, PN_FIELD_UF); // the wall reset below acts on the FP32 fields
   launch_wall_emf(s);
 }
 
@@ -409,6 +412,8 @@ void shock_load_particles(Sim& s)
   up(s.d_np, hnp.data(), g.nseg * sizeof(int32_t));
   s.h_np          = hnp;
   s.np_host_valid = true;
+  if (s.fp32)
+    launch32_from_f64(s, 0, g.nseg);
 }
 
 // MainChunk::set_boundary_particle at x = Lx (example/shock/main.cpp:418-442):
@@ -450,7 +455,10 @@ void shock_position_boundary(Sim& s)
     }
   });
   upload_vals(s, vals, nullptr);
-  launch_wall_apply(s);
+  if (s.fp32)
+    launch32_wall_apply(s);
+  else
+    launch_wall_apply(s);
 }
 
 // MainChunk::inject_particle (example/shock/main.cpp:445-535): a Poisson number of
@@ -506,7 +514,10 @@ void shock_inject(Sim& s)
   });
   upload_vals(s, vals, &ninj);
   h.last_ninj = ninj.empty() ? 0 : *std::max_element(ninj.begin(), ninj.end());
-  launch_inject_append(s);
+  if (s.fp32)
+    launch32_inject_append(s);
+  else
+    launch_inject_append(s);
   s.np_host_valid = false;
 }
 

@@ -227,6 +227,8 @@ void launch_wall_emf(Sim& s);     // MainChunk::set_boundary_emf after an EMF ex
 void launch_wall_list(Sim& s);    // particles past x = Lx -> shk.d_list / d_count
 void launch_wall_apply(Sim& s);   // their replacement values -> particles, bins, slots
 void launch_inject_append(Sim& s);
+void launch32_wall_apply(Sim& s);    // the FP32 form's versions (pn_fp32.hip)
+void launch32_inject_append(Sim& s);
 void launch_moment_energy(Sim& s, double* out /* host: particle[Ns] */);
 // minus the rho of the particles the last shock_inject appended (nmax: the most
 // injected into one segment)

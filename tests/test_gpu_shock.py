@@ -113,3 +113,46 @@ def test_shock_parity(name):
     finally:
         g.close()
         o.close()
+
+
+@pytest.mark.parametrize("name", ["shock2d_o2", "shock3d_ramp"])
+def test_shock_fp32_follows_fp64(name):
+    """The FP32 form with the same boundaries: the injections (their draws do not
+    depend on the particles) are identical, the redraws nearly so, and the
+    energies follow the FP64 run to FP32 round-off after 10 steps."""
+    import picnix_amd
+    dim, kw = CASES[name]
+    kw = dict(kw)
+    kw.pop("emf_order", None)
+    cfg64 = helpers.small_shock(**kw)
+    cfg32 = helpers.small_shock(precision="fp32", **kw)
+    dt = float(cfg64["parameter"]["delt"])
+    a = picnix_amd.Simulation(helpers.dumps(cfg64))
+    b = picnix_amd.Simulation(helpers.dumps(cfg32))
+    try:
+        inj = [0, 0]
+        red = [0, 0]
+        for _ in range(10):
+            a.step(dt)
+            b.step(dt)
+            for c in range(a.nchunks):
+                sa, sb = a.boundary_state(c), b.boundary_state(c)
+                assert sa[2:4] == sb[2:4], (c, sa, sb)  # influx
+                inj[0] += sa[2] + sa[3]
+                inj[1] += sb[2] + sb[3]
+                red[0] += sa[4] + sa[5]
+                red[1] += sb[4] + sb[5]
+        na = sum(a.np(c, s) for c in range(a.nchunks) for s in range(2))
+        nb = sum(b.np(c, s) for c in range(b.nchunks) for s in range(2))
+        ha, hb = a.history(), b.history()
+        _say(f"[shock] {name} fp32 vs fp64: particles {nb} vs {na}, injected {inj}, "
+             f"redrawn {red}, E2 {hb[2]:.6e} vs {ha[2]:.6e}, B2 {hb[3]:.6e} vs {ha[3]:.6e}, "
+             f"K {hb[4]:.6e} vs {ha[4]:.6e}")
+        assert inj[0] == inj[1] and inj[0] > 0
+        assert abs(red[0] - red[1]) <= max(2, red[0] // 20)
+        assert abs(na - nb) <= max(4, na // 1000)
+        for k in (2, 3, 4, 5):
+            assert abs(hb[k] - ha[k]) <= 2e-3 * abs(ha[k]), (k, ha[k], hb[k])
+    finally:
+        a.close()
+        b.close()
