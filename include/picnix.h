@@ -128,7 +128,8 @@ int pn_get_diverror(pn_sim* sim, double* efd, double* bfd);
 int pn_synchronize(pn_sim* sim);
 
 /* Autotuner of the particle kernels' launch shapes (the bin rows per block of the
- * current and rho march kernels, the bin-row or one-thread-per-particle gather).
+ * current and rho march kernels, 1, 2 or 4 bin rows per gather block or one
+ * thread per particle).
  * No reference counterpart (the reference picks its loop forms by the
  * [application.option] vectorization table, pic/pic_chunk.cpp:140-189).
  * enable = 1 starts a session: the next pn_get_tuning()[1] pn_step calls run
@@ -138,7 +139,8 @@ int pn_synchronize(pn_sim* sim);
  * same step, up to the order of the current sums.  enable = 0 returns to the
  * built-in heuristics.  pn_get_tuning: out5 = {state (0 off, 1 measuring,
  * 2 done), steps left, bin rows of the current march (0 = heuristic), bin rows of
- * the rho march, 1 if the gather runs one thread per particle}. */
+ * the rho march, bin rows per gather block (0 = heuristic, -1 = one thread per
+ * particle)}. */
 int pn_autotune(pn_sim* sim, int enable);
 int pn_get_tuning(pn_sim* sim, int* out5);
 

@@ -1185,19 +1185,35 @@ static void drain_timing(Sim& s)
 // on the run's own steps.  A session of 2 x max(#candidates, 2) steps runs them
 // eagerly with per-phase HIP-event timing; step k tries candidate k mod n of
 // every knob (bin rows per march block for the current and rho march kernels;
-// bin-row or one-thread-per-particle gather), each candidate is measured twice
+// 1, 2 or 4 bin rows per gather block or one thread per particle), each
+// candidate is measured twice
 // and keeps its faster time, and the fastest candidate of every knob stays.
 // Every candidate computes the same step (the current is summed in another
 // order), so the tuning steps are ordinary steps of the run.
 // ---------------------------------------------------------------------------
-static int tune_cycle(const Sim& s) { return std::max<int>((int)s.tune.cand.size(), 2); }
+static constexpr int kVelShapes          = 4;
+static constexpr int kVelRows[kVelShapes] = {1, 2, 4, 0}; // 0: one thread per particle
+
+static int tune_cycle(const Sim& s)
+{
+  return std::max<int>((int)s.tune.cand.size(), s.tune.tune_vel ? kVelShapes : 2);
+}
+
+// rows > 0: that many bin rows per gather block; 0: one thread per particle;
+// -1: the built-in heuristic
+static void set_vel_shape(Sim& s, int rows)
+{
+  s.vel_generic = rows == 0 ? 1 : 0;
+  s.vel_rows    = std::max(rows, 0);
+}
 
 void tune_begin(Sim& s)
 {
   Sim::AutoTune& t = s.tune;
   t                = Sim::AutoTune{};
   const Geom& g    = s.g;
-  s.dep_rows = s.rho_rows = s.vel_generic = 0;
+  s.dep_rows = s.rho_rows = 0;
+  set_vel_shape(s, -1);
   if (g.dim == 3 && !s.generic_kernels) {
     const int nbz = g.nbin / g.bsz;
     int       hz, nzr;
@@ -1207,7 +1223,7 @@ void tune_begin(Sim& s)
         t.cand.push_back(v);
     std::sort(t.cand.begin(), t.cand.end());
   }
-  t.tune_vel = !s.fp32 && g.dim >= 2 && !s.generic_kernels;
+  t.tune_vel = g.dim >= 2 && !s.generic_kernels && s.sorted;
   if (t.cand.empty() && !t.tune_vel) {
     t.state = 2;
     return;
@@ -1215,7 +1231,7 @@ void tune_begin(Sim& s)
   const double inf = std::numeric_limits<double>::infinity();
   t.t_dep.assign(t.cand.size(), inf);
   t.t_rho.assign(t.cand.size(), inf);
-  t.t_vel[0] = t.t_vel[1] = inf;
+  t.t_vel.assign(kVelShapes, inf);
   t.state                 = 1;
   drop_graphs(s);
 }
@@ -1232,7 +1248,12 @@ static void tune_finish(Sim& s)
   };
   s.dep_rows    = best(t.t_dep);
   s.rho_rows    = best(t.t_rho);
-  s.vel_generic = (t.tune_vel && t.t_vel[1] < t.t_vel[0]) ? 1 : 0;
+  if (t.tune_vel) {
+    const size_t k = std::min_element(t.t_vel.begin(), t.t_vel.end()) - t.t_vel.begin();
+    set_vel_shape(s, std::isfinite(t.t_vel[k]) ? kVelRows[k] : -1);
+  } else {
+    set_vel_shape(s, -1);
+  }
   t.state       = 2;
   drop_graphs(s);
 }
@@ -1245,7 +1266,7 @@ void tune_step(Sim& s, double delt)
   if (!t.cand.empty())
     s.dep_rows = s.rho_rows = t.cand[c % t.cand.size()];
   if (t.tune_vel)
-    s.vel_generic = c % 2;
+    set_vel_shape(s, kVelRows[c % kVelShapes]);
   // time this step's phases alone; the caller's accumulators are set aside
   drain_timing(s);
   double saved_ms[PN_NPHASE + 1];
@@ -1276,7 +1297,7 @@ void tune_step(Sim& s, double delt)
       t.t_rho[k] = std::min(t.t_rho[k], rho);
   }
   if (t.tune_vel && vel >= 0)
-    t.t_vel[c % 2] = std::min(t.t_vel[c % 2], vel);
+    t.t_vel[c % kVelShapes] = std::min(t.t_vel[c % kVelShapes], vel);
   t.step++;
   if (t.step >= 2 * n)
     tune_finish(s);
@@ -1293,7 +1314,8 @@ int pn_autotune(pn_sim* h, int enable)
       tune_begin(s);
     } else {
       s.tune     = Sim::AutoTune{};
-      s.dep_rows = s.rho_rows = s.vel_generic = 0;
+      s.dep_rows = s.rho_rows = 0;
+      set_vel_shape(s, -1);
       drop_graphs(s);
     }
     return PN_OK;
@@ -1309,7 +1331,7 @@ int pn_get_tuning(pn_sim* h, int* out5)
     out5[1] = s.tune.state == 1 ? 2 * tune_cycle(s) - s.tune.step : 0;
     out5[2] = s.dep_rows;
     out5[3] = s.rho_rows;
-    out5[4] = s.vel_generic;
+    out5[4] = s.vel_generic ? -1 : s.vel_rows;
     return PN_OK;
   });
 }

@@ -194,7 +194,7 @@ template <int Dim, int Order>
 __global__ void __launch_bounds__(256)
     k32_push_velocity_rows(Geom g, P32 p, const int32_t* __restrict__ cellstart,
                            const int* __restrict__ off, const float* __restrict__ E,
-                           const float* __restrict__ B, Push32 pc)
+                           const float* __restrict__ B, Push32 pc, int yrows)
 {
   constexpr int is_odd = Order % 2;
   constexpr int S      = Order + 1;
@@ -202,20 +202,26 @@ __global__ void __launch_bounds__(256)
   extern __shared__ float ft32[]; // [6][TZ][T][TXv]
   const int c   = blockIdx.y; // chunk: every species' particles of the row share the tile
   const int nbx = g.bsy;
+  // yrows consecutive bin rows of one z plane per block (the last block of a
+  // plane may hold fewer): they are contiguous in the sorted order and share
+  // one staged tile of TY = T + nr - 1 rows
+  const int nby = Dim == 3 ? g.bsz / g.bsy : g.nbin / g.bsy;
+  const int nyb = (nby + yrows - 1) / yrows;
   int       bz = 0, by, b0;
   if constexpr (Dim == 3) {
-    const int nby = g.bsz / g.bsy;
-    bz            = blockIdx.x / nby;
-    by            = blockIdx.x - bz * nby;
-    b0            = bz * g.bsz + by * g.bsy;
+    bz = blockIdx.x / nyb;
+    by = (blockIdx.x - bz * nyb) * yrows;
+    b0 = bz * g.bsz + by * g.bsy;
   } else {
-    by = blockIdx.x;
+    by = blockIdx.x * yrows;
     b0 = by * g.bsy;
   }
+  const int nr = min(yrows, nby - by);
+  const int TY = T + nr - 1;
   bool any = false;
   for (int sp = 0; sp < g.Ns; sp++) {
     const int32_t* cs = cellstart + (long)(c * g.Ns + sp) * (g.nbin + 1);
-    any               = any || cs[b0 + nbx] > cs[b0];
+    any               = any || cs[b0 + nr * nbx] > cs[b0];
   }
   if (!any)
     return;
@@ -225,14 +231,14 @@ __global__ void __launch_bounds__(256)
   const int  ty0 = by - is_odd + lb - (Order / 2);
   const int  tx0 = -is_odd + lb - (Order / 2);
   const int  TXv = nbx + Order + 1;
-  const int  pc1 = TZ * T * TXv;
+  const int  pc1 = TZ * TY * TXv;
   const long sy = g.P[2], sz = g.plane, v = g.vol;
   const int  offz = off[c * 3 + 0], offy = off[c * 3 + 1], offx = off[c * 3 + 2];
   for (int t = threadIdx.x; t < 6 * pc1; t += blockDim.x) {
     int k  = t / pc1;
     int r  = t - k * pc1;
-    int tz = r / (T * TXv);
-    int r2 = r - tz * T * TXv;
+    int tz = r / (TY * TXv);
+    int r2 = r - tz * TY * TXv;
     int ty = r2 / TXv;
     int tx = r2 - ty * TXv;
     int gz = Dim == 3 ? offz + tz0 + tz : 0;
@@ -264,7 +270,7 @@ __global__ void __launch_bounds__(256)
   for (int sp = 0; sp < g.Ns; sp++) {
   const int      seg = c * g.Ns + sp;
   const int32_t* cs  = cellstart + (long)seg * (g.nbin + 1);
-  const int      p0  = cs[b0], p1 = cs[b0 + nbx];
+  const int      p0  = cs[b0], p1 = cs[b0 + nr * nbx];
   qmdt               = pc.qmdt[sp];
   for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
     const long  i    = (long)seg * g.cap + q;
@@ -292,10 +298,10 @@ __global__ void __launch_bounds__(256)
     }
     // sorted particles sit in the row's bins, so the stencils are inside the tile
     const bool in = min(X, HX) >= 0 && max(X, HX) + S <= TXv && min(Y, HY) >= 0 &&
-                    max(Y, HY) + S <= T && min(Z, HZ) >= 0 && max(Z, HZ) + (Dim == 3 ? S : 1) <= TZ;
+                    max(Y, HY) + S <= TY && min(Z, HZ) >= 0 && max(Z, HZ) + (Dim == 3 ? S : 1) <= TZ;
     if (!in)
       continue;
-    const long bsz = (long)T * TXv, bsy = TXv;
+    const long bsz = (long)TY * TXv, bsy = TXv;
     const int  nz  = Dim == 3 ? S : 1;
     float      eb[6];
     eb[0] = interp(ft32 + 0 * pc1, bsz, bsy, Z, Y, HX, wiz, wiy, whx, nz);
@@ -1376,14 +1382,22 @@ void launch32_push_velocity(Sim& s, double delt)
     pc.rcdt[a] = 1.0f / pc.cdt[a];
   }
   const int    T    = g.order + 2;
-  const size_t lds  = sizeof(float) * 6 * (g.dim == 3 ? T : 1) * T * (g.bsy + g.order + 1);
-  const bool   rows = s.sorted && !s.generic_kernels && lds <= 65536;
-  const int    nrow = g.nbin / g.bsy;
+  const int    nby  = g.dim == 3 ? g.bsz / g.bsy : g.nbin / g.bsy;
+  auto         lds_of = [&](int r) {
+    return sizeof(float) * 6 * (g.dim == 3 ? T : 1) * (T + r - 1) * (g.bsy + g.order + 1);
+  };
+  int yr = s.vel_rows > 0 ? s.vel_rows : 4; // 0: four rows, or fewer where LDS is short
+  while (yr > 1 && lds_of(yr) > 65536)
+    yr = (yr + 1) / 2;
+  const size_t lds  = lds_of(yr);
+  const bool   rows = s.sorted && !s.generic_kernels && !s.vel_generic && lds <= 65536;
+  const int    nblk_rows = (g.nbin / g.bsy / nby) * ((nby + yr - 1) / yr);
 #define PN32_VEL(D, O)                                                                        \
   do {                                                                                        \
     if (rows)                                                                                 \
-      hipLaunchKernelGGL((k32_push_velocity_rows<D, O>), dim3(nrow, g.nchunk), dim3(256), lds, \
-                         s.stream, g, s.qa, s.ss.cellstart, s.d_off, s.f_E, s.f_B, pc);       \
+      hipLaunchKernelGGL((k32_push_velocity_rows<D, O>), dim3(nblk_rows, g.nchunk), dim3(256), \
+                         lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off, s.f_E, s.f_B, pc,  \
+                         yr);                                                                 \
     else                                                                                      \
       hipLaunchKernelGGL((k32_push_velocity<D, O>), dim3(nblk(g.cap, 256), g.nseg), dim3(256), \
                          0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc);            \

@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(256)
     k_push_velocity_rows(Geom g, PArr p, const int32_t* __restrict__ cellstart,
                          const int* __restrict__ off, const double* __restrict__ lim,
                          const double* __restrict__ E, const double* __restrict__ B,
-                         PushConsts pc)
+                         PushConsts pc, int yrows)
 {
   constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
   constexpr int S      = Order + 1;
@@ -211,20 +211,26 @@ __global__ void __launch_bounds__(256)
   extern __shared__ double ft[]; // [6][TZ][T][TXv]
   const int c   = blockIdx.y; // chunk: every species' particles of the row share the tile
   const int nbx = g.bsy;
+  // yrows consecutive bin rows of one z plane per block (the last block of a
+  // plane may hold fewer): they are contiguous in the sorted order and share
+  // one staged tile of TY = T + nr - 1 rows
+  const int nby = Dim == 3 ? g.bsz / g.bsy : g.nbin / g.bsy;
+  const int nyb = (nby + yrows - 1) / yrows;
   int       bz = 0, by, b0;
   if constexpr (Dim == 3) {
-    const int nby = g.bsz / g.bsy;
-    bz            = blockIdx.x / nby;
-    by            = blockIdx.x - bz * nby;
-    b0            = bz * g.bsz + by * g.bsy;
+    bz = blockIdx.x / nyb;
+    by = (blockIdx.x - bz * nyb) * yrows;
+    b0 = bz * g.bsz + by * g.bsy;
   } else {
-    by = blockIdx.x;
+    by = blockIdx.x * yrows;
     b0 = by * g.bsy;
   }
+  const int nr = min(yrows, nby - by);
+  const int TY = T + nr - 1;
   bool any = false;
   for (int sp = 0; sp < g.Ns; sp++) {
     const int32_t* cs = cellstart + (long)(c * g.Ns + sp) * (g.nbin + 1);
-    any               = any || cs[b0 + nbx] > cs[b0];
+    any               = any || cs[b0 + nr * nbx] > cs[b0];
   }
   if (!any)
     return;
@@ -234,14 +240,14 @@ __global__ void __launch_bounds__(256)
   const int  ty0 = by - is_odd + lb - (Order / 2);
   const int  tx0 = -is_odd + lb - (Order / 2);
   const int  TXv = nbx + Order + 1;
-  const int  pc1 = TZ * T * TXv; // per component
+  const int  pc1 = TZ * TY * TXv; // per component
   const long sy = g.P[2], sz = g.plane, v = g.vol;
   const int  offz = off[c * 3 + 0], offy = off[c * 3 + 1], offx = off[c * 3 + 2];
   for (int t = threadIdx.x; t < 6 * pc1; t += blockDim.x) {
     int k  = t / pc1;
     int r  = t - k * pc1;
-    int tz = r / (T * TXv);
-    int r2 = r - tz * T * TXv;
+    int tz = r / (TY * TXv);
+    int r2 = r - tz * TY * TXv;
     int ty = r2 / TXv;
     int tx = r2 - ty * TXv;
     int gz = Dim == 3 ? offz + tz0 + tz : 0;
@@ -299,7 +305,7 @@ __global__ void __launch_bounds__(256)
   for (int sp = 0; sp < g.Ns; sp++) {
   const int      seg = c * g.Ns + sp;
   const int32_t* cs  = cellstart + (long)seg * (g.nbin + 1);
-  const int      p0  = cs[b0], p1 = cs[b0 + nbx];
+  const int      p0  = cs[b0], p1 = cs[b0 + nr * nbx];
   qmdt               = pc.qmdt[sp];
   for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
     const long i     = (long)seg * g.cap + q;
@@ -323,7 +329,7 @@ __global__ void __launch_bounds__(256)
     hx0 += lb - (Order / 2);
     hy0 += lb - (Order / 2);
     bool in = min(ix0, hx0) >= tx0 && max(ix0, hx0) + S <= tx0 + TXv && min(iy0, hy0) >= ty0 &&
-              max(iy0, hy0) + S <= ty0 + T;
+              max(iy0, hy0) + S <= ty0 + TY;
     double eb[6];
     if constexpr (Dim == 3) {
       const double zimin  = zmin + 0.5 * dz * is_odd;
@@ -341,7 +347,7 @@ __global__ void __launch_bounds__(256)
       hz0 += lb - (Order / 2);
       in = in && min(iz0, hz0) >= tz0 && max(iz0, hz0) + S <= tz0 + T;
       if (in) {
-        const long bsz = (long)T * TXv, bsy = TXv;
+        const long bsz = (long)TY * TXv, bsy = TXv;
         const int  Z = iz0 - tz0, HZ = hz0 - tz0, Y = iy0 - ty0, HY = hy0 - ty0;
         const int  X = ix0 - tx0, HX = hx0 - tx0;
         eb[0] = interp3(ft + 0 * pc1, bsz, bsy, Z, Y, HX, wiz, wiy, whx);
@@ -545,14 +551,21 @@ void launch_push_velocity(Sim& s, double delt)
   }
   dim3         grid(nblk(g.cap, 256), g.nseg);
   const int    T     = g.order + 2;
-  const size_t lds   = sizeof(double) * 6 * (g.dim == 3 ? T : 1) * T * (g.bsy + g.order + 1);
-  const bool   rows  = g.dim >= 2 && s.sorted && !s.generic_kernels && !s.vel_generic &&
-                     lds <= 65536;
-  const int    nrow  = g.nbin / g.bsy;
+  const int    nby   = g.dim == 3 ? g.bsz / g.bsy : g.nbin / g.bsy;
+  auto         lds_of = [&](int r) {
+    return sizeof(double) * 6 * (g.dim == 3 ? T : 1) * (T + r - 1) * (g.bsy + g.order + 1);
+  };
+  int yr = s.vel_rows > 0 ? s.vel_rows : 4; // 0: four rows, or fewer where LDS is short
+  while (yr > 1 && lds_of(yr) > 65536)
+    yr = (yr + 1) / 2;
+  const size_t lds  = lds_of(yr);
+  const bool   rows = g.dim >= 2 && s.sorted && !s.generic_kernels && !s.vel_generic &&
+                    lds <= 65536;
+  const int    nrow  = (g.nbin / g.bsy / nby) * ((nby + yr - 1) / yr);
   const bool   plain = pc.pusher == 0 && pc.wt == 0; // Boris + MC: no option branches
 #define PN_VEL_ROWS(D, O, PL)                                                                 \
   hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL>), dim3(nrow, g.nchunk), dim3(256), lds, \
-                     s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc)
+                     s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc, yr)
 #define PN_VEL(D, O)                                                                          \
   do {                                                                                        \
     if (rows && plain)                                                                        \

@@ -133,12 +133,13 @@ struct Sim {
   int         dep_rows    = 0; // bin rows per block of the current march kernels
   int         rho_rows    = 0; // bin rows per block of the rho march kernels
   int         vel_generic = 0; // 1: one thread per particle instead of the bin-row gather
+  int         vel_rows    = 0; // bin rows of one z plane per gather block (0: heuristic)
   struct AutoTune {
     int                 state = 0; // 0 off, 1 measuring, 2 done
     int                 step  = 0; // tuning steps taken
     std::vector<int>    cand;      // candidate bin rows per march block
     std::vector<double> t_dep, t_rho;
-    double              t_vel[2] = {0, 0};
+    std::vector<double> t_vel; // per gather shape (1, 2, 4 rows; per particle)
     bool                tune_vel = false;
   } tune;
 

@@ -29,6 +29,7 @@ def test_autotune_session_matches_untuned_run(precision):
         print(f"[autotune] {precision}: {n} tuning steps -> {t1}", flush=True)
         assert t1["state"] == 2 and t1["steps_left"] == 0
         assert t1["deposit_rows"] >= 1 and t1["rho_rows"] >= 0
+        assert t1["gather_rows"] in (-1, 1, 2, 4)
         tol = 1e-12 if precision == "fp64" else 1e-4
         for c in range(a.nchunks):
             info = a.chunk_info(c)
@@ -42,6 +43,7 @@ def test_autotune_session_matches_untuned_run(precision):
                     assert helpers.rel_err(pa[:, :6], pb[:, :6]) <= tol
         a.autotune(False)
         assert a.tuning()["state"] == 0 and a.tuning()["deposit_rows"] == 0
+        assert a.tuning()["gather_rows"] == 0
     finally:
         a.close()
         b.close()
