@@ -87,6 +87,7 @@ CASES = {
     "thermal3d_o2": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=2)),
     "thermal3d_o1": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=1)),
     "thermal3d_o3": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=4, order=3)),
+    "thermal3d_o4": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=4, order=4)),
     "twostream2d_o1": (2, lambda: helpers.small_twostream(N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1)),
 }
 
@@ -266,10 +267,13 @@ def _moments_rel(o, g, dim):
     return worst / scale if scale > 0 else worst
 
 
-@pytest.mark.parametrize("kind", ["thermal3d_o2", "thermal3d_o1", "twostream2d_o1"])
+@pytest.mark.parametrize("kind", ["thermal3d_o2", "thermal3d_o1", "thermal3d_o3", "thermal3d_o4",
+                                  "twostream2d_o1"])
 def test_moment_deposit(kind):
     """PicChunk::deposit_moment + Mom exchange on sorted particles (LDS row
-    kernel) and on unsorted ones (after push_position: one thread per particle)."""
+    kernels: per-bin register sums up to order 3, LDS atomics per particle for
+    order 4 in 3D) and on unsorted ones (after push_position: one thread per
+    particle)."""
     import picnix_amd
     dim, make = CASES[kind]
     cfg = make()
