@@ -1185,14 +1185,17 @@ static void drain_timing(Sim& s)
 // on the run's own steps.  A session of 2 x max(#candidates, 2) steps runs them
 // eagerly with per-phase HIP-event timing; step k tries candidate k mod n of
 // every knob (bin rows per march block for the current and rho march kernels;
-// 1, 2 or 4 bin rows per gather block or one thread per particle), each
-// candidate is measured twice
+// 1, 2 or 4 bin rows per gather block for all species or 1 or 2 per species,
+// or one thread per particle), each candidate is measured twice
 // and keeps its faster time, and the fastest candidate of every knob stays.
 // Every candidate computes the same step (the current is summed in another
 // order), so the tuning steps are ordinary steps of the run.
 // ---------------------------------------------------------------------------
-static constexpr int kVelShapes          = 4;
-static constexpr int kVelRows[kVelShapes] = {1, 2, 4, 0}; // 0: one thread per particle
+// gather shapes: bin rows per block (0: one thread per particle) and whether a
+// block serves one species (1) or all of them (0)
+static constexpr int kVelShapes           = 6;
+static constexpr int kVelRows[kVelShapes]  = {1, 2, 4, 1, 2, 0};
+static constexpr int kVelSplit[kVelShapes] = {0, 0, 0, 1, 1, 0};
 
 static int tune_cycle(const Sim& s)
 {
@@ -1200,11 +1203,12 @@ static int tune_cycle(const Sim& s)
 }
 
 // rows > 0: that many bin rows per gather block; 0: one thread per particle;
-// -1: the built-in heuristic
-static void set_vel_shape(Sim& s, int rows)
+// -1: the built-in heuristic.  split: one block per species.
+static void set_vel_shape(Sim& s, int rows, int split = 0)
 {
   s.vel_generic = rows == 0 ? 1 : 0;
   s.vel_rows    = std::max(rows, 0);
+  s.vel_split   = split;
 }
 
 void tune_begin(Sim& s)
@@ -1250,7 +1254,10 @@ static void tune_finish(Sim& s)
   s.rho_rows    = best(t.t_rho);
   if (t.tune_vel) {
     const size_t k = std::min_element(t.t_vel.begin(), t.t_vel.end()) - t.t_vel.begin();
-    set_vel_shape(s, std::isfinite(t.t_vel[k]) ? kVelRows[k] : -1);
+    if (std::isfinite(t.t_vel[k]))
+      set_vel_shape(s, kVelRows[k], kVelSplit[k]);
+    else
+      set_vel_shape(s, -1);
   } else {
     set_vel_shape(s, -1);
   }
@@ -1266,7 +1273,7 @@ void tune_step(Sim& s, double delt)
   if (!t.cand.empty())
     s.dep_rows = s.rho_rows = t.cand[c % t.cand.size()];
   if (t.tune_vel)
-    set_vel_shape(s, kVelRows[c % kVelShapes]);
+    set_vel_shape(s, kVelRows[c % kVelShapes], kVelSplit[c % kVelShapes]);
   // time this step's phases alone; the caller's accumulators are set aside
   drain_timing(s);
   double saved_ms[PN_NPHASE + 1];
@@ -1322,16 +1329,17 @@ int pn_autotune(pn_sim* h, int enable)
   });
 }
 
-int pn_get_tuning(pn_sim* h, int* out5)
+int pn_get_tuning(pn_sim* h, int* out6)
 {
   return guarded(h, [&](Sim& s) {
-    if (!out5)
+    if (!out6)
       return PN_ERR_ARG;
-    out5[0] = s.tune.state;
-    out5[1] = s.tune.state == 1 ? 2 * tune_cycle(s) - s.tune.step : 0;
-    out5[2] = s.dep_rows;
-    out5[3] = s.rho_rows;
-    out5[4] = s.vel_generic ? -1 : s.vel_rows;
+    out6[0] = s.tune.state;
+    out6[1] = s.tune.state == 1 ? 2 * tune_cycle(s) - s.tune.step : 0;
+    out6[2] = s.dep_rows;
+    out6[3] = s.rho_rows;
+    out6[4] = s.vel_generic ? -1 : s.vel_rows;
+    out6[5] = s.vel_split;
     return PN_OK;
   });
 }

@@ -194,13 +194,17 @@ template <int Dim, int Order>
 __global__ void __launch_bounds__(256)
     k32_push_velocity_rows(Geom g, P32 p, const int32_t* __restrict__ cellstart,
                            const int* __restrict__ off, const float* __restrict__ E,
-                           const float* __restrict__ B, Push32 pc, int yrows)
+                           const float* __restrict__ B, Push32 pc, int yrows, int nsplit)
 {
   constexpr int is_odd = Order % 2;
   constexpr int S      = Order + 1;
   constexpr int T      = Order + 2;
   extern __shared__ float ft32[]; // [6][TZ][T][TXv]
-  const int c   = blockIdx.y; // chunk: every species' particles of the row share the tile
+  // nsplit = 1: one block serves every species of the chunk's row (the staged
+  // tile is loaded once); nsplit = Ns: one block per species
+  const int c     = blockIdx.y / nsplit;
+  const int sp_lo = nsplit == 1 ? 0 : blockIdx.y - c * nsplit;
+  const int sp_hi = nsplit == 1 ? g.Ns : sp_lo + 1;
   const int nbx = g.bsy;
   // yrows consecutive bin rows of one z plane per block (the last block of a
   // plane may hold fewer): they are contiguous in the sorted order and share
@@ -219,7 +223,7 @@ __global__ void __launch_bounds__(256)
   const int nr = min(yrows, nby - by);
   const int TY = T + nr - 1;
   bool any = false;
-  for (int sp = 0; sp < g.Ns; sp++) {
+  for (int sp = sp_lo; sp < sp_hi; sp++) {
     const int32_t* cs = cellstart + (long)(c * g.Ns + sp) * (g.nbin + 1);
     any               = any || cs[b0 + nr * nbx] > cs[b0];
   }
@@ -267,7 +271,7 @@ __global__ void __launch_bounds__(256)
     return rz * qmdt;
   };
 
-  for (int sp = 0; sp < g.Ns; sp++) {
+  for (int sp = sp_lo; sp < sp_hi; sp++) {
   const int      seg = c * g.Ns + sp;
   const int32_t* cs  = cellstart + (long)seg * (g.nbin + 1);
   const int      p0  = cs[b0], p1 = cs[b0 + nr * nbx];
@@ -1392,12 +1396,13 @@ void launch32_push_velocity(Sim& s, double delt)
   const size_t lds  = lds_of(yr);
   const bool   rows = s.sorted && !s.generic_kernels && !s.vel_generic && lds <= 65536;
   const int    nblk_rows = (g.nbin / g.bsy / nby) * ((nby + yr - 1) / yr);
+  const int    nsplit    = s.vel_split ? g.Ns : 1;
 #define PN32_VEL(D, O)                                                                        \
   do {                                                                                        \
     if (rows)                                                                                 \
-      hipLaunchKernelGGL((k32_push_velocity_rows<D, O>), dim3(nblk_rows, g.nchunk), dim3(256), \
-                         lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off, s.f_E, s.f_B, pc,  \
-                         yr);                                                                 \
+      hipLaunchKernelGGL((k32_push_velocity_rows<D, O>), dim3(nblk_rows, g.nchunk * nsplit), \
+                         dim3(256), lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off, s.f_E,   \
+                         s.f_B, pc, yr, nsplit);                                              \
     else                                                                                      \
       hipLaunchKernelGGL((k32_push_velocity<D, O>), dim3(nblk(g.cap, 256), g.nseg), dim3(256), \
                          0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc);            \

@@ -203,13 +203,17 @@ __global__ void __launch_bounds__(256)
     k_push_velocity_rows(Geom g, PArr p, const int32_t* __restrict__ cellstart,
                          const int* __restrict__ off, const double* __restrict__ lim,
                          const double* __restrict__ E, const double* __restrict__ B,
-                         PushConsts pc, int yrows)
+                         PushConsts pc, int yrows, int nsplit)
 {
   constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
   constexpr int S      = Order + 1;
   constexpr int T      = Order + 2;
   extern __shared__ double ft[]; // [6][TZ][T][TXv]
-  const int c   = blockIdx.y; // chunk: every species' particles of the row share the tile
+  // nsplit = 1: one block serves every species of the chunk's row (the staged
+  // tile is loaded once); nsplit = Ns: one block per species
+  const int c     = blockIdx.y / nsplit;
+  const int sp_lo = nsplit == 1 ? 0 : blockIdx.y - c * nsplit;
+  const int sp_hi = nsplit == 1 ? g.Ns : sp_lo + 1;
   const int nbx = g.bsy;
   // yrows consecutive bin rows of one z plane per block (the last block of a
   // plane may hold fewer): they are contiguous in the sorted order and share
@@ -228,7 +232,7 @@ __global__ void __launch_bounds__(256)
   const int nr = min(yrows, nby - by);
   const int TY = T + nr - 1;
   bool any = false;
-  for (int sp = 0; sp < g.Ns; sp++) {
+  for (int sp = sp_lo; sp < sp_hi; sp++) {
     const int32_t* cs = cellstart + (long)(c * g.Ns + sp) * (g.nbin + 1);
     any               = any || cs[b0 + nr * nbx] > cs[b0];
   }
@@ -302,7 +306,7 @@ __global__ void __launch_bounds__(256)
     return ry * qmdt;
   };
 
-  for (int sp = 0; sp < g.Ns; sp++) {
+  for (int sp = sp_lo; sp < sp_hi; sp++) {
   const int      seg = c * g.Ns + sp;
   const int32_t* cs  = cellstart + (long)seg * (g.nbin + 1);
   const int      p0  = cs[b0], p1 = cs[b0 + nr * nbx];
@@ -562,10 +566,12 @@ void launch_push_velocity(Sim& s, double delt)
   const bool   rows = g.dim >= 2 && s.sorted && !s.generic_kernels && !s.vel_generic &&
                     lds <= 65536;
   const int    nrow  = (g.nbin / g.bsy / nby) * ((nby + yr - 1) / yr);
+  const int    nsplit = s.vel_split ? g.Ns : 1;
   const bool   plain = pc.pusher == 0 && pc.wt == 0; // Boris + MC: no option branches
 #define PN_VEL_ROWS(D, O, PL)                                                                 \
-  hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL>), dim3(nrow, g.nchunk), dim3(256), lds, \
-                     s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc, yr)
+  hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL>), dim3(nrow, g.nchunk * nsplit), dim3(256), \
+                     lds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc, \
+                     yr, nsplit)
 #define PN_VEL(D, O)                                                                          \
   do {                                                                                        \
     if (rows && plain)                                                                        \

@@ -134,12 +134,13 @@ struct Sim {
   int         rho_rows    = 0; // bin rows per block of the rho march kernels
   int         vel_generic = 0; // 1: one thread per particle instead of the bin-row gather
   int         vel_rows    = 0; // bin rows of one z plane per gather block (0: heuristic)
+  int         vel_split   = 0; // 1: one gather block per species (0: all species)
   struct AutoTune {
     int                 state = 0; // 0 off, 1 measuring, 2 done
     int                 step  = 0; // tuning steps taken
     std::vector<int>    cand;      // candidate bin rows per march block
     std::vector<double> t_dep, t_rho;
-    std::vector<double> t_vel; // per gather shape (1, 2, 4 rows; per particle)
+    std::vector<double> t_vel; // per gather shape (kVelRows / kVelSplit)
     bool                tune_vel = false;
   } tune;
 

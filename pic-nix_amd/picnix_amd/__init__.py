@@ -310,10 +310,12 @@ class Simulation:
         self._chk(self.L.pn_autotune(self.h, 1 if enable else 0))
 
     def tuning(self):
-        """{state, steps_left, deposit_rows, rho_rows, gather_rows (-1: per particle)}."""
-        out = (ctypes.c_int * 5)()
+        """{state, steps_left, deposit_rows, rho_rows, gather_rows (-1: per particle),
+        gather_per_species}."""
+        out = (ctypes.c_int * 6)()
         self._chk(self.L.pn_get_tuning(self.h, out))
-        keys = ("state", "steps_left", "deposit_rows", "rho_rows", "gather_rows")
+        keys = ("state", "steps_left", "deposit_rows", "rho_rows", "gather_rows",
+                "gather_per_species")
         return dict(zip(keys, list(out)))
 
     def boundary_state(self, c):
