@@ -14,6 +14,14 @@
 #include "pn_sim.hpp"
 
 namespace picnix {
+
+int queue_blocks(Sim& s, const void* kernel, size_t lds, long nitems)
+{
+  int per = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, lds));
+  const long n = (long)std::max(per, 1) * std::max(s.ncu, 1);
+  return (int)std::max<long>(1, std::min(n, nitems));
+}
 std::vector<int> make_chunk_map(int Cz, int Cy, int Cx);
 void             shock_constants(Sim& s);
 }
@@ -178,6 +186,13 @@ void allocate(Sim& s)
   HIPCHK(hipMemset(s.d_B, 0, 3 * vol * sizeof(double)));
   HIPCHK(hipMemset(s.d_J, 0, 4 * vol * sizeof(double)));
 
+  s.d_wq = dalloc<int>(2 * kWqSlots);
+  HIPCHK(hipMemset(s.d_wq, 0, 2 * kWqSlots * sizeof(int)));
+  {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&s.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
   s.d_off = dalloc<int>(s.offset.size());
   s.d_lim = dalloc<double>(s.lim.size());
   s.d_rng = dalloc<double>(s.rng.size());
@@ -264,6 +279,7 @@ void release(Sim& s)
   f(s.d_ff);
   f(s.d_off);
   f(s.d_lim);
+  f(s.d_wq);
   f(s.d_rng);
   f(s.d_nb);
   for (PArr* p : {&s.pa, &s.pb}) {
@@ -484,6 +500,8 @@ int pn_create(const char* toml, int load, pn_sim** out)
       s.generic_kernels = std::string(k) == "generic";
     if (const char* k = std::getenv("PICNIX_MARCH_ROWS"))
       s.march_rows = std::max(0, std::atoi(k));
+    if (const char* k = std::getenv("PICNIX_WORK_QUEUE")) // persistent row kernels
+      s.dep_queue = s.vel_queue = std::atoi(k) != 0 ? 1 : 0;
     allocate(s);
     if (s.cfg.problem == 1) {
       shock_setup(s);
@@ -1202,6 +1220,11 @@ static int tune_cycle(const Sim& s)
   return std::max<int>((int)s.tune.cand.size(), s.tune.tune_vel ? kVelShapes : 2);
 }
 
+// steps of a session: two per candidate of the longest knob, then kQueueSteps
+// alternating the block-per-item and persistent work-queue launches
+static constexpr int kQueueSteps = 4;
+static int tune_steps(const Sim& s) { return 2 * tune_cycle(s) + kQueueSteps; }
+
 // rows > 0: that many bin rows per gather block; 0: one thread per particle;
 // -1: the built-in heuristic.  split: one block per species.
 static void set_vel_shape(Sim& s, int rows, int split = 0)
@@ -1217,6 +1240,7 @@ void tune_begin(Sim& s)
   t                = Sim::AutoTune{};
   const Geom& g    = s.g;
   s.dep_rows = s.rho_rows = 0;
+  s.dep_queue = s.vel_queue = 0;
   set_vel_shape(s, -1);
   if (g.dim == 3 && !s.generic_kernels) {
     const int nbz = g.nbin / g.bsz;
@@ -1240,7 +1264,8 @@ void tune_begin(Sim& s)
   drop_graphs(s);
 }
 
-static void tune_finish(Sim& s)
+// end of the shape phase: the fastest shape of every knob stays
+static void tune_pick_shapes(Sim& s)
 {
   Sim::AutoTune& t    = s.tune;
   // a knob keeps the heuristic unless at least two of its candidates ran (rho is
@@ -1261,19 +1286,23 @@ static void tune_finish(Sim& s)
   } else {
     set_vel_shape(s, -1);
   }
-  t.state       = 2;
-  drop_graphs(s);
 }
 
 void tune_step(Sim& s, double delt)
 {
-  Sim::AutoTune& t = s.tune;
-  const int      n = tune_cycle(s);
-  const int      c = t.step % n;
-  if (!t.cand.empty())
-    s.dep_rows = s.rho_rows = t.cand[c % t.cand.size()];
-  if (t.tune_vel)
-    set_vel_shape(s, kVelRows[c % kVelShapes], kVelSplit[c % kVelShapes]);
+  Sim::AutoTune& t     = s.tune;
+  const int      n     = tune_cycle(s);
+  const int      c     = t.step % n;
+  const bool     shape = t.step < 2 * n;
+  const int      qv    = (t.step - 2 * n) & 1; // queue phase: 0 block per item, 1 persistent
+  if (shape) {
+    if (!t.cand.empty())
+      s.dep_rows = s.rho_rows = t.cand[c % t.cand.size()];
+    if (t.tune_vel)
+      set_vel_shape(s, kVelRows[c % kVelShapes], kVelSplit[c % kVelShapes]);
+  } else {
+    s.dep_queue = s.vel_queue = qv;
+  }
   // time this step's phases alone; the caller's accumulators are set aside
   drain_timing(s);
   double saved_ms[PN_NPHASE + 1];
@@ -1296,18 +1325,31 @@ void tune_step(Sim& s, double delt)
     s.ph_count[i] = saved_n[i];
   }
   s.timing = saved_timing;
-  if (!t.cand.empty()) {
-    const size_t k = c % t.cand.size();
+  if (!shape) {
     if (dep >= 0)
-      t.t_dep[k] = std::min(t.t_dep[k], dep);
-    if (rho >= 0)
-      t.t_rho[k] = std::min(t.t_rho[k], rho);
+      t.t_q[0][qv] = std::min(t.t_q[0][qv], dep);
+    if (vel >= 0)
+      t.t_q[1][qv] = std::min(t.t_q[1][qv], vel);
+  } else {
+    if (!t.cand.empty()) {
+      const size_t k = c % t.cand.size();
+      if (dep >= 0)
+        t.t_dep[k] = std::min(t.t_dep[k], dep);
+      if (rho >= 0)
+        t.t_rho[k] = std::min(t.t_rho[k], rho);
+    }
+    if (t.tune_vel && vel >= 0)
+      t.t_vel[c % kVelShapes] = std::min(t.t_vel[c % kVelShapes], vel);
   }
-  if (t.tune_vel && vel >= 0)
-    t.t_vel[c % kVelShapes] = std::min(t.t_vel[c % kVelShapes], vel);
   t.step++;
-  if (t.step >= 2 * n)
-    tune_finish(s);
+  if (t.step == 2 * n)
+    tune_pick_shapes(s);
+  if (t.step >= tune_steps(s)) {
+    s.dep_queue = t.t_q[0][1] < t.t_q[0][0] ? 1 : 0;
+    s.vel_queue = t.t_q[1][1] < t.t_q[1][0] ? 1 : 0;
+    t.state     = 2;
+    drop_graphs(s);
+  }
 }
 
 } // namespace picnix
@@ -1322,6 +1364,7 @@ int pn_autotune(pn_sim* h, int enable)
     } else {
       s.tune     = Sim::AutoTune{};
       s.dep_rows = s.rho_rows = 0;
+      s.dep_queue = s.vel_queue = 0;
       set_vel_shape(s, -1);
       drop_graphs(s);
     }
@@ -1329,17 +1372,19 @@ int pn_autotune(pn_sim* h, int enable)
   });
 }
 
-int pn_get_tuning(pn_sim* h, int* out6)
+int pn_get_tuning(pn_sim* h, int* out8)
 {
   return guarded(h, [&](Sim& s) {
-    if (!out6)
+    if (!out8)
       return PN_ERR_ARG;
-    out6[0] = s.tune.state;
-    out6[1] = s.tune.state == 1 ? 2 * tune_cycle(s) - s.tune.step : 0;
-    out6[2] = s.dep_rows;
-    out6[3] = s.rho_rows;
-    out6[4] = s.vel_generic ? -1 : s.vel_rows;
-    out6[5] = s.vel_split;
+    out8[0] = s.tune.state;
+    out8[1] = s.tune.state == 1 ? tune_steps(s) - s.tune.step : 0;
+    out8[2] = s.dep_rows;
+    out8[3] = s.rho_rows;
+    out8[4] = s.vel_generic ? -1 : s.vel_rows;
+    out8[5] = s.vel_split;
+    out8[6] = s.dep_queue;
+    out8[7] = s.vel_queue;
     return PN_OK;
   });
 }

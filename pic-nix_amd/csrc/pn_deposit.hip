@@ -474,14 +474,14 @@ inline size_t march_lds(const Geom& g, int ntile)
   return sizeof(double) * (size_t)ntile + sizeof(int) * (size_t)g.Ns * (g.bsy + 1);
 }
 
-template <int Order>
+template <int Order, bool Queue>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(MarchPlan<Order>::WAVES, MarchPlan<Order>::WAVES)))
     k_deposit_march(Geom g, const PArr p, const double* __restrict__ xo,
                     const double* __restrict__ yo, const double* __restrict__ zo,
                     const int32_t* __restrict__ cellstart, const int* __restrict__ off,
                     const double* __restrict__ lim, double* __restrict__ J, DepConsts dc,
-                    int zrows)
+                    int zrows, int* __restrict__ wq, int gx, int nitems)
 {
   using MP             = MarchPlan<Order>;
   constexpr int N      = MP::N;
@@ -499,13 +499,26 @@ __global__ void __launch_bounds__(256)
   __shared__ int     next_cell;
   extern __shared__ double tile[]; // [3][N z-ring][N][TX] (Jx, Jy, Jz), then rcs
 
-  const int c   = blockIdx.y;
+  // Queue: persistent blocks take (row group, chunk) items from the work queue
+  // wq until it is drained; otherwise the item is the block's own index.
+  __shared__ int wq_item;
+  for (;;) {
+  int item_x = blockIdx.x, item_y = blockIdx.y;
+  if constexpr (Queue) {
+    const int it = wq_next(wq, &wq_item);
+    if (it >= nitems)
+      break; // uniform: every thread read the same item
+    item_y = it / gx;
+    item_x = it - item_y * gx;
+  }
+  do {
+  const int c   = item_y;
   const int ns  = g.Ns;
   const int nbx = g.bsy;
   const int nby = g.bsz / g.bsy;
   const int nbz = g.nbin / g.bsz;
-  const int by  = blockIdx.x % nby;
-  const int bz0 = (blockIdx.x / nby) * zrows;
+  const int by  = item_x % nby;
+  const int bz0 = (item_x / nby) * zrows;
   const int bz1 = min(nbz, bz0 + zrows);
 
   // nothing to do when every row of the range is empty for every species
@@ -518,7 +531,7 @@ __global__ void __launch_bounds__(256)
     any |= cs[b0 + nbx] > cs[b0] ? 1 : 0;
   }
   if (!__syncthreads_or(any))
-    return; // uniform for the block
+    break; // uniform for the block
 
   const int lb  = g.Nb;
   const int ylo = by - is_odd + lb - (Order / 2) - 1; // tile origin (local node index)
@@ -813,6 +826,13 @@ __global__ void __launch_bounds__(256)
     if (gi >= 0)
       unsafeAtomicAdd(&J[(kc + 1) * v + gi], val);
   }
+  } while (false);
+  if constexpr (!Queue)
+    return;
+  __syncthreads(); // the LDS of this item is read out before the next one
+  }
+  if constexpr (Queue)
+    wq_finish(wq);
 }
 
 // rho from the new positions (3D), after the sort: the reference's rho term
@@ -1226,9 +1246,20 @@ bool deposit_impl(Sim& s, const DepConsts& dc)
     const size_t mlds = march_lds(g, 3 * N * N * (g.bsy + N - 1));
     int zrows, nzr;
     march_split(g, s.dep_rows ? s.dep_rows : s.march_rows, zrows, nzr);
-    hipLaunchKernelGGL((k_deposit_march<Order>), dim3(nzr * (g.bsz / g.bsy), g.nchunk), dim3(256),
-                       mlds, s.stream, g, s.pa, s.d_xo[0], s.d_xo[1], s.d_xo[2], s.ss.cellstart,
-                       s.d_off, s.d_lim, s.d_J, dc, zrows);
+    const int gx     = nzr * (g.bsz / g.bsy);
+    const int nitems = gx * g.nchunk;
+    int*      wq     = s.d_wq + 2 * kWqDeposit;
+    if (s.dep_queue) {
+      const int nb = queue_blocks(s, reinterpret_cast<const void*>(&k_deposit_march<Order, true>),
+                                  mlds, nitems);
+      hipLaunchKernelGGL((k_deposit_march<Order, true>), dim3(nb), dim3(256), mlds, s.stream, g,
+                         s.pa, s.d_xo[0], s.d_xo[1], s.d_xo[2], s.ss.cellstart, s.d_off,
+                         s.d_lim, s.d_J, dc, zrows, wq, gx, nitems);
+    } else {
+      hipLaunchKernelGGL((k_deposit_march<Order, false>), dim3(gx, g.nchunk), dim3(256), mlds,
+                         s.stream, g, s.pa, s.d_xo[0], s.d_xo[1], s.d_xo[2], s.ss.cellstart,
+                         s.d_off, s.d_lim, s.d_J, dc, zrows, wq, gx, nitems);
+    }
     return true;
   }
   if (Dim >= 2 && s.sorted && lds <= 65536 && !s.generic_kernels) {

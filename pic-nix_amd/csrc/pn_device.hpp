@@ -372,6 +372,27 @@ __device__ __forceinline__ int bin_of(const Geom& g, const double* __restrict__ 
   return iz * g.bsz + iy * g.bsy + ix;
 }
 
+// Work queue of the persistent row kernels: wq[0] is the next item, wq[1]
+// counts the blocks that have drained the queue.  Thread 0 takes an item with a
+// global atomic and hands it to the block through LDS; the last block to drain
+// the queue resets both counters, so they are zero again for the next launch
+// (replays of a captured graph included) without a separate reset.
+__device__ __forceinline__ int wq_next(int* wq, int* item)
+{
+  if (threadIdx.x == 0)
+    *item = atomicAdd(&wq[0], 1);
+  __syncthreads();
+  return *item;
+}
+
+__device__ __forceinline__ void wq_finish(int* wq)
+{
+  if (threadIdx.x == 0 && atomicAdd(&wq[1], 1) == (int)(gridDim.x * gridDim.y) - 1) {
+    atomicExch(&wq[0], 0);
+    atomicExch(&wq[1], 0);
+  }
+}
+
 // Exchange direction slot (0..26, 13 = stays) of XtensorHaloParticle3D::pre_pack
 // (nix/xtensor_halo3d.hpp:230-244).
 __device__ __forceinline__ int dir_of(const double* __restrict__ rng, int c, double x, double y,

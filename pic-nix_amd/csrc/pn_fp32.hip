@@ -190,20 +190,34 @@ __device__ __forceinline__ void int_w(const Push32& pc, int axis, float r, int& 
 // One block per (bin row, segment) of sorted particles: the row's E and B
 // stencil region is staged in LDS, as in the FP64 row kernel; one thread per
 // particle, (Order+1)^Dim-point gathers with x innermost.
-template <int Dim, int Order>
+template <int Dim, int Order, bool Queue>
 __global__ void __launch_bounds__(256)
     k32_push_velocity_rows(Geom g, P32 p, const int32_t* __restrict__ cellstart,
                            const int* __restrict__ off, const float* __restrict__ E,
-                           const float* __restrict__ B, Push32 pc, int yrows, int nsplit)
+                           const float* __restrict__ B, Push32 pc, int yrows, int nsplit,
+                           int* __restrict__ wq, int gx, int nitems)
 {
   constexpr int is_odd = Order % 2;
   constexpr int S      = Order + 1;
   constexpr int T      = Order + 2;
   extern __shared__ float ft32[]; // [6][TZ][T][TXv]
+  // Queue: persistent blocks take (row group, chunk) items from the work queue
+  // wq until it is drained; otherwise the item is the block's own index.
+  __shared__ int wq_item;
+  for (;;) {
+  int item_x = blockIdx.x, item_y = blockIdx.y;
+  if constexpr (Queue) {
+    const int it = wq_next(wq, &wq_item);
+    if (it >= nitems)
+      break; // uniform: every thread read the same item
+    item_y = it / gx;
+    item_x = it - item_y * gx;
+  }
+  do {
   // nsplit = 1: one block serves every species of the chunk's row (the staged
   // tile is loaded once); nsplit = Ns: one block per species
-  const int c     = blockIdx.y / nsplit;
-  const int sp_lo = nsplit == 1 ? 0 : blockIdx.y - c * nsplit;
+  const int c     = item_y / nsplit;
+  const int sp_lo = nsplit == 1 ? 0 : item_y - c * nsplit;
   const int sp_hi = nsplit == 1 ? g.Ns : sp_lo + 1;
   const int nbx = g.bsy;
   // yrows consecutive bin rows of one z plane per block (the last block of a
@@ -213,11 +227,11 @@ __global__ void __launch_bounds__(256)
   const int nyb = (nby + yrows - 1) / yrows;
   int       bz = 0, by, b0;
   if constexpr (Dim == 3) {
-    bz = blockIdx.x / nyb;
-    by = (blockIdx.x - bz * nyb) * yrows;
+    bz = item_x / nyb;
+    by = (item_x - bz * nyb) * yrows;
     b0 = bz * g.bsz + by * g.bsy;
   } else {
-    by = blockIdx.x * yrows;
+    by = item_x * yrows;
     b0 = by * g.bsy;
   }
   const int nr = min(yrows, nby - by);
@@ -228,7 +242,7 @@ __global__ void __launch_bounds__(256)
     any               = any || cs[b0 + nr * nbx] > cs[b0];
   }
   if (!any)
-    return;
+    break; // uniform for the block
   const int  lb  = g.Nb;
   const int  TZ  = Dim == 3 ? T : 1;
   const int  tz0 = Dim == 3 ? bz - is_odd + lb - (Order / 2) : lb;
@@ -321,6 +335,13 @@ __global__ void __launch_bounds__(256)
     p.uz[i] = uz;
   }
   } // species
+  } while (false);
+  if constexpr (!Queue)
+    return;
+  __syncthreads(); // the LDS of this item is read out before the next one
+  }
+  if constexpr (Queue)
+    wq_finish(wq);
 }
 
 
@@ -781,13 +802,14 @@ struct March32 {
   static constexpr int WAVES = Order == 2 ? 5 : 2;
 };
 
-template <int Order>
+template <int Order, bool Queue>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(March32<Order>::WAVES, March32<Order>::WAVES)))
     k32_deposit_march(Geom g, const P32 p, const float* __restrict__ ro0,
                       const float* __restrict__ ro1, const float* __restrict__ ro2,
                       const int32_t* __restrict__ cellstart, const int* __restrict__ off,
-                      float* __restrict__ J, Dep32 dc, int zrows)
+                      float* __restrict__ J, Dep32 dc, int zrows,
+                      int* __restrict__ wq, int gx, int nitems)
 {
   using MP            = March32<Order>;
   constexpr int N     = MP::N;
@@ -804,13 +826,26 @@ __global__ void __launch_bounds__(256)
   __shared__ int    next_cell;
   extern __shared__ double tile[]; // [3][N z-ring][N][TX] FP64, then rcs
 
-  const int c   = blockIdx.y;
+  // Queue: persistent blocks take (row group, chunk) items from the work queue
+  // wq until it is drained; otherwise the item is the block's own index.
+  __shared__ int wq_item;
+  for (;;) {
+  int item_x = blockIdx.x, item_y = blockIdx.y;
+  if constexpr (Queue) {
+    const int it = wq_next(wq, &wq_item);
+    if (it >= nitems)
+      break; // uniform: every thread read the same item
+    item_y = it / gx;
+    item_x = it - item_y * gx;
+  }
+  do {
+  const int c   = item_y;
   const int ns  = g.Ns;
   const int nbx = g.bsy;
   const int nby = g.bsz / g.bsy;
   const int nbz = g.nbin / g.bsz;
-  const int by  = blockIdx.x % nby;
-  const int bz0 = (blockIdx.x / nby) * zrows;
+  const int by  = item_x % nby;
+  const int bz0 = (item_x / nby) * zrows;
   const int bz1 = min(nbz, bz0 + zrows);
 
   int any = 0;
@@ -822,7 +857,7 @@ __global__ void __launch_bounds__(256)
     any |= cs[b0 + nbx] > cs[b0] ? 1 : 0;
   }
   if (!__syncthreads_or(any))
-    return;
+    break; // uniform for the block
 
   const int lb  = g.Nb;
   const int ylo = by + lb - (Order / 2) - 1; // tile origin (local node index)
@@ -1033,6 +1068,13 @@ __global__ void __launch_bounds__(256)
         (unsigned)gx < (unsigned)g.P[2])
       atomicAdd(&J[(kc + 1) * v + ((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
   }
+  } while (false);
+  if constexpr (!Queue)
+    return;
+  __syncthreads(); // the LDS of this item is read out before the next one
+  }
+  if constexpr (Queue)
+    wq_finish(wq);
 }
 
 // rho from the current (sorted or not) positions, one thread per particle:
@@ -1400,9 +1442,22 @@ void launch32_push_velocity(Sim& s, double delt)
 #define PN32_VEL(D, O)                                                                        \
   do {                                                                                        \
     if (rows)                                                                                 \
-      hipLaunchKernelGGL((k32_push_velocity_rows<D, O>), dim3(nblk_rows, g.nchunk * nsplit), \
-                         dim3(256), lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off, s.f_E,   \
-                         s.f_B, pc, yr, nsplit);                                              \
+    {                                                                                         \
+      const int gy = g.nchunk * nsplit, nitems = nblk_rows * gy;                              \
+      int*      wq = s.d_wq + 2 * kWqGather32;                                                \
+      if (s.vel_queue)                                                                        \
+        hipLaunchKernelGGL((k32_push_velocity_rows<D, O, true>),                              \
+                           dim3(queue_blocks(                                                 \
+                               s, reinterpret_cast<const void*>(                              \
+                                      &k32_push_velocity_rows<D, O, true>),                   \
+                               lds, nitems)),                                                 \
+                           dim3(256), lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off,        \
+                           s.f_E, s.f_B, pc, yr, nsplit, wq, nblk_rows, nitems);              \
+      else                                                                                    \
+        hipLaunchKernelGGL((k32_push_velocity_rows<D, O, false>), dim3(nblk_rows, gy),        \
+                           dim3(256), lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off,        \
+                           s.f_E, s.f_B, pc, yr, nsplit, wq, nblk_rows, nitems);              \
+    }                                                                                         \
     else                                                                                      \
       hipLaunchKernelGGL((k32_push_velocity<D, O>), dim3(nblk(g.cap, 256), g.nseg), dim3(256), \
                          0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc);            \
@@ -1485,19 +1540,32 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
                         sizeof(int) * (size_t)g.Ns * (g.bsy + 1);
     int zrows, nzr;
     march32_split(g, s.dep_rows ? s.dep_rows : s.march_rows, zrows, nzr);
-    const dim3 mgrid(nzr * (g.bsz / g.bsy), g.nchunk);
+    const int  gx     = nzr * (g.bsz / g.bsy);
+    const int  nitems = gx * g.nchunk;
+    int*       wq     = s.d_wq + 2 * kWqDeposit32;
+#define PN32_MARCH(O)                                                                          \
+  do {                                                                                         \
+    if (s.dep_queue) {                                                                         \
+      const int nb = queue_blocks(                                                             \
+          s, reinterpret_cast<const void*>(&k32_deposit_march<O, true>), mlds, nitems);        \
+      hipLaunchKernelGGL((k32_deposit_march<O, true>), dim3(nb), dim3(256), mlds, s.stream, g, \
+                         s.qa, s.f_ro[0], s.f_ro[1], s.f_ro[2], s.ss.cellstart, s.d_off, s.f_J, \
+                         dc, zrows, wq, gx, nitems);                                           \
+    } else {                                                                                   \
+      hipLaunchKernelGGL((k32_deposit_march<O, false>), dim3(gx, g.nchunk), dim3(256), mlds,   \
+                         s.stream, g, s.qa, s.f_ro[0], s.f_ro[1], s.f_ro[2], s.ss.cellstart,    \
+                         s.d_off, s.f_J, dc, zrows, wq, gx, nitems);                           \
+    }                                                                                          \
+  } while (0)
     if (g.dim == 3 && s.sorted && !s.generic_kernels && g.order == 2 &&
         mlds + March32<2>::WLDS + 64 <= 65536) {
-      hipLaunchKernelGGL(k32_deposit_march<2>, mgrid, dim3(256), mlds, s.stream, g, s.qa,
-                         s.f_ro[0], s.f_ro[1], s.f_ro[2], s.ss.cellstart, s.d_off, s.f_J, dc,
-                         zrows);
+      PN32_MARCH(2);
       march = true;
     } else if (g.dim == 3 && s.sorted && !s.generic_kernels && g.order == 4 &&
                mlds + March32<4>::WLDS + 64 <= 65536) {
-      hipLaunchKernelGGL(k32_deposit_march<4>, mgrid, dim3(256), mlds, s.stream, g, s.qa,
-                         s.f_ro[0], s.f_ro[1], s.f_ro[2], s.ss.cellstart, s.d_off, s.f_J, dc,
-                         zrows);
+      PN32_MARCH(4);
       march = true;
+#undef PN32_MARCH
     } else {
       const dim3 grid(nblk(g.cap, 256), g.nseg);
 #define PN32_DEP(D, O)                                                                         \

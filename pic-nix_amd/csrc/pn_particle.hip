@@ -198,21 +198,35 @@ __global__ void __launch_bounds__(256)
 // summation order), reading LDS instead of global memory.  A particle whose
 // stencil leaves the staged region (a rounding edge of the bin test) reads
 // global memory.
-template <int Dim, int Order, bool Plain>
+template <int Dim, int Order, bool Plain, bool Queue>
 __global__ void __launch_bounds__(256)
     k_push_velocity_rows(Geom g, PArr p, const int32_t* __restrict__ cellstart,
                          const int* __restrict__ off, const double* __restrict__ lim,
                          const double* __restrict__ E, const double* __restrict__ B,
-                         PushConsts pc, int yrows, int nsplit)
+                         PushConsts pc, int yrows, int nsplit,
+                         int* __restrict__ wq, int gx, int nitems)
 {
   constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
   constexpr int S      = Order + 1;
   constexpr int T      = Order + 2;
   extern __shared__ double ft[]; // [6][TZ][T][TXv]
+  // Queue: persistent blocks take (row group, chunk) items from the work queue
+  // wq until it is drained; otherwise the item is the block's own index.
+  __shared__ int wq_item;
+  for (;;) {
+  int item_x = blockIdx.x, item_y = blockIdx.y;
+  if constexpr (Queue) {
+    const int it = wq_next(wq, &wq_item);
+    if (it >= nitems)
+      break; // uniform: every thread read the same item
+    item_y = it / gx;
+    item_x = it - item_y * gx;
+  }
+  do {
   // nsplit = 1: one block serves every species of the chunk's row (the staged
   // tile is loaded once); nsplit = Ns: one block per species
-  const int c     = blockIdx.y / nsplit;
-  const int sp_lo = nsplit == 1 ? 0 : blockIdx.y - c * nsplit;
+  const int c     = item_y / nsplit;
+  const int sp_lo = nsplit == 1 ? 0 : item_y - c * nsplit;
   const int sp_hi = nsplit == 1 ? g.Ns : sp_lo + 1;
   const int nbx = g.bsy;
   // yrows consecutive bin rows of one z plane per block (the last block of a
@@ -222,11 +236,11 @@ __global__ void __launch_bounds__(256)
   const int nyb = (nby + yrows - 1) / yrows;
   int       bz = 0, by, b0;
   if constexpr (Dim == 3) {
-    bz = blockIdx.x / nyb;
-    by = (blockIdx.x - bz * nyb) * yrows;
+    bz = item_x / nyb;
+    by = (item_x - bz * nyb) * yrows;
     b0 = bz * g.bsz + by * g.bsy;
   } else {
-    by = blockIdx.x * yrows;
+    by = item_x * yrows;
     b0 = by * g.bsy;
   }
   const int nr = min(yrows, nby - by);
@@ -237,7 +251,7 @@ __global__ void __launch_bounds__(256)
     any               = any || cs[b0 + nr * nbx] > cs[b0];
   }
   if (!any)
-    return;
+    break; // uniform for the block
   const int  lb  = g.Nb;
   const int  TZ  = Dim == 3 ? T : 1;
   const int  tz0 = Dim == 3 ? bz - is_odd + lb - (Order / 2) : lb;
@@ -407,6 +421,13 @@ __global__ void __launch_bounds__(256)
     p.uz[i] = xu[5];
   }
   } // species
+  } while (false);
+  if constexpr (!Queue)
+    return;
+  __syncthreads(); // the LDS of this item is read out before the next one
+  }
+  if constexpr (Queue)
+    wq_finish(wq);
 }
 
 // BasePosition::push (pic/engine/position.hpp:117-130) followed by count(reset)
@@ -568,10 +589,22 @@ void launch_push_velocity(Sim& s, double delt)
   const int    nrow  = (g.nbin / g.bsy / nby) * ((nby + yr - 1) / yr);
   const int    nsplit = s.vel_split ? g.Ns : 1;
   const bool   plain = pc.pusher == 0 && pc.wt == 0; // Boris + MC: no option branches
+  const int    vgy    = g.nchunk * nsplit, vitems = nrow * vgy;
+  int*         vwq    = s.d_wq + 2 * kWqGather;
 #define PN_VEL_ROWS(D, O, PL)                                                                 \
-  hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL>), dim3(nrow, g.nchunk * nsplit), dim3(256), \
-                     lds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B, pc, \
-                     yr, nsplit)
+  do {                                                                                        \
+    if (s.vel_queue)                                                                          \
+      hipLaunchKernelGGL(                                                                     \
+          (k_push_velocity_rows<D, O, PL, true>),                                             \
+          dim3(queue_blocks(s, reinterpret_cast<const void*>(&k_push_velocity_rows<D, O, PL, true>), \
+                            lds, vitems)),                                                    \
+          dim3(256), lds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B,  \
+          pc, yr, nsplit, vwq, nrow, vitems);                                                 \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL, false>), dim3(nrow, vgy), dim3(256), \
+                         lds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E,     \
+                         s.d_B, pc, yr, nsplit, vwq, nrow, vitems);                           \
+  } while (0)
 #define PN_VEL(D, O)                                                                          \
   do {                                                                                        \
     if (rows && plain)                                                                        \

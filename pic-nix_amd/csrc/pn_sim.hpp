@@ -135,12 +135,17 @@ struct Sim {
   int         vel_generic = 0; // 1: one thread per particle instead of the bin-row gather
   int         vel_rows    = 0; // bin rows of one z plane per gather block (0: heuristic)
   int         vel_split   = 0; // 1: one gather block per species (0: all species)
+  int         dep_queue   = 0; // 1: the march deposit runs persistent over a work queue
+  int         vel_queue   = 0; // 1: the row gather runs persistent over a work queue
+  int*        d_wq        = nullptr; // work-queue counters, 2 per launch site (kWq*)
+  int         ncu         = 0;       // compute units of the device
   struct AutoTune {
     int                 state = 0; // 0 off, 1 measuring, 2 done
     int                 step  = 0; // tuning steps taken
     std::vector<int>    cand;      // candidate bin rows per march block
     std::vector<double> t_dep, t_rho;
     std::vector<double> t_vel; // per gather shape (kVelRows / kVelSplit)
+    double              t_q[2][2] = {{1e300, 1e300}, {1e300, 1e300}}; // [deposit, gather][queue]
     bool                tune_vel = false;
   } tune;
 
@@ -191,6 +196,10 @@ void launch_push_efd(Sim& s, double delt);
 void launch_fill_ghost(Sim& s, double* f, int ncomp);
 void launch_fill_ghost(Sim& s, float* f, int ncomp);
 // zero fills and copies as kernels (no memset/memcpy nodes under graph capture)
+// work-queue slots of the persistent row kernels (Sim::d_wq, 2 ints each)
+enum { kWqDeposit = 0, kWqDeposit32 = 1, kWqGather = 2, kWqGather32 = 3, kWqSlots = 8 };
+// persistent blocks for a queue of nitems: resident blocks per CU x CUs, at most nitems
+int queue_blocks(Sim& s, const void* kernel, size_t lds, long nitems);
 void launch_zero(Sim& s, double* p, long n);
 void launch_zero(Sim& s, float* p, long n);
 void launch_zero(Sim& s, int32_t* p, long n);

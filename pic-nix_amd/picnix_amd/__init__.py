@@ -311,11 +311,11 @@ class Simulation:
 
     def tuning(self):
         """{state, steps_left, deposit_rows, rho_rows, gather_rows (-1: per particle),
-        gather_per_species}."""
-        out = (ctypes.c_int * 6)()
+        gather_per_species, deposit_queue, gather_queue}."""
+        out = (ctypes.c_int * 8)()
         self._chk(self.L.pn_get_tuning(self.h, out))
         keys = ("state", "steps_left", "deposit_rows", "rho_rows", "gather_rows",
-                "gather_per_species")
+                "gather_per_species", "deposit_queue", "gather_queue")
         return dict(zip(keys, list(out)))
 
     def boundary_state(self, c):

@@ -156,6 +156,31 @@ def test_step_parity_march_rows(rows, monkeypatch):
         o.close()
 
 
+@pytest.mark.parametrize("kind", ["thermal3d_o2", "thermal3d_o3", "twostream2d_o1"])
+def test_step_parity_work_queue(kind, monkeypatch):
+    """The march deposit and the row gather as persistent blocks walking the
+    device work queue (PICNIX_WORK_QUEUE=1): same steps as one block per item.
+    Ten steps also exercise the queue counters' reset between launches."""
+    monkeypatch.setenv("PICNIX_WORK_QUEUE", "1")
+    dim, make = CASES[kind]
+    cfg = make()
+    dt = float(cfg["parameter"]["delt"])
+    _say(f"[parity] work queue {kind}: create")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    for k in range(10):
+        o.step()
+        g.step(dt)
+    g.synchronize()
+    perr = _compare_particles(o, g, exact=False)
+    rel = _compare_fields(o, g, dim)
+    _say(f"[parity] work queue {kind}: step 10 particle rel err {perr:.3e}, fields {rel}")
+    assert perr <= TOL
+    assert max(rel.values()) <= TOL
+    g.close()
+    o.close()
+
+
 def _with_fields(cfg, E=(0.0, 0.0, 0.0), B=(0.0, 0.0, 0.0)):
     p = cfg["parameter"]
     p.update(Ex=E[0], Ey=E[1], Ez=E[2], Bx=B[0], By=B[1], Bz=B[2])
