@@ -129,8 +129,9 @@ int pn_synchronize(pn_sim* sim);
 
 /* Autotuner of the particle kernels' launch shapes (the bin rows per block of the
  * current and rho march kernels, the bin rows and species per gather block or
- * one thread per particle, and whether the march deposit and the row gather run
- * one block per work item or persistent blocks walking a device work queue).
+ * one thread per particle, whether the march deposit and the row gather run one
+ * block per work item or persistent blocks walking a device work queue, and
+ * whether the FP32 row gather also runs the position push).
  * No reference counterpart (the reference picks its loop forms by the
  * [application.option] vectorization table, pic/pic_chunk.cpp:140-189).
  * enable = 1 starts a session: the next pn_get_tuning()[1] pn_step calls run
@@ -138,13 +139,14 @@ int pn_synchronize(pn_sim* sim);
  * of every kernel stays for the rest of the run (step graphs are re-captured).
  * The tuning steps are ordinary steps of the run: every candidate computes the
  * same step, up to the order of the current sums.  enable = 0 returns to the
- * built-in heuristics.  pn_get_tuning: out8 = {state (0 off, 1 measuring,
+ * built-in heuristics.  pn_get_tuning: out9 = {state (0 off, 1 measuring,
  * 2 done), steps left, bin rows of the current march (0 = heuristic), bin rows of
  * the rho march, bin rows per gather block (0 = heuristic, -1 = one thread per
  * particle), 1 if a gather block serves one species (0: all species), 1 if the
- * march deposit runs persistent over the work queue, the same for the gather}. */
+ * march deposit runs persistent over the work queue, the same for the gather, 1
+ * if the FP32 row gather also runs the position push}. */
 int pn_autotune(pn_sim* sim, int enable);
-int pn_get_tuning(pn_sim* sim, int* out8);
+int pn_get_tuning(pn_sim* sim, int* out9);
 
 /* Upstream boundary state of one chunk for [application] problem = 'shock' (the
  * values MainChunk keeps in option["random_seed"] and option["boundary"],

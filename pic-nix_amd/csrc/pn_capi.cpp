@@ -502,6 +502,8 @@ int pn_create(const char* toml, int load, pn_sim** out)
       s.march_rows = std::max(0, std::atoi(k));
     if (const char* k = std::getenv("PICNIX_WORK_QUEUE")) // persistent row kernels
       s.dep_queue = s.vel_queue = std::atoi(k) != 0 ? 1 : 0;
+    if (const char* k = std::getenv("PICNIX_FUSED_PUSH")) // FP32 gather + position push
+      s.fuse_push = std::atoi(k) != 0 ? 1 : 0;
     allocate(s);
     if (s.cfg.problem == 1) {
       shock_setup(s);
@@ -933,9 +935,9 @@ namespace {
 void step_launches32(Sim& s, double delt)
 {
   launch_push_bfd(s, 0.5 * delt);
-  launch32_push_velocity(s, delt);
   bool was_sorted = s.sorted;
-  launch32_push_position(s, delt);
+  if (!launch32_push_velocity(s, delt, s.fuse_push != 0))
+    launch32_push_position(s, delt);
   s.sorted = was_sorted;
   const RhoDone rho = launch32_deposit(s, delt, s.rho_valid ? RhoMode::Keep : RhoMode::Defer);
   launch_push_bfd(s, 0.5 * delt);
@@ -961,15 +963,14 @@ void step_launches32(Sim& s, double delt)
 void step_launches_wall(Sim& s, double delt)
 {
   launch_push_bfd(s, 0.5 * delt);
-  if (s.fp32)
-    launch32_push_velocity(s, delt);
-  else
-    launch_push_velocity(s, delt);
   bool was_sorted = s.sorted;
-  if (s.fp32)
-    launch32_push_position(s, delt);
-  else
+  if (s.fp32) {
+    if (!launch32_push_velocity(s, delt, s.fuse_push != 0))
+      launch32_push_position(s, delt);
+  } else {
+    launch_push_velocity(s, delt);
     launch_push_position(s, delt);
+  }
   shock_position_boundary(s);
   s.sorted = was_sorted;
   const RhoDone rho =
@@ -1221,8 +1222,9 @@ static int tune_cycle(const Sim& s)
 }
 
 // steps of a session: two per candidate of the longest knob, then kQueueSteps
-// alternating the block-per-item and persistent work-queue launches
-static constexpr int kQueueSteps = 4;
+// over the block-per-item and persistent work-queue launches, with and without
+// the FP32 gather running the position push, each combination twice
+static constexpr int kQueueSteps = 8;
 static int tune_steps(const Sim& s) { return 2 * tune_cycle(s) + kQueueSteps; }
 
 // rows > 0: that many bin rows per gather block; 0: one thread per particle;
@@ -1240,7 +1242,7 @@ void tune_begin(Sim& s)
   t                = Sim::AutoTune{};
   const Geom& g    = s.g;
   s.dep_rows = s.rho_rows = 0;
-  s.dep_queue = s.vel_queue = 0;
+  s.dep_queue = s.vel_queue = s.fuse_push = 0;
   set_vel_shape(s, -1);
   if (g.dim == 3 && !s.generic_kernels) {
     const int nbz = g.nbin / g.bsz;
@@ -1294,7 +1296,8 @@ void tune_step(Sim& s, double delt)
   const int      n     = tune_cycle(s);
   const int      c     = t.step % n;
   const bool     shape = t.step < 2 * n;
-  const int      qv    = (t.step - 2 * n) & 1; // queue phase: 0 block per item, 1 persistent
+  const int      qv    = (t.step - 2 * n) & 1;        // 0 block per item, 1 persistent
+  const int      fv    = ((t.step - 2 * n) >> 1) & 1; // 1: fused gather + position push
   if (shape) {
     if (!t.cand.empty())
       s.dep_rows = s.rho_rows = t.cand[c % t.cand.size()];
@@ -1302,6 +1305,7 @@ void tune_step(Sim& s, double delt)
       set_vel_shape(s, kVelRows[c % kVelShapes], kVelSplit[c % kVelShapes]);
   } else {
     s.dep_queue = s.vel_queue = qv;
+    s.fuse_push               = fv;
   }
   // time this step's phases alone; the caller's accumulators are set aside
   drain_timing(s);
@@ -1319,7 +1323,8 @@ void tune_step(Sim& s, double delt)
   drain_timing(s);
   const double dep = s.ph_count[3] ? s.ph_ms[3] : -1.0;
   const double rho = s.ph_count[kPhaseRhoDeposit] ? s.ph_ms[kPhaseRhoDeposit] : -1.0;
-  const double vel = s.ph_count[1] ? s.ph_ms[1] : -1.0;
+  // gather + position push: the FP32 row gather may run the position push too
+  const double vel = s.ph_count[1] ? s.ph_ms[1] + s.ph_ms[2] : -1.0;
   for (int i = 0; i <= PN_NPHASE; i++) {
     s.ph_ms[i]    = saved_ms[i];
     s.ph_count[i] = saved_n[i];
@@ -1329,7 +1334,7 @@ void tune_step(Sim& s, double delt)
     if (dep >= 0)
       t.t_q[0][qv] = std::min(t.t_q[0][qv], dep);
     if (vel >= 0)
-      t.t_q[1][qv] = std::min(t.t_q[1][qv], vel);
+      t.t_v[2 * fv + qv] = std::min(t.t_v[2 * fv + qv], vel);
   } else {
     if (!t.cand.empty()) {
       const size_t k = c % t.cand.size();
@@ -1345,8 +1350,10 @@ void tune_step(Sim& s, double delt)
   if (t.step == 2 * n)
     tune_pick_shapes(s);
   if (t.step >= tune_steps(s)) {
-    s.dep_queue = t.t_q[0][1] < t.t_q[0][0] ? 1 : 0;
-    s.vel_queue = t.t_q[1][1] < t.t_q[1][0] ? 1 : 0;
+    s.dep_queue  = t.t_q[0][1] < t.t_q[0][0] ? 1 : 0;
+    const int kv = (int)(std::min_element(t.t_v, t.t_v + 4) - t.t_v);
+    s.vel_queue  = kv & 1;
+    s.fuse_push  = kv >> 1;
     t.state     = 2;
     drop_graphs(s);
   }
@@ -1364,7 +1371,7 @@ int pn_autotune(pn_sim* h, int enable)
     } else {
       s.tune     = Sim::AutoTune{};
       s.dep_rows = s.rho_rows = 0;
-      s.dep_queue = s.vel_queue = 0;
+      s.dep_queue = s.vel_queue = s.fuse_push = 0;
       set_vel_shape(s, -1);
       drop_graphs(s);
     }
@@ -1372,19 +1379,20 @@ int pn_autotune(pn_sim* h, int enable)
   });
 }
 
-int pn_get_tuning(pn_sim* h, int* out8)
+int pn_get_tuning(pn_sim* h, int* out9)
 {
   return guarded(h, [&](Sim& s) {
-    if (!out8)
+    if (!out9)
       return PN_ERR_ARG;
-    out8[0] = s.tune.state;
-    out8[1] = s.tune.state == 1 ? tune_steps(s) - s.tune.step : 0;
-    out8[2] = s.dep_rows;
-    out8[3] = s.rho_rows;
-    out8[4] = s.vel_generic ? -1 : s.vel_rows;
-    out8[5] = s.vel_split;
-    out8[6] = s.dep_queue;
-    out8[7] = s.vel_queue;
+    out9[0] = s.tune.state;
+    out9[1] = s.tune.state == 1 ? tune_steps(s) - s.tune.step : 0;
+    out9[2] = s.dep_rows;
+    out9[3] = s.rho_rows;
+    out9[4] = s.vel_generic ? -1 : s.vel_rows;
+    out9[5] = s.vel_split;
+    out9[6] = s.dep_queue;
+    out9[7] = s.vel_queue;
+    out9[8] = s.fuse_push;
     return PN_OK;
   });
 }

@@ -187,15 +187,134 @@ __device__ __forceinline__ void int_w(const Push32& pc, int axis, float r, int& 
     shape_mc<Order, float>(d, 0.0f, 1.0f, w);
 }
 
+// Destination of a particle of segment seg whose cell-relative position is r in
+// its old cell `bin` (see k32_push_position): segment ds, bin b, key k; ds < 0
+// when it is dropped.
+__device__ __forceinline__ void dest32(const Geom& g, int seg, int bin, float rx, float ry, float rz,
+                                       const int* __restrict__ nb, int& ds, int& b, int& k)
+{
+  ds = -1;
+  b  = -1;
+  k  = -1;
+  if (bin < 0)
+    return;
+  const Cell3 cl = cell_of(g, bin);
+  int         cx = cl.x + (g.has[2] ? (int)floorf(rx) : 0);
+  int         cy = cl.y + (g.has[1] ? (int)floorf(ry) : 0);
+  int         cz = cl.z + (g.has[0] ? (int)floorf(rz) : 0);
+  const int   dx = (cx >= g.dims[2]) - (cx < 0) + 1;
+  const int   dy = (cy >= g.dims[1]) - (cy < 0) + 1;
+  const int   dz = (cz >= g.dims[0]) - (cz < 0) + 1;
+  const int   d  = 9 * dz + 3 * dy + dx;
+  int         R  = seg / g.Ns;
+  if (d != 13) {
+    R = nb[R * 27 + d];
+    cx += (cx < 0) * g.dims[2] - (cx >= g.dims[2]) * g.dims[2];
+    cy += (cy < 0) * g.dims[1] - (cy >= g.dims[1]) * g.dims[1];
+    cz += (cz < 0) * g.dims[0] - (cz >= g.dims[0]) * g.dims[0];
+  }
+  if (R >= 0 && cx >= 0 && cx < g.dims[2] && cy >= 0 && cy < g.dims[1] && cz >= 0 &&
+      cz < g.dims[0]) {
+    ds = R * g.Ns + (seg - (seg / g.Ns) * g.Ns);
+    b  = bin_of3(g, cx, cy, cz);
+    k  = d == 13 ? b : -2 - (d * g.nbin + b);
+  }
+}
+
+// Arguments of the FP32 position push (k32_push_position, and the row gather
+// when it runs the position push of each particle it has pushed).
+struct Pos32 {
+  float*         ro0;
+  float*         ro1;
+  float*         ro2;
+  float          delt, rc, rdx, rdy, rdz;
+  const int*     nb;
+  const int*     off;
+  int32_t*       cellcnt;
+  int32_t*       key;
+  int32_t*       slot;
+  int            on; // the row gather: 1 runs the position push after the momentum push
+};
+
+// r_old -> ro; r += u dt / (gamma d) (unwrapped, relative to the old cell).
+// Then the particle's destination for the sort: its own segment and new bin, or
+// for a leaver the neighbour chunk in its direction (chunk table nb) and the bin
+// there -- the periodic wrap is a shift of the cell index, so r stays as it is
+// (set_boundary_periodic, nix/xtensor_particle.hpp:359-376, and the migration,
+// nix/xtensor_halo3d.hpp:192-499).  Every particle takes a slot in its
+// destination bin here, so the sort needs no migration pass.  key: the new bin
+// (>= 0) for a particle that stays in its chunk, -2 - (d * nbin + bin) for one
+// that moves to the neighbour in direction d, -1 for one that is dropped.
+
+// r_old -> ro; r += u dt / (gamma d) (unwrapped, relative to the old cell).
+// Then the particle's destination for the sort: its own segment and new bin, or
+// for a leaver the neighbour chunk in its direction (chunk table nb) and the bin
+// there -- the periodic wrap is a shift of the cell index, so r stays as it is
+// (set_boundary_periodic, nix/xtensor_particle.hpp:359-376, and the migration,
+// nix/xtensor_halo3d.hpp:192-499).  Every particle takes a slot in its
+// destination bin here, so the sort needs no migration pass.  key: the new bin
+// (>= 0) for a particle that stays in its chunk, -2 - (d * nbin + bin) for one
+// that moves to the neighbour in direction d, -1 for one that is dropped.
+// With a non-periodic x (the shock problem) a particle that crossed x = 0 is
+// reflected (MainChunk::set_boundary_particle, example/shock/main.cpp:402-416)
+// and one past x = Lx is marked kKeyReplace for its redraw.
+// One particle i of segment seg with position r in its old cell `bin` and the
+// momentum u of this step; WriteU: u is written here (else only when reflected).
+// Called by every active lane of the wave that holds a particle (wave_slot).
+template <bool WriteU>
+__device__ __forceinline__ void position32(const Geom& g, const P32& p, const Pos32& ps, long i,
+                                           int seg, float rx, float ry, float rz, float ux,
+                                           float uy, float uz, int bin)
+{
+  ps.ro0[i]      = rx;
+  ps.ro1[i]      = ry;
+  ps.ro2[i]      = rz;
+  const float gm = lorentz_factor(ux, uy, uz, ps.rc);
+  const float dt = ps.delt / gm;
+  rx += g.has[2] ? ux * dt * ps.rdx : ux * dt;
+  ry += g.has[1] ? uy * dt * ps.rdy : uy * dt;
+  rz += g.has[0] ? uz * dt * ps.rdz : uz * dt;
+  bool replace = false;
+  bool flip    = false;
+  if (g.xwall && bin >= 0) {
+    // global x cell of the old cell; |dx| < 1 cell, so only the end cells can cross
+    const int gx = ps.off[(seg / g.Ns) * 3 + 2] + cell_of(g, bin).x;
+    if (gx == 0 && rx < 0.0f) {
+      rx   = -rx;
+      flip = true;
+    } else if (gx == g.N[2] - 1 && rx >= 1.0f) {
+      replace = true;
+    }
+  }
+  if (WriteU || flip) {
+    p.ux[i] = flip ? -ux : ux;
+    p.uy[i] = flip ? -uy : uy;
+    p.uz[i] = flip ? -uz : uz;
+  }
+  p.rx[i] = rx;
+  p.ry[i] = ry;
+  p.rz[i] = rz;
+  int ds = -1, b = -1, k = -1;
+  if (replace)
+    k = kKeyReplace;
+  else
+    dest32(g, seg, bin, rx, ry, rz, ps.nb, ds, b, k);
+  const bool want = ds >= 0;
+  const int  sl   = wave_slot(ps.cellcnt, want ? ds * g.nbin + b : 0, want);
+  ps.key[i]       = k;
+  if (want)
+    ps.slot[i] = sl;
+}
+
 // One block per (bin row, segment) of sorted particles: the row's E and B
 // stencil region is staged in LDS, as in the FP64 row kernel; one thread per
 // particle, (Order+1)^Dim-point gathers with x innermost.
-template <int Dim, int Order, bool Queue>
+template <int Dim, int Order, bool Queue, bool Fuse>
 __global__ void __launch_bounds__(256)
     k32_push_velocity_rows(Geom g, P32 p, const int32_t* __restrict__ cellstart,
                            const int* __restrict__ off, const float* __restrict__ E,
-                           const float* __restrict__ B, Push32 pc, int yrows, int nsplit,
-                           int* __restrict__ wq, int gx, int nitems)
+                           const float* __restrict__ B, Push32 pc, Pos32 ps, int yrows,
+                           int nsplit, int* __restrict__ wq, int gx, int nitems)
 {
   constexpr int is_odd = Order % 2;
   constexpr int S      = Order + 1;
@@ -292,7 +411,8 @@ __global__ void __launch_bounds__(256)
   qmdt               = pc.qmdt[sp];
   for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
     const long  i    = (long)seg * g.cap + q;
-    const Cell3 cl   = cell_of(g, p.cell[i]);
+    const int   bin  = p.cell[i];
+    const Cell3 cl   = cell_of(g, bin);
     const float r[3] = {p.rx[i], p.ry[i], p.rz[i]};
     float       wix[S], wiy[S], wiz[S], whx[S], why[S], whz[S];
     int         ix0, iy0, iz0 = 0, hx0, hy0, hz0 = 0;
@@ -317,22 +437,28 @@ __global__ void __launch_bounds__(256)
     // sorted particles sit in the row's bins, so the stencils are inside the tile
     const bool in = min(X, HX) >= 0 && max(X, HX) + S <= TXv && min(Y, HY) >= 0 &&
                     max(Y, HY) + S <= TY && min(Z, HZ) >= 0 && max(Z, HZ) + (Dim == 3 ? S : 1) <= TZ;
-    if (!in)
-      continue;
-    const long bsz = (long)TY * TXv, bsy = TXv;
-    const int  nz  = Dim == 3 ? S : 1;
-    float      eb[6];
-    eb[0] = interp(ft32 + 0 * pc1, bsz, bsy, Z, Y, HX, wiz, wiy, whx, nz);
-    eb[1] = interp(ft32 + 1 * pc1, bsz, bsy, Z, HY, X, wiz, why, wix, nz);
-    eb[2] = interp(ft32 + 2 * pc1, bsz, bsy, HZ, Y, X, whz, wiy, wix, nz);
-    eb[3] = interp(ft32 + 3 * pc1, bsz, bsy, HZ, HY, X, whz, why, wix, nz);
-    eb[4] = interp(ft32 + 4 * pc1, bsz, bsy, HZ, Y, HX, whz, wiy, whx, nz);
-    eb[5] = interp(ft32 + 5 * pc1, bsz, bsy, Z, HY, HX, wiz, why, whx, nz);
     float ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
-    push_momentum<float>(pc.pusher, ux, uy, uz, eb, (float)g.cc);
-    p.ux[i] = ux;
-    p.uy[i] = uy;
-    p.uz[i] = uz;
+    if (in) {
+      const long bsz = (long)TY * TXv, bsy = TXv;
+      const int  nz  = Dim == 3 ? S : 1;
+      float      eb[6];
+      eb[0] = interp(ft32 + 0 * pc1, bsz, bsy, Z, Y, HX, wiz, wiy, whx, nz);
+      eb[1] = interp(ft32 + 1 * pc1, bsz, bsy, Z, HY, X, wiz, why, wix, nz);
+      eb[2] = interp(ft32 + 2 * pc1, bsz, bsy, HZ, Y, X, whz, wiy, wix, nz);
+      eb[3] = interp(ft32 + 3 * pc1, bsz, bsy, HZ, HY, X, whz, why, wix, nz);
+      eb[4] = interp(ft32 + 4 * pc1, bsz, bsy, HZ, Y, HX, whz, wiy, whx, nz);
+      eb[5] = interp(ft32 + 5 * pc1, bsz, bsy, Z, HY, HX, wiz, why, whx, nz);
+      push_momentum<float>(pc.pusher, ux, uy, uz, eb, (float)g.cc);
+    }
+    if constexpr (Fuse) {
+      // the position push of this particle with its new momentum (the step's
+      // next operator), so r and u are read once for both
+      position32<true>(g, p, ps, i, seg, r[0], r[1], r[2], ux, uy, uz, bin);
+    } else if (in) {
+      p.ux[i] = ux;
+      p.uy[i] = uy;
+      p.uz[i] = uz;
+    }
   }
   } // species
   } while (false);
@@ -426,113 +552,16 @@ __global__ void __launch_bounds__(256)
 // nix/xtensor_particle.hpp:324-357) in FP32
 // ---------------------------------------------------------------------------
 
-// r_old -> ro; r += u dt / (gamma d) (unwrapped, relative to the old cell).
-// Then the particle's destination for the sort: its own segment and new bin, or
-// for a leaver the neighbour chunk in its direction (chunk table nb) and the bin
-// there -- the periodic wrap is a shift of the cell index, so r stays as it is
-// (set_boundary_periodic, nix/xtensor_particle.hpp:359-376, and the migration,
-// nix/xtensor_halo3d.hpp:192-499).  Every particle takes a slot in its
-// destination bin here, so the sort needs no migration pass.  key: the new bin
-// (>= 0) for a particle that stays in its chunk, -2 - (d * nbin + bin) for one
-// that moves to the neighbour in direction d, -1 for one that is dropped.
-// Destination of a particle of segment seg whose cell-relative position is r in
-// its old cell `bin` (see k32_push_position): segment ds, bin b, key k; ds < 0
-// when it is dropped.
-__device__ __forceinline__ void dest32(const Geom& g, int seg, int bin, float rx, float ry, float rz,
-                                       const int* __restrict__ nb, int& ds, int& b, int& k)
-{
-  ds = -1;
-  b  = -1;
-  k  = -1;
-  if (bin < 0)
-    return;
-  const Cell3 cl = cell_of(g, bin);
-  int         cx = cl.x + (g.has[2] ? (int)floorf(rx) : 0);
-  int         cy = cl.y + (g.has[1] ? (int)floorf(ry) : 0);
-  int         cz = cl.z + (g.has[0] ? (int)floorf(rz) : 0);
-  const int   dx = (cx >= g.dims[2]) - (cx < 0) + 1;
-  const int   dy = (cy >= g.dims[1]) - (cy < 0) + 1;
-  const int   dz = (cz >= g.dims[0]) - (cz < 0) + 1;
-  const int   d  = 9 * dz + 3 * dy + dx;
-  int         R  = seg / g.Ns;
-  if (d != 13) {
-    R = nb[R * 27 + d];
-    cx += (cx < 0) * g.dims[2] - (cx >= g.dims[2]) * g.dims[2];
-    cy += (cy < 0) * g.dims[1] - (cy >= g.dims[1]) * g.dims[1];
-    cz += (cz < 0) * g.dims[0] - (cz >= g.dims[0]) * g.dims[0];
-  }
-  if (R >= 0 && cx >= 0 && cx < g.dims[2] && cy >= 0 && cy < g.dims[1] && cz >= 0 &&
-      cz < g.dims[0]) {
-    ds = R * g.Ns + (seg - (seg / g.Ns) * g.Ns);
-    b  = bin_of3(g, cx, cy, cz);
-    k  = d == 13 ? b : -2 - (d * g.nbin + b);
-  }
-}
-
-// r_old -> ro; r += u dt / (gamma d) (unwrapped, relative to the old cell).
-// Then the particle's destination for the sort: its own segment and new bin, or
-// for a leaver the neighbour chunk in its direction (chunk table nb) and the bin
-// there -- the periodic wrap is a shift of the cell index, so r stays as it is
-// (set_boundary_periodic, nix/xtensor_particle.hpp:359-376, and the migration,
-// nix/xtensor_halo3d.hpp:192-499).  Every particle takes a slot in its
-// destination bin here, so the sort needs no migration pass.  key: the new bin
-// (>= 0) for a particle that stays in its chunk, -2 - (d * nbin + bin) for one
-// that moves to the neighbour in direction d, -1 for one that is dropped.
-// With a non-periodic x (the shock problem) a particle that crossed x = 0 is
-// reflected (MainChunk::set_boundary_particle, example/shock/main.cpp:402-416)
-// and one past x = Lx is marked kKeyReplace for its redraw.
 __global__ void __launch_bounds__(256)
-    k32_push_position(Geom g, P32 p, float* __restrict__ ro0, float* __restrict__ ro1,
-                      float* __restrict__ ro2, const int32_t* __restrict__ np, float delt,
-                      float rc, float rdx, float rdy, float rdz, const int* __restrict__ nb,
-                      const int* __restrict__ off, int32_t* __restrict__ cellcnt,
-                      int32_t* __restrict__ key, int32_t* __restrict__ slot)
+    k32_push_position(Geom g, P32 p, Pos32 ps, const int32_t* __restrict__ np)
 {
-  const int  seg    = blockIdx.y;
-  const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = ip < np[seg];
-  const long i      = (long)seg * g.cap + ip;
-  int        ds = -1, b = -1, k = -1;
-  if (active) {
-    float       rx = p.rx[i], ry = p.ry[i], rz = p.rz[i];
-    const float ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
-    ro0[i]         = rx;
-    ro1[i]         = ry;
-    ro2[i]         = rz;
-    const float gm = lorentz_factor(ux, uy, uz, rc);
-    const float dt = delt / gm;
-    rx += g.has[2] ? ux * dt * rdx : ux * dt;
-    ry += g.has[1] ? uy * dt * rdy : uy * dt;
-    rz += g.has[0] ? uz * dt * rdz : uz * dt;
-    const int bin = p.cell[i];
-    bool      replace = false;
-    if (g.xwall && bin >= 0) {
-      // global x cell of the old cell; |dx| < 1 cell, so only the end cells can cross
-      const int gx = off[(seg / g.Ns) * 3 + 2] + cell_of(g, bin).x;
-      if (gx == 0 && rx < 0.0f) {
-        rx      = -rx;
-        p.ux[i] = -ux;
-        p.uy[i] = -uy;
-        p.uz[i] = -uz;
-      } else if (gx == g.N[2] - 1 && rx >= 1.0f) {
-        replace = true;
-      }
-    }
-    p.rx[i] = rx;
-    p.ry[i] = ry;
-    p.rz[i] = rz;
-    if (replace)
-      k = kKeyReplace;
-    else
-      dest32(g, seg, bin, rx, ry, rz, nb, ds, b, k);
-  }
-  const bool want = ds >= 0;
-  const int  sl   = wave_slot(cellcnt, want ? ds * g.nbin + b : 0, want);
-  if (!active)
+  const int  seg = blockIdx.y;
+  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
     return;
-  key[i] = k;
-  if (want)
-    slot[i] = sl;
+  const long i = (long)seg * g.cap + ip;
+  position32<false>(g, p, ps, i, seg, p.rx[i], p.ry[i], p.rz[i], p.ux[i], p.uy[i], p.uz[i],
+                    p.cell[i]);
 }
 
 // The shock problem's replacements in the FP32 form: x, ux, uy, uz of every
@@ -1414,7 +1443,30 @@ void launch32_fields_from_f64(Sim& s, int which)
   }
 }
 
-void launch32_push_velocity(Sim& s, double delt)
+static Pos32 pos32_args(Sim& s, double delt, int on)
+{
+  const Geom& g = s.g;
+  Pos32       ps{};
+  ps.ro0     = s.f_ro[0];
+  ps.ro1     = s.f_ro[1];
+  ps.ro2     = s.f_ro[2];
+  ps.delt    = (float)delt;
+  ps.rc      = (float)(1 / g.cc);
+  ps.rdx     = (float)(1 / g.del[2]);
+  ps.rdy     = (float)(1 / g.del[1]);
+  ps.rdz     = (float)(1 / g.del[0]);
+  ps.nb      = s.d_nb;
+  ps.off     = s.d_off;
+  ps.cellcnt = s.ss.cellcnt;
+  ps.key     = s.ss.key;
+  ps.slot    = s.ss.slot;
+  ps.on      = on;
+  return ps;
+}
+
+// with_position: when the row gather runs, it also runs each particle's
+// position push (k32_push_position's work); returns true if it did.
+bool launch32_push_velocity(Sim& s, double delt, bool with_position)
 {
   PhaseTimer  pt(s, 1);
   const Geom& g = s.g;
@@ -1439,28 +1491,35 @@ void launch32_push_velocity(Sim& s, double delt)
   const bool   rows = s.sorted && !s.generic_kernels && !s.vel_generic && lds <= 65536;
   const int    nblk_rows = (g.nbin / g.bsy / nby) * ((nby + yr - 1) / yr);
   const int    nsplit    = s.vel_split ? g.Ns : 1;
+  const bool   fused     = rows && with_position;
+  const Pos32  ps        = pos32_args(s, delt, fused ? 1 : 0);
+  if (fused)
+    launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
+#define PN32_VEL_ROWS(D, O, Q, F)                                                             \
+  hipLaunchKernelGGL((k32_push_velocity_rows<D, O, Q, F>),                                    \
+                     Q ? dim3(queue_blocks(                                                   \
+                             s, reinterpret_cast<const void*>(&k32_push_velocity_rows<D, O, Q, F>), \
+                             lds, nitems))                                                    \
+                       : dim3(nblk_rows, gy),                                                 \
+                     dim3(256), lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off, s.f_E, s.f_B, \
+                     pc, ps, yr, nsplit, wq, nblk_rows, nitems)
 #define PN32_VEL(D, O)                                                                        \
   do {                                                                                        \
-    if (rows)                                                                                 \
-    {                                                                                         \
+    if (rows) {                                                                               \
       const int gy = g.nchunk * nsplit, nitems = nblk_rows * gy;                              \
       int*      wq = s.d_wq + 2 * kWqGather32;                                                \
-      if (s.vel_queue)                                                                        \
-        hipLaunchKernelGGL((k32_push_velocity_rows<D, O, true>),                              \
-                           dim3(queue_blocks(                                                 \
-                               s, reinterpret_cast<const void*>(                              \
-                                      &k32_push_velocity_rows<D, O, true>),                   \
-                               lds, nitems)),                                                 \
-                           dim3(256), lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off,        \
-                           s.f_E, s.f_B, pc, yr, nsplit, wq, nblk_rows, nitems);              \
+      if (s.vel_queue && fused)                                                               \
+        PN32_VEL_ROWS(D, O, true, true);                                                      \
+      else if (s.vel_queue)                                                                   \
+        PN32_VEL_ROWS(D, O, true, false);                                                     \
+      else if (fused)                                                                         \
+        PN32_VEL_ROWS(D, O, false, true);                                                     \
       else                                                                                    \
-        hipLaunchKernelGGL((k32_push_velocity_rows<D, O, false>), dim3(nblk_rows, gy),        \
-                           dim3(256), lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off,        \
-                           s.f_E, s.f_B, pc, yr, nsplit, wq, nblk_rows, nitems);              \
-    }                                                                                         \
-    else                                                                                      \
+        PN32_VEL_ROWS(D, O, false, false);                                                    \
+    } else {                                                                                  \
       hipLaunchKernelGGL((k32_push_velocity<D, O>), dim3(nblk(g.cap, 256), g.nseg), dim3(256), \
                          0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc);            \
+    }                                                                                         \
   } while (0)
   if (g.dim == 3) {
     switch (g.order) {
@@ -1478,6 +1537,12 @@ void launch32_push_velocity(Sim& s, double delt)
     }
   }
 #undef PN32_VEL
+#undef PN32_VEL_ROWS
+  if (fused) {
+    s.counted = true;
+    s.sorted  = false;
+  }
+  return fused;
 }
 
 void launch32_push_position(Sim& s, double delt)
@@ -1486,9 +1551,7 @@ void launch32_push_position(Sim& s, double delt)
   const Geom& g = s.g;
   launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
   hipLaunchKernelGGL(k32_push_position, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa,
-                     s.f_ro[0], s.f_ro[1], s.f_ro[2], s.d_np, (float)delt, (float)(1 / g.cc),
-                     (float)(1 / g.del[2]), (float)(1 / g.del[1]), (float)(1 / g.del[0]),
-                     s.d_nb, s.d_off, s.ss.cellcnt, s.ss.key, s.ss.slot);
+                     pos32_args(s, delt, 0), s.d_np);
   s.counted = true;
   s.sorted  = false;
 }

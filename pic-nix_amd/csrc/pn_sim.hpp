@@ -137,6 +137,7 @@ struct Sim {
   int         vel_split   = 0; // 1: one gather block per species (0: all species)
   int         dep_queue   = 0; // 1: the march deposit runs persistent over a work queue
   int         vel_queue   = 0; // 1: the row gather runs persistent over a work queue
+  int         fuse_push   = 0; // FP32 step: the row gather also runs the position push
   int*        d_wq        = nullptr; // work-queue counters, 2 per launch site (kWq*)
   int         ncu         = 0;       // compute units of the device
   struct AutoTune {
@@ -145,7 +146,8 @@ struct Sim {
     std::vector<int>    cand;      // candidate bin rows per march block
     std::vector<double> t_dep, t_rho;
     std::vector<double> t_vel; // per gather shape (kVelRows / kVelSplit)
-    double              t_q[2][2] = {{1e300, 1e300}, {1e300, 1e300}}; // [deposit, gather][queue]
+    double              t_q[2][2] = {{1e300, 1e300}, {1e300, 1e300}}; // [deposit][queue]
+    double              t_v[4] = {1e300, 1e300, 1e300, 1e300}; // gather [2 fused + queue]
     bool                tune_vel = false;
   } tune;
 
@@ -256,7 +258,7 @@ void launch32_from_f64(Sim& s, int seg0, int nseg);        // pa (FP64 SoA) -> q
 void launch32_to_f64(Sim& s, int seg0, int nseg, bool old); // qa -> pa (and f_ro -> d_xo)
 void launch32_fields_to_f64(Sim& s, int which);            // f_* -> d_* mirror
 void launch32_fields_from_f64(Sim& s, int which);          // d_* mirror -> f_*
-void launch32_push_velocity(Sim& s, double delt);
+bool launch32_push_velocity(Sim& s, double delt, bool with_position = false);
 void launch32_push_position(Sim& s, double delt);
 RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode);
 void launch32_rho(Sim& s);

@@ -311,11 +311,11 @@ class Simulation:
 
     def tuning(self):
         """{state, steps_left, deposit_rows, rho_rows, gather_rows (-1: per particle),
-        gather_per_species, deposit_queue, gather_queue}."""
-        out = (ctypes.c_int * 8)()
+        gather_per_species, deposit_queue, gather_queue, fused_push}."""
+        out = (ctypes.c_int * 9)()
         self._chk(self.L.pn_get_tuning(self.h, out))
         keys = ("state", "steps_left", "deposit_rows", "rho_rows", "gather_rows",
-                "gather_per_species", "deposit_queue", "gather_queue")
+                "gather_per_species", "deposit_queue", "gather_queue", "fused_push")
         return dict(zip(keys, list(out)))
 
     def boundary_state(self, c):

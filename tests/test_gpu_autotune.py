@@ -32,6 +32,7 @@ def test_autotune_session_matches_untuned_run(precision):
         assert t1["gather_rows"] in (-1, 1, 2, 4)
         assert t1["gather_per_species"] in (0, 1)
         assert t1["deposit_queue"] in (0, 1) and t1["gather_queue"] in (0, 1)
+        assert t1["fused_push"] in (0, 1)
         tol = 1e-12 if precision == "fp64" else 1e-4
         for c in range(a.nchunks):
             info = a.chunk_info(c)

@@ -246,15 +246,19 @@ def test_step_parity_options(kind):
     o.close()
 
 
+@pytest.mark.parametrize("queue", ["0", "1"])
 @pytest.mark.parametrize("kind", ["thermal3d_o2", "twostream2d_o1"])
-def test_step_graph_replays(kind):
+def test_step_graph_replays(kind, queue, monkeypatch):
     """pn_step replayed from captured hipGraphs (one per particle-buffer parity),
     with eager work between the replays (energy reductions, a particle read-back,
-    an eager operator step), checked against the oracle after every replay."""
+    an eager operator step), checked against the oracle after every replay.
+    queue = 1: the graphs hold the persistent work-queue kernels, whose counters
+    must be back at zero for every replay."""
+    monkeypatch.setenv("PICNIX_WORK_QUEUE", queue)
     dim, make = CASES[kind]
     cfg = make()
     dt = float(cfg["parameter"]["delt"])
-    _say(f"[parity] graph {kind}: create")
+    _say(f"[parity] graph {kind} queue {queue}: create")
     o = oracle_lib.Oracle(cfg)
     g = _sim(cfg)
     g.step_graph(True)
