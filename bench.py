@@ -172,16 +172,18 @@ def main():
     dt = float(par["delt"])
     ncell = int(par["Nx"]) * int(par["Ny"]) * int(par["Nz"])
 
-    def run_form(precision):
+    def run_form(precision, batched=True):
         """Warm-up, then K timed steps of one precision form; returns the wall
-        time, the particle counts, the per-phase report and the final history."""
+        time, the particle counts, the per-phase report and the final history.
+        batched=False: the plain form, with the built-in launch shapes (no
+        autotuning session, one block per work item, eager steps)."""
         text = toml_text
         if precision == "fp32":
             text = text.replace("[application.option]",
                                 "[application.option]\n    precision = 'fp32'", 1)
         sim = picnix_amd.Simulation(text)
         tuning = None
-        if args.autotune == "on":
+        if args.autotune == "on" and batched:
             sim.autotune(True)
             nt = sim.tuning()["steps_left"]
             sim.step(dt, nt)
@@ -189,7 +191,7 @@ def main():
         npart = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
         # the FP32 batched form replays the step as a captured hipGraph; the
         # per-phase HIP events then come from eager steps after the timed ones
-        graph = precision == "fp32" and args.fp32_graph == "on"
+        graph = precision == "fp32" and args.fp32_graph == "on" and batched
         sim.step_graph(graph)
         sim.step(dt, args.warmup)
         sim.synchronize()
@@ -209,13 +211,21 @@ def main():
         rep = sim.timing_report()
         sim.timing(False)
         sim.close()
-        run_form.tuning[precision] = tuning
+        if batched:
+            run_form.tuning[precision] = tuning
         return t1 - t0, npart, npart_end, rep, hist
     run_form.tuning = {}
 
     elapsed, npart, npart_end, rep, hist64 = run_form("fp64")
     fp32 = None
+    plain = None
     if args.fp32 == "on":
+        # the plain FP64 path beside the batched forms: built-in shapes, eager steps
+        ep, np_p, np_p_end, _, _ = run_form("fp64", batched=False)
+        plain = {"value": 0.5 * (np_p + np_p_end) * args.steps / ep,
+                 "ms_per_step": ep / args.steps * 1e3,
+                 "form": "FP64, built-in launch shapes, no autotuning, no work queue, "
+                         "eager steps"}
         e32, n32, n32_end, rep32, hist32 = run_form("fp32")
         # error of the FP32 form against the FP64 form after the same steps
         names = ["div(E)-rho", "div(B)", "E^2/2", "B^2/2"] + \
@@ -225,6 +235,7 @@ def main():
         err["div(E)-rho (abs)"] = abs(hist32[0] - hist64[0])
         fp32 = {"value": n32 * args.steps / e32, "ms_per_step": e32 / args.steps * 1e3,
                 "speedup_vs_fp64": (n32 * args.steps / e32) / (npart * args.steps / elapsed),
+                "speedup_vs_fp64_plain": (n32 * args.steps / e32) / plain["value"],
                 "particles_conserved": n32_end == n32, "dtype": "f32",
                 "step_graph": args.fp32_graph == "on",
                 "rel_error_vs_fp64_after_steps": args.warmup + args.steps,
@@ -276,7 +287,7 @@ def main():
         "data": "synthetic (seeded loader of the reference's %s example)" % (
             "shock" if args.config.startswith("shock") else "thermal"),
         "config": {"workload": WORKLOADS[args.config], "config": args.config,
-                   "particles": npart, "cells": ncell, "precision": "fp64 plain",
+                   "particles": npart, "cells": ncell, "precision": "fp64",
                    "particles_conserved": npart_end == npart},
         "roofline": roofline,
         "step_roofline": {"bytes_per_step": step_bytes(npart, ncell), "achieved": step_gbs,
@@ -284,6 +295,7 @@ def main():
         "phases": phases,
         "tuning": run_form.tuning,
         "fp32_form": fp32,
+        "fp64_plain": plain,
     }
     if args.cpu_baseline == "on":
         line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_threads, args.cpu_steps)
