@@ -79,17 +79,23 @@ def profile_traffic(config, kernel):
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}.txt")
     if not os.path.exists(path):
         return None
-    total, seen = 0.0, 0
+    # per counter section, the launch form of `kernel` with the most dispatches
+    # (an autotuning session also runs the forms it rejects)
+    best = {}
+    section = None
     with open(path) as f:
         for line in f:
             parts = line.split()
-            if len(parts) >= 4 and parts[0].startswith(kernel):
+            if parts and parts[0] in ("FETCH_SIZE", "WRITE_SIZE"):
+                section = parts[0]
+            elif section and len(parts) >= 4 and parts[0].startswith(kernel):
                 try:
-                    total += float(parts[-1]) * 1e6  # calibrated MB per dispatch
-                    seen += 1
+                    n, mb = int(parts[-3]), float(parts[-1])
                 except ValueError:
-                    pass
-    return total if seen == 2 else None
+                    continue
+                if section not in best or n > best[section][0]:
+                    best[section] = (n, mb * 1e6)  # calibrated MB per dispatch
+    return sum(v[1] for v in best.values()) if len(best) == 2 else None
 
 
 def step_bytes(npart, ncell):
