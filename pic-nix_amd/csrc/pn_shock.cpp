@@ -143,12 +143,20 @@ int shock_npc(const ShockConfig& c, double x, double delh)
   return (int)std::floor(c.nppc * shock_density(c, x, delh) + 0.5);
 }
 
+// f(k, ids[k]) for every k on up to 16 host threads; threads = false runs them in
+// order on the calling thread (small amounts of work, where starting the threads
+// costs more than the work while the GPU waits)
 template <typename F>
-void parallel_chunks(const std::vector<int>& ids, F f)
+void parallel_chunks(const std::vector<int>& ids, F f, bool threads = true)
 {
   const unsigned n    = (unsigned)ids.size();
   unsigned       nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   nthr                = std::min(nthr, std::max(1u, n));
+  if (!threads) {
+    for (unsigned k = 0; k < n; k++)
+      f(k, ids[k]);
+    return;
+  }
   std::vector<std::thread> pool;
   for (unsigned t = 0; t < nthr; t++)
     pool.emplace_back([&, t] {
@@ -433,7 +441,10 @@ void shock_position_boundary(Sim& s)
                         s.stream));
   HIPCHK(hipStreamSynchronize(s.stream));
   std::vector<std::vector<double>> vals(nk);
-  const std::vector<int>           up = upper_chunks(s);
+  const std::vector<int>           up    = upper_chunks(s);
+  long                             total = 0;
+  for (int32_t n : cnt)
+    total += n;
   parallel_chunks(up, [&](unsigned u, int id) {
     ChunkRng rng;
     rng.reset(h.random_seed[id]);
@@ -453,7 +464,7 @@ void shock_position_boundary(Sim& s)
       }
       h.efflux[2 * id + sp] = cnt[k];
     }
-  });
+  }, total >= 16384); // the redraws of a step are usually few
   upload_vals(s, vals, nullptr);
   if (s.fp32)
     launch32_wall_apply(s);
