@@ -261,6 +261,9 @@ void allocate(Sim& s)
   ss.errflag      = dalloc<int32_t>(4);
   HIPCHK(hipMemset(ss.errflag, 0, 4 * sizeof(int32_t)));
   HIPCHK(hipMemset(ss.cellstart, 0, (size_t)g.nseg * (g.nbin + 1) * sizeof(int32_t)));
+  HIPCHK(hipHostMalloc((void**)&s.h_flag, 4 * sizeof(int32_t), hipHostMallocDefault));
+  std::memset(s.h_flag, 0, 4 * sizeof(int32_t));
+  HIPCHK(hipEventCreateWithFlags(&s.flag_ev, hipEventDisableTiming));
 }
 
 void release(Sim& s)
@@ -316,6 +319,10 @@ void release(Sim& s)
                   (void*)ss.nout, (void*)ss.recvoff, (void*)ss.npnew, (void*)ss.errflag})
     f(p);
   f(s.d_stage);
+  if (s.h_flag)
+    (void)hipHostFree(s.h_flag);
+  if (s.flag_ev)
+    (void)hipEventDestroy(s.flag_ev);
   shock_release(s);
   if (s.stream)
     (void)hipStreamDestroy(s.stream);
@@ -504,6 +511,8 @@ int pn_create(const char* toml, int load, pn_sim** out)
       s.dep_queue = s.vel_queue = std::atoi(k) != 0 ? 1 : 0;
     if (const char* k = std::getenv("PICNIX_FUSED_PUSH")) // FP32 gather + position push
       s.fuse_push = std::atoi(k) != 0 ? 1 : 0;
+    if (const char* k = std::getenv("PICNIX_RHO")) // "deposit": rho from particles every step
+      s.rho_deposit = std::string(k) == "deposit";
     allocate(s);
     if (s.cfg.problem == 1) {
       shock_setup(s);
@@ -932,6 +941,10 @@ namespace {
 // (launch_rho_continuity) instead of depositing it again; the first step after
 // anything else moved particles or wrote J deposits it (after the sort, where
 // the new positions are grouped by cell).
+// whether this step carries rho by continuity (PICNIX_RHO=deposit: never; every
+// step then deposits rho from the particles, as the reference does)
+bool carry_rho(const Sim& s) { return s.rho_valid && !s.rho_deposit; }
+
 void step_launches32(Sim& s, double delt)
 {
   launch_push_bfd(s, 0.5 * delt);
@@ -939,7 +952,7 @@ void step_launches32(Sim& s, double delt)
   if (!launch32_push_velocity(s, delt, s.fuse_push != 0))
     launch32_push_position(s, delt);
   s.sorted = was_sorted;
-  const RhoDone rho = launch32_deposit(s, delt, s.rho_valid ? RhoMode::Keep : RhoMode::Defer);
+  const RhoDone rho = launch32_deposit(s, delt, carry_rho(s) ? RhoMode::Keep : RhoMode::Defer);
   launch_push_bfd(s, 0.5 * delt);
   launch_fold_current(s, 1, 4);
   if (rho == RhoDone::Kept)
@@ -1010,7 +1023,7 @@ void step_launches(Sim& s, double delt)
   bool was_sorted = s.sorted;
   launch_push_position(s, delt);
   s.sorted = was_sorted;
-  const RhoDone rho = launch_deposit(s, delt, s.rho_valid ? RhoMode::Keep : RhoMode::Defer);
+  const RhoDone rho = launch_deposit(s, delt, carry_rho(s) ? RhoMode::Keep : RhoMode::Defer);
   launch_push_bfd(s, 0.5 * delt);
   launch_fold_current(s, rho == RhoDone::Deposited ? 0 : 1, 4);
   if (rho == RhoDone::Kept)
@@ -1039,9 +1052,54 @@ namespace picnix {
 void tune_step(Sim& s, double delt);
 }
 
+namespace {
+
+// A capacity overflow of an earlier step, as far as the host can see it without
+// waiting: the flags copied after the last step, once that copy has completed.
+int step_flags_seen(Sim& s)
+{
+  if (s.flag_queued && hipEventQuery(s.flag_ev) == hipSuccess && s.h_flag[0])
+    return fail(s, PN_ERR_CAPACITY,
+                "particle segment capacity exceeded (raise [application.option] buffer_ratio)");
+  return PN_OK;
+}
+
+// Queue the copy of the device flags that the next pn_step reads.
+void queue_step_flags(Sim& s)
+{
+  HIPCHK(hipMemcpyAsync(s.h_flag, s.ss.errflag, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                        s.stream));
+  HIPCHK(hipEventRecord(s.flag_ev, s.stream));
+  s.flag_queued = true;
+}
+
+int step_impl(Sim& s, double delt);
+
+} // namespace
+
+// The kernels keep every access inside the particle segments when a segment
+// overflows (the sort clamps its bin starts and counts to the capacity), and
+// pn_step reports the overflow without a synchronisation: PN_ERR_CAPACITY comes
+// back from the first pn_step whose start sees the flags of the step that
+// overflowed, and from pn_synchronize / pn_get_energy / pn_get_particles.
 int pn_step(pn_sim* h, double delt)
 {
   return guarded(h, [&](Sim& s) {
+    int rc = step_flags_seen(s);
+    if (rc)
+      return rc;
+    rc = step_impl(s, delt);
+    if (rc == PN_OK)
+      queue_step_flags(s);
+    return rc;
+  });
+}
+
+namespace {
+
+int step_impl(Sim& s, double delt)
+{
+  {
     if (s.tune.state == 1) {
       tune_step(s, delt); // a measured step of an autotuning session
       return PN_OK;
@@ -1085,8 +1143,10 @@ int pn_step(pn_sim* h, double delt)
     }
     HIPCHK(hipGraphLaunch(s.gexec[k], s.stream));
     return PN_OK;
-  });
+  }
 }
+
+} // namespace
 
 int pn_set_step_graph(pn_sim* h, int enable)
 {

@@ -121,12 +121,21 @@ struct Sim {
   bool     np_host_valid = false;
 
   SortScratch ss{};
+  // device error flags as last seen by the host without a synchronisation:
+  // every pn_step queues a copy of ss.errflag into pinned h_flag and records
+  // flag_ev; the next pn_step reads h_flag once flag_ev has completed
+  int32_t*    h_flag  = nullptr;
+  hipEvent_t  flag_ev = nullptr;
+  bool        flag_queued = false;
   ShockState  shk{};
   bool        counted = false; // keys hold a fresh count (push_position / count)
   bool        sorted  = false; // particles are in sorted order and cellstart is valid
   // rho (component 0 of J) is the deposit of the current positions, left by
   // pn_step; any other call that moves particles or writes J clears it
   bool        rho_valid = false;
+  // PICNIX_RHO=deposit: pn_step deposits rho from the particles every step
+  // instead of carrying it by continuity (an independent charge check)
+  bool        rho_deposit = false;
   bool        generic_kernels = false; // PICNIX_KERNELS=generic: one-thread-per-particle forms
   int         march_rows      = 0;     // PICNIX_MARCH_ROWS: bin rows per march block (0: auto)
   // launch shapes picked by the autotuner (pn_autotune); 0 = the heuristic

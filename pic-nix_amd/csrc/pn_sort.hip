@@ -197,16 +197,20 @@ __global__ void __launch_bounds__(256)
     part[threadIdx.x] += v;
     __syncthreads();
   }
-  int      run   = part[threadIdx.x] - sum;
+  // On overflow every start and the new count are clamped to the capacity, so
+  // that the kernels bounded by them (the record moves here, the next step's
+  // gathers, deposits and pushes) stay inside the segment; the flag reports it.
+  const int cap = (int)g.cap;
+  int       run = part[threadIdx.x] - sum;
   int32_t* start = cellstart + (long)seg * (nb + 1);
   for (int b = b0; b < b0 + per && b < nb; b++) {
-    start[b] = run;
+    start[b] = min(run, cap);
     run += cnt[b];
   }
   if (threadIdx.x == 255) {
-    start[nb]  = part[255];
-    npnew[seg] = part[255];
-    if (part[255] > g.cap)
+    start[nb]  = min(part[255], cap);
+    npnew[seg] = min(part[255], cap);
+    if (part[255] > cap)
       atomicOr(&errflag[0], 1);
   }
 }

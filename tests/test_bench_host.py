@@ -48,3 +48,18 @@ def test_more_than_one_gpu_is_refused():
                        capture_output=True, text=True, timeout=120, cwd=ROOT)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_rho_bytes_follow_the_step_form():
+    """The periodic workloads carry rho by continuity (a stencil over the grid,
+    no particle reads); the shock problem deposits it from the particles."""
+    b = _bench()
+    assert b.algorithmic_bytes("rho", 10 ** 6, 1000) == 40 * 1000
+    assert b.algorithmic_bytes("rho", 10 ** 6, 1000, rho_direct=True) == 24 * 10 ** 6 + 8 * 1000
+    assert b.PHASE_KERNEL["rho"] == "k_rho_continuity"
+
+
+def test_host_cpu_record():
+    b = _bench()
+    h = b.host_cpu()
+    assert h["nproc"] >= 1 and h["logical_cpus"] >= h["nproc"]

@@ -53,9 +53,50 @@ def test_missing_library_fails_loudly(tmp_path):
             picnix_amd._lib = saved
 
 
-def test_config_rejects_bad_keys():
-    """Config validation happens before any device allocation, so it is testable on CPU
-    only through the Python reader; the library's checks mirror nix/cfgparser.hpp."""
+BAD_CONFIGS = [
+    # (edit of the thermal64 config, text the library's error must contain)
+    (lambda c: c["parameter"].update(Nx=65), "multiple of C"),
+    (lambda c: c["parameter"].pop("delt"), "missing mandatory key 'parameter.delt'"),
+    (lambda c: c["parameter"].update(Ns=3), "fewer [[parameter.particle]] tables than Ns"),
+    (lambda c: c["application"]["option"].update(order=5), "order must be 1, 2, 3 or 4"),
+    (lambda c: c["application"]["option"].update(pusher="Leapfrog"), "invalid pusher"),
+    (lambda c: c["application"]["option"].update(interpolation="XX"), "invalid interpolation"),
+    (lambda c: c["application"]["option"].update(seed_type="sometimes"), "invalid seed type"),
+    (lambda c: c["application"]["option"].update(precision="fp16"), "precision must be"),
+    (lambda c: c["application"].update(problem="vortex"), "unknown [application] problem"),
+    (lambda c: c["parameter"].update(Nx=4, Cx=4), "smaller than the ghost margin"),
+]
+
+
+@pytest.mark.parametrize("case", range(len(BAD_CONFIGS)))
+def test_config_rejects_bad_keys(case):
+    """pn_create validates the configuration before it touches a device
+    (nix/cfgparser.hpp:145-217, pic/pic_chunk.cpp:135-262), so every rejected
+    key comes back as PN_ERR_CONFIG with its reason on this CPU-only host."""
+    import ctypes
+
     import helpers
+    import picnix_amd
+    edit, reason = BAD_CONFIGS[case]
     cfg = helpers.load_config("thermal64")
-    assert cfg["parameter"]["Nx"] % cfg["parameter"]["Cx"] == 0
+    edit(cfg)
+    L = picnix_amd.load_library()
+    h = ctypes.c_void_p()
+    rc = L.pn_create(helpers.dumps(cfg).encode(), 0, ctypes.byref(h))
+    try:
+        assert rc == -2, (rc, L.pn_last_error(h))
+        assert reason in L.pn_last_error(h).decode()
+    finally:
+        if h:
+            L.pn_destroy(h)
+
+
+def test_create_rejects_null_arguments():
+    import ctypes
+
+    import picnix_amd
+    L = picnix_amd.load_library()
+    h = ctypes.c_void_p()
+    assert L.pn_create(None, 0, ctypes.byref(h)) == -1
+    assert L.pn_step(None, 0.5) == -1
+    assert L.pn_destroy(None) == -1

@@ -1,0 +1,371 @@
+"""GPU runs at the full sizes of the benchmark workloads, against the oracle.
+
+Each test runs one workload at the size its throughput is quoted on:
+  * thermal64  -- 64^3 cells, 64 chunks of 16^3, electrons + protons, 64
+    particles/cell/species (33.5M particles), order 2, FP64: oracle parity at
+    steps 1 and 10 with the built-in launch shapes, and after an autotuning
+    session with the work-queue kernels; rho carried by continuity against a
+    direct deposit after 200 steps; div(E) - rho cell by cell with rho
+    deposited from the particles every step;
+  * twostream2d -- 1 x 16 x 512 cells, 32 chunks of 1 x 16 x 16, two beams at
+    +-0.1c, 64 particles/cell/beam, order 1, FP64: oracle parity over 10 steps
+    and the energy histories of 1000 steps, pointwise while the runs stay
+    together and statistically afterwards;
+  * thermal128 -- 128^3 cells, 512 chunks of 16^3, 32 particles/cell/species
+    (134M particles), the FP32 batched form (autotuned, step graph) against the
+    FP64 path on the same inputs: particle conservation, charge conservation and
+    energies after 200 steps;
+  * shock3d -- 512 x 64 x 64 cells of uneven density (16 -> 64 particles/cell),
+    wall and injection: particle counts and per-chunk boundary state of every
+    chunk against the oracle, and the particles of the boundary chunks;
+  * the adjoint form at 256^2 cells x 32 particles/cell with 8 density modes:
+    gradients against the CPU tangent-linear reference.
+Bars are those of the small-size parity tests: particle order bit-exact, values
+bit-exact at step 1 and within 1e-12 relative afterwards, fields within 1e-12.
+"""
+import copy
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import helpers
+import oracle_lib
+from test_gpu_parity_step import _compare_fields, _compare_particles
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+T0 = time.time()
+
+
+def _say(msg):
+    print(f"[configs {time.time() - T0:7.1f}s] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _sim(cfg):
+    import picnix_amd
+    return picnix_amd.Simulation(helpers.dumps(cfg))
+
+
+def _with_option(cfg, **opt):
+    cfg = copy.deepcopy(cfg)
+    cfg["application"]["option"].update(opt)
+    return cfg
+
+
+@pytest.mark.timeout(900)
+def test_thermal64_parity_steps_1_and_10():
+    cfg = helpers.load_config("thermal64")
+    dt = float(cfg["parameter"]["delt"])
+    _say("thermal64: oracle + GPU set-up (33.5M particles)")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    try:
+        _compare_particles(o, g, exact=True)
+        _say("thermal64: loader + initial sort identical; step 1")
+        o.step()
+        g.step(dt)
+        g.synchronize()
+        _compare_particles(o, g, exact=True)
+        rel = _compare_fields(o, g, 3)
+        _say(f"thermal64: step 1 particles bit-exact, fields {max(rel.values()):.2e}")
+        assert max(rel.values()) <= TOL, rel
+        for _ in range(9):
+            o.step()
+            g.step(dt)
+        g.synchronize()
+        perr = _compare_particles(o, g, exact=False)
+        rel = _compare_fields(o, g, 3)
+        _say(f"thermal64: step 10 particles {perr:.2e}, fields {max(rel.values()):.2e}")
+        assert perr <= TOL
+        assert max(rel.values()) <= TOL, rel
+        ho, hg = o.history(), g.history()
+        assert np.allclose(hg[2:], ho[2:], rtol=1e-10)
+    finally:
+        g.close()
+        o.close()
+
+
+@pytest.mark.timeout(900)
+def test_thermal64_parity_autotuned_work_queue(monkeypatch):
+    """The launch shapes the bench runs with: an autotuning session (which also
+    tries the persistent work-queue forms against one block per item) and two
+    more steps, all against the oracle."""
+    monkeypatch.setenv("PICNIX_WORK_QUEUE", "1")
+    cfg = helpers.load_config("thermal64")
+    dt = float(cfg["parameter"]["delt"])
+    _say("thermal64 autotuned: oracle + GPU set-up")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    try:
+        g.autotune(True)
+        nt = g.tuning()["steps_left"] + 2
+        for _ in range(nt):
+            o.step()
+            g.step(dt)
+        g.synchronize()
+        tun = g.tuning()
+        _say(f"thermal64 autotuned: {nt} steps, shapes {tun}")
+        assert tun["state"] == 2
+        perr = _compare_particles(o, g, exact=False)
+        rel = _compare_fields(o, g, 3)
+        _say(f"thermal64 autotuned: step {nt} particles {perr:.2e}, fields {max(rel.values()):.2e}")
+        assert perr <= TOL
+        assert max(rel.values()) <= TOL, rel
+    finally:
+        g.close()
+        o.close()
+
+
+def _rho_interior(g):
+    out = []
+    for c in range(g.nchunks):
+        info = g.chunk_info(c)
+        out.append(helpers.interior(g.field(c, 1), info, 3)[..., 0].copy())
+    return out
+
+
+@pytest.mark.timeout(600)
+def test_thermal64_rho_carried_equals_deposit_after_200_steps():
+    """pn_step carries rho by rho -= dt div J after its first step; after 200
+    steps a direct deposit of the same positions gives the same rho."""
+    import picnix_amd
+    cfg = helpers.load_config("thermal64")
+    dt = float(cfg["parameter"]["delt"])
+    g = _sim(cfg)
+    try:
+        for k in range(4):
+            g.step(dt, 50)
+            g.synchronize()
+            _say(f"thermal64 rho carried: step {50 * (k + 1)}")
+        carried = _rho_interior(g)
+        g.push_position(0.0)
+        g.deposit_current(dt)
+        g.exchange(picnix_amd.BOUNDARY_CUR)
+        direct = _rho_interior(g)
+        worst = max(float(np.max(np.abs(a - b))) for a, b in zip(carried, direct))
+        # relative to the charge density of one species: |qm * ro| = 1 per cell
+        # (the total rho of the neutral plasma is a small difference of the two)
+        _say(f"thermal64 rho carried vs direct after 200 steps: max diff {worst:.3e}")
+        assert worst <= TOL
+    finally:
+        g.close()
+
+
+def _div_e_minus_rho(g):
+    """max over cells of |div E - rho| (forward differences of E, as
+    get_diverror), from the ghost-inclusive chunk arrays."""
+    worst = 0.0
+    for c in range(g.nchunks):
+        info = g.chunk_info(c)
+        nb = int(info[9])
+        nz, ny, nx = (int(v) for v in info[6:9])
+        uf = g.field(c, 0)
+        uj = g.field(c, 1)
+        z, y, x = slice(nb, nz - nb), slice(nb, ny - nb), slice(nb, nx - nb)
+        zp, yp, xp = slice(nb + 1, nz - nb + 1), slice(nb + 1, ny - nb + 1), slice(nb + 1, nx - nb + 1)
+        d = (uf[z, y, xp, 0] - uf[z, y, x, 0]) + (uf[z, yp, x, 1] - uf[z, y, x, 1]) + \
+            (uf[zp, y, x, 2] - uf[z, y, x, 2]) - uj[z, y, x, 0]
+        worst = max(worst, float(np.max(np.abs(d))))
+    return worst
+
+
+@pytest.mark.timeout(600)
+def test_thermal64_charge_conservation_with_deposited_rho(monkeypatch):
+    """With rho deposited from the particles every step (PICNIX_RHO=deposit),
+    div(E) - rho is an independent check of the deposit: Esirkepov's current
+    and the Yee update keep it at its initial value cell by cell.  E starts at
+    zero and the species start at the same positions, so the residual must stay
+    at round-off of the per-species charge density (1 per cell) for 100 steps."""
+    monkeypatch.setenv("PICNIX_RHO", "deposit")
+    cfg = helpers.load_config("thermal64")
+    dt = float(cfg["parameter"]["delt"])
+    g = _sim(cfg)
+    try:
+        r0 = _div_e_minus_rho(g)
+        done = 0
+        for k in (1, 50, 100):
+            g.step(dt, k - done)
+            done = k
+            g.synchronize()
+            r = _div_e_minus_rho(g)
+            _say(f"thermal64 deposited rho: step {k} max |div E - rho| {r:.3e} (step 0: {r0:.3e})")
+            assert r <= 1e-12
+    finally:
+        g.close()
+
+
+@pytest.mark.timeout(600)
+def test_twostream_full_parity_and_histories():
+    """Config 1 at full size: 10 steps against the oracle, then 1000 steps of
+    energy history.  The runs separate at round-off growth; the histories must
+    agree pointwise while they stay together (1e-10 through step 100) and
+    statistically afterwards (mean field energy over the last 300 steps within
+    5 %, kinetic energies within 0.5 %)."""
+    cfg = helpers.load_config("twostream2d")
+    dt = float(cfg["parameter"]["delt"])
+    _say("twostream2d: oracle + GPU set-up")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    try:
+        _compare_particles(o, g, exact=True)
+        o.step()
+        g.step(dt)
+        g.synchronize()
+        _compare_particles(o, g, exact=True)
+        assert max(_compare_fields(o, g, 2).values()) <= TOL
+        for _ in range(9):
+            o.step()
+            g.step(dt)
+        g.synchronize()
+        perr = _compare_particles(o, g, exact=False)
+        rel = _compare_fields(o, g, 2)
+        _say(f"twostream2d: step 10 particles {perr:.2e}, fields {max(rel.values()):.2e}")
+        assert perr <= TOL and max(rel.values()) <= TOL
+        ho, hg = [], []
+        for k in range(99):
+            o.step(10)
+            g.step(dt, 10)
+            ho.append(o.history())
+            hg.append(g.history())
+            if k % 20 == 0:
+                _say(f"twostream2d: step {10 * (k + 2)} E2 {hg[-1][2]:.6e} vs {ho[-1][2]:.6e}")
+        ho, hg = np.array(ho), np.array(hg)
+        # pointwise while the runs stay together
+        for k in range(9):
+            assert np.allclose(hg[k, 2:], ho[k, 2:], rtol=1e-9), (k, hg[k], ho[k])
+        # the instability grew (field energy up by orders of magnitude)
+        assert ho[-1, 2] > 100 * ho[0, 2]
+        tail = slice(-30, None)
+        me_o, me_g = ho[tail, 2].mean(), hg[tail, 2].mean()
+        _say(f"twostream2d: mean E2 over the last 300 steps {me_g:.6e} vs {me_o:.6e}")
+        assert abs(me_g - me_o) <= 0.05 * me_o
+        for s in (4, 5):
+            assert abs(hg[tail, s].mean() - ho[tail, s].mean()) <= 5e-3 * ho[tail, s].mean()
+        # total energy is conserved to the same degree by both
+        to = ho[:, 2] + ho[:, 3] + ho[:, 4] + ho[:, 5]
+        tg = hg[:, 2] + hg[:, 3] + hg[:, 4] + hg[:, 5]
+        _say(f"twostream2d: total energy drift GPU {tg[-1] / tg[0] - 1:+.3e}, "
+             f"oracle {to[-1] / to[0] - 1:+.3e}")
+        assert abs((tg[-1] - tg[0]) - (to[-1] - to[0])) <= 0.05 * abs(to[0])
+    finally:
+        g.close()
+        o.close()
+
+
+@pytest.mark.timeout(900)
+def test_thermal128_fp32_batched_against_fp64():
+    """The FP32 batched form at 128^3 (autotuned launch shapes, replayed step
+    graph) against the FP64 path on the same inputs after 200 steps: particles
+    conserved, energies within 1e-4 relative, div(E) - rho summed over the grid
+    within 1e-9 of the total charge of one species."""
+    cfg = helpers.load_config("thermal128")
+    dt = float(cfg["parameter"]["delt"])
+    _say("thermal128: FP64 set-up (134M particles)")
+    a = _sim(cfg)
+    n0 = sum(a.np(c, s) for c in range(a.nchunks) for s in range(a.Ns))
+    for k in range(4):
+        a.step(dt, 50)
+        a.synchronize()
+    ha = a.history()
+    na = sum(a.np(c, s) for c in range(a.nchunks) for s in range(a.Ns))
+    a.close()
+    _say(f"thermal128: FP64 200 steps done {ha}")
+    b = _sim(_with_option(cfg, precision="fp32"))
+    try:
+        b.autotune(True)
+        nt = b.tuning()["steps_left"]
+        b.step(dt, nt)
+        b.step_graph(True)
+        left = 200 - nt
+        while left > 0:
+            k = min(50, left)
+            b.step(dt, k)
+            b.synchronize()
+            left -= k
+        hb = b.history()
+        nb = sum(b.np(c, s) for c in range(b.nchunks) for s in range(b.Ns))
+        _say(f"thermal128: FP32 200 steps done {hb}")
+        assert na == n0 and nb == n0
+        for k in (2, 3, 4, 5):
+            rel = abs(hb[k] - ha[k]) / abs(ha[k])
+            _say(f"thermal128: history[{k}] FP32 vs FP64 rel {rel:.2e}")
+            assert rel <= 1e-4, (k, ha[k], hb[k])
+        ncell = 128 ** 3
+        assert abs(hb[0]) <= 1e-9 * ncell, hb[0]
+        assert abs(ha[0]) <= 1e-12 * ncell, ha[0]
+    finally:
+        b.close()
+
+
+@pytest.mark.timeout(900)
+def test_shock3d_full_counts_and_boundary_state():
+    """Config 4 at full size (512 x 64 x 64, 16 -> 64 particles per cell): every
+    chunk's particle counts and upstream boundary state (random seed, last id,
+    influx, efflux) equal the oracle's after the loader and after 2 steps, and
+    the particles of the wall chunks and the injecting chunks are identical
+    (bit-exact at step 1, within 1e-12 at step 2)."""
+    import picnix_amd  # noqa: F401
+    cfg = helpers.load_config("shock3d")
+    cfg["application"]["option"]["emf_order"] = 1
+    dt = float(cfg["parameter"]["delt"])
+    _say("shock3d: oracle + GPU set-up (~160M particles)")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    try:
+        ncx = int(cfg["parameter"]["Cx"])
+        edge = [c for c in range(o.nchunks) if int(o.chunk_info(c)[2]) in (0, ncx - 1)]
+
+        def check(exact, label):
+            tot = 0
+            for c in range(o.nchunks):
+                for s in range(2):
+                    n = o.np(c, s)
+                    assert g.np(c, s) == n, (label, c, s, g.np(c, s), n)
+                    tot += n
+                assert g.boundary_state(c) == o.shock_state(c), (label, c)
+            worst = 0.0
+            for c in edge:
+                for s in range(2):
+                    a, b = o.particles(c, s), g.particles(c, s)
+                    assert np.array_equal(a[:, 6].view(np.int64), b[:, 6].view(np.int64)), (label, c, s)
+                    if exact:
+                        assert np.array_equal(a[:, :6], b[:, :6]), (label, c, s)
+                    else:
+                        worst = max(worst, helpers.rel_err(b[:, :6], a[:, :6]))
+            _say(f"shock3d {label}: {tot} particles, counts and boundary state identical in "
+                 f"{o.nchunks} chunks; {len(edge)} boundary chunks' particles worst {worst:.2e}")
+            assert worst <= TOL
+
+        check(True, "initial")
+        o.step()
+        g.step(dt)
+        g.synchronize()
+        check(True, "step 1")
+        o.step()
+        g.step(dt)
+        g.synchronize()
+        check(False, "step 2")
+        ho, hg = o.history(), g.history()
+        for k in range(2, len(ho)):
+            assert abs(hg[k] - ho[k]) <= 1e-10 * abs(ho[k]), (k, ho[k], hg[k])
+    finally:
+        g.close()
+        o.close()
+
+
+@pytest.mark.timeout(900)
+def test_adjoint_weibel_full_size():
+    """The adjoint form at its full size (256^2 cells, 32 particles/cell, the
+    anisotropy ratio and 8 density-mode amplitudes of 1e-3), 40 steps with a
+    checkpoint every 20: every gradient component within 1e-3 of the CPU
+    tangent-linear reference.  (The 200-step run is the same check with
+    PICNIX_FULL=1 in test_gpu_adjoint.py.)"""
+    from test_gpu_adjoint import _tangent_check
+    _say("adjoint 256^2: forward, backward, tangent-linear")
+    rows = _tangent_check(256, 32, 8, 40, 20, 12345, [6.0] + [1e-3] * 8)
+    for ga, gt in rows:
+        assert abs(ga - gt) <= 1e-3 * abs(gt), (ga, gt)
