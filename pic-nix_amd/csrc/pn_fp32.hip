@@ -99,11 +99,11 @@ __global__ void __launch_bounds__(256)
     const double u  = (x[a] - lim[c * 6 + 2 * ax]) / g.del[ax];
     const double fl = floor(u);
     float        rf = (float)(u - fl);
-    int          ci = (int)fl;
-    if (rf >= 1.0f) {
-      rf = 0.0f;
-      ci += 1;
-    }
+    const int    ci = (int)fl;
+    // u - floor(u) < 1 can round up to 1.0f: keep the FP64 cell (the reference's
+    // bin) with the largest offset below 1, as the sort does
+    if (rf >= 1.0f)
+      rf = 0.99999994f;
     cl[a] = ci;
     r[a]  = rf;
     in    = in && ci >= 0 && ci < g.dims[ax];

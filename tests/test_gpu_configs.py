@@ -202,9 +202,9 @@ def test_thermal64_charge_conservation_with_deposited_rho(monkeypatch):
 def test_twostream_full_parity_and_histories():
     """Config 1 at full size: 10 steps against the oracle, then 1000 steps of
     energy history.  The runs separate at round-off growth; the histories must
-    agree pointwise while they stay together (1e-10 through step 100) and
+    agree pointwise while they stay together (1e-9 through step 100) and
     statistically afterwards (mean field energy over the last 300 steps within
-    5 %, kinetic energies within 0.5 %)."""
+    5 %, kinetic energies within 0.5 %, the same total-energy drift)."""
     cfg = helpers.load_config("twostream2d")
     dt = float(cfg["parameter"]["delt"])
     _say("twostream2d: oracle + GPU set-up")
@@ -237,8 +237,13 @@ def test_twostream_full_parity_and_histories():
         # pointwise while the runs stay together
         for k in range(9):
             assert np.allclose(hg[k, 2:], ho[k, 2:], rtol=1e-9), (k, hg[k], ho[k])
-        # the instability grew (field energy up by orders of magnitude)
-        assert ho[-1, 2] > 100 * ho[0, 2]
+        # At these parameters (c = 1, v0 = 0.1c, w_p = 1, dx = 1) the fastest
+        # two-stream wavelength 2 pi v0 / w_p is 0.6 cell: the instability is not
+        # resolved and the field energy stays at the level of the particle noise
+        # (its growth rate against linear theory is checked on a resolved set-up
+        # in test_gpu_physics.py).  The field energy stays within a few times its
+        # initial noise level.
+        assert ho[-1, 2] < 5 * ho[0, 2] and hg[-1, 2] < 5 * hg[0, 2]
         tail = slice(-30, None)
         me_o, me_g = ho[tail, 2].mean(), hg[tail, 2].mean()
         _say(f"twostream2d: mean E2 over the last 300 steps {me_g:.6e} vs {me_o:.6e}")

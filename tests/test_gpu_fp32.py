@@ -247,3 +247,37 @@ def test_fp32_fused_push(kind, monkeypatch):
     assert dx <= 1e-5 * float(cfg["parameter"]["delh"]) and du <= 1e-4 and fe <= 1e-4
     a.close()
     b.close()
+
+
+def test_fp32_keeps_particles_at_the_upper_chunk_face():
+    """A particle just below a chunk's upper face (x = xmax - 1e-12 in FP64) has
+    an offset u - floor(u) that rounds to 1.0f in FP32; it stays in its FP64
+    cell (the reference's bin) with the largest offset below 1, and is neither
+    dropped nor moved to the next chunk by the conversion, the sort or a step."""
+    import picnix_amd
+    cfg = _fp32(helpers.small_thermal(N=(8, 8, 16), C=(1, 1, 2), np_=1, order=2))
+    sim = _sim(cfg)
+    try:
+        for c in range(sim.nchunks):
+            for s in range(sim.Ns):
+                sim.set_particles(c, s, np.zeros((0, 7)))
+        info = sim.chunk_info(0)
+        x1 = float(info[5]) + 8.0  # upper x face of chunk 0 (8 cells, dx = 1)
+        a = np.zeros((4, 7))
+        a[:, 0] = [x1 - 1e-12, x1 - 1e-9, np.nextafter(x1, 0.0), x1 - 0.5]
+        a[:, 1] = [0.5, 7.999999999999, 3.0, 4.0]
+        a[:, 2] = [np.nextafter(8.0, 0.0), 0.25, 5.0, 1.0]
+        a[:, 6] = np.arange(4, dtype=np.int64).view(np.float64)
+        sim.set_particles(0, 0, a)
+        sim.sort_particle()
+        assert sim.np(0, 0) == 4, sim.np(0, 0)
+        b = sim.particles(0, 0)
+        assert np.all(b[:, 0] < x1) and np.all(b[:, 0] >= x1 - 1.0)
+        assert np.all(b[:, 1] < 8.0) and np.all(b[:, 2] < 8.0)
+        assert np.max(np.abs(np.sort(b[:, 0]) - np.sort(a[:, 0]))) <= 1e-6
+        sim.step(float(cfg["parameter"]["delt"]))
+        sim.synchronize()
+        n = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
+        assert n == 4, n
+    finally:
+        sim.close()
