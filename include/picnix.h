@@ -122,6 +122,20 @@ int pn_set_step_graph(pn_sim* sim, int enable);
 /* Field energy E^2/2, B^2/2 and per-species kinetic energy (PicChunk::get_energy,
  * pic/pic_chunk.cpp:407-439, summed over chunks as HistoryDiag does). */
 int pn_get_energy(pn_sim* sim, double* efd, double* bfd, double* particle);
+/* pn_get_energy without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (tight serving loops that read
+ * the history every few steps, harnesses that forbid hipMalloc).  It
+ * synchronises the handle's stream, so it is not for graph capture.
+ * pn_get_energy takes a
+ * partial-sum buffer with hipMallocAsync on every call; this variant keeps its
+ * partial sums (at most 2048 rows of 12 doubles, 197 KB) in a file-scope device
+ * array that each GPU holds until the process ends, and calls no allocating
+ * routine.  Every call on one GPU uses the same array (each call writes every
+ * entry it reads, so nothing carries over), so calls that overlap on one GPU
+ * race.  Periodic problems only (PN_ERR_UNSUPPORTED for a physical boundary);
+ * the handle's device must be current (PN_ERR_STATE otherwise).  Same outputs as
+ * pn_get_energy, summed in a fixed order. */
+int pn_get_energy_static(pn_sim* sim, double* efd, double* bfd, double* particle);
 /* Signed sums of div(E) - rho and div(B) (PicChunk::get_diverror, pic/pic_chunk.cpp:441-445). */
 int pn_get_diverror(pn_sim* sim, double* efd, double* bfd);
 /* Block until queued device work on the handle's stream is done. */

@@ -192,6 +192,7 @@ void allocate(Sim& s)
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&s.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    s.device = dev;
   }
   s.d_off = dalloc<int>(s.offset.size());
   s.d_lim = dalloc<double>(s.lim.size());
@@ -514,6 +515,7 @@ int pn_create(const char* toml, int load, pn_sim** out)
     if (const char* k = std::getenv("PICNIX_RHO")) // "deposit": rho from particles every step
       s.rho_deposit = std::string(k) == "deposit";
     allocate(s);
+    energy_static_init(s);
     if (s.cfg.problem == 1) {
       shock_setup(s);
       shock_init_fields(s);
@@ -1163,6 +1165,31 @@ int pn_get_energy(pn_sim* h, double* efd, double* bfd, double* particle)
   return guarded(h, [&](Sim& s) {
     std::vector<double> out(4 + s.g.Ns);
     launch_energy(s, out.data());
+    if (efd)
+      *efd = out[0];
+    if (bfd)
+      *bfd = out[1];
+    if (particle)
+      for (int sp = 0; sp < s.g.Ns; sp++)
+        particle[sp] = out[4 + sp];
+    return check_device_flags(s);
+  });
+}
+
+int pn_get_energy_static(pn_sim* h, double* efd, double* bfd, double* particle)
+{
+  return guarded(h, [&](Sim& s) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != s.device)
+      return fail(s, PN_ERR_STATE, "pn_get_energy_static: the handle's device is not current");
+    double out[4 + kMaxSpecies];
+    const int rc = launch_energy_static(s, out);
+    if (rc == PN_ERR_UNSUPPORTED)
+      return fail(s, rc, "pn_get_energy_static: periodic problems only (a physical boundary "
+                         "forms the kinetic energy from the moment deposit, which allocates)");
+    if (rc)
+      return fail(s, rc, "pn_get_energy_static: no scratch address for this device");
     if (efd)
       *efd = out[0];
     if (bfd)

@@ -1,5 +1,6 @@
 // pn_diag.hip -- energy / divergence reductions (FP64) and chunk views of the
 // global fields for the C-ABI boundary.
+#include <mutex>
 #include <vector>
 
 #include "../../include/picnix.h"
@@ -36,7 +37,47 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* out)
 
 // Per interior cell: E^2/2, B^2/2 (PicChunk::get_energy, pic/pic_chunk.cpp:407-431),
 // div(E) - rho and div(B) (BaseMaxwell::get_diverror_{2,3}d,
-// pic/engine/maxwell.hpp:203-229,350-381).  Block partials, summed on the host.
+// pic/engine/maxwell.hpp:203-229,350-381) of interior cell `tid` (x fastest).
+template <int Dim>
+__device__ __forceinline__ void field_terms(const Geom& g, const double* __restrict__ E,
+                                            const double* __restrict__ B,
+                                            const double* __restrict__ J, long tid, double acc[4])
+{
+  int  x = (int)(tid % g.N[2]) + g.pad[2];
+  long r = tid / g.N[2];
+  int  y = (int)(r % g.N[1]) + g.pad[1];
+  int  z = (int)(r / g.N[1]) + g.pad[0];
+  long i = gidx(g, z, y, x);
+  long v = g.vol, sy = g.P[2], sz = g.plane;
+  const double rdx = 1 / g.del[2], rdy = 1 / g.del[1], rdz = 1 / g.del[0];
+  double ex = E[i], ey = E[v + i], ez = E[2 * v + i];
+  double bx = B[i], by = B[v + i], bz = B[2 * v + i];
+  acc[0] = 0.5 * (ex * ex + ey * ey + ez * ez);
+  acc[1] = 0.5 * (bx * bx + by * by + bz * bz);
+  if constexpr (Dim == 1) {
+    // get_diverror_1d (pic/engine/maxwell.hpp:63-83)
+    acc[2] = (E[i + 1] - ex) * rdx - J[i];
+    acc[3] = (bx - B[i - 1]) * rdx;
+    (void)rdy;
+    (void)rdz;
+    (void)sz;
+  } else if constexpr (Dim == 3) {
+    acc[2] = (E[i + 1] - ex) * rdx + (E[v + i + sy] - ey) * rdy + (E[2 * v + i + sz] - ez) * rdz -
+             J[i];
+    acc[3] = (bx - B[i - 1]) * rdx + (by - B[v + i - sy]) * rdy + (bz - B[2 * v + i - sz]) * rdz;
+  } else {
+    acc[2] = (E[i + 1] - ex) * rdx + (E[v + i + sy] - ey) * rdy - J[i];
+    acc[3] = (bx - B[i - 1]) * rdx + (by - B[v + i - sy]) * rdy;
+  }
+  // cells next to a physical boundary are left out of the divergence errors
+  // (pic/pic_engine.hpp:42-71)
+  if (g.xwall && (x < g.pad[2] + g.Nb || x >= g.pad[2] + g.N[2] - g.Nb)) {
+    acc[2] = 0;
+    acc[3] = 0;
+  }
+}
+
+// One thread per interior cell; block partials, summed on the host.
 template <int Dim>
 __global__ void __launch_bounds__(256) k_field_energy(Geom g, const double* __restrict__ E,
                                                       const double* __restrict__ B,
@@ -46,46 +87,23 @@ __global__ void __launch_bounds__(256) k_field_energy(Geom g, const double* __re
   long   n   = (long)g.N[0] * g.N[1] * g.N[2];
   long   tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double acc[4] = {0, 0, 0, 0};
-  if (tid < n) {
-    int  x = (int)(tid % g.N[2]) + g.pad[2];
-    long r = tid / g.N[2];
-    int  y = (int)(r % g.N[1]) + g.pad[1];
-    int  z = (int)(r / g.N[1]) + g.pad[0];
-    long i = gidx(g, z, y, x);
-    long v = g.vol, sy = g.P[2], sz = g.plane;
-    const double rdx = 1 / g.del[2], rdy = 1 / g.del[1], rdz = 1 / g.del[0];
-    double ex = E[i], ey = E[v + i], ez = E[2 * v + i];
-    double bx = B[i], by = B[v + i], bz = B[2 * v + i];
-    acc[0] = 0.5 * (ex * ex + ey * ey + ez * ez);
-    acc[1] = 0.5 * (bx * bx + by * by + bz * bz);
-    if constexpr (Dim == 1) {
-      // get_diverror_1d (pic/engine/maxwell.hpp:63-83)
-      acc[2] = (E[i + 1] - ex) * rdx - J[i];
-      acc[3] = (bx - B[i - 1]) * rdx;
-      (void)rdy;
-      (void)rdz;
-      (void)sz;
-    } else if constexpr (Dim == 3) {
-      acc[2] = (E[i + 1] - ex) * rdx + (E[v + i + sy] - ey) * rdy + (E[2 * v + i + sz] - ez) * rdz -
-               J[i];
-      acc[3] = (bx - B[i - 1]) * rdx + (by - B[v + i - sy]) * rdy + (bz - B[2 * v + i - sz]) * rdz;
-    } else {
-      acc[2] = (E[i + 1] - ex) * rdx + (E[v + i + sy] - ey) * rdy - J[i];
-      acc[3] = (bx - B[i - 1]) * rdx + (by - B[v + i - sy]) * rdy;
-    }
-    // cells next to a physical boundary are left out of the divergence errors
-    // (pic/pic_engine.hpp:42-71)
-    if (g.xwall && (x < g.pad[2] + g.Nb || x >= g.pad[2] + g.N[2] - g.Nb)) {
-      acc[2] = 0;
-      acc[3] = 0;
-    }
-  }
+  if (tid < n)
+    field_terms<Dim>(g, E, B, J, tid, acc);
   block_sum<4>(acc, partial + 4 * blockIdx.x);
 }
 
-// Kinetic energy m c^2 (gamma - 1) per particle; the reference forms the same
-// sum through the moment deposit, sum(um[...,4] c - um[...,0] c^2)
+// Kinetic energy m c^2 (gamma - 1) of particle i of species sp; the reference
+// forms the same sum through the moment deposit, sum(um[...,4] c - um[...,0] c^2)
 // (pic/pic_chunk.cpp:433-438).
+__device__ __forceinline__ double kinetic(const Geom& g, const PArr& p, long i, int sp)
+{
+  const double rc = 1 / g.cc;
+  double       ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+  double       u2 = (ux * ux + uy * uy + uz * uz) * rc * rc;
+  double       gm = sqrt(1 + u2);
+  return g.m[sp] * g.cc * g.cc * (u2 / (gm + 1));
+}
+
 __global__ void __launch_bounds__(256) k_particle_energy(Geom g, PArr p,
                                                          const int32_t* __restrict__ np,
                                                          double* __restrict__ partial, int ntile)
@@ -93,17 +111,84 @@ __global__ void __launch_bounds__(256) k_particle_energy(Geom g, PArr p,
   const int  seg = blockIdx.y;
   const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double     acc[1] = {0};
-  if (ip < np[seg]) {
-    const int    sp = seg % g.Ns;
-    const long   i  = (long)seg * g.cap + ip;
-    const double rc = 1 / g.cc;
-    double       ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
-    double       u2 = (ux * ux + uy * uy + uz * uz) * rc * rc;
-    double       gm = sqrt(1 + u2);
-    acc[0]          = g.m[sp] * g.cc * g.cc * (u2 / (gm + 1));
-  }
+  if (ip < np[seg])
+    acc[0] = kinetic(g, p, (long)seg * g.cap + ip, seg % g.Ns);
   block_sum<1>(acc, partial + ((long)seg * ntile + blockIdx.x));
 }
+
+// ---- the same reductions into a fixed number of partial rows ----
+// A grid of at most kEnergyRows blocks walks the cells (or the particle tiles)
+// by stride; every block writes its row of partial sums, and one block sums the
+// rows in row order.  All of it lives in the file-scope scratch array below, so
+// these kernels need no allocation and no clearing (every row is written before
+// it is read).
+constexpr int kEnergyRows = 2048;
+constexpr int kEnergyCols = 4 + kMaxSpecies;
+
+template <int Dim>
+__global__ void __launch_bounds__(256) k_field_energy_rows(Geom g, const double* __restrict__ E,
+                                                           const double* __restrict__ B,
+                                                           const double* __restrict__ J,
+                                                           double* __restrict__ rows)
+{
+  const long n      = (long)g.N[0] * g.N[1] * g.N[2];
+  const long stride = (long)gridDim.x * blockDim.x;
+  double     acc[4] = {0, 0, 0, 0};
+  for (long tid = (long)blockIdx.x * blockDim.x + threadIdx.x; tid < n; tid += stride) {
+    double t[4];
+    field_terms<Dim>(g, E, B, J, tid, t);
+    for (int k = 0; k < 4; k++)
+      acc[k] += t[k];
+  }
+  block_sum<4>(acc, rows + 4 * blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_particle_energy_rows(Geom g, PArr p,
+                                                             const int32_t* __restrict__ np,
+                                                             double* __restrict__ rows, int ntile)
+{
+  const long nitem = (long)g.nseg * ntile;
+  double     acc[kMaxSpecies];
+#pragma unroll
+  for (int k = 0; k < kMaxSpecies; k++)
+    acc[k] = 0;
+  for (long it = blockIdx.x; it < nitem; it += gridDim.x) {
+    const int  seg = (int)(it / ntile);
+    const long ip  = (it - (long)seg * ntile) * blockDim.x + threadIdx.x;
+    if (ip < np[seg]) {
+      const int    sp = seg % g.Ns;
+      const double e  = kinetic(g, p, (long)seg * g.cap + ip, sp);
+#pragma unroll
+      for (int k = 0; k < kMaxSpecies; k++)
+        acc[k] += k == sp ? e : 0.0;
+    }
+  }
+  block_sum<kMaxSpecies>(acc, rows + kMaxSpecies * blockIdx.x);
+}
+
+// out[0..3] = column sums of the field rows, out[4 + s] = of the particle rows
+__global__ void __launch_bounds__(64) k_energy_rows_sum(const double* __restrict__ frows, int nf,
+                                                        const double* __restrict__ prows, int npr,
+                                                        double* __restrict__ out)
+{
+  const int k = threadIdx.x;
+  if (k >= kEnergyCols)
+    return;
+  double s = 0;
+  if (k < 4)
+    for (int r = 0; r < nf; r++)
+      s += frows[4 * r + k];
+  else
+    for (int r = 0; r < npr; r++)
+      s += prows[kMaxSpecies * r + (k - 4)];
+  out[k] = s;
+}
+
+// Field rows, particle rows, then the kEnergyCols sums.  Each GPU holds its own
+// copy until the process ends; every call on one GPU uses it, so calls that
+// overlap on one GPU race.
+static __device__ __align__(256) double
+    scratch[kEnergyRows * 4 + kEnergyRows * kMaxSpecies + kEnergyCols];
 
 __device__ __forceinline__ int wrapg(int l, int off, int pad, int n)
 {
@@ -211,6 +296,70 @@ void launch_energy(Sim& s, double* out)
     launch_moment_exchange(s, true);
     launch_moment_energy(s, out + 4);
   }
+}
+
+namespace {
+
+// Device address of `scratch` on every GPU the library has run on (each GPU has
+// its own copy of the array at its own address).  On the host the array's name
+// is only a stand-in: kernels and copies take these addresses.
+constexpr int kMaxDevices = 64;
+std::mutex    scratch_mu;
+double*       scratch_ptr[kMaxDevices] = {};
+
+} // namespace
+
+void energy_static_init(Sim& s)
+{
+  if (s.device < 0 || s.device >= kMaxDevices)
+    throw HipError{"device index out of the range of the energy scratch table"};
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[s.device]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    scratch_ptr[s.device] = static_cast<double*>(ptr);
+  }
+  s.energy_scratch = scratch_ptr[s.device];
+}
+
+// launch_energy without any allocation: the partial rows and the sums live in
+// the file-scope scratch array of this GPU.
+int launch_energy_static(Sim& s, double* out)
+{
+  const Geom& g = s.g;
+  if (!s.energy_scratch)
+    return PN_ERR_STATE;
+  if (g.xwall || g.Ns > kMaxSpecies)
+    return PN_ERR_UNSUPPORTED;
+  double*    frows = s.energy_scratch;
+  double*    prows = frows + 4 * kEnergyRows;
+  double*    sums  = prows + kMaxSpecies * kEnergyRows;
+  const long n     = (long)g.N[0] * g.N[1] * g.N[2];
+  const int  nf    = (int)std::min<long>(kEnergyRows, nblk(n, 256));
+  const int  npr   = (int)std::min<long>(kEnergyRows, (long)g.nseg * s.ss.ntile);
+  if (s.fp32) {
+    // FP64 reductions over the FP64 mirrors of the FP32 state
+    launch32_fields_to_f64(s, PN_FIELD_UF);
+    launch32_fields_to_f64(s, PN_FIELD_UJ);
+    launch32_to_f64(s, 0, g.nseg, false);
+  }
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_field_energy_rows<3>, dim3(nf), dim3(256), 0, s.stream, g, s.d_E, s.d_B,
+                       s.d_J, frows);
+  else if (g.dim == 1)
+    hipLaunchKernelGGL(k_field_energy_rows<1>, dim3(nf), dim3(256), 0, s.stream, g, s.d_E, s.d_B,
+                       s.d_J, frows);
+  else
+    hipLaunchKernelGGL(k_field_energy_rows<2>, dim3(nf), dim3(256), 0, s.stream, g, s.d_E, s.d_B,
+                       s.d_J, frows);
+  hipLaunchKernelGGL(k_particle_energy_rows, dim3(npr), dim3(256), 0, s.stream, g, s.pa, s.d_np,
+                     prows, s.ss.ntile);
+  hipLaunchKernelGGL(k_energy_rows_sum, dim3(1), dim3(64), 0, s.stream, frows, nf, prows, npr,
+                     sums);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, sums, sizeof(double) * (4 + g.Ns), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  return PN_OK;
 }
 
 void launch_chunk_fields_get(Sim& s, int chunk, int which, double* d_out)

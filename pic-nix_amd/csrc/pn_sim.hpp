@@ -149,6 +149,8 @@ struct Sim {
   int         fuse_push   = 0; // FP32 step: the row gather also runs the position push
   int*        d_wq        = nullptr; // work-queue counters, 2 per launch site (kWq*)
   int         ncu         = 0;       // compute units of the device
+  int         device      = 0;       // HIP device the handle was created on
+  double*     energy_scratch = nullptr; // this device's address of the energy scratch array
   struct AutoTune {
     int                 state = 0; // 0 off, 1 measuring, 2 done
     int                 step  = 0; // tuning steps taken
@@ -241,6 +243,9 @@ void launch_rho_continuity(Sim& s, double delt);
 // rho from the current positions; sorted_by_new_cell: cellstart describes them
 void launch_rho(Sim& s, bool sorted_by_new_cell);
 void launch_energy(Sim& s, double* out /* host: efd, bfd, divE, divB, particle[Ns] */);
+// the same sums with no allocation (scratch array of the GPU, see pn_diag.hip)
+void energy_static_init(Sim& s); // once per handle, at creation (device current)
+int  launch_energy_static(Sim& s, double* out /* host, 4 + Ns */);
 // moments (PicChunk::deposit_moment, when `deposit`) and the Mom exchange
 void launch_moment_exchange(Sim& s, bool deposit);
 

@@ -31,7 +31,7 @@ EXPORTS = [
     "pn_set_fields", "pn_push_bfd", "pn_push_efd", "pn_push_velocity", "pn_push_position",
     "pn_deposit_current", "pn_deposit_moment", "pn_init_friedman", "pn_set_step_graph",
     "pn_sort_particle", "pn_exchange", "pn_step",
-    "pn_get_energy",
+    "pn_get_energy", "pn_get_energy_static",
     "pn_get_diverror", "pn_synchronize", "pn_timing", "pn_timing_report",
     "pn_write_field_diag", "pn_write_particle_diag", "pn_write_profile",
     "pn_save_state", "pn_load_state", "pn_get_boundary_state", "pn_autotune",
@@ -97,6 +97,7 @@ def load_library(path=LIB_PATH):
         "pn_exchange": [P, i],
         "pn_step": [P, d],
         "pn_get_energy": [P, dp, dp, dp],
+        "pn_get_energy_static": [P, dp, dp, dp],
         "pn_get_diverror": [P, dp, dp],
         "pn_synchronize": [P],
         "pn_timing": [P, i],
@@ -272,6 +273,13 @@ class Simulation:
         e, b = ctypes.c_double(), ctypes.c_double()
         p = np.zeros(self.Ns, dtype=np.float64)
         self._chk(self.L.pn_get_energy(self.h, ctypes.byref(e), ctypes.byref(b), _dp(p)))
+        return e.value, b.value, p
+
+    def energy_static(self):
+        """pn_get_energy_static: the same sums with no device allocation."""
+        e, b = ctypes.c_double(), ctypes.c_double()
+        p = np.zeros(self.Ns, dtype=np.float64)
+        self._chk(self.L.pn_get_energy_static(self.h, ctypes.byref(e), ctypes.byref(b), _dp(p)))
         return e.value, b.value, p
 
     def diverror(self):
