@@ -124,6 +124,11 @@ def host_cpu():
             "logical_cpus": os.cpu_count()}
 
 
+def progress(msg):
+    """One line on stderr as each part of the run finishes."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(cfg_name, threads, steps):
     """Time the FP64 oracle on the host: `steps` full steps of the same workload."""
     import helpers
@@ -214,12 +219,14 @@ def main():
             text = text.replace("[application.option]",
                                 "[application.option]\n    precision = 'fp32'", 1)
         sim = picnix_amd.Simulation(text)
+        progress(f"{precision} {'batched' if batched else 'plain'}: set up")
         tuning = None
         if args.autotune == "on" and batched:
             sim.autotune(True)
             nt = sim.tuning()["steps_left"]
             sim.step(dt, nt)
             tuning = dict(sim.tuning(), session_steps=nt)
+            progress(f"{precision}: tuning session done ({nt} steps)")
         npart = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
         # the FP32 batched form replays the step as a captured hipGraph; the
         # per-phase HIP events then come from eager steps after the timed ones
@@ -233,6 +240,8 @@ def main():
         sim.step(dt, args.steps)
         sim.synchronize()
         t1 = time.perf_counter()
+        progress(f"{precision} {'batched' if batched else 'plain'}: {args.steps} timed steps, "
+                 f"{(t1 - t0) / args.steps * 1e3:.3f} ms/step")
         npart_end = sum(sim.np(c, s) for c in range(sim.nchunks) for s in range(sim.Ns))
         hist = sim.history()  # after warm-up + timed steps, as for the other form
         if graph:
@@ -333,6 +342,7 @@ def main():
         "fp64_plain": plain,
     }
     if args.cpu_baseline == "on":
+        progress("CPU baseline")
         line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_threads, args.cpu_steps)
     else:
         line["cpu_baseline"] = None
