@@ -320,6 +320,7 @@ void release(Sim& s)
                   (void*)ss.nout, (void*)ss.recvoff, (void*)ss.npnew, (void*)ss.errflag})
     f(p);
   f(s.d_stage);
+  f(s.d_cs_saved);
   if (s.h_flag)
     (void)hipHostFree(s.h_flag);
   if (s.flag_ev)
@@ -453,11 +454,114 @@ void init_fields(Sim& s)
     launch32_fields_from_f64(s, PN_FIELD_UF);
 }
 
+// chunk = -1: every chunk (one omp-for phase of push_openmp); chunk >= 0: that
+// chunk alone, as PicChunk's operators act (pic/pic_chunk.hpp:90-142).
 int check_chunk_arg(Sim& s, int chunk)
 {
-  if (chunk != -1)
-    return fail(s, PN_ERR_UNSUPPORTED, "per-chunk operator calls are not supported; pass chunk = -1");
+  if (chunk == -1)
+    return PN_OK;
+  if (chunk < 0 || chunk >= s.g.nchunk)
+    return fail(s, PN_ERR_ARG, "chunk index out of range (-1 selects every chunk)");
+  if (s.fp32)
+    return fail(s, PN_ERR_UNSUPPORTED,
+                "per-chunk operator calls are built for precision = fp64; pass chunk = -1");
+  if (s.cfg.problem != 0)
+    return fail(s, PN_ERR_UNSUPPORTED,
+                "per-chunk operator calls are built for periodic problems; pass chunk = -1");
   return PN_OK;
+}
+
+// The particle operators on one chunk run the one-thread-per-particle kernels,
+// which stop at each segment's count: the other chunks' counts are set to zero
+// for the call and restored afterwards (the chunk's own count stays as the
+// operator leaves it).
+struct ChunkWindow {
+  Sim&                 s;
+  int                  c;
+  bool                 generic;
+  std::vector<int32_t> saved;
+  ChunkWindow(Sim& s_, int c_) : s(s_), c(c_), generic(s_.generic_kernels)
+  {
+    const Geom& g = s.g;
+    refresh_np(s);
+    saved = s.h_np;
+    std::vector<int32_t> m(g.nseg, 0);
+    for (int sp = 0; sp < g.Ns; sp++)
+      m[c * g.Ns + sp] = saved[c * g.Ns + sp];
+    HIPCHK(hipMemcpyAsync(s.d_np, m.data(), sizeof(int32_t) * g.nseg, hipMemcpyHostToDevice,
+                          s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    s.generic_kernels = true;
+  }
+  void restore()
+  {
+    const Geom&          g = s.g;
+    std::vector<int32_t> now(g.nseg);
+    HIPCHK(hipMemcpyAsync(now.data(), s.d_np, sizeof(int32_t) * g.nseg, hipMemcpyDeviceToHost,
+                          s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    for (int seg = 0; seg < g.nseg; seg++)
+      if (seg / g.Ns != c)
+        now[seg] = saved[seg];
+    HIPCHK(hipMemcpyAsync(s.d_np, now.data(), sizeof(int32_t) * g.nseg, hipMemcpyHostToDevice,
+                          s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    s.h_np          = now;
+    s.np_host_valid = true;
+    s.generic_kernels = generic;
+  }
+};
+
+// PicChunk::sort_particle on one chunk: count and sort its segments with the
+// others' counts masked, then put back the other segments' records (the sort
+// writes the other record set), their bin starts and their counts.
+void sort_one_chunk(Sim& s, int c)
+{
+  const Geom&  g   = s.g;
+  const size_t ncs = (size_t)g.nseg * (g.nbin + 1);
+  if (!s.d_cs_saved)
+    s.d_cs_saved = dalloc<int32_t>(ncs);
+  HIPCHK(hipMemcpyAsync(s.d_cs_saved, s.ss.cellstart, ncs * sizeof(int32_t),
+                        hipMemcpyDeviceToDevice, s.stream));
+  const bool  was_sorted = s.sorted;
+  ChunkWindow win(s, c);
+  launch_sort(s); // records of chunk c -> the other set, which becomes current
+  const size_t cap  = (size_t)g.cap;
+  const size_t lo   = (size_t)c * g.Ns * cap;        // first record of chunk c
+  const size_t hi   = (size_t)(c + 1) * g.Ns * cap;  // one past its last
+  const size_t tot  = (size_t)g.nseg * cap;
+  auto         keep = [&](auto* dst, const auto* src) {
+    using T = std::remove_pointer_t<decltype(dst)>;
+    if (lo > 0)
+      HIPCHK(hipMemcpyAsync(dst, src, lo * sizeof(T), hipMemcpyDeviceToDevice, s.stream));
+    if (hi < tot)
+      HIPCHK(hipMemcpyAsync(dst + hi, src + hi, (tot - hi) * sizeof(T), hipMemcpyDeviceToDevice,
+                            s.stream));
+  };
+  keep(s.pa.x, s.pb.x);
+  keep(s.pa.y, s.pb.y);
+  keep(s.pa.z, s.pb.z);
+  keep(s.pa.ux, s.pb.ux);
+  keep(s.pa.uy, s.pb.uy);
+  keep(s.pa.uz, s.pb.uz);
+  keep(s.pa.id, s.pb.id);
+  const size_t clo = (size_t)c * g.Ns * (g.nbin + 1), chi = (size_t)(c + 1) * g.Ns * (g.nbin + 1);
+  win.restore();
+  // the sort clears every segment's bin starts it does not own: put them back
+  if (clo > 0)
+    HIPCHK(hipMemcpyAsync(s.ss.cellstart, s.d_cs_saved, clo * sizeof(int32_t),
+                          hipMemcpyDeviceToDevice, s.stream));
+  if (chi < ncs)
+    HIPCHK(hipMemcpyAsync(s.ss.cellstart + chi, s.d_cs_saved + chi, (ncs - chi) * sizeof(int32_t),
+                          hipMemcpyDeviceToDevice, s.stream));
+  if (s.chunk_sorted.empty())
+    s.chunk_sorted.assign(g.nchunk, was_sorted ? 1 : 0);
+  s.chunk_sorted[c] = 1;
+  bool all = true;
+  for (char v : s.chunk_sorted)
+    all = all && v;
+  s.sorted  = all;
+  s.counted = false;
 }
 
 int check_seg(Sim& s, int chunk, int species)
@@ -657,6 +761,7 @@ int pn_set_particles(pn_sim* h, int chunk, int species, const double* xu7, int64
       launch32_from_f64(s, seg, 1);
     HIPCHK(hipStreamSynchronize(s.stream));
     s.sorted  = false;
+    s.chunk_sorted.clear();
     s.counted = false;
     return PN_OK;
   });
@@ -786,7 +891,7 @@ int pn_push_bfd(pn_sim* h, int chunk, double delt)
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
-    launch_push_bfd(s, delt);
+    launch_push_bfd(s, delt, chunk);
     return PN_OK;
   });
 }
@@ -797,7 +902,10 @@ int pn_push_efd(pn_sim* h, int chunk, double delt)
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
-    launch_push_efd(s, delt);
+    if (chunk >= 0 && s.cfg.friedman != 0.0)
+      return fail(s, PN_ERR_UNSUPPORTED,
+                  "per-chunk push_efd with the Friedman filter is not built; pass chunk = -1");
+    launch_push_efd(s, delt, chunk);
     return PN_OK;
   });
 }
@@ -808,10 +916,15 @@ int pn_push_velocity(pn_sim* h, int chunk, double delt)
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
-    if (s.fp32)
-      launch32_push_velocity(s, delt);
-    else
+    if (chunk >= 0) {
+      ChunkWindow win(s, chunk);
       launch_push_velocity(s, delt);
+      win.restore();
+    } else if (s.fp32) {
+      launch32_push_velocity(s, delt);
+    } else {
+      launch_push_velocity(s, delt);
+    }
     return PN_OK;
   });
 }
@@ -827,7 +940,7 @@ int pn_push_position(pn_sim* h, int chunk, double delt)
     if (s.fp32)
       launch32_push_position(s, delt);
     else
-      launch_push_position(s, delt);
+      launch_push_position(s, delt, chunk);
     if (s.g.xwall)
       shock_position_boundary(s);
     // the particle order is unchanged, so cellstart still describes the old
@@ -844,10 +957,30 @@ int pn_deposit_current(pn_sim* h, int chunk, double delt)
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
-    if (s.fp32)
-      launch32_deposit(s, delt, RhoMode::Now);
-    else
+    if (chunk >= 0) {
+      // The reference clears each chunk's uj (ghosts included) and the Cur
+      // exchange adds the ghosts into the neighbours.  Here every chunk adds
+      // into one global J, so J is cleared once per round of per-chunk
+      // deposits: at the first call, or when this chunk has already deposited
+      // since the last clear.
+      const Geom& g = s.g;
+      if (s.dep_round.empty() || s.dep_round[chunk]) {
+        launch_zero(s, s.d_J, 4 * g.vol);
+        s.dep_round.assign(g.nchunk, 0);
+      }
+      s.dep_round[chunk] = 1;
+      ChunkWindow win(s, chunk);
+      s.dep_accumulate = true;
       launch_deposit(s, delt);
+      s.dep_accumulate = false;
+      win.restore();
+    } else if (s.fp32) {
+      s.dep_round.clear();
+      launch32_deposit(s, delt, RhoMode::Now);
+    } else {
+      s.dep_round.clear();
+      launch_deposit(s, delt);
+    }
     return PN_OK;
   });
 }
@@ -863,9 +996,8 @@ int pn_init_friedman(pn_sim* h)
 int pn_deposit_moment(pn_sim* h, int chunk)
 {
   return guarded(h, [&](Sim& s) {
-    int rc = check_chunk_arg(s, chunk);
-    if (rc)
-      return rc;
+    if (chunk != -1)
+      return fail(s, PN_ERR_UNSUPPORTED, "the moment deposit runs on every chunk; pass chunk = -1");
     if (s.fp32)
       return fail(s, PN_ERR_UNSUPPORTED, "moments are not built for precision = fp32");
     if (!s.d_M)
@@ -882,10 +1014,15 @@ int pn_sort_particle(pn_sim* h, int chunk)
     int rc = check_chunk_arg(s, chunk);
     if (rc)
       return rc;
-    if (s.fp32)
-      launch32_sort(s);
-    else
-      launch_sort(s);
+    if (chunk >= 0) {
+      sort_one_chunk(s, chunk);
+    } else {
+      s.chunk_sorted.clear();
+      if (s.fp32)
+        launch32_sort(s);
+      else
+        launch_sort(s);
+    }
     return PN_OK;
   });
 }
@@ -906,6 +1043,7 @@ int pn_exchange(pn_sim* h, int mode)
       return PN_OK;
     case PN_BOUNDARY_CUR:
       s.rho_valid = false;
+      s.dep_round.clear(); // the fold ends a round of per-chunk deposits
       launch_fold_current(s);
       return PN_OK;
     case PN_BOUNDARY_PARTICLE:
@@ -914,6 +1052,7 @@ int pn_exchange(pn_sim* h, int mode)
         return fail(s, PN_ERR_STATE, "particle exchange needs the count of pn_push_position");
       if (s.g.xwall)
         shock_inject(s); // set_boundary_pack(BoundaryParticle), pic/pic_chunk.cpp:305-309
+      s.chunk_sorted.clear();
       if (s.fp32)
         launch32_migrate(s);
       else
@@ -1090,6 +1229,8 @@ int pn_step(pn_sim* h, double delt)
     int rc = step_flags_seen(s);
     if (rc)
       return rc;
+    s.dep_round.clear();
+    s.chunk_sorted.clear();
     rc = step_impl(s, delt);
     if (rc == PN_OK)
       queue_step_flags(s);

@@ -1306,7 +1306,7 @@ RhoDone launch_deposit(Sim& s, double delt, RhoMode mode)
   dc.dzdt = g.del[0] / delt;
   // the 3D march kernel deposits J only; the others deposit rho with J
   const bool keep = mode == RhoMode::Keep && deposit_marches(s);
-  {
+  if (!s.dep_accumulate) {
     PhaseTimer pz(s, 4);
     if (keep)
       launch_zero(s, s.d_J + g.vol, 3 * g.vol);

@@ -267,10 +267,27 @@ void launch_copy(Sim& s, int32_t* dst, const int32_t* src, long n)
 
 namespace {
 
-template <typename T>
-void push_bfd_impl(Sim& s, const T* E, T* B, T* ff, double delt)
+// The interior cells the field pushes update: the whole domain (chunk < 0), or
+// one chunk's interior (PicChunk::push_bfd / push_efd on one chunk).  The
+// kernels walk g.N cells from g.pad, so a chunk is a Geom whose N is the chunk's
+// extent and whose pad is shifted by the chunk's cell offset.
+Geom update_box(const Sim& s, int chunk)
 {
-  const Geom& g    = s.g;
+  Geom b = s.g;
+  if (chunk < 0)
+    return b;
+  for (int d = 0; d < 3; d++)
+    if (b.has[d]) {
+      b.N[d] = b.dims[d];
+      b.pad[d] += s.offset[chunk * 3 + d];
+    }
+  return b;
+}
+
+template <typename T>
+void push_bfd_impl(Sim& s, const T* E, T* B, T* ff, double delt, int chunk)
+{
+  const Geom  g    = update_box(s, chunk);
   const T     cflx = (T)(g.cc * delt / g.del[2]);
   const T     cfly = (T)(g.cc * delt / g.del[1]);
   const T     cflz = (T)(g.cc * delt / g.del[0]);
@@ -281,8 +298,8 @@ void push_bfd_impl(Sim& s, const T* E, T* B, T* ff, double delt)
     const double A  = 1 + 0.5 * th;
     const double B  = -th * (1 - 0.5 * th);
     const double C  = 0.5 * th * (1 - th) * (1 - th);
-    hipLaunchKernelGGL(k_friedman_combine<T>, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g,
-                       E, ff, (T)A, (T)B, (T)C);
+    hipLaunchKernelGGL(k_friedman_combine<T>, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream,
+                       s.g, E, ff, (T)A, (T)B, (T)C);
     F = ff;
   }
   if (g.dim == 3)
@@ -294,20 +311,20 @@ void push_bfd_impl(Sim& s, const T* E, T* B, T* ff, double delt)
   else
     hipLaunchKernelGGL((k_push_bfd<2, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, F, B,
                        cflx, cfly, cflz);
-  hipLaunchKernelGGL(k_fill_ghost<T>, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g, B, 3);
+  hipLaunchKernelGGL(k_fill_ghost<T>, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream, s.g, B, 3);
 }
 
 template <typename T>
-void push_efd_impl(Sim& s, T* E, const T* B, const T* J, T* ff, double delt)
+void push_efd_impl(Sim& s, T* E, const T* B, const T* J, T* ff, double delt, int chunk)
 {
-  const Geom& g    = s.g;
+  const Geom  g    = update_box(s, chunk);
   const T     cflx = (T)(g.cc * delt / g.del[2]);
   const T     cfly = (T)(g.cc * delt / g.del[1]);
   const T     cflz = (T)(g.cc * delt / g.del[0]);
   long        n    = (long)g.N[0] * g.N[1] * g.N[2];
   if (ff)
-    hipLaunchKernelGGL(k_friedman_shift<T>, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g, E,
-                       ff, (T)s.cfg.friedman);
+    hipLaunchKernelGGL(k_friedman_shift<T>, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream,
+                       s.g, E, ff, (T)s.cfg.friedman);
   if (g.dim == 3)
     hipLaunchKernelGGL((k_push_efd<3, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, E, B, J,
                        cflx, cfly, cflz, (T)delt);
@@ -317,7 +334,7 @@ void push_efd_impl(Sim& s, T* E, const T* B, const T* J, T* ff, double delt)
   else
     hipLaunchKernelGGL((k_push_efd<2, T>), dim3(nblk(n, 256)), dim3(256), 0, s.stream, g, E, B, J,
                        cflx, cfly, cflz, (T)delt);
-  hipLaunchKernelGGL(k_fill_ghost<T>, dim3(nblk(g.vol, 256)), dim3(256), 0, s.stream, g, E, 3);
+  hipLaunchKernelGGL(k_fill_ghost<T>, dim3(nblk(s.g.vol, 256)), dim3(256), 0, s.stream, s.g, E, 3);
 }
 
 // rho^{n+1} from the continuity equation that the Esirkepov current satisfies
@@ -382,22 +399,22 @@ void fold_impl(Sim& s, T* F, int k0, int k1)
 
 } // namespace
 
-void launch_push_bfd(Sim& s, double delt)
+void launch_push_bfd(Sim& s, double delt, int chunk)
 {
   PhaseTimer pt(s, 0);
   if (s.fp32)
-    push_bfd_impl<float>(s, s.f_E, s.f_B, s.f_ff, delt);
+    push_bfd_impl<float>(s, s.f_E, s.f_B, s.f_ff, delt, chunk);
   else
-    push_bfd_impl<double>(s, s.d_E, s.d_B, s.d_ff, delt);
+    push_bfd_impl<double>(s, s.d_E, s.d_B, s.d_ff, delt, chunk);
 }
 
-void launch_push_efd(Sim& s, double delt)
+void launch_push_efd(Sim& s, double delt, int chunk)
 {
   PhaseTimer pt(s, 5);
   if (s.fp32)
-    push_efd_impl<float>(s, s.f_E, s.f_B, s.f_J, s.f_ff, delt);
+    push_efd_impl<float>(s, s.f_E, s.f_B, s.f_J, s.f_ff, delt, chunk);
   else
-    push_efd_impl<double>(s, s.d_E, s.d_B, s.d_J, s.d_ff, delt);
+    push_efd_impl<double>(s, s.d_E, s.d_B, s.d_J, s.d_ff, delt, chunk);
 }
 
 void launch_init_friedman(Sim& s)

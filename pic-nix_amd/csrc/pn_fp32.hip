@@ -13,8 +13,8 @@
 //     cell changes its cell index, and one that leaves its chunk keeps r and
 //     gets the neighbour's cell index (the periodic wrap is exact);
 //   * per-cell current sums run in FP32 registers and the block's ring tile
-//     accumulates in FP64 LDS before the FP32 global add (compensated tile
-//     accumulation: each node's tile value is exact to FP64 rounding).
+//     accumulates them as compensated FP32 pairs in LDS (hi + lo, every add's
+//     rounding error kept in lo) before the FP32 global add.
 // FP64 stays for the chunk origins (positions are rebuilt as
 // xmin + (cell + r) dx in FP64 at the C-ABI boundary) and for the energy and
 // charge reductions, which read FP64 mirrors of the FP32 fields.
@@ -809,8 +809,24 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Compensated FP32 accumulation into an LDS pair (hi, lo).  The atomic add on
+// hi returns the value it replaced; the lane forms old + x again with the same
+// IEEE operation the LDS unit applied, and the rounding error of that addition
+// (Knuth's TwoSum, exact in round-to-nearest) goes to lo.  Concurrent adds keep
+// this exact: each one captures the error of the one addition it made.  The
+// node's value is hi + lo (formed in FP64 at the flush).
+__device__ __forceinline__ void tile_add2(float* hi, float* lo, float x)
+{
+  const float old = atomicAdd(hi, x);
+  const float s   = old + x;
+  const float bb  = s - old;
+  const float e   = (old - (s - bb)) + (x - bb);
+  if (e != 0.0f)
+    atomicAdd(lo, e);
+}
+
 // Plane-lane march kernel (3D, even orders): the FP64 k_deposit_march in FP32
-// registers, with the block's ring tile in FP64 LDS.  All particles of a bin
+// registers, with the block's ring tile as compensated FP32 pairs in LDS.  All particles of a bin
 // share the tile origin (even orders: the old cell), so no particle needs the
 // own-origin path.
 template <int Order>
@@ -853,7 +869,7 @@ __global__ void __launch_bounds__(256)
 
   __shared__ float2 wts[NWAVE][MP::WPAIR];
   __shared__ int    next_cell;
-  extern __shared__ double tile[]; // [3][N z-ring][N][TX] FP64, then rcs
+  extern __shared__ float tile[]; // compensated pairs: hi [3][N z-ring][N][TX], lo (same), then rcs
 
   // Queue: persistent blocks take (row group, chunk) items from the work queue
   // wq until it is drained; otherwise the item is the block's own index.
@@ -894,9 +910,11 @@ __global__ void __launch_bounds__(256)
   const int TX  = nbx + N - 1;
   const int psz = N * TX;
   const int tsz = N * psz;
-  int*      rcs = reinterpret_cast<int*>(tile + 3 * tsz);
+  float*    thi = tile;
+  float*    tlo = tile + 3 * tsz;
+  int*      rcs = reinterpret_cast<int*>(tile + 2 * 3 * tsz);
   for (int q = threadIdx.x; q < 3 * tsz; q += blockDim.x)
-    tile[q] = 0.0;
+    thi[q] = tlo[q] = 0.0f;
   auto load_row = [&](int bz) {
     for (int q = threadIdx.x; q < ns * (nbx + 1); q += blockDim.x) {
       const int sp = q / (nbx + 1);
@@ -1046,7 +1064,10 @@ __global__ void __launch_bounds__(256)
             const int oy  = comp == 0 ? t : (comp == 1 ? j + 1 : r);
             const int oxx = comp == 0 ? j + 1 : (comp == 1 ? r : t);
             const int sl  = zs + oz >= N ? zs + oz - N : zs + oz;
-            atomicAdd(&tile[comp * tsz + sl * psz + oy * TX + bx + oxx], (double)acc[r][j]);
+            {
+              const int ti = comp * tsz + sl * psz + oy * TX + bx + oxx;
+              tile_add2(&thi[ti], &tlo[ti], acc[r][j]);
+            }
           }
       }
       have = more;
@@ -1060,7 +1081,7 @@ __global__ void __launch_bounds__(256)
       const int    kc  = q / psz;
       const int    r2  = q - kc * psz;
       const int    idx = kc * tsz + zs * psz + r2;
-      const double val = tile[idx];
+      const double val = (double)thi[idx] + (double)tlo[idx];
       if (val != 0.0) {
         const int ty = r2 / TX;
         const int tx = r2 - ty * TX;
@@ -1069,7 +1090,7 @@ __global__ void __launch_bounds__(256)
         if ((unsigned)gz < (unsigned)g.P[0] && (unsigned)gy < (unsigned)g.P[1] &&
             (unsigned)gx < (unsigned)g.P[2])
           atomicAdd(&J[(kc + 1) * v + ((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
-        tile[idx] = 0.0;
+        thi[idx] = tlo[idx] = 0.0f;
       }
     }
     if (bz + 1 < bz1)
@@ -1086,7 +1107,7 @@ __global__ void __launch_bounds__(256)
     const int    dz  = 1 + r / psz;
     const int    r2  = r - (dz - 1) * psz;
     const int    sl  = zs + dz >= N ? zs + dz - N : zs + dz;
-    const double val = tile[kc * tsz + sl * psz + r2];
+    const double val = (double)thi[kc * tsz + sl * psz + r2] + (double)tlo[kc * tsz + sl * psz + r2];
     if (val == 0.0)
       continue;
     const int ty = r2 / TX;
@@ -1158,7 +1179,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // rho after the sort (3D, even orders): the FP64 k_rho_march walk in FP32
-// registers with an FP64 LDS ring tile.  Sorted FP32 particles carry
+// registers with a compensated FP32 LDS ring tile.  Sorted FP32 particles carry
 // r in [0, 1) and sit in their cell's bin, so every particle of a bin has the
 // bin's stencil origin.
 template <int Order>
@@ -1184,7 +1205,7 @@ __global__ void __launch_bounds__(256)
 
   __shared__ float wts[NWAVE][BATCH][3 * S];
   __shared__ int   next_cell;
-  extern __shared__ double tile[]; // [S z-ring][S][TX] FP64, then rcs
+  extern __shared__ float tile[]; // compensated pairs: hi [S z-ring][S][TX], lo (same), then rcs
 
   const int c   = blockIdx.y;
   const int ns  = g.Ns;
@@ -1212,9 +1233,11 @@ __global__ void __launch_bounds__(256)
   const int TX  = nbx + Order;
   const int psz = S * TX;
   const int tsz = S * psz;
-  int*      rcs = reinterpret_cast<int*>(tile + tsz);
+  float*    thi = tile;
+  float*    tlo = tile + tsz;
+  int*      rcs = reinterpret_cast<int*>(tile + 2 * tsz);
   for (int q = threadIdx.x; q < tsz; q += blockDim.x)
-    tile[q] = 0.0;
+    thi[q] = tlo[q] = 0.0f;
   auto load_row = [&](int bz) {
     for (int q = threadIdx.x; q < ns * (nbx + 1); q += blockDim.x) {
       const int sp = q / (nbx + 1);
@@ -1336,7 +1359,7 @@ __global__ void __launch_bounds__(256)
         for (int jy = 0; jy < S; jy++)
 #pragma unroll
           for (int jx = 0; jx < S; jx++)
-            atomicAdd(&tile[sl * psz + jy * TX + bx + jx], (double)acc[jy][jx]);
+            tile_add2(&thi[sl * psz + jy * TX + bx + jx], &tlo[sl * psz + jy * TX + bx + jx], acc[jy][jx]);
       }
       have = more;
       sp   = spn;
@@ -1347,7 +1370,7 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     for (int q = threadIdx.x; q < psz; q += blockDim.x) {
       const int    idx = zs * psz + q;
-      const double val = tile[idx];
+      const double val = (double)thi[idx] + (double)tlo[idx];
       if (val != 0.0) {
         const int ty = q / TX;
         const int tx = q - ty * TX;
@@ -1356,7 +1379,7 @@ __global__ void __launch_bounds__(256)
         if ((unsigned)gz < (unsigned)g.P[0] && (unsigned)gy < (unsigned)g.P[1] &&
             (unsigned)gx < (unsigned)g.P[2])
           atomicAdd(&J[((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
-        tile[idx] = 0.0;
+        thi[idx] = tlo[idx] = 0.0f;
       }
     }
     if (bz + 1 < bz1)
@@ -1371,7 +1394,7 @@ __global__ void __launch_bounds__(256)
     const int    dz  = 1 + q / psz;
     const int    r2  = q - (dz - 1) * psz;
     const int    sl  = zs + dz >= S ? zs + dz - S : zs + dz;
-    const double val = tile[sl * psz + r2];
+    const double val = (double)thi[sl * psz + r2] + (double)tlo[sl * psz + r2];
     if (val == 0.0)
       continue;
     const int ty = r2 / TX;

@@ -440,13 +440,14 @@ __global__ void __launch_bounds__(256)
                     double* __restrict__ zo, const int32_t* __restrict__ np,
                     const double* __restrict__ lim, const double* __restrict__ rng, double delt,
                     double rc, int32_t* __restrict__ cellcnt, int32_t* __restrict__ key,
-                    int32_t* __restrict__ slot, int32_t* __restrict__ tilecnt, int ntile)
+                    int32_t* __restrict__ slot, int32_t* __restrict__ tilecnt, int ntile,
+                    int seg0)
 {
   __shared__ int cnt[27];
   if (threadIdx.x < 27)
     cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int  seg    = blockIdx.y;
+  const int  seg    = seg0 + blockIdx.y;
   const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = ip < np[seg];
   const int  c      = seg / g.Ns;
@@ -641,15 +642,19 @@ void launch_push_velocity(Sim& s, double delt)
 #undef PN_VEL_ROWS
 }
 
-void launch_push_position(Sim& s, double delt)
+// chunk >= 0: the segments of that chunk only (their bin counters, keys, slots
+// and tile counts; the other segments' keep what their own push left)
+void launch_push_position(Sim& s, double delt, int chunk)
 {
   PhaseTimer  pt(s, 2);
-  const Geom& g = s.g;
-  launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
-  dim3 grid(s.ss.ntile, g.nseg);
+  const Geom& g    = s.g;
+  const int   seg0 = chunk < 0 ? 0 : chunk * g.Ns;
+  const int   nsg  = chunk < 0 ? g.nseg : g.Ns;
+  launch_zero(s, s.ss.cellcnt + (long)seg0 * g.nbin, (long)nsg * g.nbin);
+  dim3 grid(s.ss.ntile, nsg);
   hipLaunchKernelGGL(k_push_position, grid, dim3(256), 0, s.stream, g, s.pa, s.d_xo[0],
                      s.d_xo[1], s.d_xo[2], s.d_np, s.d_lim, s.d_rng, delt, 1 / g.cc,
-                     s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.tilecnt, s.ss.ntile);
+                     s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.tilecnt, s.ss.ntile, seg0);
   s.counted = true;
   s.sorted  = false;
 }

@@ -137,6 +137,15 @@ struct Sim {
   // instead of carrying it by continuity (an independent charge check)
   bool        rho_deposit = false;
   bool        generic_kernels = false; // PICNIX_KERNELS=generic: one-thread-per-particle forms
+  // per-chunk operator calls (chunk >= 0): chunks that deposited since J was
+  // last cleared (the first deposit of a new round clears it; the Cur
+  // exchange and whole-domain deposits end a round), the deposit's
+  // "accumulate into J" switch, which chunks a per-chunk sort has sorted,
+  // and device scratch for the saved counts and bin starts
+  std::vector<char> dep_round;
+  bool              dep_accumulate = false;
+  std::vector<char> chunk_sorted;
+  int32_t*          d_cs_saved     = nullptr;
   int         march_rows      = 0;     // PICNIX_MARCH_ROWS: bin rows per march block (0: auto)
   // launch shapes picked by the autotuner (pn_autotune); 0 = the heuristic
   int         dep_rows    = 0; // bin rows per block of the current march kernels
@@ -204,8 +213,8 @@ struct PhaseTimer {
 void march_split(const Geom& g, int fixed, int& zrows, int& nzr);
 
 // kernels / launchers (pn_field.hip, pn_particle.hip, pn_sort.hip, pn_deposit.hip, pn_diag.hip)
-void launch_push_bfd(Sim& s, double delt);
-void launch_push_efd(Sim& s, double delt);
+void launch_push_bfd(Sim& s, double delt, int chunk = -1);
+void launch_push_efd(Sim& s, double delt, int chunk = -1);
 void launch_fill_ghost(Sim& s, double* f, int ncomp);
 void launch_fill_ghost(Sim& s, float* f, int ncomp);
 // zero fills and copies as kernels (no memset/memcpy nodes under graph capture)
@@ -226,7 +235,7 @@ void launch_fold_current(Sim& s, int k0 = 0, int k1 = 4);
 // moment deposit into d_M (allocated on first use), without the Mom exchange
 void launch_deposit_moment(Sim& s);
 void launch_push_velocity(Sim& s, double delt);
-void launch_push_position(Sim& s, double delt);
+void launch_push_position(Sim& s, double delt, int chunk = -1);
 void launch_count(Sim& s);
 void launch_migrate(Sim& s);
 void launch_sort(Sim& s);

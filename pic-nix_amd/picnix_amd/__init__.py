@@ -6,7 +6,8 @@ can silently fall back to a CPU implementation.
 
 Method names mirror the reference's PicChunk operator surface
 (pic/pic_chunk.hpp:90-142); each call acts on every chunk, as one
-``omp for`` phase of PicApplication::push_openmp does.
+``omp for`` phase of PicApplication::push_openmp does, or on the one chunk
+its ``chunk`` argument names, as a PicChunk method does.
 """
 import ctypes
 import os
@@ -219,20 +220,20 @@ class Simulation:
         self._chk(self.L.pn_set_fields(self.h, c, which, _dp(arr)))
 
     # ---- operators (PicChunk names) ----
-    def push_bfd(self, dt):
-        self._chk(self.L.pn_push_bfd(self.h, -1, dt))
+    def push_bfd(self, dt, chunk=-1):
+        self._chk(self.L.pn_push_bfd(self.h, chunk, dt))
 
-    def push_efd(self, dt):
-        self._chk(self.L.pn_push_efd(self.h, -1, dt))
+    def push_efd(self, dt, chunk=-1):
+        self._chk(self.L.pn_push_efd(self.h, chunk, dt))
 
-    def push_velocity(self, dt):
-        self._chk(self.L.pn_push_velocity(self.h, -1, dt))
+    def push_velocity(self, dt, chunk=-1):
+        self._chk(self.L.pn_push_velocity(self.h, chunk, dt))
 
-    def push_position(self, dt):
-        self._chk(self.L.pn_push_position(self.h, -1, dt))
+    def push_position(self, dt, chunk=-1):
+        self._chk(self.L.pn_push_position(self.h, chunk, dt))
 
-    def deposit_current(self, dt):
-        self._chk(self.L.pn_deposit_current(self.h, -1, dt))
+    def deposit_current(self, dt, chunk=-1):
+        self._chk(self.L.pn_deposit_current(self.h, chunk, dt))
 
     def deposit_moment(self):
         """PicChunk::deposit_moment; follow with exchange(BOUNDARY_MOM)."""
@@ -245,8 +246,8 @@ class Simulation:
     def init_friedman(self):
         self._chk(self.L.pn_init_friedman(self.h))
 
-    def sort_particle(self):
-        self._chk(self.L.pn_sort_particle(self.h, -1))
+    def sort_particle(self, chunk=-1):
+        self._chk(self.L.pn_sort_particle(self.h, chunk))
 
     def exchange(self, mode):
         self._chk(self.L.pn_exchange(self.h, mode))

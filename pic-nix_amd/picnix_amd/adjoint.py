@@ -2,13 +2,16 @@
 
     python -m picnix_amd.adjoint --check [--steps 200 --every 20 --nx 256 --nppc 32]
 
-`AdjointForm` drives the C ABI; `WeibelLoss` is a torch.autograd.Function whose
-forward runs the steps (checkpointing every K) and whose backward runs the
+`AdjointForm` drives the C ABI; `weibel_loss` goes through the libtorch
+autograd function (torch::autograd::Function in pic-nix_amd/torch_ext/pn_torch.cpp)
+whose forward runs the steps (checkpointing every K) and whose backward runs the
 hand-written adjoint kernels, so that
 
     theta = torch.tensor([6.0] + [1e-3] * 8, dtype=torch.float64, requires_grad=True)
-    loss = WeibelLoss.apply(theta, form, 200, 20)
+    loss = weibel_loss(theta, form, 200, 20)
     loss.backward()          # theta.grad = dL/dtheta from the adjoint
+
+`WeibelLoss` is the same loss as a Python torch.autograd.Function.
 
 The default set-up is the 2D Weibel case: 256 x 256 periodic cells, 32
 electrons per cell with vth_y = vth_z = 0.05c and vth_x = theta[0] * 0.05c over
@@ -81,6 +84,7 @@ class AdjointForm:
         self._chk(rc)
         self.nmodes = nmodes
         self.nx, self.ny = nx, ny
+        self.tape = 0
 
     def _chk(self, rc):
         if rc != 0:
@@ -105,6 +109,7 @@ class AdjointForm:
         loss = ctypes.c_double()
         self._chk(self.L.pn_adj_forward(self.h, _dp(th), int(nsteps), int(every),
                                         ctypes.byref(loss)))
+        self.tape += 1  # the checkpoints pn_adj_backward differentiates
         return loss.value
 
     def backward(self):
@@ -135,22 +140,36 @@ class AdjointForm:
         self._chk(self.L.pn_adj_step(self.h, int(n)))
 
 
+def weibel_loss(theta, form, nsteps, every):
+    """L(theta) through the libtorch autograd function of pic-nix_amd/torch_ext
+    (picnix_torch*.so, a torch::autograd::Function in C++): forward runs the
+    steps with a checkpoint every `every`, backward the adjoint kernels; the
+    gradient comes back on theta's device in theta's dtype."""
+    import picnix_torch
+    return picnix_torch.weibel_loss(theta, int(form.h.value), int(nsteps), int(every))
+
+
 try:
     import torch
 
     class WeibelLoss(torch.autograd.Function):
-        """L(theta) = 1/2 sum B^2 after `nsteps` steps; backward = the adjoint kernels."""
+        """The same loss as a Python torch.autograd.Function over the C ABI."""
 
         @staticmethod
         def forward(ctx, theta, form, nsteps, every):
             ctx.form = form
             loss = form.forward(theta.detach().cpu().numpy(), nsteps, every)
-            return torch.tensor(loss, dtype=torch.float64)
+            ctx.tape = form.tape
+            ctx.device, ctx.dtype = theta.device, theta.dtype
+            return torch.tensor(loss, dtype=theta.dtype, device=theta.device)
 
         @staticmethod
         def backward(ctx, grad_out):
-            g = ctx.form.backward()
-            return torch.from_numpy(g) * grad_out, None, None, None
+            if ctx.form.tape != ctx.tape:
+                raise RuntimeError("another forward ran on this adjoint form after the one "
+                                   "being differentiated; backward needs its own checkpoints")
+            g = torch.from_numpy(ctx.form.backward()).to(ctx.device, ctx.dtype)
+            return g * grad_out, None, None, None
 except ImportError:  # torch is optional for the C ABI itself
     WeibelLoss = None
 

@@ -368,7 +368,7 @@ def test_adjoint_weibel_full_size():
     anisotropy ratio and 8 density-mode amplitudes of 1e-3), 40 steps with a
     checkpoint every 20: every gradient component within 1e-3 of the CPU
     tangent-linear reference.  (The 200-step run is the same check with
-    PICNIX_FULL=1 in test_gpu_adjoint.py.)"""
+    test_gpu_adjoint.py.)"""
     from test_gpu_adjoint import _tangent_check
     _say("adjoint 256^2: forward, backward, tangent-linear")
     rows = _tangent_check(256, 32, 8, 40, 20, 12345, [6.0] + [1e-3] * 8)

@@ -336,3 +336,115 @@ def test_rho_carried_by_continuity(kind):
     _say(f"[ops] {kind} rho continuity: max |carried - direct| {worst:.3e}, max |rho| {scale:.3e}")
     assert worst <= TOL * scale
     g.close()
+
+
+@pytest.mark.parametrize("kind", ["thermal3d_o2", "thermal3d_o1", "twostream2d_o1"])
+def test_operator_sequence_per_chunk(kind):
+    """The step driven one chunk at a time, as the reference's PicChunk methods
+    are called inside push_openmp's omp-for loops (pic/pic_application.cpp:
+    219-292), against the oracle's whole-domain operators: field pushes on each
+    chunk's interior, the particle operators on each chunk's segments (the
+    others untouched), the deposits of all chunks summed once the Cur exchange
+    folds them, and a per-chunk sort.  Two steps; the bars of the whole-domain
+    sequence."""
+    dim, make = CASES[kind]
+    cfg = make()
+    dt = float(cfg["parameter"]["delt"])
+    _say(f"[ops] {kind} per chunk: create")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    chunks = list(range(o.nchunks))
+    # a chunk order that differs from the chunk ids
+    order = chunks[1::2] + chunks[0::2]
+    for it in range(2):
+        tag = f"{kind} per chunk step {it + 1}"
+        exact = it == 0
+        o.push_bfd(0.5 * dt)
+        for c in order:
+            g.push_bfd(0.5 * dt, c)
+        g.exchange(EMF)
+        rel = _fields_rel(o, g, 0, dim, 6)
+        assert np.all(rel == 0.0) if exact else np.all(rel <= TOL), rel
+        o.push_velocity(dt)
+        for c in order:
+            g.push_velocity(dt, c)
+        _particles_equal(o, g, f"{tag} push_velocity", exact)
+        o.push_position(dt)
+        for c in order:
+            g.push_position(dt, c)
+        _particles_equal(o, g, f"{tag} push_position", exact)
+        _gindex_equal(o, g, f"{tag} count")
+        o.deposit_current(dt)
+        for c in order:
+            g.deposit_current(dt, c)
+        o.push_bfd(0.5 * dt)
+        for c in order:
+            g.push_bfd(0.5 * dt, c)
+        o.exchange(CUR)
+        g.exchange(CUR)
+        rel = _fields_rel(o, g, 1, dim, 4)
+        _say(f"[ops] {tag}: per-chunk deposits + Cur exchange rel {rel}")
+        assert np.all(rel <= TOL), rel
+        o.push_efd(dt)
+        for c in order:
+            g.push_efd(dt, c)
+        o.exchange(EMF)
+        g.exchange(EMF)
+        rel = _fields_rel(o, g, 0, dim, 6)
+        assert np.all(rel <= TOL), rel
+        o.exchange(PARTICLE)
+        g.exchange(PARTICLE)
+        _particles_equal(o, g, f"{tag} particle exchange", exact)
+        _pindex_equal(o, g, f"{tag} particle exchange")
+    # a per-chunk sort: each chunk's particles end in the reference order, the
+    # other chunks untouched; particles placed outside the chunk are dropped
+    rng = np.random.default_rng(3)
+    for c in order:
+        for s in range(o.Ns):
+            if c in order[:2]:
+                xu = _placed_particles(o, c, s, 50, rng, outside=3, edges=2)
+            else:
+                xu = o.particles(c, s)  # the same records on both sides
+            o.set_particles(c, s, xu)
+            g.set_particles(c, s, xu)
+    o.sort_particle()
+    before = {(c, s): g.particles(c, s) for c in order[1:] for s in range(o.Ns)}
+    g.sort_particle(order[0])
+    for (c, s), a in before.items():
+        assert np.array_equal(a, g.particles(c, s)), (c, s)
+    for c in order[1:]:
+        g.sort_particle(c)
+    _particles_equal(o, g, f"{kind} per-chunk sort", True)
+    _pindex_equal(o, g, f"{kind} per-chunk sort")
+    g.close()
+    o.close()
+
+
+def test_per_chunk_errors():
+    """Chunk indices out of range, the FP32 form and the moment deposit reject a
+    per-chunk call with a clear error."""
+    import picnix_amd
+    dim, make = CASES["thermal3d_o2"]
+    cfg = make()
+    g = _sim(cfg)
+    try:
+        with pytest.raises(picnix_amd.PicnixError) as e:
+            g.push_bfd(0.25, g.nchunks)
+        assert e.value.code == -1
+        with pytest.raises(picnix_amd.PicnixError) as e:
+            g.push_velocity(0.25, -2)
+        assert e.value.code == -1
+        with pytest.raises(picnix_amd.PicnixError) as e:
+            g._chk(g.L.pn_deposit_moment(g.h, 0))
+        assert e.value.code == -5
+    finally:
+        g.close()
+    cfg32 = make()
+    cfg32["application"]["option"]["precision"] = "fp32"
+    g = _sim(cfg32)
+    try:
+        with pytest.raises(picnix_amd.PicnixError) as e:
+            g.push_velocity(0.25, 0)
+        assert e.value.code == -5
+    finally:
+        g.close()
