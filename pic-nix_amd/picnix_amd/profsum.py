@@ -2,6 +2,7 @@
 
     python -m picnix_amd.profsum trace <dir>/..._results.db
     python -m picnix_amd.profsum pmc [--probe-mb=M] FETCH_SIZE <fetch.db> [WRITE_SIZE <write.db>]
+    python -m picnix_amd.profsum sq <sq.db> [kernel-prefix ...]
 
 `trace` prints calls, total and average duration per kernel (from a
 `--kernel-trace --stats` run), then, for kernels launched with several grids
@@ -84,6 +85,30 @@ def pmc(pairs, probe_mb=None):
     return "\n".join(out)
 
 
+def sq(db, prefixes=()):
+    """Per-dispatch averages of every counter of a multi-counter --pmc run, one row
+    per kernel (the kernels whose names start with one of `prefixes`, or all)."""
+    c = sqlite3.connect(db)
+    rows = c.execute("select kernel_name, counter_name, count(*), avg(value) from "
+                     "counters_collection group by kernel_name, counter_name").fetchall()
+    table, counters = {}, []
+    for name, counter, n, avg in rows:
+        short = _short(name, 40)
+        if prefixes and not any(short.startswith(p) for p in prefixes):
+            continue
+        table.setdefault(short, {})[counter] = (n, avg)
+        if counter not in counters:
+            counters.append(counter)
+    counters.sort()
+    out = [f"{'kernel':<40} {'n':>4} " + " ".join(f"{k:>22}" for k in counters)]
+    for name in sorted(table):
+        vals = table[name]
+        n = max(v[0] for v in vals.values())
+        out.append(f"{name:<40} {n:>4} " +
+                   " ".join(f"{vals[k][1]:>22.4g}" if k in vals else f"{'':>22}" for k in counters))
+    return "\n".join(out)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(trace(argv[1]))
@@ -100,6 +125,9 @@ def main(argv):
             args = args[1:]
         pairs = [(args[i], args[i + 1]) for i in range(0, len(args) - 1, 2)]
         print(pmc(pairs, probe))
+        return 0
+    if len(argv) >= 2 and argv[0] == "sq":
+        print(sq(argv[1], tuple(argv[2:])))
         return 0
     print(__doc__, file=sys.stderr)
     return 2
