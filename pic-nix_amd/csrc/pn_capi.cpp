@@ -614,8 +614,14 @@ int pn_create(const char* toml, int load, pn_sim** out)
       s.march_rows = std::max(0, std::atoi(k));
     if (const char* k = std::getenv("PICNIX_WORK_QUEUE")) // persistent row kernels
       s.dep_queue = s.vel_queue = std::atoi(k) != 0 ? 1 : 0;
-    if (const char* k = std::getenv("PICNIX_FUSED_PUSH")) // FP32 gather + position push
+    if (const char* k = std::getenv("PICNIX_FUSED_PUSH")) // gather + position push in one kernel
       s.fuse_push = std::atoi(k) != 0 ? 1 : 0;
+    if (const char* k = std::getenv("PICNIX_FP32_TILE")) { // FP32 deposit tile form
+      const std::string v(k);
+      s.tile32 = v == "f64" ? 1 : (v == "plain" ? 2 : (v == "comp" ? 3 : 0));
+    }
+    if (const char* k = std::getenv("PICNIX_STAGE_CELLS")) // FP32 staged tile: cells per phase
+      s.stage_cells = std::max(0, std::atoi(k));
     if (const char* k = std::getenv("PICNIX_RHO")) // "deposit": rho from particles every step
       s.rho_deposit = std::string(k) == "deposit";
     allocate(s);
@@ -1122,8 +1128,8 @@ void step_launches_wall(Sim& s, double delt)
     if (!launch32_push_velocity(s, delt, s.fuse_push != 0))
       launch32_push_position(s, delt);
   } else {
-    launch_push_velocity(s, delt);
-    launch_push_position(s, delt);
+    if (!launch_push_velocity(s, delt, s.fuse_push != 0))
+      launch_push_position(s, delt);
   }
   shock_position_boundary(s);
   s.sorted = was_sorted;
@@ -1160,9 +1166,9 @@ void step_launches(Sim& s, double delt)
     return;
   }
   launch_push_bfd(s, 0.5 * delt);
-  launch_push_velocity(s, delt);
   bool was_sorted = s.sorted;
-  launch_push_position(s, delt);
+  if (!launch_push_velocity(s, delt, s.fuse_push != 0))
+    launch_push_position(s, delt);
   s.sorted = was_sorted;
   const RhoDone rho = launch_deposit(s, delt, carry_rho(s) ? RhoMode::Keep : RhoMode::Defer);
   launch_push_bfd(s, 0.5 * delt);
@@ -1551,7 +1557,7 @@ void tune_step(Sim& s, double delt)
   drain_timing(s);
   const double dep = s.ph_count[3] ? s.ph_ms[3] : -1.0;
   const double rho = s.ph_count[kPhaseRhoDeposit] ? s.ph_ms[kPhaseRhoDeposit] : -1.0;
-  // gather + position push: the FP32 row gather may run the position push too
+  // gather + position push: the row gather may run the position push too
   const double vel = s.ph_count[1] ? s.ph_ms[1] + s.ph_ms[2] : -1.0;
   for (int i = 0; i <= PN_NPHASE; i++) {
     s.ph_ms[i]    = saved_ms[i];

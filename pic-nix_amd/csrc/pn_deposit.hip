@@ -585,22 +585,31 @@ __global__ void __launch_bounds__(256)
   double2*      W     = &wts[wv][0];
   int4*         FO    = &forg[wv][0];
 
-  // claim the next non-empty cell of the row (uniform per wave); false when done
-  auto claim = [&](int& sp, int& bx, int& p0, int& p1) {
+  // claim the next cell of the row that holds particles of any species
+  // (uniform per wave); false when the row is done
+  auto claim = [&](int& bx) {
     for (;;) {
       int q = 0;
       if (lane == 0)
         q = atomicAdd(&next_cell, 1);
       q = __builtin_amdgcn_readfirstlane(q);
-      if (q >= ns * nbx)
+      if (q >= nbx)
         return false;
-      sp = q / nbx;
-      bx = q - sp * nbx;
+      bx = q;
+      for (int sp = 0; sp < ns; sp++)
+        if (rcs[sp * (nbx + 1) + bx + 1] > rcs[sp * (nbx + 1) + bx])
+          return true;
+    }
+  };
+  // the first species >= sp of cell bx that holds particles, and its range
+  auto species_from = [&](int bx, int& sp, int& p0, int& p1) {
+    for (; sp < ns; sp++) {
       p0 = rcs[sp * (nbx + 1) + bx];
       p1 = rcs[sp * (nbx + 1) + bx + 1];
       if (p1 > p0)
         return true;
     }
+    return false;
   };
   // coordinates of batch particle k1 (axis a1) of [pb, p1) in segment seg
   auto fetch = [&](int seg, int pb, int p1, double& xv, double& xu) {
@@ -614,17 +623,21 @@ __global__ void __launch_bounds__(256)
   for (int bz = bz0; bz < bz1; bz++) {
     const int zlo = bz - is_odd + lb - (Order / 2) - 1;
     const int zs  = ((zlo % N) + N) % N; // ring slot of z-plane zlo
-    int       sp, bx, p0, p1;
-    bool      have = claim(sp, bx, p0, p1);
-    double    nxv = 0, nxu = 0; // prefetched coordinates
-    if (have)
+    // One claimed cell at a time; every species of the cell accumulates into
+    // the same registers (the tile origin is the cell's), flushed once per cell.
+    int    bx, sp = 0, p0 = 0, p1 = 0;
+    bool   have = claim(bx);
+    double nxv = 0, nxu = 0; // prefetched coordinates
+    if (have) {
+      species_from(bx, sp, p0, p1);
       fetch(c * ns + sp, p0, p1, nxv, nxu);
+    }
     while (have) {
-      int        spn, bxn, p0n, p1n;
-      const bool more = claim(spn, bxn, p0n, p1n);
-      const int  seg  = c * ns + sp;
-      const int  ox   = bx + xlo;
-      const double qd = dc.q[sp] * ddt; // qs * dxdt etc. (esirkepov.hpp:172-174)
+      int        bxn, spn = 0, p0n = 0, p1n = 0;
+      const bool more = claim(bxn);
+      if (more)
+        species_from(bxn, spn, p0n, p1n);
+      const int ox = bx + xlo;
 
       double acc[N][N - 1];
 #pragma unroll
@@ -633,11 +646,18 @@ __global__ void __launch_bounds__(256)
         for (int j = 0; j < N - 1; j++)
           acc[r][j] = 0.0;
 
+      for (;;) { // species of cell bx
+      const int    seg = c * ns + sp;
+      const double qd  = dc.q[sp] * ddt; // qs * dxdt etc. (esirkepov.hpp:172-174)
+      int          sp2 = sp + 1, q0 = 0, q1 = 0;
+      const bool   next_sp = species_from(bx, sp2, q0, q1);
       for (int pb = p0; pb < p1; pb += BATCH) {
         const int    nbt = min(BATCH, p1 - pb);
         const double xv  = nxv, xu = nxu;
         if (pb + BATCH < p1)
           fetch(seg, pb + BATCH, p1, nxv, nxu);
+        else if (next_sp)
+          fetch(c * ns + sp2, q0, q1, nxv, nxu);
         else if (more)
           fetch(c * ns + spn, p0n, p1n, nxv, nxu);
         // ---- phase 1: shape weights, one lane per (particle, axis) ----
@@ -756,6 +776,12 @@ __global__ void __launch_bounds__(256)
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
+      if (!next_sp)
+        break;
+      sp = sp2;
+      p0 = q0;
+      p1 = q1;
+      } // species
       // ---- sum the PPS partial planes, add them to the ring tile ----
 #pragma unroll
       for (int r = 0; r < N; r++)
@@ -769,16 +795,13 @@ __global__ void __launch_bounds__(256)
           acc[r][j] = s;
         }
       if (act && k == 0) {
+        int ro[N], co[N - 1];
+        plane_offsets<N>(comp, t, zs, TX, psz, tsz, bx, ro, co);
 #pragma unroll
         for (int r = 0; r < N; r++)
 #pragma unroll
-          for (int j = 0; j < N - 1; j++) {
-            const int oz  = comp == 0 ? r : (comp == 1 ? t : j + 1);
-            const int oy  = comp == 0 ? t : (comp == 1 ? j + 1 : r);
-            const int oxx = comp == 0 ? j + 1 : (comp == 1 ? r : t);
-            const int sl  = zs + oz >= N ? zs + oz - N : zs + oz;
-            atomicAdd(&tile[comp * tsz + sl * psz + oy * TX + bx + oxx], acc[r][j]);
-          }
+          for (int j = 0; j < N - 1; j++)
+            atomicAdd(&tile[ro[r] + co[j]], acc[r][j]);
       }
       have = more;
       sp   = spn;

@@ -406,6 +406,35 @@ __device__ __forceinline__ int dir_of(const double* __restrict__ rng, int c, dou
 
 
 
+// Node offsets of a plane lane of the march deposits.  The block's ring tile
+// is [3 components][N z-planes][N][TX] (z ring slot, y, x), its lowest z-plane
+// in ring slot zs; the lane holds component comp at index t along that
+// component's axis, and its accumulator entry (r, j) is the node
+//   Jx: z = r,     y = t,     x = j + 1
+//   Jy: z = t,     y = j + 1, x = r
+//   Jz: z = j + 1, y = r,     x = t
+// relative to the cell bx's tile origin.  Its offset is ro[r] + co[j]; the
+// wrap of the z ring is folded into the two tables, so the flush adds them
+// with no per-node select.
+template <int N>
+__device__ __forceinline__ void plane_offsets(int comp, int t, int zs, int TX, int psz, int tsz,
+                                              int bx, int (&ro)[N], int (&co)[N - 1])
+{
+  const int lim  = N - zs; // z offsets >= lim wrap to the start of the ring
+  const int wrap = N * psz;
+  const int zt   = zs + t >= N ? zs + t - N : zs + t;
+  const int base = comp == 0 ? zs * psz + t * TX
+                             : (comp == 1 ? tsz + zt * psz : 2 * tsz + zs * psz + t);
+  const int sr   = comp == 0 ? psz : (comp == 1 ? 1 : TX);
+  const int sj   = comp == 0 ? 1 : (comp == 1 ? TX : psz);
+#pragma unroll
+  for (int r = 0; r < N; r++)
+    ro[r] = base + bx + r * sr - ((comp == 0 && r >= lim) ? wrap : 0);
+#pragma unroll
+  for (int j = 0; j < N - 1; j++)
+    co[j] = (j + 1) * sj - ((comp == 2 && j + 1 >= lim) ? wrap : 0);
+}
+
 // FP32 particle storage (precision = "fp32"): position as the cell index (the
 // chunk-local bin, packed like the sort bins) plus the offset inside the cell
 // in cell units, r in [0, 1) after every sort; between the position push and

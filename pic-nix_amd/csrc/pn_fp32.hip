@@ -809,20 +809,38 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Compensated FP32 accumulation into an LDS pair (hi, lo).  The atomic add on
-// hi returns the value it replaced; the lane forms old + x again with the same
+// Compensated FP32 accumulation into LDS pairs (hi, lo).  The atomic add on hi
+// returns the value it replaced; the lane forms old + x again with the same
 // IEEE operation the LDS unit applied, and the rounding error of that addition
 // (Knuth's TwoSum, exact in round-to-nearest) goes to lo.  Concurrent adds keep
 // this exact: each one captures the error of the one addition it made.  The
-// node's value is hi + lo (formed in FP64 at the flush).
-__device__ __forceinline__ void tile_add2(float* hi, float* lo, float x)
+// node's value is hi + lo (formed in FP64 at the flush).  A lane first issues
+// all its hi adds and only then the lo adds, so the returns of the hi adds are
+// awaited once, not one at a time.
+__device__ __forceinline__ float two_sum_err(float old, float x)
 {
-  const float old = atomicAdd(hi, x);
-  const float s   = old + x;
-  const float bb  = s - old;
-  const float e   = (old - (s - bb)) + (x - bb);
-  if (e != 0.0f)
-    atomicAdd(lo, e);
+  const float s  = old + x;
+  const float bb = s - old;
+  return (old - (s - bb)) + (x - bb);
+}
+
+constexpr int kTileStaged = 0;
+constexpr int kTileF64    = 1;
+constexpr int kTilePlain  = 2;
+constexpr int kTileComp   = 3;
+
+// The value of ring-tile node idx (tile_d: the FP64 form's nodes; hi, lo: the
+// FP32 forms' pairs, lo unused by kTilePlain).
+template <int Tile>
+__device__ __forceinline__ double tile_value(const double* tile_d, const float* hi,
+                                             const float* lo, int idx)
+{
+  if constexpr (Tile == kTileF64)
+    return tile_d[idx];
+  else if constexpr (Tile == kTileComp || Tile == kTileStaged)
+    return (double)hi[idx] + (double)lo[idx];
+  else
+    return (double)hi[idx];
 }
 
 // Plane-lane march kernel (3D, even orders): the FP64 k_deposit_march in FP32
@@ -845,16 +863,30 @@ struct March32 {
   // five waves measured 1.27 -> 1.14 ms at 64^3 against the unconstrained four;
   // six the same, three slower)
   static constexpr int WAVES = Order == 2 ? 5 : 2;
+  static constexpr int waves(int) { return WAVES; }
 };
 
-template <int Order, bool Queue>
+// Tile selects how cell sums reach the ring tile.
+//   kTileStaged (the default): the compensated FP32 pairs (hi, lo) are updated
+//     without atomics.  A claimed cell's wave stores its per-cell sums (one
+//     N x (N-1) block per plane lane) into the cell's slot of a row stage; at the
+//     end of the bin row one thread per tile node adds the slots that reach it,
+//     in ascending cell order, with TwoSum into (hi, lo).  The block's tile is
+//     therefore the same whichever wave claimed which cell.
+//   kTileComp: the pairs take LDS float atomics; the error of each add comes
+//     from its returned old value (TwoSum).
+//   kTileF64: one FP64 value per node (LDS double atomics).
+//   kTilePlain: one FP32 value per node, no compensation.
+// The last three exist to measure the first against (LDS float atomics are
+// several times slower than double ones on this GPU).
+template <int Order, bool Queue, int Tile>
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(March32<Order>::WAVES, March32<Order>::WAVES)))
+    __attribute__((amdgpu_waves_per_eu(March32<Order>::waves(Tile), March32<Order>::waves(Tile))))
     k32_deposit_march(Geom g, const P32 p, const float* __restrict__ ro0,
                       const float* __restrict__ ro1, const float* __restrict__ ro2,
                       const int32_t* __restrict__ cellstart, const int* __restrict__ off,
                       float* __restrict__ J, Dep32 dc, int zrows,
-                      int* __restrict__ wq, int gx, int nitems)
+                      int* __restrict__ wq, int gx, int nitems, int kst)
 {
   using MP            = March32<Order>;
   constexpr int N     = MP::N;
@@ -869,7 +901,10 @@ __global__ void __launch_bounds__(256)
 
   __shared__ float2 wts[NWAVE][MP::WPAIR];
   __shared__ int    next_cell;
-  extern __shared__ float tile[]; // compensated pairs: hi [3][N z-ring][N][TX], lo (same), then rcs
+  // kTileComp: hi [3][N z-ring][N][TX] and lo (same) as floats; kTileF64: the
+  // same nodes as doubles; kTilePlain: hi only.  Then rcs.
+  extern __shared__ double tile_d[];
+  float* tile = reinterpret_cast<float*>(tile_d);
 
   // Queue: persistent blocks take (row group, chunk) items from the work queue
   // wq until it is drained; otherwise the item is the block's own index.
@@ -910,11 +945,24 @@ __global__ void __launch_bounds__(256)
   const int TX  = nbx + N - 1;
   const int psz = N * TX;
   const int tsz = N * psz;
-  float*    thi = tile;
-  float*    tlo = tile + 3 * tsz;
-  int*      rcs = reinterpret_cast<int*>(tile + 2 * 3 * tsz);
-  for (int q = threadIdx.x; q < 3 * tsz; q += blockDim.x)
-    thi[q] = tlo[q] = 0.0f;
+  // kTileStaged: the stage holds the per-cell sums of the kst cells of one
+  // phase of the row, [kst][NPL][N][N - 1], and two phases of cell flags
+  // [2][kst]; the ring (hi, lo) is updated only by the phase reductions.
+  constexpr int CB    = NPL * N * (N - 1); // staged floats per cell
+  float*        thi   = tile;
+  float*        tlo   = tile + 3 * tsz;
+  float*        stage = tile + 2 * 3 * tsz;
+  int*          cflag = reinterpret_cast<int*>(stage + (Tile == kTileStaged ? kst * CB : 0));
+  int*          rcs   = cflag + (Tile == kTileStaged ? 2 * kst : 0);
+  for (int q = threadIdx.x; q < 3 * tsz; q += blockDim.x) {
+    if constexpr (Tile == kTileF64)
+      tile_d[q] = 0.0;
+    else
+      thi[q] = tlo[q] = 0.0f;
+  }
+  if constexpr (Tile == kTileStaged)
+    for (int q = threadIdx.x; q < 2 * kst; q += blockDim.x)
+      cflag[q] = 0;
   auto load_row = [&](int bz) {
     for (int q = threadIdx.x; q < ns * (nbx + 1); q += blockDim.x) {
       const int sp = q / (nbx + 1);
@@ -947,21 +995,32 @@ __global__ void __launch_bounds__(256)
   const float* R1   = a1 == 0 ? p.rx : (a1 == 1 ? p.ry : p.rz);
   float2*      W    = &wts[wv][0];
 
-  auto claim = [&](int& sp, int& bx, int& p0, int& p1) {
+  // the next cell of [next_cell, cend) that holds particles of any species
+  // (uniform per wave); false when the range is done
+  int  cbeg = 0, cend = nbx; // the cells of the current phase (kTileStaged) or row
+  auto claim = [&](int& bx) {
     for (;;) {
       int q = 0;
       if (lane == 0)
         q = atomicAdd(&next_cell, 1);
       q = __builtin_amdgcn_readfirstlane(q);
-      if (q >= ns * nbx)
+      if (q >= cend)
         return false;
-      sp = q / nbx;
-      bx = q - sp * nbx;
+      bx = q;
+      for (int sp = 0; sp < ns; sp++)
+        if (rcs[sp * (nbx + 1) + bx + 1] > rcs[sp * (nbx + 1) + bx])
+          return true;
+    }
+  };
+  // the first species >= sp of cell bx that holds particles, and its range
+  auto species_from = [&](int bx, int& sp, int& p0, int& p1) {
+    for (; sp < ns; sp++) {
       p0 = rcs[sp * (nbx + 1) + bx];
       p1 = rcs[sp * (nbx + 1) + bx + 1];
       if (p1 > p0)
         return true;
     }
+    return false;
   };
   auto fetch = [&](int seg, int pb, int p1, float& xo, float& xn) {
     if (a1 < 3 && pb + k1 < p1) {
@@ -971,30 +1030,46 @@ __global__ void __launch_bounds__(256)
     }
   };
 
+  const int nph = Tile == kTileStaged ? (nbx + kst - 1) / kst : 1; // phases per row
   for (int bz = bz0; bz < bz1; bz++) {
     const int zlo = bz + lb - (Order / 2) - 1;
     const int zs  = ((zlo % N) + N) % N;
-    int       sp, bx, p0, p1;
-    bool      have = claim(sp, bx, p0, p1);
+    for (int ph = 0; ph < nph; ph++) {
+    cbeg = Tile == kTileStaged ? ph * kst : 0;
+    cend = Tile == kTileStaged ? min(nbx, cbeg + kst) : nbx;
+    int* const fl = cflag + ((bz * nph + ph) & 1) * kst; // this phase's cell flags
+    // one claimed cell at a time; all its species accumulate into the same
+    // registers, flushed to the tile once per cell
+    int       bx, sp = 0, p0 = 0, p1 = 0;
+    bool      have = claim(bx);
     float     nxo = 0, nxn = 0;
-    if (have)
+    if (have) {
+      species_from(bx, sp, p0, p1);
       fetch(c * ns + sp, p0, p1, nxo, nxn);
+    }
     while (have) {
-      int         spn, bxn, p0n, p1n;
-      const bool  more = claim(spn, bxn, p0n, p1n);
-      const int   seg  = c * ns + sp;
-      const float qd   = dc.q[sp] * ddt;
+      int         bxn, spn = 0, p0n = 0, p1n = 0;
+      const bool  more = claim(bxn);
+      if (more)
+        species_from(bxn, spn, p0n, p1n);
       float       acc[N][N - 1];
 #pragma unroll
       for (int r = 0; r < N; r++)
 #pragma unroll
         for (int j = 0; j < N - 1; j++)
           acc[r][j] = 0.0f;
+      for (;;) { // species of cell bx
+      const int   seg = c * ns + sp;
+      const float qd  = dc.q[sp] * ddt;
+      int         sp2 = sp + 1, q0 = 0, q1 = 0;
+      const bool  next_sp = species_from(bx, sp2, q0, q1);
       for (int pb = p0; pb < p1; pb += BATCH) {
         const int   nbt = min(BATCH, p1 - pb);
         const float xo  = nxo, xn = nxn;
         if (pb + BATCH < p1)
           fetch(seg, pb + BATCH, p1, nxo, nxn);
+        else if (next_sp)
+          fetch(c * ns + sp2, q0, q1, nxo, nxn);
         else if (more)
           fetch(c * ns + spn, p0n, p1n, nxo, nxn);
         if (a1 < 3 && k1 < nbt) {
@@ -1044,6 +1119,12 @@ __global__ void __launch_bounds__(256)
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
+      if (!next_sp)
+        break;
+      sp = sp2;
+      p0 = q0;
+      p1 = q1;
+      } // species
 #pragma unroll
       for (int r = 0; r < N; r++)
 #pragma unroll
@@ -1056,19 +1137,41 @@ __global__ void __launch_bounds__(256)
           acc[r][j] = s;
         }
       if (act && k == 0) {
+        int ro[N], co[N - 1];
+        if constexpr (Tile != kTileStaged)
+          plane_offsets<N>(comp, t, zs, TX, psz, tsz, bx, ro, co);
+        if constexpr (Tile == kTileStaged) {
+          float* st = stage + (bx - cbeg) * CB + plane * (N * (N - 1));
 #pragma unroll
-        for (int r = 0; r < N; r++)
+          for (int r = 0; r < N; r++)
 #pragma unroll
-          for (int j = 0; j < N - 1; j++) {
-            const int oz  = comp == 0 ? r : (comp == 1 ? t : j + 1);
-            const int oy  = comp == 0 ? t : (comp == 1 ? j + 1 : r);
-            const int oxx = comp == 0 ? j + 1 : (comp == 1 ? r : t);
-            const int sl  = zs + oz >= N ? zs + oz - N : zs + oz;
-            {
-              const int ti = comp * tsz + sl * psz + oy * TX + bx + oxx;
-              tile_add2(&thi[ti], &tlo[ti], acc[r][j]);
-            }
+            for (int j = 0; j < N - 1; j++)
+              st[r * (N - 1) + j] = acc[r][j];
+          if (plane == 0)
+            fl[bx - cbeg] = 1;
+        } else if constexpr (Tile == kTileComp) {
+          // per row: the N - 1 hi adds, then their N - 1 error adds
+#pragma unroll
+          for (int r = 0; r < N; r++) {
+            float old[N - 1];
+#pragma unroll
+            for (int j = 0; j < N - 1; j++)
+              old[j] = atomicAdd(&thi[ro[r] + co[j]], acc[r][j]);
+#pragma unroll
+            for (int j = 0; j < N - 1; j++)
+              atomicAdd(&tlo[ro[r] + co[j]], two_sum_err(old[j], acc[r][j]));
           }
+        } else {
+#pragma unroll
+          for (int r = 0; r < N; r++)
+#pragma unroll
+            for (int j = 0; j < N - 1; j++) {
+              if constexpr (Tile == kTileF64)
+                atomicAdd(&tile_d[ro[r] + co[j]], (double)acc[r][j]);
+              else
+                atomicAdd(&thi[ro[r] + co[j]], acc[r][j]);
+            }
+        }
       }
       have = more;
       sp   = spn;
@@ -1077,11 +1180,77 @@ __global__ void __launch_bounds__(256)
       p1   = p1n;
     }
     __syncthreads();
+    if constexpr (Tile == kTileStaged) {
+      // Every tile node that the phase's cells reach takes their staged sums in
+      // ascending cell order (TwoSum into hi, lo; one thread per node).  After
+      // the row's last phase, plane zlo is complete and goes to J.  The flags
+      // of the next phase are cleared here.
+      const bool last = ph == nph - 1;
+      const int  wx   = last ? TX : (cend - cbeg) + N - 1; // node columns to visit
+      const int  x0   = last ? 0 : cbeg;
+      const int  nq   = 3 * N * N * wx;
+      for (int q2 = threadIdx.x; q2 < nq; q2 += blockDim.x) {
+        const int rq = q2 / wx;
+        const int tx = x0 + (q2 - rq * wx);
+        const int ty = rq % N;
+        const int r3 = rq / N;
+        const int sl = r3 % N;
+        const int kc = r3 / N;
+        const int q  = kc * tsz + sl * psz + ty * TX + tx;
+        const int oz = sl >= zs ? sl - zs : sl - zs + N; // z offset from zlo
+        float     hi = thi[q], lo = tlo[q];
+        // the cells that reach this node are cb = tx - N + 1 + i (ascending),
+        // their staged entries base + i * step:
+        //   Jx: z = r, y = t, x = j + 1 -> i < N - 1, j = N - 2 - i
+        //   Jy: z = t, y = j + 1, x = r -> i < N, r = N - 1 - i
+        //   Jz: z = j + 1, y = r, x = t -> i < N, t = N - 1 - i
+        constexpr int PB   = N * (N - 1); // floats per plane block
+        const int     cb0  = tx - N + 1;
+        const int     cnt  = kc == 0 ? N - 1 : N;
+        const bool    ok   = kc == 0 || (kc == 1 ? ty >= 1 : oz >= 1);
+        const int     base = (cb0 - cbeg) * CB +
+                         (kc == 0 ? ty * PB + oz * (N - 1) + N - 2
+                          : kc == 1 ? (N + oz) * PB + (N - 1) * (N - 1) + ty - 1
+                                    : (3 * N - 1) * PB + ty * (N - 1) + oz - 1);
+        const int     step = CB - (kc == 0 ? 1 : kc == 1 ? N - 1 : PB);
+        if (ok) {
+          for (int i = 0; i < cnt; i++) {
+            const int cs = cb0 + i - cbeg; // stage slot
+            if ((unsigned)cs < (unsigned)(cend - cbeg) && fl[cs] != 0) {
+              const float x = stage[base + i * step];
+              lo += two_sum_err(hi, x);
+              hi += x;
+            }
+          }
+        }
+        if (last && sl == zs) {
+          const double val = (double)hi + (double)lo;
+          if (val != 0.0) {
+            const int gz = off[c * 3 + 0] + zlo, gy = off[c * 3 + 1] + ylo + ty,
+                      gx = off[c * 3 + 2] + xlo + tx;
+            if ((unsigned)gz < (unsigned)g.P[0] && (unsigned)gy < (unsigned)g.P[1] &&
+                (unsigned)gx < (unsigned)g.P[2])
+              atomicAdd(&J[(kc + 1) * v + ((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
+          }
+          hi = lo = 0.0f;
+        }
+        thi[q] = hi;
+        tlo[q] = lo;
+      }
+      int* const fn = cflag + ((bz * nph + ph + 1) & 1) * kst;
+      for (int q = threadIdx.x; q < kst; q += blockDim.x)
+        fn[q] = 0;
+      if (!last) {
+        if (threadIdx.x == 0)
+          next_cell = cend;
+        __syncthreads();
+      }
+    } else {
     for (int q = threadIdx.x; q < 3 * psz; q += blockDim.x) {
       const int    kc  = q / psz;
       const int    r2  = q - kc * psz;
       const int    idx = kc * tsz + zs * psz + r2;
-      const double val = (double)thi[idx] + (double)tlo[idx];
+      const double val = tile_value<Tile>(tile_d, thi, tlo, idx);
       if (val != 0.0) {
         const int ty = r2 / TX;
         const int tx = r2 - ty * TX;
@@ -1090,9 +1259,14 @@ __global__ void __launch_bounds__(256)
         if ((unsigned)gz < (unsigned)g.P[0] && (unsigned)gy < (unsigned)g.P[1] &&
             (unsigned)gx < (unsigned)g.P[2])
           atomicAdd(&J[(kc + 1) * v + ((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
-        thi[idx] = tlo[idx] = 0.0f;
+        if constexpr (Tile == kTileF64)
+          tile_d[idx] = 0.0;
+        else
+          thi[idx] = tlo[idx] = 0.0f;
       }
     }
+    }
+    } // phase
     if (bz + 1 < bz1)
       load_row(bz + 1);
     if (threadIdx.x == 0)
@@ -1107,7 +1281,7 @@ __global__ void __launch_bounds__(256)
     const int    dz  = 1 + r / psz;
     const int    r2  = r - (dz - 1) * psz;
     const int    sl  = zs + dz >= N ? zs + dz - N : zs + dz;
-    const double val = (double)thi[kc * tsz + sl * psz + r2] + (double)tlo[kc * tsz + sl * psz + r2];
+    const double val = tile_value<Tile>(tile_d, thi, tlo, kc * tsz + sl * psz + r2);
     if (val == 0.0)
       continue;
     const int ty = r2 / TX;
@@ -1355,11 +1529,17 @@ __global__ void __launch_bounds__(256)
         }
       if (act && k == 0) {
         const int sl = zs + t >= S ? zs + t - S : zs + t;
+        float     old[S][S];
 #pragma unroll
         for (int jy = 0; jy < S; jy++)
 #pragma unroll
           for (int jx = 0; jx < S; jx++)
-            tile_add2(&thi[sl * psz + jy * TX + bx + jx], &tlo[sl * psz + jy * TX + bx + jx], acc[jy][jx]);
+            old[jy][jx] = atomicAdd(&thi[sl * psz + jy * TX + bx + jx], acc[jy][jx]);
+#pragma unroll
+        for (int jy = 0; jy < S; jy++)
+#pragma unroll
+          for (int jx = 0; jx < S; jx++)
+            atomicAdd(&tlo[sl * psz + jy * TX + bx + jx], two_sum_err(old[jy][jx], acc[jy][jx]));
       }
       have = more;
       sp   = spn;
@@ -1622,26 +1802,46 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
   {
     PhaseTimer pt(s, 3);
     const int    N    = g.order + 3;
-    const size_t mlds = sizeof(double) * 3 * (size_t)N * N * (g.bsy + N - 1) +
-                        sizeof(int) * (size_t)g.Ns * (g.bsy + 1);
+    const size_t mbase = sizeof(double) * 3 * (size_t)N * N * (g.bsy + N - 1) +
+                         sizeof(int) * (size_t)g.Ns * (g.bsy + 1);
+    // kTileStaged adds the stage of one phase of kst cells (per cell: 3N planes
+    // x N x (N-1) floats) and two phases of cell flags
+    const int    kst    = std::max(1, std::min(s.stage_cells > 0 ? s.stage_cells : 4, g.bsy));
+    const size_t mstage = mbase + sizeof(float) * (size_t)kst * 3 * N * N * (N - 1) +
+                          sizeof(int) * 2 * (size_t)kst;
+    const size_t wlds  = g.order == 2 ? March32<2>::WLDS : March32<4>::WLDS;
+    const int    tile  = s.tile32 != kTileStaged ? s.tile32
+                         : (mstage + wlds + 64 <= 65536 ? kTileStaged : kTileComp);
+    const size_t mlds  = tile == kTileStaged ? mstage : mbase;
     int zrows, nzr;
     march32_split(g, s.dep_rows ? s.dep_rows : s.march_rows, zrows, nzr);
     const int  gx     = nzr * (g.bsz / g.bsy);
     const int  nitems = gx * g.nchunk;
     int*       wq     = s.d_wq + 2 * kWqDeposit32;
-#define PN32_MARCH(O)                                                                          \
+#define PN32_MARCH_T(O, T)                                                                     \
   do {                                                                                         \
     if (s.dep_queue) {                                                                         \
       const int nb = queue_blocks(                                                             \
-          s, reinterpret_cast<const void*>(&k32_deposit_march<O, true>), mlds, nitems);        \
-      hipLaunchKernelGGL((k32_deposit_march<O, true>), dim3(nb), dim3(256), mlds, s.stream, g, \
-                         s.qa, s.f_ro[0], s.f_ro[1], s.f_ro[2], s.ss.cellstart, s.d_off, s.f_J, \
-                         dc, zrows, wq, gx, nitems);                                           \
+          s, reinterpret_cast<const void*>(&k32_deposit_march<O, true, T>), mlds, nitems);     \
+      hipLaunchKernelGGL((k32_deposit_march<O, true, T>), dim3(nb), dim3(256), mlds, s.stream, \
+                         g, s.qa, s.f_ro[0], s.f_ro[1], s.f_ro[2], s.ss.cellstart, s.d_off,     \
+                         s.f_J, dc, zrows, wq, gx, nitems, kst);                               \
     } else {                                                                                   \
-      hipLaunchKernelGGL((k32_deposit_march<O, false>), dim3(gx, g.nchunk), dim3(256), mlds,   \
-                         s.stream, g, s.qa, s.f_ro[0], s.f_ro[1], s.f_ro[2], s.ss.cellstart,    \
-                         s.d_off, s.f_J, dc, zrows, wq, gx, nitems);                           \
+      hipLaunchKernelGGL((k32_deposit_march<O, false, T>), dim3(gx, g.nchunk), dim3(256),      \
+                         mlds, s.stream, g, s.qa, s.f_ro[0], s.f_ro[1], s.f_ro[2],              \
+                         s.ss.cellstart, s.d_off, s.f_J, dc, zrows, wq, gx, nitems, kst);      \
     }                                                                                          \
+  } while (0)
+#define PN32_MARCH(O)                                                                          \
+  do {                                                                                         \
+    if (tile == kTileF64)                                                                      \
+      PN32_MARCH_T(O, kTileF64);                                                               \
+    else if (tile == kTilePlain)                                                               \
+      PN32_MARCH_T(O, kTilePlain);                                                             \
+    else if (tile == kTileComp)                                                                \
+      PN32_MARCH_T(O, kTileComp);                                                              \
+    else                                                                                       \
+      PN32_MARCH_T(O, kTileStaged);                                                            \
   } while (0)
     if (g.dim == 3 && s.sorted && !s.generic_kernels && g.order == 2 &&
         mlds + March32<2>::WLDS + 64 <= 65536) {
@@ -1652,6 +1852,7 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
       PN32_MARCH(4);
       march = true;
 #undef PN32_MARCH
+#undef PN32_MARCH_T
     } else {
       const dim3 grid(nblk(g.cap, 256), g.nseg);
 #define PN32_DEP(D, O)                                                                         \

@@ -191,6 +191,71 @@ __global__ void __launch_bounds__(256)
   p.uz[i] = xu[5];
 }
 
+// Arguments of the position push (BasePosition::push, pic/engine/position.hpp:
+// 117-130, and count(reset), pic/pic_engine.hpp:292-303) when the row gather
+// runs it for the particles it holds.
+struct Pos64 {
+  double*       xo;
+  double*       yo;
+  double*       zo;
+  const double* rng;
+  double        delt;
+  double        rc;
+  int32_t*      cellcnt; // bin counters, nbin per segment
+  int32_t*      key;
+  int32_t*      slot;
+};
+
+// The position push of particle i of chunk c with momentum (ux, uy, uz): writes
+// the old and the new position, and the momentum when WriteU or when the
+// reflecting wall flips it.  Returns the new bin when the particle stays in the
+// chunk; otherwise -1, with key[i] set (-1 - d for a leaver in direction d, also
+// returned in dir, or kKeyReplace past the open end).
+template <bool WriteU>
+__device__ __forceinline__ int position64(const Geom& g, const PArr& p, const Pos64& ps,
+                                          const double* __restrict__ lim, int c, long i,
+                                          double x, double y, double z, double ux, double uy,
+                                          double uz, int& dir)
+{
+  dir       = -1;
+  ps.xo[i]  = x;
+  ps.yo[i]  = y;
+  ps.zo[i]  = z;
+  double gm = lorentz_factor(ux, uy, uz, ps.rc);
+  double dt = ps.delt / gm;
+  x += ux * dt;
+  y += uy * dt;
+  z += uz * dt;
+  if (g.xwall && x < g.glim[2][0]) {
+    // reflecting wall, MainChunk::set_boundary_particle (example/shock/main.cpp:402-416)
+    x  = -x + 2 * g.glim[2][0];
+    ux = -ux;
+    uy = -uy;
+    uz = -uz;
+    p.ux[i] = ux;
+    p.uy[i] = uy;
+    p.uz[i] = uz;
+  } else if (WriteU) {
+    p.ux[i] = ux;
+    p.uy[i] = uy;
+    p.uz[i] = uz;
+  }
+  p.x[i] = x;
+  p.y[i] = y;
+  p.z[i] = z;
+  if (g.xwall && x >= g.glim[2][1]) {
+    // past the open end: replaced by an injected particle (k_wall_apply)
+    ps.key[i] = kKeyReplace;
+    return -1;
+  }
+  const int b = bin_of(g, lim, ps.rng, c, x, y, z);
+  if (b < 0) {
+    dir       = dir_of(ps.rng, c, x, y, z);
+    ps.key[i] = -1 - dir;
+  }
+  return b;
+}
+
 // Row form of the gather: one block per (bin row, segment).  The E and B
 // values every particle of the row can touch -- (Order+2) nodes around the
 // row's (z, y) bin and its whole x extent -- are staged in LDS once; each
@@ -198,13 +263,17 @@ __global__ void __launch_bounds__(256)
 // summation order), reading LDS instead of global memory.  A particle whose
 // stencil leaves the staged region (a rounding edge of the bin test) reads
 // global memory.
-template <int Dim, int Order, bool Plain, bool Queue>
+// Fuse: each particle's position push follows its velocity push here (the
+// step's next operator), so its position and momentum are read once for both;
+// the bin counters then count it and key holds its bin or exchange direction,
+// as after k_push_position (whose per-tile leaver counts k_leaver_tiles adds).
+template <int Dim, int Order, bool Plain, bool Queue, bool Fuse>
 __global__ void __launch_bounds__(256)
     k_push_velocity_rows(Geom g, PArr p, const int32_t* __restrict__ cellstart,
                          const int* __restrict__ off, const double* __restrict__ lim,
                          const double* __restrict__ E, const double* __restrict__ B,
                          PushConsts pc, int yrows, int nsplit,
-                         int* __restrict__ wq, int gx, int nitems)
+                         int* __restrict__ wq, int gx, int nitems, Pos64 ps)
 {
   constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
   constexpr int S      = Order + 1;
@@ -348,6 +417,7 @@ __global__ void __launch_bounds__(256)
     hy0 += lb - (Order / 2);
     bool in = min(ix0, hx0) >= tx0 && max(ix0, hx0) + S <= tx0 + TXv && min(iy0, hy0) >= ty0 &&
               max(iy0, hy0) + S <= ty0 + TY;
+    bool   ok = true; // false: the stencil leaves the padded arrays, no velocity push
     double eb[6];
     if constexpr (Dim == 3) {
       const double zimin  = zmin + 0.5 * dz * is_odd;
@@ -378,14 +448,18 @@ __global__ void __launch_bounds__(256)
         const int Z = iz0 + offz, HZ = hz0 + offz, Y = iy0 + offy, HY = hy0 + offy;
         const int X = ix0 + offx, HX = hx0 + offx;
         if (min(Z, HZ) < 0 || max(Z, HZ) > g.P[0] - S || min(Y, HY) < 0 ||
-            max(Y, HY) > g.P[1] - S || min(X, HX) < 0 || max(X, HX) > g.P[2] - S)
-          continue;
-        eb[0] = interp3(E, sz, sy, Z, Y, HX, wiz, wiy, whx);
-        eb[1] = interp3(E + v, sz, sy, Z, HY, X, wiz, why, wix);
-        eb[2] = interp3(E + 2 * v, sz, sy, HZ, Y, X, whz, wiy, wix);
-        eb[3] = interp3(B, sz, sy, HZ, HY, X, whz, why, wix);
-        eb[4] = interp3(B + v, sz, sy, HZ, Y, HX, whz, wiy, whx);
-        eb[5] = interp3(B + 2 * v, sz, sy, Z, HY, HX, wiz, why, whx);
+            max(Y, HY) > g.P[1] - S || min(X, HX) < 0 || max(X, HX) > g.P[2] - S) {
+          if constexpr (!Fuse)
+            continue;
+          ok = false; // not pushed; its position push still runs below
+        } else {
+          eb[0] = interp3(E, sz, sy, Z, Y, HX, wiz, wiy, whx);
+          eb[1] = interp3(E + v, sz, sy, Z, HY, X, wiz, why, wix);
+          eb[2] = interp3(E + 2 * v, sz, sy, HZ, Y, X, whz, wiy, wix);
+          eb[3] = interp3(B, sz, sy, HZ, HY, X, whz, why, wix);
+          eb[4] = interp3(B + v, sz, sy, HZ, Y, HX, whz, wiy, whx);
+          eb[5] = interp3(B + 2 * v, sz, sy, Z, HY, HX, wiz, why, whx);
+        }
       }
     } else {
       (void)wiz;
@@ -402,23 +476,41 @@ __global__ void __launch_bounds__(256)
       } else {
         const int Y = iy0 + offy, HY = hy0 + offy, X = ix0 + offx, HX = hx0 + offx;
         if (min(Y, HY) < 0 || max(Y, HY) > g.P[1] - S || min(X, HX) < 0 ||
-            max(X, HX) > g.P[2] - S)
-          continue;
-        eb[0] = interp2(E, sy, Y, HX, wiy, whx);
-        eb[1] = interp2(E + v, sy, HY, X, why, wix);
-        eb[2] = interp2(E + 2 * v, sy, Y, X, wiy, wix);
-        eb[3] = interp2(B, sy, HY, X, why, wix);
-        eb[4] = interp2(B + v, sy, Y, HX, wiy, whx);
-        eb[5] = interp2(B + 2 * v, sy, HY, HX, why, whx);
+            max(X, HX) > g.P[2] - S) {
+          if constexpr (!Fuse)
+            continue;
+          ok = false;
+        } else {
+          eb[0] = interp2(E, sy, Y, HX, wiy, whx);
+          eb[1] = interp2(E + v, sy, HY, X, why, wix);
+          eb[2] = interp2(E + 2 * v, sy, Y, X, wiy, wix);
+          eb[3] = interp2(B, sy, HY, X, why, wix);
+          eb[4] = interp2(B + v, sy, Y, HX, wiy, whx);
+          eb[5] = interp2(B + 2 * v, sy, HY, HX, why, whx);
+        }
       }
     }
-    if constexpr (Plain)
-      push_boris(xu[3], xu[4], xu[5], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], g.cc);
-    else
-      push_momentum(pc.pusher, xu[3], xu[4], xu[5], eb, g.cc);
-    p.ux[i] = xu[3];
-    p.uy[i] = xu[4];
-    p.uz[i] = xu[5];
+    if (ok) {
+      if constexpr (Plain)
+        push_boris(xu[3], xu[4], xu[5], eb[0], eb[1], eb[2], eb[3], eb[4], eb[5], g.cc);
+      else
+        push_momentum(pc.pusher, xu[3], xu[4], xu[5], eb, g.cc);
+    }
+    if constexpr (Fuse) {
+      int       dir;
+      const int b    = position64<true>(g, p, ps, lim, c, i, xu[0], xu[1], xu[2], xu[3], xu[4],
+                                        xu[5], dir);
+      const bool stay = b >= 0;
+      const int  sl   = wave_slot(ps.cellcnt + (long)seg * g.nbin, b, stay);
+      if (stay) {
+        ps.key[i]  = b;
+        ps.slot[i] = sl;
+      }
+    } else {
+      p.ux[i] = xu[3];
+      p.uy[i] = xu[4];
+      p.uz[i] = xu[5];
+    }
   }
   } // species
   } while (false);
@@ -454,43 +546,41 @@ __global__ void __launch_bounds__(256)
   const long i      = (long)seg * g.cap + ip;
   int        b      = -1;
   if (active) {
-    double x = p.x[i], y = p.y[i], z = p.z[i];
-    double ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
-    xo[i]     = x;
-    yo[i]     = y;
-    zo[i]     = z;
-    double gm = lorentz_factor(ux, uy, uz, rc);
-    double dt = delt / gm;
-    x += ux * dt;
-    y += uy * dt;
-    z += uz * dt;
-    if (g.xwall && x < g.glim[2][0]) {
-      // reflecting wall, MainChunk::set_boundary_particle (example/shock/main.cpp:402-416)
-      x       = -x + 2 * g.glim[2][0];
-      p.ux[i] = -ux;
-      p.uy[i] = -uy;
-      p.uz[i] = -uz;
-    }
-    p.x[i] = x;
-    p.y[i] = y;
-    p.z[i] = z;
-    if (g.xwall && x >= g.glim[2][1]) {
-      // past the open end: replaced by an injected particle (k_wall_apply)
-      key[i] = kKeyReplace;
-    } else {
-      b = bin_of(g, lim, rng, c, x, y, z);
-      if (b < 0) {
-        int d  = dir_of(rng, c, x, y, z);
-        key[i] = -1 - d;
-        atomicAdd(&cnt[d], 1);
-      }
-    }
+    const Pos64 ps{xo, yo, zo, rng, delt, rc, cellcnt, key, slot};
+    int         d;
+    b = position64<false>(g, p, ps, lim, c, i, p.x[i], p.y[i], p.z[i], p.ux[i], p.uy[i],
+                          p.uz[i], d);
+    if (d >= 0)
+      atomicAdd(&cnt[d], 1);
   }
   const bool stay = active && b >= 0;
   int        sl   = wave_slot(cellcnt + (long)seg * g.nbin, b, stay);
   if (stay) {
     slot[i] = sl;
     key[i]  = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < 27)
+    tilecnt[((long)seg * ntile + blockIdx.x) * 27 + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// Per-tile leaver counts by exchange direction (tile = 256 consecutive
+// particles of a segment), as k_push_position writes them, for a position push
+// that ran inside the row gather: from the keys it left.
+__global__ void __launch_bounds__(256)
+    k_leaver_tiles(Geom g, const int32_t* __restrict__ np, const int32_t* __restrict__ key,
+                   int32_t* __restrict__ tilecnt, int ntile)
+{
+  __shared__ int cnt[27];
+  if (threadIdx.x < 27)
+    cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int  seg = blockIdx.y;
+  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip < np[seg]) {
+    const int k = key[(long)seg * g.cap + ip];
+    if (k < 0 && k >= -27)
+      atomicAdd(&cnt[-1 - k], 1);
   }
   __syncthreads();
   if (threadIdx.x < 27)
@@ -562,7 +652,10 @@ __global__ void __launch_bounds__(256) k_soa_to_aos(PArr p, long base, long n,
 
 } // namespace
 
-void launch_push_velocity(Sim& s, double delt)
+// with_position: the row gather also runs the position push (+ count) when the
+// row form applies; returns whether it did (then launch_push_position is not
+// needed).
+bool launch_push_velocity(Sim& s, double delt, bool with_position)
 {
   PhaseTimer  pt(s, 1);
   const Geom& g = s.g;
@@ -587,31 +680,41 @@ void launch_push_velocity(Sim& s, double delt)
   const size_t lds  = lds_of(yr);
   const bool   rows = g.dim >= 2 && s.sorted && !s.generic_kernels && !s.vel_generic &&
                     lds <= 65536;
+  const bool   fuse = rows && with_position;
   const int    nrow  = (g.nbin / g.bsy / nby) * ((nby + yr - 1) / yr);
   const int    nsplit = s.vel_split ? g.Ns : 1;
   const bool   plain = pc.pusher == 0 && pc.wt == 0; // Boris + MC: no option branches
   const int    vgy    = g.nchunk * nsplit, vitems = nrow * vgy;
   int*         vwq    = s.d_wq + 2 * kWqGather;
-#define PN_VEL_ROWS(D, O, PL)                                                                 \
+  const Pos64  ps{s.d_xo[0], s.d_xo[1], s.d_xo[2], s.d_rng,     delt,
+                 1 / g.cc,  s.ss.cellcnt, s.ss.key, s.ss.slot};
+  if (fuse)
+    launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
+#define PN_VEL_ROWS(D, O, PL, FU)                                                             \
   do {                                                                                        \
     if (s.vel_queue)                                                                          \
       hipLaunchKernelGGL(                                                                     \
-          (k_push_velocity_rows<D, O, PL, true>),                                             \
-          dim3(queue_blocks(s, reinterpret_cast<const void*>(&k_push_velocity_rows<D, O, PL, true>), \
-                            lds, vitems)),                                                    \
+          (k_push_velocity_rows<D, O, PL, true, FU>),                                         \
+          dim3(queue_blocks(                                                                  \
+              s, reinterpret_cast<const void*>(&k_push_velocity_rows<D, O, PL, true, FU>),    \
+              lds, vitems)),                                                                  \
           dim3(256), lds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B,  \
-          pc, yr, nsplit, vwq, nrow, vitems);                                                 \
+          pc, yr, nsplit, vwq, nrow, vitems, ps);                                             \
     else                                                                                      \
-      hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL, false>), dim3(nrow, vgy), dim3(256), \
-                         lds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E,     \
-                         s.d_B, pc, yr, nsplit, vwq, nrow, vitems);                           \
+      hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL, false, FU>), dim3(nrow, vgy),        \
+                         dim3(256), lds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, \
+                         s.d_E, s.d_B, pc, yr, nsplit, vwq, nrow, vitems, ps);                \
   } while (0)
 #define PN_VEL(D, O)                                                                          \
   do {                                                                                        \
-    if (rows && plain)                                                                        \
-      PN_VEL_ROWS(D, O, true);                                                                \
+    if (rows && plain && fuse)                                                                \
+      PN_VEL_ROWS(D, O, true, true);                                                          \
+    else if (rows && plain)                                                                   \
+      PN_VEL_ROWS(D, O, true, false);                                                         \
+    else if (rows && fuse)                                                                    \
+      PN_VEL_ROWS(D, O, false, true);                                                         \
     else if (rows)                                                                            \
-      PN_VEL_ROWS(D, O, false);                                                               \
+      PN_VEL_ROWS(D, O, false, false);                                                        \
     else                                                                                      \
       hipLaunchKernelGGL((k_push_velocity<D, O>), grid, dim3(256), 0, s.stream, g, s.pa,      \
                          s.d_np, s.d_off, s.d_lim, s.d_E, s.d_B, pc);                         \
@@ -640,6 +743,13 @@ void launch_push_velocity(Sim& s, double delt)
   }
 #undef PN_VEL
 #undef PN_VEL_ROWS
+  if (fuse) {
+    hipLaunchKernelGGL(k_leaver_tiles, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g,
+                       s.d_np, s.ss.key, s.ss.tilecnt, s.ss.ntile);
+    s.counted = true;
+    s.sorted  = false;
+  }
+  return fuse;
 }
 
 // chunk >= 0: the segments of that chunk only (their bin counters, keys, slots

@@ -155,7 +155,10 @@ struct Sim {
   int         vel_split   = 0; // 1: one gather block per species (0: all species)
   int         dep_queue   = 0; // 1: the march deposit runs persistent over a work queue
   int         vel_queue   = 0; // 1: the row gather runs persistent over a work queue
-  int         fuse_push   = 0; // FP32 step: the row gather also runs the position push
+  int         fuse_push   = 0; // pn_step: the row gather also runs the position push
+  int         tile32      = 0; // FP32 march deposit tile: 0 staged compensated pairs, 1 FP64,
+                               // 2 plain FP32, 3 compensated pairs by LDS atomics
+  int         stage_cells = 0; // FP32 staged tile: cells per phase (0: 4)
   int*        d_wq        = nullptr; // work-queue counters, 2 per launch site (kWq*)
   int         ncu         = 0;       // compute units of the device
   int         device      = 0;       // HIP device the handle was created on
@@ -234,7 +237,7 @@ void launch_fold(Sim& s, float* F, int k0, int k1, int phase);
 void launch_fold_current(Sim& s, int k0 = 0, int k1 = 4);
 // moment deposit into d_M (allocated on first use), without the Mom exchange
 void launch_deposit_moment(Sim& s);
-void launch_push_velocity(Sim& s, double delt);
+bool launch_push_velocity(Sim& s, double delt, bool with_position = false);
 void launch_push_position(Sim& s, double delt, int chunk = -1);
 void launch_count(Sim& s);
 void launch_migrate(Sim& s);

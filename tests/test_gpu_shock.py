@@ -74,6 +74,19 @@ def _compare(o, g, dim, exact):
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_shock_parity(name):
+    _shock_parity(name)
+
+
+@pytest.mark.parametrize("name", ["shock2d_o2", "shock3d_o2", "shock3d_ramp"])
+def test_shock_parity_fused_push(name, monkeypatch):
+    """The row gather runs the position push (wall reflection, upstream
+    marking, bins and leaver directions) for the particles it holds
+    (PICNIX_FUSED_PUSH=1): the same bars as the separate kernels."""
+    monkeypatch.setenv("PICNIX_FUSED_PUSH", "1")
+    _shock_parity(name)
+
+
+def _shock_parity(name):
     import picnix_amd
     dim, kw = CASES[name]
     cfg = helpers.small_shock(**kw)
