@@ -282,26 +282,34 @@ __device__ __forceinline__ void shape_wt(T x, T X, T rdx, T dt, T rdt, T* s)
 
 // Slot allocation with one atomic per distinct counter in the wave: lanes that
 // want a slot from the same counter share a single atomicAdd and take
-// consecutive slots in lane order.  Every lane of the wave must call this with
-// uniform control flow; lanes with want = false get -1.
+// consecutive slots in lane order.  The groups are formed first (ballots and
+// shuffles only); then every group's lowest lane issues its atomic in the same
+// instruction, so the wave waits for one memory round trip, not one per group.
+// Every active lane of the wave must call this; lanes with want = false get -1.
 __device__ __forceinline__ int wave_slot(int32_t* counters, int key, bool want)
 {
   const int          lane    = (int)__lane_id();
   unsigned long long pending = __ballot(want);
-  int                slot    = -1;
+  int                leader  = lane; // lowest lane of this lane's group
+  int                rank    = 0;    // place in the group
+  int                count   = 0;    // group size (on the leader)
   while (pending) {
-    const int          leader = __builtin_ctzll(pending);
-    const int          lk     = __shfl(key, leader);
-    unsigned long long m      = __ballot(want && key == lk);
-    int                base   = 0;
-    if (lane == leader)
-      base = atomicAdd(&counters[lk], __popcll(m));
-    base = __shfl(base, leader);
-    if ((m >> lane) & 1ull)
-      slot = base + __popcll(m & ((1ull << lane) - 1ull));
+    const int          ld = __builtin_ctzll(pending);
+    const int          lk = __shfl(key, ld);
+    unsigned long long m  = __ballot(want && key == lk);
+    if ((m >> lane) & 1ull) {
+      leader = ld;
+      rank   = __popcll(m & ((1ull << lane) - 1ull));
+    }
+    if (lane == ld)
+      count = __popcll(m);
     pending &= ~m;
   }
-  return slot;
+  int base = 0;
+  if (want && leader == lane)
+    base = atomicAdd(&counters[key], count);
+  base = __shfl(base, leader);
+  return want ? base + rank : -1;
 }
 
 // Workgroups of a launch are dispatched to the 8 XCDs of an MI355X round-robin

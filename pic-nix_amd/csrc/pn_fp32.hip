@@ -245,16 +245,6 @@ struct Pos32 {
 // destination bin here, so the sort needs no migration pass.  key: the new bin
 // (>= 0) for a particle that stays in its chunk, -2 - (d * nbin + bin) for one
 // that moves to the neighbour in direction d, -1 for one that is dropped.
-
-// r_old -> ro; r += u dt / (gamma d) (unwrapped, relative to the old cell).
-// Then the particle's destination for the sort: its own segment and new bin, or
-// for a leaver the neighbour chunk in its direction (chunk table nb) and the bin
-// there -- the periodic wrap is a shift of the cell index, so r stays as it is
-// (set_boundary_periodic, nix/xtensor_particle.hpp:359-376, and the migration,
-// nix/xtensor_halo3d.hpp:192-499).  Every particle takes a slot in its
-// destination bin here, so the sort needs no migration pass.  key: the new bin
-// (>= 0) for a particle that stays in its chunk, -2 - (d * nbin + bin) for one
-// that moves to the neighbour in direction d, -1 for one that is dropped.
 // With a non-periodic x (the shock problem) a particle that crossed x = 0 is
 // reflected (MainChunk::set_boundary_particle, example/shock/main.cpp:402-416)
 // and one past x = Lx is marked kKeyReplace for its redraw.
@@ -946,23 +936,19 @@ __global__ void __launch_bounds__(256)
   const int psz = N * TX;
   const int tsz = N * psz;
   // kTileStaged: the stage holds the per-cell sums of the kst cells of one
-  // phase of the row, [kst][NPL][N][N - 1], and two phases of cell flags
-  // [2][kst]; the ring (hi, lo) is updated only by the phase reductions.
+  // phase of the row, [kst][NPL][N][N - 1] (zeros for an empty cell); the
+  // ring (hi, lo) is updated only by the phase reductions.
   constexpr int CB    = NPL * N * (N - 1); // staged floats per cell
   float*        thi   = tile;
   float*        tlo   = tile + 3 * tsz;
   float*        stage = tile + 2 * 3 * tsz;
-  int*          cflag = reinterpret_cast<int*>(stage + (Tile == kTileStaged ? kst * CB : 0));
-  int*          rcs   = cflag + (Tile == kTileStaged ? 2 * kst : 0);
+  int*          rcs   = reinterpret_cast<int*>(stage + (Tile == kTileStaged ? kst * CB : 0));
   for (int q = threadIdx.x; q < 3 * tsz; q += blockDim.x) {
     if constexpr (Tile == kTileF64)
       tile_d[q] = 0.0;
     else
       thi[q] = tlo[q] = 0.0f;
   }
-  if constexpr (Tile == kTileStaged)
-    for (int q = threadIdx.x; q < 2 * kst; q += blockDim.x)
-      cflag[q] = 0;
   auto load_row = [&](int bz) {
     for (int q = threadIdx.x; q < ns * (nbx + 1); q += blockDim.x) {
       const int sp = q / (nbx + 1);
@@ -1010,6 +996,9 @@ __global__ void __launch_bounds__(256)
       for (int sp = 0; sp < ns; sp++)
         if (rcs[sp * (nbx + 1) + bx + 1] > rcs[sp * (nbx + 1) + bx])
           return true;
+      if constexpr (Tile == kTileStaged) // an empty cell stages zeros
+        for (int e = lane; e < CB; e += 64)
+          stage[(bx - cbeg) * CB + e] = 0.0f;
     }
   };
   // the first species >= sp of cell bx that holds particles, and its range
@@ -1037,7 +1026,6 @@ __global__ void __launch_bounds__(256)
     for (int ph = 0; ph < nph; ph++) {
     cbeg = Tile == kTileStaged ? ph * kst : 0;
     cend = Tile == kTileStaged ? min(nbx, cbeg + kst) : nbx;
-    int* const fl = cflag + ((bz * nph + ph) & 1) * kst; // this phase's cell flags
     // one claimed cell at a time; all its species accumulate into the same
     // registers, flushed to the tile once per cell
     int       bx, sp = 0, p0 = 0, p1 = 0;
@@ -1147,8 +1135,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int j = 0; j < N - 1; j++)
               st[r * (N - 1) + j] = acc[r][j];
-          if (plane == 0)
-            fl[bx - cbeg] = 1;
+
         } else if constexpr (Tile == kTileComp) {
           // per row: the N - 1 hi adds, then their N - 1 error adds
 #pragma unroll
@@ -1181,65 +1168,71 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
     if constexpr (Tile == kTileStaged) {
-      // Every tile node that the phase's cells reach takes their staged sums in
-      // ascending cell order (TwoSum into hi, lo; one thread per node).  After
-      // the row's last phase, plane zlo is complete and goes to J.  The flags
-      // of the next phase are cleared here.
+      // Every tile node that the phase's cells reach (columns cbeg .. cend+N-2)
+      // takes their staged sums in ascending cell order, TwoSum into (hi, lo),
+      // one thread per node: thread -> (column, row group), wp = the window
+      // width rounded up to a power of two.
       const bool last = ph == nph - 1;
-      const int  wx   = last ? TX : (cend - cbeg) + N - 1; // node columns to visit
-      const int  x0   = last ? 0 : cbeg;
-      const int  nq   = 3 * N * N * wx;
-      for (int q2 = threadIdx.x; q2 < nq; q2 += blockDim.x) {
-        const int rq = q2 / wx;
-        const int tx = x0 + (q2 - rq * wx);
-        const int ty = rq % N;
-        const int r3 = rq / N;
-        const int sl = r3 % N;
-        const int kc = r3 / N;
-        const int q  = kc * tsz + sl * psz + ty * TX + tx;
-        const int oz = sl >= zs ? sl - zs : sl - zs + N; // z offset from zlo
-        float     hi = thi[q], lo = tlo[q];
-        // the cells that reach this node are cb = tx - N + 1 + i (ascending),
-        // their staged entries base + i * step:
-        //   Jx: z = r, y = t, x = j + 1 -> i < N - 1, j = N - 2 - i
-        //   Jy: z = t, y = j + 1, x = r -> i < N, r = N - 1 - i
-        //   Jz: z = j + 1, y = r, x = t -> i < N, t = N - 1 - i
-        constexpr int PB   = N * (N - 1); // floats per plane block
-        const int     cb0  = tx - N + 1;
-        const int     cnt  = kc == 0 ? N - 1 : N;
-        const bool    ok   = kc == 0 || (kc == 1 ? ty >= 1 : oz >= 1);
-        const int     base = (cb0 - cbeg) * CB +
-                         (kc == 0 ? ty * PB + oz * (N - 1) + N - 2
-                          : kc == 1 ? (N + oz) * PB + (N - 1) * (N - 1) + ty - 1
-                                    : (3 * N - 1) * PB + ty * (N - 1) + oz - 1);
-        const int     step = CB - (kc == 0 ? 1 : kc == 1 ? N - 1 : PB);
-        if (ok) {
-          for (int i = 0; i < cnt; i++) {
-            const int cs = cb0 + i - cbeg; // stage slot
-            if ((unsigned)cs < (unsigned)(cend - cbeg) && fl[cs] != 0) {
-              const float x = stage[base + i * step];
-              lo += two_sum_err(hi, x);
-              hi += x;
-            }
+      const int  wx   = (cend - cbeg) + N - 1;
+      int        lw   = 0;
+      while ((1 << lw) < wx)
+        lw++;
+      const int col = (int)threadIdx.x & ((1 << lw) - 1);
+      const int tx  = cbeg + col;
+      if (col < wx) {
+        constexpr int PB = N * (N - 1); // floats per plane block
+        const int     cb0 = tx - N + 1;
+        for (int rr = (int)threadIdx.x >> lw; rr < 3 * N * N; rr += 256 >> lw) {
+          const int ty = rr % N;
+          const int sl = (rr / N) % N;
+          const int kc = rr / (N * N);
+          const int q  = kc * tsz + sl * psz + ty * TX + tx;
+          const int oz = sl >= zs ? sl - zs : sl - zs + N; // z offset from zlo
+          // the cells that reach this node are cb0 + i (ascending), their staged
+          // entries base + i * step:
+          //   Jx: z = r, y = t, x = j + 1 -> i < N - 1, j = N - 2 - i
+          //   Jy: z = t, y = j + 1, x = r -> i < N, r = N - 1 - i
+          //   Jz: z = j + 1, y = r, x = t -> i < N, t = N - 1 - i
+          if (kc == 1 ? ty < 1 : (kc == 2 && oz < 1))
+            continue;
+          const int cnt  = kc == 0 ? N - 1 : N;
+          const int base = (cb0 - cbeg) * CB +
+                           (kc == 0 ? ty * PB + oz * (N - 1) + N - 2
+                            : kc == 1 ? (N + oz) * PB + (N - 1) * (N - 1) + ty - 1
+                                      : (3 * N - 1) * PB + ty * (N - 1) + oz - 1);
+          const int step = CB - (kc == 0 ? 1 : kc == 1 ? N - 1 : PB);
+          // the phase's cells among cb0 .. cb0 + cnt - 1
+          const int i0 = max(0, cbeg - cb0), i1 = min(cnt, cend - cb0);
+          float     hi = thi[q], lo = tlo[q];
+          for (int i = i0; i < i1; i++) {
+            const float x = stage[base + i * step];
+            lo += two_sum_err(hi, x);
+            hi += x;
           }
+          thi[q] = hi;
+          tlo[q] = lo;
         }
-        if (last && sl == zs) {
-          const double val = (double)hi + (double)lo;
+      }
+      if (last) {
+        // plane zlo is complete: to J, and its ring slot cleared
+        __syncthreads();
+        for (int q = threadIdx.x; q < 3 * psz; q += blockDim.x) {
+          const int    kc  = q / psz;
+          const int    r2  = q - kc * psz;
+          const int    idx = kc * tsz + zs * psz + r2;
+          const double val = (double)thi[idx] + (double)tlo[idx];
           if (val != 0.0) {
+            const int ty = r2 / TX;
+            const int tx2 = r2 - ty * TX;
             const int gz = off[c * 3 + 0] + zlo, gy = off[c * 3 + 1] + ylo + ty,
-                      gx = off[c * 3 + 2] + xlo + tx;
+                      gx = off[c * 3 + 2] + xlo + tx2;
             if ((unsigned)gz < (unsigned)g.P[0] && (unsigned)gy < (unsigned)g.P[1] &&
                 (unsigned)gx < (unsigned)g.P[2])
               atomicAdd(&J[(kc + 1) * v + ((long)gz * g.P[1] + gy) * g.P[2] + gx], (float)val);
+            thi[idx] = tlo[idx] = 0.0f;
           }
-          hi = lo = 0.0f;
         }
-        thi[q] = hi;
-        tlo[q] = lo;
       }
-      int* const fn = cflag + ((bz * nph + ph + 1) & 1) * kst;
-      for (int q = threadIdx.x; q < kst; q += blockDim.x)
-        fn[q] = 0;
       if (!last) {
         if (threadIdx.x == 0)
           next_cell = cend;
@@ -1805,10 +1798,9 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
     const size_t mbase = sizeof(double) * 3 * (size_t)N * N * (g.bsy + N - 1) +
                          sizeof(int) * (size_t)g.Ns * (g.bsy + 1);
     // kTileStaged adds the stage of one phase of kst cells (per cell: 3N planes
-    // x N x (N-1) floats) and two phases of cell flags
+    // x N x (N-1) floats)
     const int    kst    = std::max(1, std::min(s.stage_cells > 0 ? s.stage_cells : 4, g.bsy));
-    const size_t mstage = mbase + sizeof(float) * (size_t)kst * 3 * N * N * (N - 1) +
-                          sizeof(int) * 2 * (size_t)kst;
+    const size_t mstage = mbase + sizeof(float) * (size_t)kst * 3 * N * N * (N - 1);
     const size_t wlds  = g.order == 2 ? March32<2>::WLDS : March32<4>::WLDS;
     const int    tile  = s.tile32 != kTileStaged ? s.tile32
                          : (mstage + wlds + 64 <= 65536 ? kTileStaged : kTileComp);

@@ -281,3 +281,51 @@ def test_fp32_keeps_particles_at_the_upper_chunk_face():
         assert n == 4, n
     finally:
         sim.close()
+
+
+@pytest.mark.parametrize("tile", ["stage1", "stage3", "stage16", "comp", "f64", "plain"])
+def test_fp32_deposit_tile_forms(tile, monkeypatch):
+    """The FP32 march deposit's tile forms against the default (staged
+    compensated pairs, 4 cells per phase): 1, 3 (a row of 8 cells in phases
+    of 3, 3 and 2) and 16 cells per phase, the pairs updated by LDS atomics, an
+    FP64 tile and a plain FP32 tile.  From the same state, one step gives
+    particles identical by id (the current does not act on them within the step)
+    and currents equal to FP32 summation round-off: within 1e-5 relative (the
+    plain tile, with no compensation, within 1e-4), and every form within 1e-4 of
+    the FP64 oracle."""
+    base = CASES["thermal3d_o2"][1]()
+    base["parameter"].update(Nx=32, Ny=8, Nz=8, Cx=4, Cy=1, Cz=1)  # chunks of 8 x 8 x 8 cells
+    cfg = _fp32(base)
+    dt = float(cfg["parameter"]["delt"])
+    if tile.startswith("stage"):
+        monkeypatch.setenv("PICNIX_STAGE_CELLS", tile[5:])
+    else:
+        monkeypatch.setenv("PICNIX_FP32_TILE", tile)
+    g = _sim(cfg)
+    monkeypatch.delenv("PICNIX_STAGE_CELLS", raising=False)
+    monkeypatch.delenv("PICNIX_FP32_TILE", raising=False)
+    d = _sim(cfg)
+    o = oracle_lib.Oracle(base)
+    for sim in (g, d):
+        sim.step(dt)
+        sim.synchronize()
+    o.step()
+    worst = 0.0
+    for c in range(g.nchunks):
+        for s in range(g.Ns):
+            ia, a = _by_id(g.particles(c, s))
+            ib, b = _by_id(d.particles(c, s))
+            assert np.array_equal(ia, ib) and np.array_equal(a[:, :6], b[:, :6]), (tile, c, s)
+        info = o.chunk_info(c)
+        ja = helpers.interior(g.field(c, 1), info, 3)
+        jb = helpers.interior(d.field(c, 1), info, 3)
+        for k in range(1, 4):
+            sc = max(float(np.max(np.abs(jb[..., k]))), 1e-30)
+            worst = max(worst, float(np.max(np.abs(ja[..., k] - jb[..., k]))) / sc)
+    rel = _field_errors(o, g, 3)
+    _say(f"[fp32] tile {tile}: J vs default {worst:.2e}, fields vs oracle {rel.max():.2e}")
+    assert worst <= (1e-4 if tile == "plain" else 1e-5)
+    assert rel.max() <= 1e-4
+    g.close()
+    d.close()
+    o.close()
