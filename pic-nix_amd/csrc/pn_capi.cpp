@@ -38,12 +38,16 @@ namespace {
 
 
 // Run `f` and convert failures into an error code + message on the handle.
+// keep_perm: the call may leave a deferred sort pending (pn_step); every other
+// entry point sees the particles in sorted order
 template <typename F>
-int guarded(pn_sim* h, F f)
+int guarded(pn_sim* h, F f, bool keep_perm = false)
 {
   if (!h)
     return PN_ERR_ARG;
   try {
+    if (!keep_perm)
+      materialize(h->s);
     int rc = f(h->s);
     if (rc == PN_OK) {
       hipError_t e = hipGetLastError();
@@ -255,6 +259,7 @@ void allocate(Sim& s)
   ss.didx         = dalloc<int32_t>(np);
   ss.mkey         = dalloc<int32_t>(np + kMemberPad);
   ss.msrc         = dalloc<int32_t>(np);
+  ss.perm         = dalloc<int32_t>(np);
   ss.tilecnt      = dalloc<int32_t>((size_t)g.nseg * ss.ntile * 27);
   ss.nout         = dalloc<int32_t>((size_t)g.nseg * 27);
   ss.recvoff      = dalloc<int32_t>((size_t)g.nseg * 27);
@@ -316,7 +321,7 @@ void release(Sim& s)
   f(s.d_np);
   SortScratch& ss = s.ss;
   for (void* p : {(void*)ss.cellcnt, (void*)ss.cellstart, (void*)ss.key, (void*)ss.slot,
-                  (void*)ss.dseg, (void*)ss.didx, (void*)ss.mkey, (void*)ss.msrc, (void*)ss.tilecnt,
+                  (void*)ss.dseg, (void*)ss.didx, (void*)ss.mkey, (void*)ss.msrc, (void*)ss.perm, (void*)ss.tilecnt,
                   (void*)ss.nout, (void*)ss.recvoff, (void*)ss.npnew, (void*)ss.errflag})
     f(p);
   f(s.d_stage);
@@ -622,6 +627,8 @@ int pn_create(const char* toml, int load, pn_sim** out)
     }
     if (const char* k = std::getenv("PICNIX_STAGE_CELLS")) // FP32 staged tile: cells per phase
       s.stage_cells = std::max(0, std::atoi(k));
+    if (const char* k = std::getenv("PICNIX_DEFER_SORT")) // 0: pn_step sorts in place
+      s.defer_sort = std::atoi(k) != 0;
     if (const char* k = std::getenv("PICNIX_RHO")) // "deposit": rho from particles every step
       s.rho_deposit = std::string(k) == "deposit";
     allocate(s);
@@ -1092,6 +1099,13 @@ namespace {
 // step then deposits rho from the particles, as the reference does)
 bool carry_rho(const Sim& s) { return s.rho_valid && !s.rho_deposit; }
 
+// pn_step leaves the record move of its sort to the next step's row gather
+// (periodic FP64 problems whose gather takes the row form)
+bool defers_sort(const Sim& s)
+{
+  return s.defer_sort && !s.fp32 && !s.g.xwall && gather_takes_perm(s);
+}
+
 void step_launches32(Sim& s, double delt)
 {
   launch_push_bfd(s, 0.5 * delt);
@@ -1176,7 +1190,9 @@ void step_launches(Sim& s, double delt)
   if (rho == RhoDone::Kept)
     launch_rho_continuity(s, delt);
   launch_push_efd(s, delt);
-  launch_migrate(s);
+  // a deferred rho deposit reads the particles in their new order, so the
+  // sort's record move is left to the next row gather only when rho is carried
+  launch_migrate(s, defers_sort(s) && rho != RhoDone::Deferred);
   if (rho == RhoDone::Deferred) {
     launch_rho(s, true);
     launch_fold_current(s, 0, 1);
@@ -1241,7 +1257,7 @@ int pn_step(pn_sim* h, double delt)
     if (rc == PN_OK)
       queue_step_flags(s);
     return rc;
-  });
+  }, /*keep_perm=*/true);
 }
 
 namespace {
@@ -1257,7 +1273,10 @@ int step_impl(Sim& s, double delt)
     // no phase timing (event records are host-side bookkeeping)
     // (and rho carried from the previous step: the recording uses the continuity update)
     // (the shock problem's boundary draws run on the host between launches)
-    if (!s.step_graph || !s.sorted || s.timing || !s.rho_valid || s.g.xwall) {
+    // (and, where pn_step defers its sort, the deferred sort of the previous
+    // step pending, as the recorded step leaves it)
+    if (!s.step_graph || !s.sorted || s.timing || !s.rho_valid || s.g.xwall ||
+        (defers_sort(s) && carry_rho(s) && !s.perm_pending)) {
       step_launches(s, delt);
       return PN_OK;
     }
@@ -1289,6 +1308,7 @@ int step_impl(Sim& s, double delt)
       s.np_host_valid = false;
       s.counted       = false;
       s.sorted        = true;
+      s.perm_pending  = defers_sort(s) && carry_rho(s);
     }
     HIPCHK(hipGraphLaunch(s.gexec[k], s.stream));
     return PN_OK;

@@ -1319,6 +1319,7 @@ void rho_impl(Sim& s, bool sorted_by_new_cell)
 
 RhoDone launch_deposit(Sim& s, double delt, RhoMode mode)
 {
+  materialize(s);
   const Geom& g = s.g;
   DepConsts   dc{};
   for (int sp = 0; sp < g.Ns; sp++)
@@ -1421,6 +1422,7 @@ void launch_rho_unject(Sim& s, int nmax)
 
 void launch_rho(Sim& s, bool sorted_by_new_cell)
 {
+  materialize(s);
   PhaseTimer  pt(s, 7);
   PhaseTimer  pr(s, kPhaseRhoDeposit);
   const Geom& g = s.g;

@@ -267,13 +267,17 @@ __device__ __forceinline__ int position64(const Geom& g, const PArr& p, const Po
 // step's next operator), so its position and momentum are read once for both;
 // the bin counters then count it and key holds its bin or exchange direction,
 // as after k_push_position (whose per-tile leaver counts k_leaver_tiles adds).
+// perm (a sort left pending by pn_step): sorted slot i holds record perm[i] of
+// p; the kernel reads it there and writes the whole record, pushed, to slot i
+// of dst.  Without perm, dst is p.
 template <int Dim, int Order, bool Plain, bool Queue, bool Fuse>
 __global__ void __launch_bounds__(256)
     k_push_velocity_rows(Geom g, PArr p, const int32_t* __restrict__ cellstart,
                          const int* __restrict__ off, const double* __restrict__ lim,
                          const double* __restrict__ E, const double* __restrict__ B,
                          PushConsts pc, int yrows, int nsplit,
-                         int* __restrict__ wq, int gx, int nitems, Pos64 ps)
+                         int* __restrict__ wq, int gx, int nitems, Pos64 ps,
+                         const int32_t* __restrict__ perm, PArr dst)
 {
   constexpr int is_odd = Order % 2 == 0 ? 0 : 1;
   constexpr int S      = Order + 1;
@@ -396,7 +400,8 @@ __global__ void __launch_bounds__(256)
   qmdt               = pc.qmdt[sp];
   for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
     const long i     = (long)seg * g.cap + q;
-    double     xu[6] = {p.x[i], p.y[i], p.z[i], p.ux[i], p.uy[i], p.uz[i]};
+    const long src   = perm ? (long)perm[i] : i;
+    double     xu[6] = {p.x[src], p.y[src], p.z[src], p.ux[src], p.uy[src], p.uz[src]};
     double     wix[S], wiy[S], wiz[S], whx[S], why[S], whz[S];
     int        ix0 = digitize(xu[0], ximin, rdx);
     int        iy0 = digitize(xu[1], yimin, rdy);
@@ -449,9 +454,7 @@ __global__ void __launch_bounds__(256)
         const int X = ix0 + offx, HX = hx0 + offx;
         if (min(Z, HZ) < 0 || max(Z, HZ) > g.P[0] - S || min(Y, HY) < 0 ||
             max(Y, HY) > g.P[1] - S || min(X, HX) < 0 || max(X, HX) > g.P[2] - S) {
-          if constexpr (!Fuse)
-            continue;
-          ok = false; // not pushed; its position push still runs below
+          ok = false; // not pushed (its record and position push still go on below)
         } else {
           eb[0] = interp3(E, sz, sy, Z, Y, HX, wiz, wiy, whx);
           eb[1] = interp3(E + v, sz, sy, Z, HY, X, wiz, why, wix);
@@ -477,8 +480,6 @@ __global__ void __launch_bounds__(256)
         const int Y = iy0 + offy, HY = hy0 + offy, X = ix0 + offx, HX = hx0 + offx;
         if (min(Y, HY) < 0 || max(Y, HY) > g.P[1] - S || min(X, HX) < 0 ||
             max(X, HX) > g.P[2] - S) {
-          if constexpr (!Fuse)
-            continue;
           ok = false;
         } else {
           eb[0] = interp2(E, sy, Y, HX, wiy, whx);
@@ -496,10 +497,12 @@ __global__ void __launch_bounds__(256)
       else
         push_momentum(pc.pusher, xu[3], xu[4], xu[5], eb, g.cc);
     }
+    if (perm)
+      dst.id[i] = p.id[src];
     if constexpr (Fuse) {
       int       dir;
-      const int b    = position64<true>(g, p, ps, lim, c, i, xu[0], xu[1], xu[2], xu[3], xu[4],
-                                        xu[5], dir);
+      const int b    = position64<true>(g, dst, ps, lim, c, i, xu[0], xu[1], xu[2], xu[3],
+                                        xu[4], xu[5], dir);
       const bool stay = b >= 0;
       const int  sl   = wave_slot(ps.cellcnt + (long)seg * g.nbin, b, stay);
       if (stay) {
@@ -507,9 +510,16 @@ __global__ void __launch_bounds__(256)
         ps.slot[i] = sl;
       }
     } else {
-      p.ux[i] = xu[3];
-      p.uy[i] = xu[4];
-      p.uz[i] = xu[5];
+      if (ok || perm) {
+        dst.ux[i] = xu[3];
+        dst.uy[i] = xu[4];
+        dst.uz[i] = xu[5];
+      }
+      if (perm) {
+        dst.x[i] = xu[0];
+        dst.y[i] = xu[1];
+        dst.z[i] = xu[2];
+      }
     }
   }
   } // species
@@ -652,6 +662,30 @@ __global__ void __launch_bounds__(256) k_soa_to_aos(PArr p, long base, long n,
 
 } // namespace
 
+// The row gather's launch shape: yr bin rows per block and the LDS they stage;
+// false when the row form does not apply (1D, unsorted particles, the generic
+// kernels, or a stencil tile beyond the LDS of a block).
+static bool vel_rows_shape(const Sim& s, int& yr, size_t& lds)
+{
+  const Geom& g      = s.g;
+  const int   T      = g.order + 2;
+  auto        lds_of = [&](int r) {
+    return sizeof(double) * 6 * (g.dim == 3 ? T : 1) * (T + r - 1) * (g.bsy + g.order + 1);
+  };
+  yr = s.vel_rows > 0 ? s.vel_rows : 4; // 0: four rows, or fewer where LDS is short
+  while (yr > 1 && lds_of(yr) > 65536)
+    yr = (yr + 1) / 2;
+  lds = lds_of(yr);
+  return g.dim >= 2 && s.sorted && !s.generic_kernels && !s.vel_generic && lds <= 65536;
+}
+
+bool gather_takes_perm(const Sim& s)
+{
+  int    yr;
+  size_t lds;
+  return vel_rows_shape(s, yr, lds);
+}
+
 // with_position: the row gather also runs the position push (+ count) when the
 // row form applies; returns whether it did (then launch_push_position is not
 // needed).
@@ -671,16 +705,17 @@ bool launch_push_velocity(Sim& s, double delt, bool with_position)
   dim3         grid(nblk(g.cap, 256), g.nseg);
   const int    T     = g.order + 2;
   const int    nby   = g.dim == 3 ? g.bsz / g.bsy : g.nbin / g.bsy;
-  auto         lds_of = [&](int r) {
-    return sizeof(double) * 6 * (g.dim == 3 ? T : 1) * (T + r - 1) * (g.bsy + g.order + 1);
-  };
-  int yr = s.vel_rows > 0 ? s.vel_rows : 4; // 0: four rows, or fewer where LDS is short
-  while (yr > 1 && lds_of(yr) > 65536)
-    yr = (yr + 1) / 2;
-  const size_t lds  = lds_of(yr);
-  const bool   rows = g.dim >= 2 && s.sorted && !s.generic_kernels && !s.vel_generic &&
-                    lds <= 65536;
+  int          yr;
+  size_t       lds;
+  const bool   rows = vel_rows_shape(s, yr, lds);
   const bool   fuse = rows && with_position;
+  // a sort left pending by the previous step: the row gather applies it;
+  // any other form needs the particles in place first
+  if (s.perm_pending && !rows)
+    materialize(s);
+  const bool     use_perm = s.perm_pending;
+  const int32_t* perm     = use_perm ? s.ss.perm : nullptr;
+  const PArr     dst      = use_perm ? s.pb : s.pa;
   const int    nrow  = (g.nbin / g.bsy / nby) * ((nby + yr - 1) / yr);
   const int    nsplit = s.vel_split ? g.Ns : 1;
   const bool   plain = pc.pusher == 0 && pc.wt == 0; // Boris + MC: no option branches
@@ -699,11 +734,11 @@ bool launch_push_velocity(Sim& s, double delt, bool with_position)
               s, reinterpret_cast<const void*>(&k_push_velocity_rows<D, O, PL, true, FU>),    \
               lds, vitems)),                                                                  \
           dim3(256), lds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_E, s.d_B,  \
-          pc, yr, nsplit, vwq, nrow, vitems, ps);                                             \
+          pc, yr, nsplit, vwq, nrow, vitems, ps, perm, dst);                                  \
     else                                                                                      \
       hipLaunchKernelGGL((k_push_velocity_rows<D, O, PL, false, FU>), dim3(nrow, vgy),        \
                          dim3(256), lds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, \
-                         s.d_E, s.d_B, pc, yr, nsplit, vwq, nrow, vitems, ps);                \
+                         s.d_E, s.d_B, pc, yr, nsplit, vwq, nrow, vitems, ps, perm, dst);     \
   } while (0)
 #define PN_VEL(D, O)                                                                          \
   do {                                                                                        \
@@ -743,6 +778,10 @@ bool launch_push_velocity(Sim& s, double delt, bool with_position)
   }
 #undef PN_VEL
 #undef PN_VEL_ROWS
+  if (use_perm) {
+    std::swap(s.pa, s.pb);
+    s.perm_pending = false;
+  }
   if (fuse) {
     hipLaunchKernelGGL(k_leaver_tiles, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g,
                        s.d_np, s.ss.key, s.ss.tilecnt, s.ss.ntile);
@@ -756,6 +795,7 @@ bool launch_push_velocity(Sim& s, double delt, bool with_position)
 // and tile counts; the other segments' keep what their own push left)
 void launch_push_position(Sim& s, double delt, int chunk)
 {
+  materialize(s);
   PhaseTimer  pt(s, 2);
   const Geom& g    = s.g;
   const int   seg0 = chunk < 0 ? 0 : chunk * g.Ns;

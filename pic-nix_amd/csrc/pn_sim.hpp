@@ -40,6 +40,7 @@ struct SortScratch {
   int32_t* didx      = nullptr; // [nseg*cap]     index in the destination's pre-sort array
   int32_t* mkey      = nullptr; // [nseg*cap+pad] member list of every bin (pre-sort indices)
   int32_t* msrc      = nullptr; // [nseg*cap]     where each member's record currently is
+  int32_t* perm      = nullptr; // [nseg*cap]     deferred sort: source record of each sorted slot
   int32_t* tilecnt   = nullptr; // [nseg][ntile][27] leavers per 256-particle tile and direction
   int32_t* nout      = nullptr; // [nseg][27]     leavers per direction
   int32_t* recvoff   = nullptr; // [nseg][27]     first pre-sort index of each receive slot
@@ -130,6 +131,11 @@ struct Sim {
   ShockState  shk{};
   bool        counted = false; // keys hold a fresh count (push_position / count)
   bool        sorted  = false; // particles are in sorted order and cellstart is valid
+  // the last sort (in pn_step) left its record move to the next row gather:
+  // sorted slot i of the buffer pb-to-be holds record perm[i] of pa.  Every
+  // other reader of the particles applies it first (materialize).
+  bool        perm_pending = false;
+  bool        defer_sort   = true; // PICNIX_DEFER_SORT=0: pn_step sorts in place
   // rho (component 0 of J) is the deposit of the current positions, left by
   // pn_step; any other call that moves particles or writes J clears it
   bool        rho_valid = false;
@@ -240,8 +246,12 @@ void launch_deposit_moment(Sim& s);
 bool launch_push_velocity(Sim& s, double delt, bool with_position = false);
 void launch_push_position(Sim& s, double delt, int chunk = -1);
 void launch_count(Sim& s);
-void launch_migrate(Sim& s);
+void launch_migrate(Sim& s, bool defer = false);
 void launch_sort(Sim& s);
+// apply a pending deferred sort (record move pa -> pb, swap); no-op otherwise
+void materialize(Sim& s);
+// whether the next row gather can read the particles through a pending sort
+bool gather_takes_perm(const Sim& s);
 // What the Esirkepov deposit does with rho (component 0 of J):
 //   Now   -- deposits it from the new positions, as the reference does;
 //   Defer -- where the kernel allows, leaves it for launch_rho after the sort;

@@ -256,20 +256,18 @@ __global__ void __launch_bounds__(256)
 // scalar loads.
 typedef const __attribute__((address_space(4))) int32_t* member_ptr;
 
-// Ranks of the lane's members (keys m0, and m1 when two) among all n members
-// of a bin.  The other members' keys are wave-uniform: they are read 16 at a
-// time with scalar loads and compared with the lane's keys (one compare and one
-// add per key and member).  The last group reads up to 15 entries past n and
-// ignores them (the member list is allocated with kMemberPad spare entries).
+// Adds to r0 (and r1 when Two) the number of members in [lo, hi) of a member
+// list whose keys are below m0 (m1).  The keys are wave-uniform: they are read
+// 16 at a time with scalar loads.  A lane whose member is not in this range
+// passes key 0, which no key is below.  The last group reads up to 15 entries
+// past hi and ignores them (the member list is allocated with kMemberPad spare
+// entries).
 template <bool Two>
-__device__ __forceinline__ void bin_ranks(const int32_t* mk, int n, unsigned int m0,
-                                          unsigned int m1, int& r0, int& r1)
+__device__ __forceinline__ void range_ranks(member_ptr ck, int lo, int hi, unsigned int m0,
+                                            unsigned int m1, int& r0, int& r1)
 {
-  const member_ptr ck = (member_ptr)mk;
-  r0                  = 0;
-  r1                  = 0;
-  int c1              = 0;
-  for (; c1 + 16 <= n; c1 += 16) {
+  int c1 = lo;
+  for (; c1 + 16 <= hi; c1 += 16) {
 #pragma unroll
     for (int j = 0; j < 16; j++) {
       const unsigned int o = (unsigned int)ck[c1 + j];
@@ -278,35 +276,15 @@ __device__ __forceinline__ void bin_ranks(const int32_t* mk, int n, unsigned int
         r1 += o < m1;
     }
   }
-  if (c1 < n) {
+  if (c1 < hi) {
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-      const unsigned int o = c1 + j < n ? (unsigned int)ck[c1 + j] : 0xFFFFFFFFu;
+      const unsigned int o = c1 + j < hi ? (unsigned int)ck[c1 + j] : 0xFFFFFFFFu;
       r0 += o < m0;
       if constexpr (Two)
         r1 += o < m1;
     }
   }
-}
-
-// Ranks of members c0 + lane and c0 + 64 + lane (keys m0, m1; valid below n).
-__device__ __forceinline__ void pair_ranks(const int32_t* mk, int n, int c0, unsigned int m0,
-                                           unsigned int m1, int& r0, int& r1)
-{
-  if (c0 + 64 < n)
-    bin_ranks<true>(mk, n, m0, m1, r0, r1);
-  else
-    bin_ranks<false>(mk, n, m0, m1, r0, r1);
-}
-
-// The lane's two member keys, loaded before the records so that the ranking
-// waits for them only (vector loads complete in order).
-__device__ __forceinline__ void lane_keys(const int32_t* mk, int n, int c0, int lane,
-                                          unsigned int& m0, unsigned int& m1)
-{
-  const int q0 = c0 + lane, q1 = q0 + 64;
-  m0           = q0 < n ? (unsigned int)mk[q0] : 0xFFFFFFFFu;
-  m1           = q1 < n ? (unsigned int)mk[q1] : 0xFFFFFFFFu;
 }
 
 struct Rec64 {
@@ -330,69 +308,117 @@ __device__ __forceinline__ void store_rec(const PArr& p, long i, const Rec64& r)
   p.id[i] = r.id;
 }
 
-// Rank the members of bin w (segment-major flat index) by (idx mod 8, idx) --
-// the order of XtensorParticle::sort -- and move each record to
-// start(bin) + rank.  One wavefront, two members per lane per pass; all lanes
-// call.
-__device__ __forceinline__ void rank_one_bin(const Geom& g, const PArr& src, const PArr& dst,
-                                             const int32_t* __restrict__ cellstart,
-                                             const int32_t* __restrict__ mkey,
-                                             const int32_t* __restrict__ msrc, long w, int lane)
+// Rank the members of bins b and b + 1 of segment seg (b even; b + 1 may be
+// past the last bin) by (idx mod 8, idx) -- the order of XtensorParticle::sort
+// -- each within its own bin, and move each record to start(bin) + rank
+// (Defer: record only where it comes from, perm[start(bin) + rank] = source).
+// The two bins' members are contiguous in the member list, so one wavefront
+// covers both with one member per lane (two per lane per pass of 128
+// members); all lanes call.
+template <bool Defer>
+__device__ __forceinline__ void rank_bin_pair(const Geom& g, const PArr& src, const PArr& dst,
+                                              const int32_t* __restrict__ cellstart,
+                                              const int32_t* __restrict__ mkey,
+                                              const int32_t* __restrict__ msrc,
+                                              int32_t* __restrict__ perm, int seg, int b,
+                                              int lane)
 {
-  const int      seg  = (int)(w / g.nbin);
-  const int      b    = (int)(w - (long)seg * g.nbin);
   const int32_t* cs   = cellstart + (long)seg * (g.nbin + 1);
-  const int      base = cs[b];
-  const int      end  = min(cs[b + 1], (int)g.cap);
+  const int      cap  = (int)g.cap;
+  const int      base = min(cs[b], cap);
+  const int      mid  = min(cs[b + 1], cap);
+  const int      end  = b + 1 < g.nbin ? min(cs[b + 2], cap) : mid;
   if (base >= end)
     return;
-  const int      n  = end - base;
-  const int32_t* mk = mkey + (long)seg * g.cap + base;
-  const int32_t* ms = msrc + (long)seg * g.cap + base;
+  const int        n  = end - base;
+  const int        nA = mid - base;
+  const int32_t*   mk = mkey + (long)seg * g.cap + base;
+  const int32_t*   ms = msrc + (long)seg * g.cap + base;
+  const member_ptr ck = (member_ptr)mk;
+  const long       d0 = (long)seg * g.cap;
   for (int c0 = 0; c0 < n; c0 += 128) {
-    const int    q0 = c0 + lane, q1 = q0 + 64;
-    unsigned int m0, m1;
-    lane_keys(mk, n, c0, lane, m0, m1);
+    const int          q0 = c0 + lane, q1 = q0 + 64;
+    const unsigned int m0 = q0 < n ? (unsigned int)mk[q0] : 0xFFFFFFFFu;
+    const unsigned int m1 = q1 < n ? (unsigned int)mk[q1] : 0xFFFFFFFFu;
     // the records are loaded before the ranking, which overlaps their latency
-    Rec64 a{}, c{};
-    if (q0 < n)
-      a = load_rec(src, ms[q0]);
-    if (q1 < n)
-      c = load_rec(src, ms[q1]);
-    int r0, r1;
-    pair_ranks(mk, n, c0, m0, m1, r0, r1);
-    const long d0 = (long)seg * g.cap + base;
-    if (q0 < n)
-      store_rec(dst, d0 + r0, a);
-    if (q1 < n)
-      store_rec(dst, d0 + r1, c);
+    Rec64   a{}, c{};
+    int32_t s0 = 0, s1 = 0;
+    if (q0 < n) {
+      s0 = ms[q0];
+      if constexpr (!Defer)
+        a = load_rec(src, s0);
+    }
+    if (q1 < n) {
+      s1 = ms[q1];
+      if constexpr (!Defer)
+        c = load_rec(src, s1);
+    }
+    // each member is ranked among its own bin's members only
+    const bool inA0 = q0 < nA, inA1 = q1 < nA;
+    int        r0 = 0, r1 = 0;
+    if (c0 + 64 < n) {
+      range_ranks<true>(ck, 0, nA, inA0 ? m0 : 0u, inA1 ? m1 : 0u, r0, r1);
+      range_ranks<true>(ck, nA, n, inA0 ? 0u : m0, inA1 ? 0u : m1, r0, r1);
+    } else {
+      range_ranks<false>(ck, 0, nA, inA0 ? m0 : 0u, 0u, r0, r1);
+      range_ranks<false>(ck, nA, n, inA0 ? 0u : m0, 0u, r0, r1);
+    }
+    if constexpr (Defer) {
+      if (q0 < n)
+        perm[d0 + (inA0 ? base : mid) + r0] = s0;
+      if (q1 < n)
+        perm[d0 + (inA1 ? base : mid) + r1] = s1;
+    } else {
+      if (q0 < n)
+        store_rec(dst, d0 + (inA0 ? base : mid) + r0, a);
+      if (q1 < n)
+        store_rec(dst, d0 + (inA1 ? base : mid) + r1, c);
+    }
   }
 }
 
-// A block owns kRankBins consecutive bins, one per wavefront (more bins per
-// block, walked in turn, measured slower: fewer bins in flight per CU);
-// consecutive blocks run on one XCD (xcd_block), so records moving between
-// neighbouring bins are read through the same L2.
-constexpr int kRankBins = 4;
+// A block owns kRankPairs consecutive bin pairs of one segment, one per
+// wavefront; consecutive blocks run on one XCD (xcd_block), so records moving
+// between neighbouring bins are read through the same L2.
+constexpr int kRankPairs = 4;
 
+template <bool Defer>
 __global__ void __launch_bounds__(256)
     k_rank_bin(Geom g, PArr src, PArr dst, const int32_t* __restrict__ cellstart,
-               const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc, long nblocks)
+               const int32_t* __restrict__ mkey, const int32_t* __restrict__ msrc,
+               int32_t* __restrict__ perm, int npairs, long nblocks)
 {
   const long blk = xcd_block(blockIdx.x, nblocks);
   if (blk >= nblocks)
     return;
-  const int  lane  = threadIdx.x & 63;
-  const long nbins = (long)g.nseg * g.nbin;
-  const long wend  = min(nbins, (blk + 1) * kRankBins);
-  // the wavefront index, marked wave-uniform so that the bin's member list is
+  const int lane = threadIdx.x & 63;
+  // the wavefront index, marked wave-uniform so that the member lists are
   // read with scalar loads
-  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  for (long w = blk * kRankBins + wv; w < wend; w += blockDim.x >> 6)
-    rank_one_bin(g, src, dst, cellstart, mkey, msrc, w, lane);
+  const int  wv   = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const long pair = blk * kRankPairs + wv; // segment-major: npairs per segment
+  if (pair >= (long)g.nseg * npairs)
+    return;
+  const int seg = (int)(pair / npairs);
+  const int b   = 2 * (int)(pair - (long)seg * npairs);
+  rank_bin_pair<Defer>(g, src, dst, cellstart, mkey, msrc, perm, seg, b, lane);
 }
 
-void finish_sort(Sim& s)
+// The record move of a deferred sort: sorted slot i of every segment takes
+// record perm[i] (any segment) of src.
+__global__ void __launch_bounds__(256)
+    k_apply_perm(Geom g, PArr src, PArr dst, const int32_t* __restrict__ np,
+                 const int32_t* __restrict__ perm)
+{
+  const int  seg = blockIdx.y;
+  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
+    return;
+  const long i = (long)seg * g.cap + ip;
+  store_rec(dst, i, load_rec(src, perm[i]));
+}
+
+// defer: the record move is left to the next row gather (perm_pending)
+void finish_sort(Sim& s, bool defer = false)
 {
   const Geom& g = s.g;
   hipLaunchKernelGGL(k_cell_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.ss.cellcnt,
@@ -400,12 +426,19 @@ void finish_sort(Sim& s)
   hipLaunchKernelGGL(k_members, dim3(xcd_grid((long)g.nseg * s.ss.ntile)), dim3(256), 0, s.stream,
                      g, s.d_np, s.ss.cellstart, s.ss.key, s.ss.slot, s.ss.dseg, s.ss.didx,
                      s.ss.mkey, s.ss.msrc, s.ss.ntile);
-  const long nwave = (long)g.nseg * g.nbin;
-  const long nrb   = nblk(nwave, kRankBins);
-  hipLaunchKernelGGL(k_rank_bin, dim3(xcd_grid(nrb)), dim3(256), 0, s.stream, g, s.pa, s.pb,
-                     s.ss.cellstart, s.ss.mkey, s.ss.msrc, nrb);
+  const int  npairs = (g.nbin + 1) / 2;
+  const long nrb    = nblk((long)g.nseg * npairs, kRankPairs);
+  if (defer)
+    hipLaunchKernelGGL(k_rank_bin<true>, dim3(xcd_grid(nrb)), dim3(256), 0, s.stream, g, s.pa,
+                       s.pb, s.ss.cellstart, s.ss.mkey, s.ss.msrc, s.ss.perm, npairs, nrb);
+  else
+    hipLaunchKernelGGL(k_rank_bin<false>, dim3(xcd_grid(nrb)), dim3(256), 0, s.stream, g, s.pa,
+                       s.pb, s.ss.cellstart, s.ss.mkey, s.ss.msrc, s.ss.perm, npairs, nrb);
   launch_copy(s, s.d_np, s.ss.npnew, g.nseg);
-  std::swap(s.pa, s.pb);
+  if (defer)
+    s.perm_pending = true;
+  else
+    std::swap(s.pa, s.pb);
   s.np_host_valid = false;
   s.counted       = false;
   s.sorted        = true;
@@ -541,8 +574,20 @@ void finish_sort32(Sim& s)
 
 } // namespace
 
-void launch_migrate(Sim& s)
+void materialize(Sim& s)
 {
+  if (!s.perm_pending)
+    return;
+  const Geom& g = s.g;
+  hipLaunchKernelGGL(k_apply_perm, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.pa,
+                     s.pb, s.d_np, s.ss.perm);
+  std::swap(s.pa, s.pb);
+  s.perm_pending = false;
+}
+
+void launch_migrate(Sim& s, bool defer)
+{
+  materialize(s);
   PhaseTimer  pt(s, 6);
   const Geom& g = s.g;
   hipLaunchKernelGGL(k_tile_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.d_np,
@@ -553,11 +598,12 @@ void launch_migrate(Sim& s)
   hipLaunchKernelGGL(k_migrate, grid, dim3(256), 0, s.stream, g, s.pa, s.d_np, s.d_lim, s.d_rng,
                      s.d_nb, s.ss.tilecnt, s.ss.recvoff, s.ss.cellcnt, s.ss.key, s.ss.slot,
                      s.ss.dseg, s.ss.didx, s.ss.ntile);
-  finish_sort(s);
+  finish_sort(s, defer);
 }
 
 void launch_sort(Sim& s)
 {
+  materialize(s);
   PhaseTimer pt(s, 6);
   launch_count(s);
   finish_sort(s);

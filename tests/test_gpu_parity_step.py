@@ -217,6 +217,31 @@ def test_step_parity_fused_push(kind, queue, monkeypatch):
     o.close()
 
 
+@pytest.mark.parametrize("kind", ["thermal3d_o2", "twostream2d_o1"])
+def test_step_parity_sort_in_place(kind, monkeypatch):
+    """pn_step's sort moves the records itself (PICNIX_DEFER_SORT=0) instead of
+    leaving the move to the next step's row gather (the default, which every
+    other parity test runs): same bars."""
+    monkeypatch.setenv("PICNIX_DEFER_SORT", "0")
+    dim, make = CASES[kind]
+    cfg = make()
+    dt = float(cfg["parameter"]["delt"])
+    _say(f"[parity] sort in place {kind}: create")
+    o = oracle_lib.Oracle(cfg)
+    g = _sim(cfg)
+    for k in range(10):
+        o.step()
+        g.step(dt)
+    g.synchronize()
+    perr = _compare_particles(o, g, exact=False)
+    rel = _compare_fields(o, g, dim)
+    _say(f"[parity] sort in place {kind}: step 10 particle rel err {perr:.3e}, fields {rel}")
+    assert perr <= TOL
+    assert max(rel.values()) <= TOL
+    g.close()
+    o.close()
+
+
 def _with_fields(cfg, E=(0.0, 0.0, 0.0), B=(0.0, 0.0, 0.0)):
     p = cfg["parameter"]
     p.update(Ex=E[0], Ey=E[1], Ez=E[2], Bx=B[0], By=B[1], Bz=B[2])
