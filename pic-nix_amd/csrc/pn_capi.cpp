@@ -1100,10 +1100,10 @@ namespace {
 bool carry_rho(const Sim& s) { return s.rho_valid && !s.rho_deposit; }
 
 // pn_step leaves the record move of its sort to the next step's row gather
-// (periodic FP64 problems whose gather takes the row form)
+// (periodic problems whose gather takes the row form)
 bool defers_sort(const Sim& s)
 {
-  return s.defer_sort && !s.fp32 && !s.g.xwall && gather_takes_perm(s);
+  return s.defer_sort && !s.g.xwall && gather_takes_perm(s);
 }
 
 void step_launches32(Sim& s, double delt)
@@ -1119,7 +1119,7 @@ void step_launches32(Sim& s, double delt)
   if (rho == RhoDone::Kept)
     launch_rho_continuity(s, delt);
   launch_push_efd(s, delt);
-  launch32_migrate(s);
+  launch32_migrate(s, defers_sort(s) && rho != RhoDone::Deferred);
   if (rho == RhoDone::Deferred) {
     launch32_rho(s);
     launch_fold_current(s, 0, 1);

@@ -443,6 +443,14 @@ __device__ __forceinline__ void plane_offsets(int comp, int t, int zs, int TX, i
     co[j] = (j + 1) * sj - ((comp == 2 && j + 1 >= lim) ? wrap : 0);
 }
 
+// An FP32 offset folded back into [0, 1) (the largest float below 1 when the
+// subtraction rounds up to 1).
+__device__ __forceinline__ float frac01(float r)
+{
+  const float f = r - floorf(r);
+  return f < 1.0f ? f : 0.99999994f;
+}
+
 // FP32 particle storage (precision = "fp32"): position as the cell index (the
 // chunk-local bin, packed like the sort bins) plus the offset inside the cell
 // in cell units, r in [0, 1) after every sort; between the position push and

@@ -299,12 +299,17 @@ __device__ __forceinline__ void position32(const Geom& g, const P32& p, const Po
 // One block per (bin row, segment) of sorted particles: the row's E and B
 // stencil region is staged in LDS, as in the FP64 row kernel; one thread per
 // particle, (Order+1)^Dim-point gathers with x innermost.
+// perm (a sort left pending by pn_step): sorted slot i holds record perm[i] of
+// p, with its offset relative to its old cell; the kernel reads it there, finds
+// its bin (as the position push did, dest32) and its offset in [0, 1), and
+// writes the whole record, pushed, to slot i of dst.  Without perm, dst is p.
 template <int Dim, int Order, bool Queue, bool Fuse>
 __global__ void __launch_bounds__(256)
     k32_push_velocity_rows(Geom g, P32 p, const int32_t* __restrict__ cellstart,
                            const int* __restrict__ off, const float* __restrict__ E,
                            const float* __restrict__ B, Push32 pc, Pos32 ps, int yrows,
-                           int nsplit, int* __restrict__ wq, int gx, int nitems)
+                           int nsplit, int* __restrict__ wq, int gx, int nitems,
+                           const int32_t* __restrict__ perm, P32 dst)
 {
   constexpr int is_odd = Order % 2;
   constexpr int S      = Order + 1;
@@ -400,10 +405,26 @@ __global__ void __launch_bounds__(256)
   const int      p0  = cs[b0], p1 = cs[b0 + nr * nbx];
   qmdt               = pc.qmdt[sp];
   for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
-    const long  i    = (long)seg * g.cap + q;
-    const int   bin  = p.cell[i];
-    const Cell3 cl   = cell_of(g, bin);
-    const float r[3] = {p.rx[i], p.ry[i], p.rz[i]};
+    const long i   = (long)seg * g.cap + q;
+    long       src = i;
+    int        bin;
+    float      r[3];
+    if (perm) {
+      src           = perm[i];
+      const float x = p.rx[src], y = p.ry[src], z = p.rz[src];
+      int         ds, b, k;
+      dest32(g, (int)(src / g.cap), p.cell[src], x, y, z, ps.nb, ds, b, k);
+      bin  = b;
+      r[0] = g.has[2] ? frac01(x) : x;
+      r[1] = g.has[1] ? frac01(y) : y;
+      r[2] = g.has[0] ? frac01(z) : z;
+    } else {
+      bin  = p.cell[i];
+      r[0] = p.rx[i];
+      r[1] = p.ry[i];
+      r[2] = p.rz[i];
+    }
+    const Cell3 cl = cell_of(g, bin);
     float       wix[S], wiy[S], wiz[S], whx[S], why[S], whz[S];
     int         ix0, iy0, iz0 = 0, hx0, hy0, hz0 = 0;
     int_w<Order>(pc, 0, r[0], ix0, wix);
@@ -427,7 +448,7 @@ __global__ void __launch_bounds__(256)
     // sorted particles sit in the row's bins, so the stencils are inside the tile
     const bool in = min(X, HX) >= 0 && max(X, HX) + S <= TXv && min(Y, HY) >= 0 &&
                     max(Y, HY) + S <= TY && min(Z, HZ) >= 0 && max(Z, HZ) + (Dim == 3 ? S : 1) <= TZ;
-    float ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+    float ux = p.ux[src], uy = p.uy[src], uz = p.uz[src];
     if (in) {
       const long bsz = (long)TY * TXv, bsy = TXv;
       const int  nz  = Dim == 3 ? S : 1;
@@ -440,14 +461,25 @@ __global__ void __launch_bounds__(256)
       eb[5] = interp(ft32 + 5 * pc1, bsz, bsy, Z, HY, HX, wiz, why, whx, nz);
       push_momentum<float>(pc.pusher, ux, uy, uz, eb, (float)g.cc);
     }
+    if (perm) {
+      dst.id[i]   = p.id[src];
+      dst.cell[i] = bin;
+    }
     if constexpr (Fuse) {
       // the position push of this particle with its new momentum (the step's
       // next operator), so r and u are read once for both
-      position32<true>(g, p, ps, i, seg, r[0], r[1], r[2], ux, uy, uz, bin);
-    } else if (in) {
-      p.ux[i] = ux;
-      p.uy[i] = uy;
-      p.uz[i] = uz;
+      position32<true>(g, dst, ps, i, seg, r[0], r[1], r[2], ux, uy, uz, bin);
+    } else {
+      if (in || perm) {
+        dst.ux[i] = ux;
+        dst.uy[i] = uy;
+        dst.uz[i] = uz;
+      }
+      if (perm) {
+        dst.rx[i] = r[0];
+        dst.ry[i] = r[1];
+        dst.rz[i] = r[2];
+      }
     }
   }
   } // species
@@ -1662,6 +1694,55 @@ static Pos32 pos32_args(Sim& s, double delt, int on)
 
 // with_position: when the row gather runs, it also runs each particle's
 // position push (k32_push_position's work); returns true if it did.
+// The FP32 row gather's launch shape: yr bin rows per block and the LDS they
+// stage; false when the row form does not apply.
+static bool vel32_rows_shape(const Sim& s, int& yr, size_t& lds)
+{
+  const Geom& g      = s.g;
+  const int   T      = g.order + 2;
+  auto        lds_of = [&](int r) {
+    return sizeof(float) * 6 * (g.dim == 3 ? T : 1) * (T + r - 1) * (g.bsy + g.order + 1);
+  };
+  yr = s.vel_rows > 0 ? s.vel_rows : 4; // 0: four rows, or fewer where LDS is short
+  while (yr > 1 && lds_of(yr) > 65536)
+    yr = (yr + 1) / 2;
+  lds = lds_of(yr);
+  return s.sorted && !s.generic_kernels && !s.vel_generic && lds <= 65536;
+}
+
+bool gather32_takes_perm(const Sim& s)
+{
+  int    yr;
+  size_t lds;
+  return vel32_rows_shape(s, yr, lds);
+}
+
+// The record move of a deferred FP32 sort: sorted slot i of every segment
+// takes record perm[i] (any segment), its bin found again (dest32) and its
+// offset folded into [0, 1), as the sort's scatter writes it.
+__global__ void __launch_bounds__(256)
+    k32_apply_perm(Geom g, P32 src, P32 dst, const int32_t* __restrict__ np,
+                   const int32_t* __restrict__ perm, const int* __restrict__ nb)
+{
+  const int  seg = blockIdx.y;
+  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np[seg])
+    return;
+  const long  i = (long)seg * g.cap + ip;
+  const long  j = perm[i];
+  const float x = src.rx[j], y = src.ry[j], z = src.rz[j];
+  int         ds, b, k;
+  dest32(g, (int)(j / g.cap), src.cell[j], x, y, z, nb, ds, b, k);
+  dst.rx[i]   = g.has[2] ? frac01(x) : x;
+  dst.ry[i]   = g.has[1] ? frac01(y) : y;
+  dst.rz[i]   = g.has[0] ? frac01(z) : z;
+  dst.ux[i]   = src.ux[j];
+  dst.uy[i]   = src.uy[j];
+  dst.uz[i]   = src.uz[j];
+  dst.id[i]   = src.id[j];
+  dst.cell[i] = b;
+}
+
 bool launch32_push_velocity(Sim& s, double delt, bool with_position)
 {
   PhaseTimer  pt(s, 1);
@@ -1675,16 +1756,17 @@ bool launch32_push_velocity(Sim& s, double delt, bool with_position)
     pc.cdt[a]  = (float)(g.cc * delt / g.del[2 - a]);
     pc.rcdt[a] = 1.0f / pc.cdt[a];
   }
-  const int    T    = g.order + 2;
   const int    nby  = g.dim == 3 ? g.bsz / g.bsy : g.nbin / g.bsy;
-  auto         lds_of = [&](int r) {
-    return sizeof(float) * 6 * (g.dim == 3 ? T : 1) * (T + r - 1) * (g.bsy + g.order + 1);
-  };
-  int yr = s.vel_rows > 0 ? s.vel_rows : 4; // 0: four rows, or fewer where LDS is short
-  while (yr > 1 && lds_of(yr) > 65536)
-    yr = (yr + 1) / 2;
-  const size_t lds  = lds_of(yr);
-  const bool   rows = s.sorted && !s.generic_kernels && !s.vel_generic && lds <= 65536;
+  int          yr;
+  size_t       lds;
+  const bool   rows = vel32_rows_shape(s, yr, lds);
+  // a sort left pending by the previous step: the row gather applies it;
+  // any other form needs the particles in place first
+  if (s.perm_pending && !rows)
+    materialize(s);
+  const bool     use_perm = s.perm_pending;
+  const int32_t* perm     = use_perm ? s.ss.perm : nullptr;
+  const P32      dst      = use_perm ? s.qb : s.qa;
   const int    nblk_rows = (g.nbin / g.bsy / nby) * ((nby + yr - 1) / yr);
   const int    nsplit    = s.vel_split ? g.Ns : 1;
   const bool   fused     = rows && with_position;
@@ -1698,7 +1780,7 @@ bool launch32_push_velocity(Sim& s, double delt, bool with_position)
                              lds, nitems))                                                    \
                        : dim3(nblk_rows, gy),                                                 \
                      dim3(256), lds, s.stream, g, s.qa, s.ss.cellstart, s.d_off, s.f_E, s.f_B, \
-                     pc, ps, yr, nsplit, wq, nblk_rows, nitems)
+                     pc, ps, yr, nsplit, wq, nblk_rows, nitems, perm, dst)
 #define PN32_VEL(D, O)                                                                        \
   do {                                                                                        \
     if (rows) {                                                                               \
@@ -1734,6 +1816,10 @@ bool launch32_push_velocity(Sim& s, double delt, bool with_position)
   }
 #undef PN32_VEL
 #undef PN32_VEL_ROWS
+  if (use_perm) {
+    std::swap(s.qa, s.qb);
+    s.perm_pending = false;
+  }
   if (fused) {
     s.counted = true;
     s.sorted  = false;
@@ -1741,8 +1827,20 @@ bool launch32_push_velocity(Sim& s, double delt, bool with_position)
   return fused;
 }
 
+void materialize32(Sim& s)
+{
+  if (!s.perm_pending)
+    return;
+  const Geom& g = s.g;
+  hipLaunchKernelGGL(k32_apply_perm, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.qa,
+                     s.qb, s.d_np, s.ss.perm, s.d_nb);
+  std::swap(s.qa, s.qb);
+  s.perm_pending = false;
+}
+
 void launch32_push_position(Sim& s, double delt)
 {
+  materialize(s);
   PhaseTimer  pt(s, 2);
   const Geom& g = s.g;
   launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
@@ -1776,6 +1874,7 @@ void launch32_inject_append(Sim& s)
 // The FP32 kernels deposit J only; rho follows as the mode asks.
 RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
 {
+  materialize(s);
   const Geom& g = s.g;
   Dep32       dc{};
   for (int sp = 0; sp < g.Ns; sp++)
@@ -1880,6 +1979,7 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
 
 void launch32_rho(Sim& s)
 {
+  materialize(s);
   PhaseTimer  pt(s, 7);
   PhaseTimer  pr(s, kPhaseRhoDeposit);
   const Geom& g = s.g;

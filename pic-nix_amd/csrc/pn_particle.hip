@@ -681,6 +681,8 @@ static bool vel_rows_shape(const Sim& s, int& yr, size_t& lds)
 
 bool gather_takes_perm(const Sim& s)
 {
+  if (s.fp32)
+    return gather32_takes_perm(s);
   int    yr;
   size_t lds;
   return vel_rows_shape(s, yr, lds);

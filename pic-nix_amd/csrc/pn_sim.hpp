@@ -298,7 +298,9 @@ bool launch32_push_velocity(Sim& s, double delt, bool with_position = false);
 void launch32_push_position(Sim& s, double delt);
 RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode);
 void launch32_rho(Sim& s);
-void launch32_migrate(Sim& s);
+void launch32_migrate(Sim& s, bool defer = false);
+void materialize32(Sim& s);
+bool gather32_takes_perm(const Sim& s);
 void launch32_sort(Sim& s);
 void launch_aos_to_soa(Sim& s, int seg, int64_t np);
 void launch_soa_to_aos(Sim& s, int seg, int64_t np);

@@ -486,14 +486,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// An FP32 offset folded back into [0, 1) (the largest float below 1 when the
-// subtraction rounds up to 1).
-__device__ __forceinline__ float frac01_s(float r)
-{
-  const float f = r - floorf(r);
-  return f < 1.0f ? f : 0.99999994f;
-}
-
 struct Rec32 {
   float   rx, ry, rz, ux, uy, uz;
   int64_t id;
@@ -507,9 +499,9 @@ __device__ __forceinline__ Rec32 load_rec32(const P32& p, long i)
 __device__ __forceinline__ void store_rec32(const Geom& g, const P32& p, long i, const Rec32& r,
                                             int b)
 {
-  p.rx[i]   = g.has[2] ? frac01_s(r.rx) : r.rx;
-  p.ry[i]   = g.has[1] ? frac01_s(r.ry) : r.ry;
-  p.rz[i]   = g.has[0] ? frac01_s(r.rz) : r.rz;
+  p.rx[i]   = g.has[2] ? frac01(r.rx) : r.rx;
+  p.ry[i]   = g.has[1] ? frac01(r.ry) : r.ry;
+  p.rz[i]   = g.has[0] ? frac01(r.rz) : r.rz;
   p.ux[i]   = r.ux;
   p.uy[i]   = r.uy;
   p.uz[i]   = r.uz;
@@ -522,10 +514,14 @@ __device__ __forceinline__ void store_rec32(const Geom& g, const P32& p, long i,
 // set to the bin.  One pass, no member lists and no ranking: inside a bin the
 // order is the order of the slot atomics, not the reference's (idx mod 8, idx)
 // (the FP32 form is compared with the reference particle by particle, by id).
+// Defer: the record stays where it is; perm[destination] = its index (the
+// next row gather, or materialize, moves it).
+template <bool Defer>
 __global__ void __launch_bounds__(256)
     k32_scatter(Geom g, P32 src, P32 dst, const int32_t* __restrict__ np,
                 const int32_t* __restrict__ cellstart, const int32_t* __restrict__ key,
-                const int32_t* __restrict__ slot, const int* __restrict__ nb, int ntile)
+                const int32_t* __restrict__ slot, const int* __restrict__ nb, int ntile,
+                int32_t* __restrict__ perm)
 {
   // consecutive tiles on one XCD (as in k_members): records moving to
   // neighbouring bins meet in one L2
@@ -554,19 +550,29 @@ __global__ void __launch_bounds__(256)
   const long pos = (long)cellstart[(long)ds * (g.nbin + 1) + b] + slot[i];
   if (pos >= g.cap)
     return; // overflow, flagged by k_cell_scan
-  store_rec32(g, dst, (long)ds * g.cap + pos, load_rec32(src, i), b);
+  if constexpr (Defer)
+    perm[(long)ds * g.cap + pos] = (int32_t)i;
+  else
+    store_rec32(g, dst, (long)ds * g.cap + pos, load_rec32(src, i), b);
 }
 
-void finish_sort32(Sim& s)
+void finish_sort32(Sim& s, bool defer = false)
 {
   const Geom& g = s.g;
   hipLaunchKernelGGL(k_cell_scan, dim3(g.nseg), dim3(256), 0, s.stream, g, s.ss.cellcnt,
                      s.ss.cellstart, s.ss.npnew, s.ss.errflag);
-  hipLaunchKernelGGL(k32_scatter, dim3(xcd_grid((long)g.nseg * s.ss.ntile)), dim3(256), 0,
-                     s.stream, g, s.qa, s.qb, s.d_np, s.ss.cellstart, s.ss.key, s.ss.slot, s.d_nb,
-                     s.ss.ntile);
+  const dim3 grid(xcd_grid((long)g.nseg * s.ss.ntile));
+  if (defer)
+    hipLaunchKernelGGL(k32_scatter<true>, grid, dim3(256), 0, s.stream, g, s.qa, s.qb, s.d_np,
+                       s.ss.cellstart, s.ss.key, s.ss.slot, s.d_nb, s.ss.ntile, s.ss.perm);
+  else
+    hipLaunchKernelGGL(k32_scatter<false>, grid, dim3(256), 0, s.stream, g, s.qa, s.qb, s.d_np,
+                       s.ss.cellstart, s.ss.key, s.ss.slot, s.d_nb, s.ss.ntile, s.ss.perm);
   launch_copy(s, s.d_np, s.ss.npnew, g.nseg);
-  std::swap(s.qa, s.qb);
+  if (defer)
+    s.perm_pending = true;
+  else
+    std::swap(s.qa, s.qb);
   s.np_host_valid = false;
   s.counted       = false;
   s.sorted        = true;
@@ -578,6 +584,10 @@ void materialize(Sim& s)
 {
   if (!s.perm_pending)
     return;
+  if (s.fp32) {
+    materialize32(s);
+    return;
+  }
   const Geom& g = s.g;
   hipLaunchKernelGGL(k_apply_perm, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g, s.pa,
                      s.pb, s.d_np, s.ss.perm);
@@ -612,14 +622,16 @@ void launch_sort(Sim& s)
 // The FP32 position push already gave every particle its destination segment,
 // bin and slot, leavers included, so the migration + sort is the bin scan and
 // the record scatter.
-void launch32_migrate(Sim& s)
+void launch32_migrate(Sim& s, bool defer)
 {
+  materialize(s);
   PhaseTimer pt(s, 6);
-  finish_sort32(s);
+  finish_sort32(s, defer);
 }
 
 void launch32_sort(Sim& s)
 {
+  materialize(s);
   PhaseTimer  pt(s, 6);
   const Geom& g = s.g;
   launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
