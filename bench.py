@@ -46,7 +46,7 @@ WORKLOADS = {
 }
 
 
-def algorithmic_bytes(phase, npart, ncell, rho_direct=False):
+def algorithmic_bytes(phase, npart, ncell, rho_direct=False, fused=False, deferred=False):
     """Bytes each launch must move at minimum (FP64 operator-boundary model).
 
     Per particle / per cell, one launch:
@@ -60,11 +60,18 @@ def algorithmic_bytes(phase, npart, ncell, rho_direct=False):
       rho      : carried by continuity, rho -= dt div J: read rho, Jx, Jy, Jz and
                  write rho (40 B/cell), no particle reads; a direct deposit instead
                  reads x (24 B/particle) and writes rho (8 B/cell)
+    fused (the gather runs the position push): velocity reads x, u once and writes
+    u, x_old, x and the cell index, 124 B/particle; deferred (the sort's record
+    move runs in the next gather): the gather also reads the source index and
+    moves the id, +20 B/particle, and the sort reads the cell index and writes
+    the source index, 8 B/particle.
     """
     if phase == "rho" and not rho_direct:
         return 40 * ncell
-    per_p = {"bfd": 0, "velocity": 72, "position": 100, "deposit": 48, "fold": 0, "efd": 0,
-             "sort": 116, "rho": 24}[phase]
+    per_p = {"bfd": 0, "velocity": 124 if fused else 72, "position": 100, "deposit": 48,
+             "fold": 0, "efd": 0, "sort": 8 if deferred else 116, "rho": 24}[phase]
+    if phase == "velocity" and deferred:
+        per_p += 20
     per_c = {"bfd": 72, "velocity": 48, "position": 0, "deposit": 24, "fold": 32, "efd": 96,
              "sort": 12, "rho": 8}[phase]
     return per_p * npart + per_c * ncell
@@ -77,11 +84,25 @@ PHASE_KERNEL = {"bfd": "k_push_bfd", "velocity": "k_push_velocity_rows",
                 "rho": "k_rho_continuity"}
 
 
+def launch_form(phase, tuning, fused, dim=3, order=2):
+    """The template form of the phase's kernel that the tuned FP64 step launches,
+    as rocprofv3 names it (None when the phase has no such form)."""
+    t = tuning or {}
+    b = lambda v: "true" if v else "false"
+    if phase == "velocity":
+        return (f"k_push_velocity_rows<{dim}, {order}, true, {b(t.get('gather_queue'))}, "
+                f"{b(fused)}>")
+    if phase == "deposit":
+        return f"k_deposit_march<{order}, {b(t.get('deposit_queue'))}>"
+    return None
+
+
 def profile_traffic(config, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     profiles/hbm_traffic_<config>.txt: FETCH_SIZE + WRITE_SIZE per dispatch,
     each scaled by the copy-probe calibration recorded in the same file.
-    None when the summary or the kernel is missing."""
+    `kernel` is a full form (matched exactly) or a name prefix (the form with
+    the most dispatches).  None when the summary or the kernel is missing."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}.txt")
     if not os.path.exists(path):
         return None
@@ -94,7 +115,9 @@ def profile_traffic(config, kernel):
             parts = line.split()
             if parts and parts[0] in ("FETCH_SIZE", "WRITE_SIZE"):
                 section = parts[0]
-            elif section and len(parts) >= 4 and parts[0].startswith(kernel):
+            elif section and len(parts) >= 4 and " ".join(parts[:-3]).startswith(kernel):
+                if "<" in kernel and " ".join(parts[:-3]) != kernel:
+                    continue
                 try:
                     n, mb = int(parts[-3]), float(parts[-1])
                 except ValueError:
@@ -293,24 +316,33 @@ def main():
     # count the mean of the first and last step
     value = 0.5 * (npart + npart_end) * args.steps / elapsed
     phases = {}
+    # the gather ran the position push when no position launch was timed; pn_step
+    # defers its sort's record move on the periodic workloads (PICNIX_DEFER_SORT)
+    fused = rep.get("position", (0.0, 0))[1] == 0
+    deferred = (not args.config.startswith("shock") and
+                os.environ.get("PICNIX_DEFER_SORT", "1") != "0")
     for ph, (ms, n) in rep.items():
         if n == 0:
             continue
         per = ms / n
         # the shock problem deposits rho every step; the periodic workloads carry it
-        b = algorithmic_bytes(ph, npart, ncell, rho_direct=args.config.startswith("shock"))
+        b = algorithmic_bytes(ph, npart, ncell, rho_direct=args.config.startswith("shock"),
+                              fused=fused, deferred=deferred)
         phases[ph] = {"ms_per_launch": per, "launches_per_step": n / args.steps,
                       "bytes_per_launch": b, "GBps": b / (per * 1e-3) / 1e9}
     dom = max(phases, key=lambda p: phases[p]["ms_per_launch"] * phases[p]["launches_per_step"])
     d = phases[dom]
+    # the tuned form of the dominant kernel (thermal workloads: 3D, order 2)
+    form = (launch_form(dom, (run_form.tuning.get("fp64") or {}), fused)
+            if args.config.startswith("thermal") else None)
     roofline = {
         "bound": "hbm",
         "achieved": d["GBps"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": d["GBps"] / HBM_PEAK_GBS,
-        "traffic": profile_traffic(args.config, PHASE_KERNEL.get(dom, dom)),
-        "kernel": PHASE_KERNEL.get(dom, dom),
+        "traffic": profile_traffic(args.config, form or PHASE_KERNEL.get(dom, dom)),
+        "kernel": form or PHASE_KERNEL.get(dom, dom),
         "phase": dom,
         "bytes_per_launch": d["bytes_per_launch"],
     }

@@ -56,6 +56,12 @@ def test_rho_bytes_follow_the_step_form():
     b = _bench()
     assert b.algorithmic_bytes("rho", 10 ** 6, 1000) == 40 * 1000
     assert b.algorithmic_bytes("rho", 10 ** 6, 1000, rho_direct=True) == 24 * 10 ** 6 + 8 * 1000
+    # the gather with the position push (124 B) and the deferred record move (+20 B);
+    # the deferred sort reads the cell index and writes the source index
+    assert b.algorithmic_bytes("velocity", 10 ** 6, 1000, fused=True) == 124 * 10 ** 6 + 48 * 1000
+    assert (b.algorithmic_bytes("velocity", 10 ** 6, 1000, fused=True, deferred=True)
+            == 144 * 10 ** 6 + 48 * 1000)
+    assert b.algorithmic_bytes("sort", 10 ** 6, 1000, deferred=True) == 8 * 10 ** 6 + 12 * 1000
     assert b.PHASE_KERNEL["rho"] == "k_rho_continuity"
 
 
