@@ -2,7 +2,7 @@
  * picnix_adjoint.h -- C ABI of the adjoint form of the PIC step (libpicnix_amd.so).
  *
  * No counterpart in the reference: pic-nix has no differentiable form.  The form
- * is a 2D periodic FP64 run of one species (electrons, immobile neutralising
+ * is a 2D periodic run (FP64 or FP32, pn_adj_params.precision) of one species (electrons, immobile neutralising
  * ions) with the hot path's operators at order 1 -- push_bfd / push_efd
  * (pic/engine/maxwell.hpp:231-348), the MC gather and the Boris push
  * (pic/engine/velocity.hpp:390-418, nix/primitives.hpp:163-189), the position
@@ -44,6 +44,9 @@ typedef struct pn_adj_params {
   int      nmodes;      /* density modes M (<= PN_ADJ_MAX_MODES) */
   int      kmax;        /* mode numbers m_x, m_y drawn from [-kmax, kmax] */
   uint64_t seed;        /* positions, velocities, modes and phases */
+  int      precision;   /* 0: FP64; 1: FP32 particles, fields and co-states (theta, the
+                           loss and the gradient stay FP64; the loss and gradient sums
+                           are reduced in FP64) */
 } pn_adj_params;
 
 int         pn_adj_create(const pn_adj_params* p, pn_adj** out);

@@ -607,11 +607,6 @@ int pn_create(const char* toml, int load, pn_sim** out)
       *out  = h;
       return PN_ERR_UNSUPPORTED;
     }
-    if (s.fp32 && s.cfg.N[0] == 1 && s.cfg.C[0] == 1 && s.cfg.N[1] == 1 && s.cfg.C[1] == 1) {
-      s.err = "precision = fp32 is built for 2D and 3D domains; 1D runs use fp64";
-      *out  = h;
-      return PN_ERR_UNSUPPORTED;
-    }
     setup_geometry(s);
     if (const char* k = std::getenv("PICNIX_KERNELS"))
       s.generic_kernels = std::string(k) == "generic";

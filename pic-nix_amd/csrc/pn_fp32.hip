@@ -515,6 +515,45 @@ __global__ void __launch_bounds__(256)
   float       wix[S], wiy[S], wiz[S], whx[S], why[S], whz[S];
   int         ix0, iy0, iz0 = 0, hx0, hy0, hz0 = 0;
   int_w<Order>(pc, 0, r[0], ix0, wix);
+  if constexpr (Dim == 1) {
+    // the x line of the grid (weights1d + push_scalar1d, pic/engine/velocity.hpp:
+    // 223-251,362-388): E_x, B_y, B_z on the half-integer nodes, the rest on the
+    // integer ones, as the FP64 1D gather
+    shape_mc<Order, float>(half_delta<Order>(r[0], hx0), 0.0f, 1.0f, whx);
+    const int X  = off[c * 3 + 2] + cl.x + ix0 + g.Nb - (Order / 2);
+    const int HX = off[c * 3 + 2] + cl.x + hx0 + g.Nb - (Order / 2);
+    if (min(X, HX) < 0 || max(X, HX) > g.P[2] - S)
+      return;
+    const long  v    = g.vol;
+    const float qmdt = pc.qmdt[sp];
+    auto        line = [&](const float* F, int x0, const float* w) {
+      float rr = 0;
+      for (int jx = 0; jx < S; jx++)
+        rr += F[x0 + jx] * w[jx];
+      return rr * qmdt;
+    };
+    float eb[6];
+    eb[0] = line(E, HX, whx);
+    eb[1] = line(E + v, X, wix);
+    eb[2] = line(E + 2 * v, X, wix);
+    eb[3] = line(B, X, wix);
+    eb[4] = line(B + v, HX, whx);
+    eb[5] = line(B + 2 * v, HX, whx);
+    float ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+    push_momentum<float>(pc.pusher, ux, uy, uz, eb, (float)g.cc);
+    p.ux[i] = ux;
+    p.uy[i] = uy;
+    p.uz[i] = uz;
+    (void)wiy;
+    (void)wiz;
+    (void)why;
+    (void)whz;
+    (void)iy0;
+    (void)iz0;
+    (void)hy0;
+    (void)hz0;
+    return;
+  }
   int_w<Order>(pc, 1, r[1], iy0, wiy);
   shape_mc<Order, float>(half_delta<Order>(r[0], hx0), 0.0f, 1.0f, whx);
   shape_mc<Order, float>(half_delta<Order>(r[1], hy0), 0.0f, 1.0f, why);
@@ -756,6 +795,36 @@ __global__ void __launch_bounds__(256)
   const Cell3 cl = cell_of(g, bin);
   float       s0x[N], s0y[N], s0z[N], dsx[N], dsy[N], dsz[N];
   const int   fx = esk_axis<Order>(ro0[i], p.rx[i], s0x, dsx);
+  if constexpr (Dim == 1) {
+    // local1d + deposit1d (pic/engine/current.hpp:207-258, nix/esirkepov.hpp:18-67):
+    // Jx by the prefix sum, Jy and Jz from the velocities (y, z keep absolute
+    // coordinates in 1D) times the time-averaged weight
+    const int   gx  = off[c * 3 + 2] + cl.x + fx + g.Nb;
+    const long  v   = g.vol;
+    const float qs  = dc.q[sp];
+    const float vy  = (p.ry[i] - ro1[i]) / dc.delt;
+    const float vz  = (p.rz[i] - ro2[i]) / dc.delt;
+    auto        add = [&](int x, int k, float val) {
+      if ((unsigned)x < (unsigned)g.P[2])
+        atomicAdd(&J[k * v + x], val);
+    };
+    const float wx = -(qs * dc.dxdt);
+    float       ww = 0;
+    for (int jx = 0; jx < N - 1; jx++) {
+      ww += dsx[jx] * wx;
+      add(gx + jx + 1, 1, ww);
+    }
+    for (int jx = 0; jx < N; jx++) {
+      const float wt = s0x[jx] + 0.5f * dsx[jx];
+      add(gx + jx, 2, wt * (qs * vy));
+      add(gx + jx, 3, wt * (qs * vz));
+    }
+    (void)s0y;
+    (void)s0z;
+    (void)dsy;
+    (void)dsz;
+    return;
+  }
   const int   fy = esk_axis<Order>(ro1[i], p.ry[i], s0y, dsy);
   int         fz = 0;
   if constexpr (Dim == 3)
@@ -1364,11 +1433,13 @@ __global__ void __launch_bounds__(256)
   }
   const float qs = (float)g.q[sp];
   for (int jz = 0; jz < (Dim == 3 ? S : 1); jz++)
-    for (int jy = 0; jy < S; jy++)
+    for (int jy = 0; jy < (Dim >= 2 ? S : 1); jy++)
       for (int jx = 0; jx < S; jx++) {
-        const float val = Dim == 3 ? qs * w[0][jx] * w[1][jy] * w[2][jz] : qs * w[0][jx] * w[1][jy];
+        const float val = Dim == 3   ? qs * w[0][jx] * w[1][jy] * w[2][jz]
+                          : Dim == 2 ? qs * w[0][jx] * w[1][jy]
+                                     : qs * w[0][jx];
         const int   gz  = g.has[0] ? off[c * 3 + 0] + o[2] + jz : 0;
-        const int   gy  = off[c * 3 + 1] + o[1] + jy;
+        const int   gy  = g.has[1] ? off[c * 3 + 1] + o[1] + jy : 0;
         const int   gx  = off[c * 3 + 2] + o[0] + jx;
         if ((unsigned)gz >= (unsigned)g.P[0] || (unsigned)gy >= (unsigned)g.P[1] ||
             (unsigned)gx >= (unsigned)g.P[2])
@@ -1707,7 +1778,7 @@ static bool vel32_rows_shape(const Sim& s, int& yr, size_t& lds)
   while (yr > 1 && lds_of(yr) > 65536)
     yr = (yr + 1) / 2;
   lds = lds_of(yr);
-  return s.sorted && !s.generic_kernels && !s.vel_generic && lds <= 65536;
+  return g.dim >= 2 && s.sorted && !s.generic_kernels && !s.vel_generic && lds <= 65536;
 }
 
 bool gather32_takes_perm(const Sim& s)
@@ -1799,7 +1870,15 @@ bool launch32_push_velocity(Sim& s, double delt, bool with_position)
                          0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc);            \
     }                                                                                         \
   } while (0)
-  if (g.dim == 3) {
+  if (g.dim == 1) {
+    const dim3 grid(nblk(g.cap, 256), g.nseg);
+    switch (g.order) {
+    case 1: hipLaunchKernelGGL((k32_push_velocity<1, 1>), grid, dim3(256), 0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc); break;
+    case 2: hipLaunchKernelGGL((k32_push_velocity<1, 2>), grid, dim3(256), 0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc); break;
+    case 3: hipLaunchKernelGGL((k32_push_velocity<1, 3>), grid, dim3(256), 0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc); break;
+    default: hipLaunchKernelGGL((k32_push_velocity<1, 4>), grid, dim3(256), 0, s.stream, g, s.qa, s.d_np, s.d_off, s.f_E, s.f_B, pc); break;
+    }
+  } else if (g.dim == 3) {
     switch (g.order) {
     case 1: PN32_VEL(3, 1); break;
     case 2: PN32_VEL(3, 2); break;
@@ -1956,6 +2035,13 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
         case 3: PN32_DEP(3, 3); break;
         default: PN32_DEP(3, 4); break;
         }
+      } else if (g.dim == 1) {
+        switch (g.order) {
+        case 1: PN32_DEP(1, 1); break;
+        case 2: PN32_DEP(1, 2); break;
+        case 3: PN32_DEP(1, 3); break;
+        default: PN32_DEP(1, 4); break;
+        }
       } else {
         switch (g.order) {
         case 1: PN32_DEP(2, 1); break;
@@ -2011,6 +2097,13 @@ void launch32_rho(Sim& s)
     case 2: PN32_RHO(3, 2); break;
     case 3: PN32_RHO(3, 3); break;
     default: PN32_RHO(3, 4); break;
+    }
+  } else if (g.dim == 1) {
+    switch (g.order) {
+    case 1: PN32_RHO(1, 1); break;
+    case 2: PN32_RHO(1, 2); break;
+    case 3: PN32_RHO(1, 3); break;
+    default: PN32_RHO(1, 4); break;
     }
   } else {
     switch (g.order) {

@@ -38,7 +38,7 @@ class Params(ctypes.Structure):
                 ("delh", ctypes.c_double), ("delt", ctypes.c_double), ("cc", ctypes.c_double),
                 ("nppc", ctypes.c_int), ("qm", ctypes.c_double), ("ro", ctypes.c_double),
                 ("vth", ctypes.c_double), ("nmodes", ctypes.c_int), ("kmax", ctypes.c_int),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("precision", ctypes.c_int)]
 
 
 def _lib():
@@ -73,12 +73,18 @@ def _dp(a):
 
 
 class AdjointForm:
-    """One adjoint-form set-up (seeded base particles and modes) on the GPU."""
+    """One adjoint-form set-up (seeded base particles and modes) on the GPU.
+    precision: "fp64" or "fp32" (the particles, fields and co-states; theta, the
+    loss and the gradient stay FP64)."""
 
     def __init__(self, nx=256, ny=256, delh=1.0, delt=0.5, cc=1.0, nppc=32, qm=-1.0, ro=1.0,
-                 vth=0.05, nmodes=8, kmax=4, seed=12345):
+                 vth=0.05, nmodes=8, kmax=4, seed=12345, precision="fp64"):
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
         self.L = _lib()
-        self.p = Params(nx, ny, delh, delt, cc, nppc, qm, ro, vth, nmodes, kmax, seed)
+        self.p = Params(nx, ny, delh, delt, cc, nppc, qm, ro, vth, nmodes, kmax, seed,
+                        1 if precision == "fp32" else 0)
+        self.precision = precision
         self.h = ctypes.c_void_p()
         rc = self.L.pn_adj_create(ctypes.byref(self.p), ctypes.byref(self.h))
         self._chk(rc)

@@ -84,6 +84,10 @@ CASES = {
     "thermal3d_o1": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=1)),
     "thermal3d_o4": (3, lambda: helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=4, order=4)),
     "twostream2d_o1": (2, lambda: helpers.small_twostream(N=(1, 16, 64), C=(1, 1, 4), np_=16, order=1)),
+    # 1D (Nz = Ny = 1): y and z stay absolute coordinates, x is cell-relative
+    "twostream1d_o1": (1, lambda: helpers.small_twostream(N=(1, 1, 128), C=(1, 1, 8), np_=64, order=1)),
+    "thermal1d_o2": (1, lambda: helpers.small_thermal(N=(1, 1, 64), C=(1, 1, 4), np_=32, order=2)),
+    "thermal1d_o3": (1, lambda: helpers.small_thermal(N=(1, 1, 64), C=(1, 1, 4), np_=32, order=3)),
 }
 
 

@@ -128,7 +128,7 @@ def _shock_parity(name):
         o.close()
 
 
-@pytest.mark.parametrize("name", ["shock2d_o2", "shock3d_ramp"])
+@pytest.mark.parametrize("name", ["shock1d_o2", "shock2d_o2", "shock3d_ramp"])
 def test_shock_fp32_follows_fp64(name):
     """The FP32 form with the same boundaries: the injections (their draws do not
     depend on the particles) are identical, the redraws nearly so, and the
