@@ -307,7 +307,7 @@ def main():
                 "rel_error_vs_fp64_after_steps": args.warmup + args.steps,
                 "rel_error_vs_fp64": err,
                 "phases_ms_per_launch": {k: (ms / n if n else 0.0) for k, (ms, n) in rep32.items()},
-                "form": "FP32 cell-relative particles and fields, compensated FP32 LDS current tiles, "
+                "form": "FP32 cell-relative particles and fields, FP64 LDS current tiles, "
                         "FP64 energy/charge reductions"}
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb)

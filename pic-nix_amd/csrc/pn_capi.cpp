@@ -618,7 +618,7 @@ int pn_create(const char* toml, int load, pn_sim** out)
       s.fuse_push = std::atoi(k) != 0 ? 1 : 0;
     if (const char* k = std::getenv("PICNIX_FP32_TILE")) { // FP32 deposit tile form
       const std::string v(k);
-      s.tile32 = v == "f64" ? 1 : (v == "plain" ? 2 : (v == "comp" ? 3 : 0));
+      s.tile32 = v == "staged" ? 0 : (v == "plain" ? 2 : (v == "comp" ? 3 : 1));
     }
     if (const char* k = std::getenv("PICNIX_STAGE_CELLS")) // FP32 staged tile: cells per phase
       s.stage_cells = std::max(0, std::atoi(k));

@@ -958,18 +958,21 @@ struct March32 {
 };
 
 // Tile selects how cell sums reach the ring tile.
-//   kTileStaged (the default): the compensated FP32 pairs (hi, lo) are updated
-//     without atomics.  A claimed cell's wave stores its per-cell sums (one
-//     N x (N-1) block per plane lane) into the cell's slot of a row stage; at the
-//     end of the bin row one thread per tile node adds the slots that reach it,
-//     in ascending cell order, with TwoSum into (hi, lo).  The block's tile is
-//     therefore the same whichever wave claimed which cell.
+//   kTileF64 (the default): one FP64 value per node (LDS double atomics).
+//   kTileStaged: compensated FP32 pairs (hi, lo) updated without atomics.  A
+//     claimed cell's wave stores its per-cell sums (one N x (N-1) block per
+//     plane lane) into the cell's slot of a stage of kst cells; at the end of
+//     each phase of kst cells one thread per tile node adds the slots that
+//     reach it, in ascending cell order, with TwoSum into (hi, lo).  The
+//     block's tile is therefore the same whichever wave claimed which cell.
 //   kTileComp: the pairs take LDS float atomics; the error of each add comes
 //     from its returned old value (TwoSum).
-//   kTileF64: one FP64 value per node (LDS double atomics).
 //   kTilePlain: one FP32 value per node, no compensation.
-// The last three exist to measure the first against (LDS float atomics are
-// several times slower than double ones on this GPU).
+// All four flush to J with FP32 global atomics, whose rounding bounds J's
+// error (a few 1e-7 of max |J|) for every form.  At 128^3 the FP64 tile is the
+// fastest (LDS float atomics are several times slower than double ones on this
+// GPU, and the staged form pays for its stage), so it is the default; the
+// others stay selectable to measure against it.
 template <int Order, bool Queue, int Tile>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(March32<Order>::waves(Tile), March32<Order>::waves(Tile))))
@@ -1981,7 +1984,7 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
     const size_t mstage = mbase + sizeof(float) * (size_t)kst * 3 * N * N * (N - 1);
     const size_t wlds  = g.order == 2 ? March32<2>::WLDS : March32<4>::WLDS;
     const int    tile  = s.tile32 != kTileStaged ? s.tile32
-                         : (mstage + wlds + 64 <= 65536 ? kTileStaged : kTileComp);
+                         : (mstage + wlds + 64 <= 65536 ? kTileStaged : kTileF64);
     const size_t mlds  = tile == kTileStaged ? mstage : mbase;
     int zrows, nzr;
     march32_split(g, s.dep_rows ? s.dep_rows : s.march_rows, zrows, nzr);

@@ -162,8 +162,9 @@ struct Sim {
   int         dep_queue   = 0; // 1: the march deposit runs persistent over a work queue
   int         vel_queue   = 0; // 1: the row gather runs persistent over a work queue
   int         fuse_push   = 0; // pn_step: the row gather also runs the position push
-  int         tile32      = 0; // FP32 march deposit tile: 0 staged compensated pairs, 1 FP64,
-                               // 2 plain FP32, 3 compensated pairs by LDS atomics
+  int         tile32      = 1; // FP32 march deposit tile: 1 FP64 (default), 0 staged
+                               // compensated pairs, 2 plain FP32, 3 compensated pairs by
+                               // LDS atomics
   int         stage_cells = 0; // FP32 staged tile: cells per phase (0: 4)
   int*        d_wq        = nullptr; // work-queue counters, 2 per launch site (kWq*)
   int         ncu         = 0;       // compute units of the device

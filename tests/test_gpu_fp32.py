@@ -287,12 +287,12 @@ def test_fp32_keeps_particles_at_the_upper_chunk_face():
         sim.close()
 
 
-@pytest.mark.parametrize("tile", ["stage1", "stage3", "stage16", "comp", "f64", "plain"])
+@pytest.mark.parametrize("tile", ["stage1", "stage3", "stage4", "stage16", "comp", "plain"])
 def test_fp32_deposit_tile_forms(tile, monkeypatch):
-    """The FP32 march deposit's tile forms against the default (staged
-    compensated pairs, 4 cells per phase): 1, 3 (a row of 8 cells in phases
-    of 3, 3 and 2) and 16 cells per phase, the pairs updated by LDS atomics, an
-    FP64 tile and a plain FP32 tile.  From the same state, one step gives
+    """The FP32 march deposit's tile forms against the default (an FP64 tile):
+    staged compensated pairs with 1, 3 (a row of 8 cells in phases of 3, 3 and
+    2), 4 and 16 cells per phase, the pairs updated by LDS atomics, and a plain
+    FP32 tile.  From the same state, one step gives
     particles identical by id (the current does not act on them within the step)
     and currents equal to FP32 summation round-off: within 1e-5 relative (the
     plain tile, with no compensation, within 1e-4), and every form within 1e-4 of
@@ -302,6 +302,7 @@ def test_fp32_deposit_tile_forms(tile, monkeypatch):
     cfg = _fp32(base)
     dt = float(cfg["parameter"]["delt"])
     if tile.startswith("stage"):
+        monkeypatch.setenv("PICNIX_FP32_TILE", "staged")
         monkeypatch.setenv("PICNIX_STAGE_CELLS", tile[5:])
     else:
         monkeypatch.setenv("PICNIX_FP32_TILE", tile)
