@@ -1098,7 +1098,8 @@ bool carry_rho(const Sim& s) { return s.rho_valid && !s.rho_deposit; }
 // (periodic problems whose gather takes the row form)
 bool defers_sort(const Sim& s)
 {
-  return s.defer_sort && !s.g.xwall && gather_takes_perm(s);
+  // the FP32 wall step's rho deposit reads the sorted records in place
+  return s.defer_sort && !(s.g.xwall && s.fp32) && gather_takes_perm(s);
 }
 
 void step_launches32(Sim& s, double delt)
@@ -1149,10 +1150,12 @@ void step_launches_wall(Sim& s, double delt)
   launch_push_efd(s, delt);
   launch_wall_emf(s);
   shock_inject(s);
+  // the rho deposit below reads the particles through a pending sort, so the
+  // record move is left to the next row gather
   if (s.fp32)
     launch32_migrate(s);
   else
-    launch_migrate(s);
+    launch_migrate(s, defers_sort(s));
   if (rho == RhoDone::Deferred) {
     if (s.fp32)
       launch32_rho(s);

@@ -879,7 +879,7 @@ template <int Order>
 __global__ void __launch_bounds__(256)
     k_rho_march(Geom g, const PArr p, const int32_t* __restrict__ cellstart,
                 const int* __restrict__ off, const double* __restrict__ lim,
-                double* __restrict__ J, int zrows)
+                double* __restrict__ J, int zrows, const int32_t* __restrict__ perm)
 {
   using RP             = RhoPlan<Order>;
   constexpr int S      = RP::S;
@@ -960,10 +960,11 @@ __global__ void __launch_bounds__(256)
   };
   auto fetch = [&](int seg, int pb, int p1, double* x3) {
     if (pb + lane < p1) {
-      const long i = (long)seg * g.cap + pb + lane;
-      x3[0]        = p.x[i];
-      x3[1]        = p.y[i];
-      x3[2]        = p.z[i];
+      const long i   = (long)seg * g.cap + pb + lane;
+      const long src = perm ? (long)perm[i] : i;
+      x3[0]          = p.x[src];
+      x3[1]          = p.y[src];
+      x3[2]          = p.z[src];
     }
   };
 
@@ -1126,7 +1127,7 @@ template <int Dim, int Order>
 __global__ void __launch_bounds__(256)
     k_rho_global(Geom g, const PArr p, const int32_t* __restrict__ np,
                  const int* __restrict__ off, const double* __restrict__ lim,
-                 double* __restrict__ J)
+                 double* __restrict__ J, const int32_t* __restrict__ perm)
 {
   constexpr int is_odd = Order % 2;
   constexpr int S      = Order + 1;
@@ -1137,7 +1138,8 @@ __global__ void __launch_bounds__(256)
   const int    c     = seg / g.Ns;
   const int    sp    = seg - c * g.Ns;
   const long   i     = (long)seg * g.cap + ip;
-  const double xu[3] = {p.x[i], p.y[i], p.z[i]};
+  const long   src   = perm ? (long)perm[i] : i;
+  const double xu[3] = {p.x[src], p.y[src], p.z[src]};
   const double d3[3] = {g.del[2], g.del[1], g.del[0]};
   const double mn[3] = {lim[c * 6 + 4], lim[c * 6 + 2], lim[c * 6 + 0]};
   double       s[3][S];
@@ -1297,10 +1299,12 @@ bool deposit_impl(Sim& s, const DepConsts& dc)
   return false;
 }
 
+// A sort left pending (perm) is read through: sorted slot i holds record perm[i].
 template <int Dim, int Order>
 void rho_impl(Sim& s, bool sorted_by_new_cell)
 {
-  const Geom& g = s.g;
+  const Geom&    g    = s.g;
+  const int32_t* perm = s.perm_pending ? s.ss.perm : nullptr;
   const int   S = Order + 1;
   const size_t mlds = march_lds(g, S * S * (g.bsy + Order));
   if (Dim == 3 && sorted_by_new_cell && !s.generic_kernels &&
@@ -1308,10 +1312,11 @@ void rho_impl(Sim& s, bool sorted_by_new_cell)
     int zrows, nzr;
     march_split(g, s.rho_rows ? s.rho_rows : s.march_rows, zrows, nzr);
     hipLaunchKernelGGL((k_rho_march<Order>), dim3(nzr * (g.bsz / g.bsy), g.nchunk), dim3(256),
-                       mlds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_J, zrows);
+                       mlds, s.stream, g, s.pa, s.ss.cellstart, s.d_off, s.d_lim, s.d_J, zrows,
+                       perm);
   } else {
     hipLaunchKernelGGL((k_rho_global<Dim, Order>), dim3(nblk(g.cap, 256), g.nseg), dim3(256), 0,
-                       s.stream, g, s.pa, s.d_np, s.d_off, s.d_lim, s.d_J);
+                       s.stream, g, s.pa, s.d_np, s.d_off, s.d_lim, s.d_J, perm);
   }
 }
 
@@ -1422,7 +1427,6 @@ void launch_rho_unject(Sim& s, int nmax)
 
 void launch_rho(Sim& s, bool sorted_by_new_cell)
 {
-  materialize(s);
   PhaseTimer  pt(s, 7);
   PhaseTimer  pr(s, kPhaseRhoDeposit);
   const Geom& g = s.g;

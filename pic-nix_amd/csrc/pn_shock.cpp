@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <limits>
 #include <random>
 #include <thread>
@@ -188,30 +189,65 @@ std::vector<int> upper_chunks(const Sim& s)
   return out;
 }
 
+// The next pinned staging buffer, at least `bytes` long, once its previous copy
+// has finished.
+char* stage_buffer(Sim& s, size_t bytes, int& slot)
+{
+  ShockState& h = s.shk;
+  slot          = h.stage_next;
+  h.stage_next  = (slot + 1) % ShockState::kStages;
+  if (h.stage_ev[slot])
+    HIPCHK(hipEventSynchronize(h.stage_ev[slot]));
+  else
+    HIPCHK(hipEventCreateWithFlags(&h.stage_ev[slot], hipEventDisableTiming));
+  if (bytes > h.stage_cap[slot]) {
+    if (h.h_stage[slot])
+      HIPCHK(hipHostFree(h.h_stage[slot]));
+    h.h_stage[slot]   = nullptr;
+    h.stage_cap[slot] = 0;
+    const size_t cap  = std::max<size_t>(bytes + bytes / 2, 1 << 16);
+    HIPCHK(hipHostMalloc(&h.h_stage[slot], cap, hipHostMallocDefault));
+    h.stage_cap[slot] = cap;
+  }
+  return static_cast<char*>(h.h_stage[slot]);
+}
+
+// Values per upper segment (and, for an injection, the counts) to the device,
+// through a pinned staging buffer: the copies run in stream order with no wait on
+// the host.
 void upload_vals(Sim& s, const std::vector<std::vector<double>>& vals,
                  const std::vector<int32_t>* counts)
 {
-  ShockState&          h  = s.shk;
-  const size_t         nk = h.useg.size();
-  std::vector<int64_t> off(nk);
-  size_t               tot = 0;
-  for (size_t k = 0; k < nk; k++) {
-    off[k] = (int64_t)tot;
-    tot += vals[k].size();
-  }
-  ensure_vals(s, tot);
-  std::vector<double> flat(std::max<size_t>(tot, 1));
+  ShockState&  h  = s.shk;
+  const size_t nk = h.useg.size();
+  size_t       tot = 0;
   for (size_t k = 0; k < nk; k++)
-    std::copy(vals[k].begin(), vals[k].end(), flat.begin() + off[k]);
-  HIPCHK(hipMemcpyAsync(h.d_vals, flat.data(), tot * sizeof(double), hipMemcpyHostToDevice,
-                        s.stream));
-  HIPCHK(hipMemcpyAsync(h.d_voff, off.data(), nk * sizeof(int64_t), hipMemcpyHostToDevice,
-                        s.stream));
-  if (counts)
-    HIPCHK(hipMemcpyAsync(h.d_count, counts->data(), nk * sizeof(int32_t),
-                          hipMemcpyHostToDevice, s.stream));
-  // the host vectors go out of scope after the call: wait for the copies
-  HIPCHK(hipStreamSynchronize(s.stream));
+    tot += vals[k].size();
+  ensure_vals(s, tot);
+  // staging layout: offsets (int64) [nk], counts (int32) [nk, padded], values
+  const size_t boff = 0, bcnt = nk * sizeof(int64_t);
+  const size_t bval = bcnt + ((nk * sizeof(int32_t) + 7) & ~(size_t)7);
+  int          slot;
+  char*        st   = stage_buffer(s, bval + tot * sizeof(double), slot);
+  int64_t*     off  = reinterpret_cast<int64_t*>(st + boff);
+  double*      flat = reinterpret_cast<double*>(st + bval);
+  size_t       at   = 0;
+  for (size_t k = 0; k < nk; k++) {
+    off[k] = (int64_t)at;
+    std::copy(vals[k].begin(), vals[k].end(), flat + at);
+    at += vals[k].size();
+  }
+  if (tot)
+    HIPCHK(hipMemcpyAsync(h.d_vals, flat, tot * sizeof(double), hipMemcpyHostToDevice,
+                          s.stream));
+  HIPCHK(hipMemcpyAsync(h.d_voff, off, nk * sizeof(int64_t), hipMemcpyHostToDevice, s.stream));
+  if (counts) {
+    int32_t* cnt = reinterpret_cast<int32_t*>(st + bcnt);
+    std::copy(counts->begin(), counts->end(), cnt);
+    HIPCHK(hipMemcpyAsync(h.d_count, cnt, nk * sizeof(int32_t), hipMemcpyHostToDevice,
+                          s.stream));
+  }
+  HIPCHK(hipEventRecord(h.stage_ev[slot], s.stream));
 }
 
 } // namespace
@@ -286,10 +322,19 @@ void shock_setup(Sim& s)
 void shock_release(Sim& s)
 {
   ShockState& h = s.shk;
+  h.ahead.reset(); // waits for draws still running on the host thread
+  if (s.stream)
+    (void)hipStreamSynchronize(s.stream);
   for (void* p : {(void*)h.d_useg, (void*)h.d_list, (void*)h.d_count, (void*)h.d_voff,
                   (void*)h.d_vals})
     if (p)
       (void)hipFree(p);
+  for (int k = 0; k < ShockState::kStages; k++) {
+    if (h.h_stage[k])
+      (void)hipHostFree(h.h_stage[k]);
+    if (h.stage_ev[k])
+      (void)hipEventDestroy(h.stage_ev[k]);
+  }
   h = ShockState{};
 }
 
@@ -472,64 +517,163 @@ void shock_position_boundary(Sim& s)
     launch_wall_apply(s);
 }
 
+namespace {
+
+// What one injection call needs: the upper chunks, their random_seed and lastid
+// when the call starts, and the constants of the draws.
+struct InjectInput {
+  std::vector<int>     up;     // upper chunk ids, chunk-id order
+  std::vector<int>     seed;   // random_seed[id] before the call's reset
+  std::vector<int64_t> lastid; // lastid[id] before the call
+  std::vector<double>  lim;    // [u][6] chunk ranges
+  int    nz = 0, ny = 0, ns = 0;
+  double dy = 0, dz = 0, flux = 0, vte = 0, vti = 0, u0 = 0, delt = 0;
+};
+
+// Its results: the values per upper segment and the particles per chunk.
+struct InjectDraws {
+  std::vector<std::vector<double>> vals;  // [u][sp] x, y, z, ux, uy, uz, id bits
+  std::vector<int>                 total; // [u]
+};
+
 // MainChunk::inject_particle (example/shock/main.cpp:445-535): a Poisson number of
 // particles per (z, y) column of every upper chunk, the same number for both
-// species, appended column by column with ids from lastid.
-void shock_inject(Sim& s)
+// species, appended column by column with ids from lastid.  The chunk's
+// generator is reseeded at the start of the call, so the draws depend on the
+// seed and lastid only, not on the particles.
+InjectDraws draw_injection(const InjectInput& in, bool threads)
 {
-  ShockState& h  = s.shk;
-  const Geom& g  = s.g;
-  const size_t nk = h.useg.size();
-  if (nk == 0)
-    return;
-  std::vector<std::vector<double>> vals(nk);
-  std::vector<int32_t>             ninj(nk, 0);
-  const std::vector<int>           up = upper_chunks(s);
-  parallel_chunks(up, [&](unsigned u, int id) {
+  const size_t nu = in.up.size();
+  InjectDraws  d;
+  d.vals.resize(nu * in.ns);
+  d.total.assign(nu, 0);
+  parallel_chunks(in.up, [&](unsigned u, int) {
     ChunkRng rng;
-    rng.reset(h.random_seed[id]);
-    std::poisson_distribution<int> poisson(h.flux);
-    MaxwellJuttner mj[2] = {MaxwellJuttner(h.vte * h.vte, -h.u0),
-                            MaxwellJuttner(h.vti * h.vti, -h.u0)};
-    const double*  lim   = &s.lim[id * 6];
-    const int      ncol  = g.dims[0] * g.dims[1];
+    int      seed = in.seed[u];
+    rng.reset(seed);
+    std::poisson_distribution<int> poisson(in.flux);
+    MaxwellJuttner mj[2] = {MaxwellJuttner(in.vte * in.vte, -in.u0),
+                            MaxwellJuttner(in.vti * in.vti, -in.u0)};
+    const double*  lim   = &in.lim[6 * u];
+    const int      ncol  = in.nz * in.ny;
     std::vector<int> col(ncol);
     int              total = 0;
     for (int k = 0; k < ncol; k++) {
       col[k] = poisson(rng.mt);
       total += col[k];
     }
-    std::vector<int64_t> id64(g.Ns, h.lastid[id]);
-    for (int sp = 0; sp < g.Ns; sp++)
-      vals[(size_t)u * g.Ns + sp].reserve(7 * (size_t)total);
-    for (int iz = 0; iz < g.dims[0]; iz++)
-      for (int iy = 0; iy < g.dims[1]; iy++)
-        for (int sp = 0; sp < g.Ns; sp++) {
-          std::vector<double>& v = vals[(size_t)u * g.Ns + sp];
-          for (int t = 0; t < col[iz * g.dims[1] + iy]; t++) {
+    std::vector<int64_t> id64(in.ns, in.lastid[u]);
+    for (int sp = 0; sp < in.ns; sp++)
+      d.vals[(size_t)u * in.ns + sp].reserve(7 * (size_t)total);
+    for (int iz = 0; iz < in.nz; iz++)
+      for (int iy = 0; iy < in.ny; iy++)
+        for (int sp = 0; sp < in.ns; sp++) {
+          std::vector<double>& v = d.vals[(size_t)u * in.ns + sp];
+          for (int t = 0; t < col[iz * in.ny + iy]; t++) {
             double x, ux, uy, uz;
-            injection_particle(rng, mj[sp], h.delt, lim[5], x, ux, uy, uz);
-            double y = lim[2] + (iy + rng.uniform(rng.mt)) * g.del[1];
-            double z = lim[0] + (iz + rng.uniform(rng.mt)) * g.del[0];
+            injection_particle(rng, mj[sp], in.delt, lim[5], x, ux, uy, uz);
+            double y = lim[2] + (iy + rng.uniform(rng.mt)) * in.dy;
+            double z = lim[0] + (iz + rng.uniform(rng.mt)) * in.dz;
             double idbits;
             std::memcpy(&idbits, &id64[sp], sizeof(double));
             id64[sp]++;
             v.insert(v.end(), {x, y, z, ux, uy, uz, idbits});
           }
         }
-    h.lastid[id] += total;
-    for (int sp = 0; sp < g.Ns; sp++) {
-      ninj[(size_t)u * g.Ns + sp] = total;
-      h.influx[2 * id + sp]       = total;
+    d.total[u] = total;
+  }, threads);
+  return d;
+}
+
+} // namespace
+
+// The next injection's draws, started on a host thread when the previous
+// injection is done; `in` holds the seed and lastid they were drawn for.
+struct ShockAhead {
+  InjectInput              in;
+  std::future<InjectDraws> draws;
+};
+
+namespace {
+
+InjectInput inject_input(const Sim& s, const std::vector<int>& up)
+{
+  const ShockState& h = s.shk;
+  const Geom&       g = s.g;
+  InjectInput       in;
+  in.up = up;
+  for (int id : up) {
+    in.seed.push_back(h.random_seed[id]);
+    in.lastid.push_back(h.lastid[id]);
+    in.lim.insert(in.lim.end(), &s.lim[id * 6], &s.lim[id * 6] + 6);
+  }
+  in.nz   = g.dims[0];
+  in.ny   = g.dims[1];
+  in.ns   = g.Ns;
+  in.dy   = g.del[1];
+  in.dz   = g.del[0];
+  in.flux = h.flux;
+  in.vte  = h.vte;
+  in.vti  = h.vti;
+  in.u0   = h.u0;
+  in.delt = h.delt;
+  return in;
+}
+
+} // namespace
+
+// Injection at x = Lx.  The draws of this call were usually made ahead, on a host
+// thread, while the device ran the step; they are used when they were drawn for
+// the seed and lastid the chunks hold now (otherwise -- the first call, a restart,
+// another operator order -- they are drawn here).  Then the draws of the next
+// call start: every step resets each upper chunk's generator once in the
+// position boundary (seed + 1) before the injection resets it again.
+void shock_inject(Sim& s)
+{
+  ShockState&  h  = s.shk;
+  const Geom&  g  = s.g;
+  const size_t nk = h.useg.size();
+  if (nk == 0)
+    return;
+  const std::vector<int> up = upper_chunks(s);
+  InjectInput            in = inject_input(s, up);
+  InjectDraws            d;
+  bool                   have = false;
+  if (h.ahead) {
+    InjectDraws r = h.ahead->draws.get();
+    if (h.ahead->in.seed == in.seed && h.ahead->in.lastid == in.lastid) {
+      d    = std::move(r);
+      have = true;
     }
-  });
-  upload_vals(s, vals, &ninj);
+    h.ahead.reset();
+  }
+  if (!have)
+    d = draw_injection(in, true);
+  std::vector<int32_t> ninj(nk, 0);
+  for (size_t u = 0; u < up.size(); u++) {
+    const int id = up[u];
+    h.random_seed[id] += 1; // the call's reset
+    h.lastid[id] += d.total[u];
+    for (int sp = 0; sp < g.Ns; sp++) {
+      ninj[u * g.Ns + sp]   = d.total[u];
+      h.influx[2 * id + sp] = d.total[u];
+    }
+  }
+  upload_vals(s, d.vals, &ninj);
   h.last_ninj = ninj.empty() ? 0 : *std::max_element(ninj.begin(), ninj.end());
   if (s.fp32)
     launch32_inject_append(s);
   else
     launch_inject_append(s);
   s.np_host_valid = false;
+  // the next call's draws
+  InjectInput next = inject_input(s, up);
+  for (int& sd : next.seed)
+    sd += 1; // the next position boundary's reset
+  auto a   = std::make_shared<ShockAhead>();
+  a->in    = next;
+  a->draws = std::async(std::launch::async, [next] { return draw_injection(next, true); });
+  h.ahead  = std::move(a);
 }
 
 } // namespace picnix

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -53,6 +54,8 @@ struct SortScratch {
 // (example/shock/main.cpp:56-92), the per-chunk state the example keeps in
 // option["random_seed"] and option["boundary"], and the device buffers of the
 // upstream boundary.
+struct ShockAhead; // the next injection's draws, computed ahead (pn_shock.cpp)
+
 struct ShockState {
   double delt = 0, u0 = 0, vte = 0, vti = 0, taper = 0;
   double E0[3] = {0, 0, 0}, B0[3] = {0, 0, 0};
@@ -70,6 +73,14 @@ struct ShockState {
   int64_t* d_voff  = nullptr;       // [nk] offsets into d_vals
   double*  d_vals  = nullptr;       // replacement (4 per particle) / injected (7) values
   size_t   vals_cap = 0;
+  // pinned host staging of the uploads, used in turn; each buffer's event marks
+  // the end of its last copy
+  static constexpr int kStages = 3;
+  void*      h_stage[kStages]   = {};
+  size_t     stage_cap[kStages] = {};
+  hipEvent_t stage_ev[kStages]  = {};
+  int        stage_next         = 0;
+  std::shared_ptr<ShockAhead> ahead; // draws of the next shock_inject, or null
 };
 
 struct Sim;
