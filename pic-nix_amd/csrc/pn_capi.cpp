@@ -1098,8 +1098,7 @@ bool carry_rho(const Sim& s) { return s.rho_valid && !s.rho_deposit; }
 // (periodic problems whose gather takes the row form)
 bool defers_sort(const Sim& s)
 {
-  // the FP32 wall step's rho deposit reads the sorted records in place
-  return s.defer_sort && !(s.g.xwall && s.fp32) && gather_takes_perm(s);
+  return s.defer_sort && gather_takes_perm(s);
 }
 
 void step_launches32(Sim& s, double delt)
@@ -1153,7 +1152,7 @@ void step_launches_wall(Sim& s, double delt)
   // the rho deposit below reads the particles through a pending sort, so the
   // record move is left to the next row gather
   if (s.fp32)
-    launch32_migrate(s);
+    launch32_migrate(s, defers_sort(s));
   else
     launch_migrate(s, defers_sort(s));
   if (rho == RhoDone::Deferred) {

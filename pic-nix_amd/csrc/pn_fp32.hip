@@ -1403,7 +1403,8 @@ __global__ void __launch_bounds__(256)
 template <int Dim, int Order>
 __global__ void __launch_bounds__(256)
     k32_rho_global(Geom g, P32 p, const int32_t* __restrict__ np, const int* __restrict__ off,
-                   float* __restrict__ J)
+                   float* __restrict__ J, const int32_t* __restrict__ perm,
+                   const int* __restrict__ nb)
 {
   constexpr int S   = Order + 1;
   const int     seg = blockIdx.y;
@@ -1413,11 +1414,27 @@ __global__ void __launch_bounds__(256)
   const int  c   = seg / g.Ns;
   const int  sp  = seg - c * g.Ns;
   const long i   = (long)seg * g.cap + ip;
-  const int  bin = p.cell[i];
+  int        bin;
+  float      r[3];
+  if (perm) {
+    // a pending sort: sorted slot i holds record perm[i], whose bin is found
+    // again and whose offset is folded into [0, 1), as k32_apply_perm does
+    const long  j = perm[i];
+    const float x = p.rx[j], y = p.ry[j], z = p.rz[j];
+    int         ds, k;
+    dest32(g, (int)(j / g.cap), p.cell[j], x, y, z, nb, ds, bin, k);
+    r[0] = g.has[2] ? frac01(x) : x;
+    r[1] = g.has[1] ? frac01(y) : y;
+    r[2] = g.has[0] ? frac01(z) : z;
+  } else {
+    bin  = p.cell[i];
+    r[0] = p.rx[i];
+    r[1] = p.ry[i];
+    r[2] = p.rz[i];
+  }
   if (bin < 0)
     return;
   const Cell3 cl   = cell_of(g, bin);
-  const float r[3] = {p.rx[i], p.ry[i], p.rz[i]};
   const int   ci[3] = {cl.x, cl.y, cl.z};
   float       w[3][S];
   int         o[3];
@@ -1467,7 +1484,8 @@ struct Rho32 {
 template <int Order>
 __global__ void __launch_bounds__(256)
     k32_rho_march(Geom g, P32 p, const int32_t* __restrict__ cellstart,
-                  const int* __restrict__ off, float* __restrict__ J, int zrows)
+                  const int* __restrict__ off, float* __restrict__ J, int zrows,
+                  const int32_t* __restrict__ perm)
 {
   using RP            = Rho32<Order>;
   constexpr int S     = RP::S;
@@ -1546,12 +1564,21 @@ __global__ void __launch_bounds__(256)
         return true;
     }
   };
+  // a pending sort: sorted slot i holds record perm[i], whose offset (still
+  // relative to its old cell) is folded into [0, 1) of the slot's bin
   auto fetch = [&](int seg, int pb, int p1, float* x3) {
     if (pb + lane < p1) {
       const long i = (long)seg * g.cap + pb + lane;
-      x3[0]        = p.rx[i];
-      x3[1]        = p.ry[i];
-      x3[2]        = p.rz[i];
+      if (perm) {
+        const long j = perm[i];
+        x3[0]        = g.has[2] ? frac01(p.rx[j]) : p.rx[j];
+        x3[1]        = g.has[1] ? frac01(p.ry[j]) : p.ry[j];
+        x3[2]        = g.has[0] ? frac01(p.rz[j]) : p.rz[j];
+      } else {
+        x3[0] = p.rx[i];
+        x3[1] = p.ry[i];
+        x3[2] = p.rz[i];
+      }
     }
   };
 
@@ -2068,10 +2095,10 @@ RhoDone launch32_deposit(Sim& s, double delt, RhoMode mode)
 
 void launch32_rho(Sim& s)
 {
-  materialize(s);
-  PhaseTimer  pt(s, 7);
-  PhaseTimer  pr(s, kPhaseRhoDeposit);
-  const Geom& g = s.g;
+  PhaseTimer     pt(s, 7);
+  PhaseTimer     pr(s, kPhaseRhoDeposit);
+  const Geom&    g    = s.g;
+  const int32_t* perm = s.perm_pending ? s.ss.perm : nullptr; // read through by the kernels
   if (g.dim == 3 && s.sorted && !s.generic_kernels && (g.order == 2 || g.order == 4)) {
     const int    S    = g.order + 1;
     const size_t mlds = sizeof(double) * (size_t)S * S * (g.bsy + g.order) +
@@ -2083,17 +2110,17 @@ void launch32_rho(Sim& s)
       const dim3 mgrid(nzr * (g.bsz / g.bsy), g.nchunk);
       if (g.order == 2)
         hipLaunchKernelGGL(k32_rho_march<2>, mgrid, dim3(256), mlds, s.stream, g, s.qa,
-                           s.ss.cellstart, s.d_off, s.f_J, zrows);
+                           s.ss.cellstart, s.d_off, s.f_J, zrows, perm);
       else
         hipLaunchKernelGGL(k32_rho_march<4>, mgrid, dim3(256), mlds, s.stream, g, s.qa,
-                           s.ss.cellstart, s.d_off, s.f_J, zrows);
+                           s.ss.cellstart, s.d_off, s.f_J, zrows, perm);
       return;
     }
   }
   const dim3 grid(nblk(g.cap, 256), g.nseg);
 #define PN32_RHO(D, O)                                                                        \
   hipLaunchKernelGGL((k32_rho_global<D, O>), grid, dim3(256), 0, s.stream, g, s.qa, s.d_np,    \
-                     s.d_off, s.f_J)
+                     s.d_off, s.f_J, perm, s.d_nb)
   if (g.dim == 3) {
     switch (g.order) {
     case 1: PN32_RHO(3, 1); break;

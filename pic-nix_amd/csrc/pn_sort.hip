@@ -120,21 +120,40 @@ __global__ void __launch_bounds__(256)
               int32_t* __restrict__ slot, int32_t* __restrict__ dseg,
               int32_t* __restrict__ didx, int ntile)
 {
-  __shared__ int dirs[256];
+  __shared__ int wcnt[4][27]; // leavers per direction in each wavefront of the tile
   const int  seg    = blockIdx.y;
   const long ip     = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = ip < np[seg];
   const long i      = (long)seg * g.cap + ip;
   int        k      = active ? key[i] : 0;
   int        d      = (active && k < 0) ? (-1 - k) : -1;
-  dirs[threadIdx.x] = d;
+  const int  lane   = (int)(threadIdx.x & 63);
+  const int  wv     = (int)(threadIdx.x >> 6);
+  for (int q = threadIdx.x; q < 4 * 27; q += blockDim.x)
+    (&wcnt[0][0])[q] = 0;
+  __syncthreads();
+  // a leaver's rank in the tile counts the earlier threads with its direction:
+  // the earlier lanes of its wavefront (one ballot per direction present in the
+  // wavefront), then the earlier wavefronts' totals
+  int                rank_w  = 0;
+  unsigned long long pending = __ballot(d >= 0);
+  while (pending) {
+    const int                ld = __builtin_ctzll(pending);
+    const int                dd = __shfl(d, ld);
+    const unsigned long long m  = __ballot(d == dd);
+    if (d == dd)
+      rank_w = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == ld)
+      wcnt[wv][dd] = __popcll(m);
+    pending &= ~m;
+  }
   __syncthreads();
   // arrivals: destination segment, pre-sort index, wrapped position and bin
   int  sR = -1, b = -1, idx = 0;
   if (d >= 0) {
-    int rank = tilecnt[((long)seg * ntile + blockIdx.x) * 27 + d];
-    for (int t = 0; t < (int)threadIdx.x; t++)
-      rank += (dirs[t] == d);
+    int rank = tilecnt[((long)seg * ntile + blockIdx.x) * 27 + d] + rank_w;
+    for (int w = 0; w < wv; w++)
+      rank += wcnt[w][d];
     const int c  = seg / g.Ns;
     const int sp = seg - c * g.Ns;
     const int R  = nb[c * 27 + d];
