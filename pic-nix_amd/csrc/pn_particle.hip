@@ -204,6 +204,8 @@ struct Pos64 {
   int32_t*      cellcnt; // bin counters, nbin per segment
   int32_t*      key;
   int32_t*      slot;
+  int32_t*      tilecnt = nullptr; // leavers per (256-particle tile, direction): fused gather
+  int           ntile   = 0;
 };
 
 // The position push of particle i of chunk c with momentum (ux, uy, uz): writes
@@ -266,7 +268,7 @@ __device__ __forceinline__ int position64(const Geom& g, const PArr& p, const Po
 // Fuse: each particle's position push follows its velocity push here (the
 // step's next operator), so its position and momentum are read once for both;
 // the bin counters then count it and key holds its bin or exchange direction,
-// as after k_push_position (whose per-tile leaver counts k_leaver_tiles adds).
+// as after k_push_position, and the per-tile leaver counts are added here too.
 // perm (a sort left pending by pn_step): sorted slot i holds record perm[i] of
 // p; the kernel reads it there and writes the whole record, pushed, to slot i
 // of dst.  Without perm, dst is p.
@@ -505,6 +507,9 @@ __global__ void __launch_bounds__(256)
                                         xu[4], xu[5], dir);
       const bool stay = b >= 0;
       const int  sl   = wave_slot(ps.cellcnt + (long)seg * g.nbin, b, stay);
+      // the leavers of each 256-particle tile per direction, which the migration
+      // ranks by (one atomic per direction and tile present in the wavefront)
+      (void)wave_slot(ps.tilecnt, (seg * ps.ntile + q / 256) * 27 + dir, dir >= 0);
       if (stay) {
         ps.key[i]  = b;
         ps.slot[i] = sl;
@@ -556,7 +561,7 @@ __global__ void __launch_bounds__(256)
   const long i      = (long)seg * g.cap + ip;
   int        b      = -1;
   if (active) {
-    const Pos64 ps{xo, yo, zo, rng, delt, rc, cellcnt, key, slot};
+    const Pos64 ps{xo, yo, zo, rng, delt, rc, cellcnt, key, slot, nullptr, 0};
     int         d;
     b = position64<false>(g, p, ps, lim, c, i, p.x[i], p.y[i], p.z[i], p.ux[i], p.uy[i],
                           p.uz[i], d);
@@ -568,29 +573,6 @@ __global__ void __launch_bounds__(256)
   if (stay) {
     slot[i] = sl;
     key[i]  = b;
-  }
-  __syncthreads();
-  if (threadIdx.x < 27)
-    tilecnt[((long)seg * ntile + blockIdx.x) * 27 + threadIdx.x] = cnt[threadIdx.x];
-}
-
-// Per-tile leaver counts by exchange direction (tile = 256 consecutive
-// particles of a segment), as k_push_position writes them, for a position push
-// that ran inside the row gather: from the keys it left.
-__global__ void __launch_bounds__(256)
-    k_leaver_tiles(Geom g, const int32_t* __restrict__ np, const int32_t* __restrict__ key,
-                   int32_t* __restrict__ tilecnt, int ntile)
-{
-  __shared__ int cnt[27];
-  if (threadIdx.x < 27)
-    cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int  seg = blockIdx.y;
-  const long ip  = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip < np[seg]) {
-    const int k = key[(long)seg * g.cap + ip];
-    if (k < 0 && k >= -27)
-      atomicAdd(&cnt[-1 - k], 1);
   }
   __syncthreads();
   if (threadIdx.x < 27)
@@ -724,9 +706,11 @@ bool launch_push_velocity(Sim& s, double delt, bool with_position)
   const int    vgy    = g.nchunk * nsplit, vitems = nrow * vgy;
   int*         vwq    = s.d_wq + 2 * kWqGather;
   const Pos64  ps{s.d_xo[0], s.d_xo[1], s.d_xo[2], s.d_rng,     delt,
-                 1 / g.cc,  s.ss.cellcnt, s.ss.key, s.ss.slot};
-  if (fuse)
+                 1 / g.cc,  s.ss.cellcnt, s.ss.key, s.ss.slot, s.ss.tilecnt, s.ss.ntile};
+  if (fuse) {
     launch_zero(s, s.ss.cellcnt, (long)g.nseg * g.nbin);
+    launch_zero(s, s.ss.tilecnt, (long)g.nseg * s.ss.ntile * 27);
+  }
 #define PN_VEL_ROWS(D, O, PL, FU)                                                             \
   do {                                                                                        \
     if (s.vel_queue)                                                                          \
@@ -785,8 +769,6 @@ bool launch_push_velocity(Sim& s, double delt, bool with_position)
     s.perm_pending = false;
   }
   if (fuse) {
-    hipLaunchKernelGGL(k_leaver_tiles, dim3(s.ss.ntile, g.nseg), dim3(256), 0, s.stream, g,
-                       s.d_np, s.ss.key, s.ss.tilecnt, s.ss.ntile);
     s.counted = true;
     s.sorted  = false;
   }
