@@ -702,8 +702,10 @@ __global__ void __launch_bounds__(256)
           for (int l = 0; l < NS / 2; l++)
             pr[(N + l) * PITCH] = make_double2(pre[2 * l], pre[2 * l + 1]);
         }
-        {
-          // per particle: all three axes at the bin's origin?  and its own origin
+        // Odd orders only: per particle, are all three axes at the bin's origin, and
+        // its own origin.  For even orders the bin (the sort's digitize of the same
+        // position with the same offsets) is always the stencil's origin.
+        if constexpr (is_odd) {
           const unsigned long long m    = __ballot(on1 && same);
           const int                orgy = __shfl(org, (lane + BATCH) & 63);
           const int                orgz = __shfl(org, (lane + 2 * BATCH) & 63);
@@ -731,7 +733,7 @@ __global__ void __launch_bounds__(256)
               P[2 * j]         = d2.x;
               P[2 * j + 1]     = d2.y;
             }
-            const int4 fo = FO[kk];
+            const int4 fo = is_odd ? FO[kk] : make_int4(1, 0, 0, 0);
             if (fo.x != 0) {
 #pragma unroll
               for (int r = 0; r < N; r++) {
