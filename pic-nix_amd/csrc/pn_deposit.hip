@@ -724,6 +724,10 @@ __global__ void __launch_bounds__(256)
           if (act && kk < nbt) {
             const double2* pp  = W + kk;
             const double2  pf  = pp[(axF * ROWS + t) * PITCH]; // (s0F[t], dsF[t])
+            double2        psr[N]; // (s0S[r], dsS[r]), every row read before use
+#pragma unroll
+            for (int r = 0; r < N; r++)
+              psr[r] = pp[(axS * ROWS + r) * PITCH];
             const double   aF  = fma(A, pf.y, pf.x);                 // 1 s0F + A dsF
             const double   bF  = fma(B, pf.y, A * pf.x);             // A s0F + B dsF
             double         P[NS];
@@ -737,7 +741,7 @@ __global__ void __launch_bounds__(256)
             if (fo.x != 0) {
 #pragma unroll
               for (int r = 0; r < N; r++) {
-                const double2 ps = pp[(axS * ROWS + r) * PITCH]; // (s0S[r], dsS[r])
+                const double2 ps = psr[r];
                 const double  w  = -fma(aF, ps.x, bF * ps.y) * qd;
 #pragma unroll
                 for (int j = 0; j < N - 1; j++)
@@ -1027,7 +1031,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int jx = 0; jx < S; jx++)
               sx[jx] = rec[jx];
-            if (rec[RP::FLAG] != 0.0) {
+            // even orders: the sort's bin is always the stencil's origin
+            if (!is_odd || rec[RP::FLAG] != 0.0) {
 #pragma unroll
               for (int jy = 0; jy < S; jy++) {
                 const double sy = rec[S + jy];
