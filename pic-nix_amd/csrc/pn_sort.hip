@@ -20,51 +20,50 @@ namespace {
 inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 // Exclusive prefix over tiles of the per-direction leaver counts of each
-// segment: one block per segment, each thread owns a contiguous run of tiles.
+// segment: one block per segment.  The counts ([tile][27]) are read and written in
+// order, 9 tiles x 27 directions per pass (one element per thread, so every access
+// is coalesced); each pass scans its 9 tiles per direction in LDS and carries the
+// running sums to the next.
 __global__ void __launch_bounds__(256)
     k_tile_scan(Geom g, const int32_t* __restrict__ np, int32_t* __restrict__ tilecnt,
                 int32_t* __restrict__ nout, int ntile)
 {
-  __shared__ int part[27][257];
-  const int seg = blockIdx.x;
-  const int nt  = (np[seg] + 255) / 256;
-  const int per = (nt + 255) / 256;
-  const int t0  = threadIdx.x * per;
-  const int t1  = min(t0 + per, nt);
-  int32_t*  tc  = tilecnt + (long)seg * ntile * 27;
-  int       loc[27];
-  for (int d = 0; d < 27; d++)
-    loc[d] = 0;
-  for (int t = t0; t < t1; t++)
-    for (int d = 0; d < 27; d++)
-      loc[d] += tc[(long)t * 27 + d];
-  for (int d = 0; d < 27; d++)
-    part[d][threadIdx.x + 1] = loc[d];
-  if (threadIdx.x == 0)
-    for (int d = 0; d < 27; d++)
-      part[d][0] = 0;
-  __syncthreads();
-  // inclusive scan of part[d][1..256] (Hillis-Steele), 27 independent lines
-  for (int off = 1; off < 256; off <<= 1) {
-    int v[27];
-    for (int d = 0; d < 27; d++)
-      v[d] = (int)threadIdx.x >= off ? part[d][threadIdx.x + 1 - off] : 0;
-    __syncthreads();
-    for (int d = 0; d < 27; d++)
-      part[d][threadIdx.x + 1] += v[d];
-    __syncthreads();
-  }
-  for (int d = 0; d < 27; d++) {
-    int run = part[d][threadIdx.x];
-    for (int t = t0; t < t1; t++) {
-      long a = (long)t * 27 + d;
-      int  v = tc[a];
-      tc[a]  = run;
-      run += v;
-    }
-  }
+  constexpr int TPP = 9; // tiles per pass: 9 x 27 = 243 threads
+  __shared__ int v[TPP][27];
+  __shared__ int carry[27];
+  const int  seg = blockIdx.x;
+  const int  nt  = (np[seg] + 255) / 256;
+  int32_t*   tc  = tilecnt + (long)seg * ntile * 27;
+  const bool on  = threadIdx.x < TPP * 27;
+  const int  s   = (int)threadIdx.x / 27;
+  const int  d   = (int)threadIdx.x - s * 27;
   if (threadIdx.x < 27)
-    nout[seg * 27 + threadIdx.x] = part[threadIdx.x][256];
+    carry[threadIdx.x] = 0;
+  __syncthreads();
+  for (int base = 0; base < nt; base += TPP) {
+    const int  t   = base + s;
+    const bool in  = on && t < nt;
+    const int  val = in ? tc[(long)t * 27 + d] : 0;
+    if (on)
+      v[s][d] = val;
+    __syncthreads();
+    if (in) {
+      int pre = carry[d];
+      for (int q = 0; q < s; q++)
+        pre += v[q][d];
+      tc[(long)t * 27 + d] = pre;
+    }
+    __syncthreads();
+    if (threadIdx.x < 27) {
+      int sum = 0;
+      for (int q = 0; q < TPP; q++)
+        sum += v[q][threadIdx.x];
+      carry[threadIdx.x] += sum;
+    }
+    __syncthreads(); // carry is read, and v rewritten, by the next pass
+  }
+  if (threadIdx.x < 27) // carry[d] is only ever written by thread d
+    nout[seg * 27 + threadIdx.x] = carry[threadIdx.x];
 }
 
 // Receive offsets: slot order (iz, iy, ix) lexicographic, skipping the centre;
