@@ -1188,6 +1188,10 @@ __global__ void __launch_bounds__(256)
           if (act && kk < nbt) {
             const float2* pp = W + kk;
             const float2  pf = pp[(axF * ROWS + t) * PITCH];
+            float2        psr[N]; // (s0S[r], dsS[r]), every row read before use
+#pragma unroll
+            for (int r = 0; r < N; r++)
+              psr[r] = pp[(axS * ROWS + r) * PITCH];
             const float   aF = fmaf(A, pf.y, pf.x);
             const float   bF = fmaf(B, pf.y, A * pf.x);
             float         P[NS];
@@ -1199,7 +1203,7 @@ __global__ void __launch_bounds__(256)
             }
 #pragma unroll
             for (int r = 0; r < N; r++) {
-              const float2 ps = pp[(axS * ROWS + r) * PITCH];
+              const float2 ps = psr[r];
               const float  w  = -fmaf(aF, ps.x, bF * ps.y) * qd;
 #pragma unroll
               for (int j = 0; j < N - 1; j++)
