@@ -88,6 +88,44 @@ def test_restart_continues_the_run(kind, tmp_path):
     b.close()
 
 
+def test_restart_shock_draws_made_ahead(tmp_path):
+    """The shock's injection draws for the next step are made ahead of it on a host
+    thread, and a step uses them only when the chunks' random_seed and lastid are
+    the ones they were drawn for.  A run restarted from a checkpoint in a fresh
+    handle (no draws ahead) and one loaded back into a handle that had run on (its
+    draws ahead are for a later step) both continue exactly as the uninterrupted
+    run, whose draws were made ahead: the same particles in the same order, the
+    same fields and boundary states after 5 more steps."""
+    cfg = helpers.small_shock(N=(8, 8, 32), C=(2, 2, 2), nppc=2, order=2, emf_order=1)
+    dt = float(cfg["parameter"]["delt"])
+    a = _sim(cfg)
+    for _ in range(4):
+        a.step(dt)
+    a.synchronize()
+    prefix = tmp_path / "state"
+    a.save_state(prefix, 4, 4 * dt)
+    b = _sim(cfg, load=False)
+    b.load_state(prefix)
+    c = _sim(cfg)
+    for _ in range(6):
+        c.step(dt)
+    c.synchronize()
+    c.load_state(prefix)
+    for _ in range(5):
+        for sim in (a, b, c):
+            sim.step(dt)
+    for sim in (a, b, c):
+        sim.synchronize()
+    for name, other in (("fresh handle", b), ("handle loaded back", c)):
+        wp, wf = _compare(a, other, 3)
+        _say(f"[ckpt] shock3d_o2 restarted in a {name}: particles {wp:.2e} fields {wf:.2e}")
+        assert wp <= 1e-12 and wf <= 1e-12
+        for k in range(a.nchunks):
+            assert other.boundary_state(k) == a.boundary_state(k), (name, k)
+    for sim in (a, b, c):
+        sim.close()
+
+
 def test_restart_refuses_other_configuration(tmp_path):
     import picnix_amd
     cfg = helpers.small_thermal(N=(16, 16, 16), C=(2, 2, 2), np_=8, order=2)
