@@ -375,8 +375,24 @@ __device__ __forceinline__ void rank_bin_pair(const Geom& g, const PArr& src, co
     const bool inA0 = q0 < nA, inA1 = q1 < nA;
     int        r0 = 0, r1 = 0;
     if (c0 + 64 < n) {
-      range_ranks<true>(ck, 0, nA, inA0 ? m0 : 0u, inA1 ? m1 : 0u, r0, r1);
-      range_ranks<true>(ck, nA, n, inA0 ? 0u : m0, inA1 ? 0u : m1, r0, r1);
+      // two members per lane: each range is compared only with the members that
+      // can lie in it (wave-uniform tests), e.g. for two bins of 64 the first
+      // members are all in bin b and the second all in bin b + 1
+      const bool a0 = c0 < nA, a1 = c0 + 64 < nA;        // some q0 / q1 in bin b
+      const bool b0 = c0 + 63 >= nA, b1 = c0 + 127 >= nA; // some q0 / q1 in bin b + 1
+      int        none = 0;
+      if (a0 && a1)
+        range_ranks<true>(ck, 0, nA, inA0 ? m0 : 0u, inA1 ? m1 : 0u, r0, r1);
+      else if (a0)
+        range_ranks<false>(ck, 0, nA, inA0 ? m0 : 0u, 0u, r0, none);
+      else if (a1)
+        range_ranks<false>(ck, 0, nA, inA1 ? m1 : 0u, 0u, r1, none);
+      if (b0 && b1)
+        range_ranks<true>(ck, nA, n, inA0 ? 0u : m0, inA1 ? 0u : m1, r0, r1);
+      else if (b0)
+        range_ranks<false>(ck, nA, n, inA0 ? 0u : m0, 0u, r0, none);
+      else if (b1)
+        range_ranks<false>(ck, nA, n, inA1 ? 0u : m1, 0u, r1, none);
     } else {
       range_ranks<false>(ck, 0, nA, inA0 ? m0 : 0u, 0u, r0, r1);
       range_ranks<false>(ck, nA, n, inA0 ? 0u : m0, 0u, r0, r1);
