@@ -992,8 +992,10 @@ __global__ void __launch_bounds__(256)
   constexpr int PITCH = MP::PITCH;
   constexpr int NWAVE = MP::NWAVE;
   static_assert(Order % 2 == 0, "the FP32 march kernel takes even orders");
+  static_assert(PITCH % 2 == 0 && (MP::WPAIR * sizeof(float2)) % 16 == 0,
+                "phase 2 reads the weights of particle pairs as 16-byte values");
 
-  __shared__ float2 wts[NWAVE][MP::WPAIR];
+  __shared__ __align__(16) float2 wts[NWAVE][MP::WPAIR];
   __shared__ int    next_cell;
   // kTileComp: hi [3][N z-ring][N][TX] and lo (same) as floats; kTileF64: the
   // same nodes as doubles; kTilePlain: hi only.  Then rcs.
@@ -1084,6 +1086,10 @@ __global__ void __launch_bounds__(256)
   const float* R0   = a1 == 0 ? ro0 : (a1 == 1 ? ro1 : ro2);
   const float* R1   = a1 == 0 ? p.rx : (a1 == 1 ? p.ry : p.rz);
   float2*      W    = &wts[wv][0];
+  // phase 2 reads the weights of two neighbouring particles at once, the second
+  // possibly past the batch: the slots start as zeros, so such a read is finite
+  for (int q = lane; q < MP::WPAIR; q += 64)
+    W[q] = make_float2(0.0f, 0.0f);
 
   // the next cell of [next_cell, cend) that holds particles of any species
   // (uniform per wave); false when the range is done
@@ -1183,31 +1189,41 @@ __global__ void __launch_bounds__(256)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int k0 = 0; k0 < nbt; k0 += PPS) {
-          const int kk = k0 + k;
+        // each lane takes two neighbouring particles kk, kk + 1 per pass, whose
+        // weights one 16-byte read returns (the second counts only inside the
+        // batch: its weights are zeroed otherwise)
+        for (int k0 = 0; k0 < nbt; k0 += 2 * PPS) {
+          const int kk = k0 + 2 * k;
           if (act && kk < nbt) {
-            const float2* pp = W + kk;
-            const float2  pf = pp[(axF * ROWS + t) * PITCH];
-            float2        psr[N]; // (s0S[r], dsS[r]), every row read before use
+            const float4* pp = reinterpret_cast<const float4*>(W + kk);
+            constexpr int P4 = PITCH / 2; // the pitch in float4
+            const float4  pf = pp[(axF * ROWS + t) * P4];
+            float4        psr[N]; // (s0S[r], dsS[r]) of both, every row read before use
 #pragma unroll
             for (int r = 0; r < N; r++)
-              psr[r] = pp[(axS * ROWS + r) * PITCH];
-            const float   aF = fmaf(A, pf.y, pf.x);
-            const float   bF = fmaf(B, pf.y, A * pf.x);
-            float         P[NS];
+              psr[r] = pp[(axS * ROWS + r) * P4];
+            const bool  two = kk + 1 < nbt;
+            const float aF0 = fmaf(A, pf.y, pf.x);
+            const float bF0 = fmaf(B, pf.y, A * pf.x);
+            const float aF1 = two ? fmaf(A, pf.w, pf.z) : 0.0f;
+            const float bF1 = two ? fmaf(B, pf.w, A * pf.z) : 0.0f;
+            float       P0[NS], P1[NS];
 #pragma unroll
             for (int j = 0; j < NS / 2; j++) {
-              const float2 d2 = pp[(axC * ROWS + N + j) * PITCH];
-              P[2 * j]        = d2.x;
-              P[2 * j + 1]    = d2.y;
+              const float4 d4 = pp[(axC * ROWS + N + j) * P4];
+              P0[2 * j]       = d4.x;
+              P0[2 * j + 1]   = d4.y;
+              P1[2 * j]       = d4.z;
+              P1[2 * j + 1]   = d4.w;
             }
 #pragma unroll
             for (int r = 0; r < N; r++) {
-              const float2 ps = psr[r];
-              const float  w  = -fmaf(aF, ps.x, bF * ps.y) * qd;
+              const float4 ps = psr[r];
+              const float  w0 = -fmaf(aF0, ps.x, bF0 * ps.y) * qd;
+              const float  w1 = -fmaf(aF1, ps.z, bF1 * ps.w) * qd;
 #pragma unroll
               for (int j = 0; j < N - 1; j++)
-                acc[r][j] = fmaf(P[j], w, acc[r][j]);
+                acc[r][j] = fmaf(P1[j], w1, fmaf(P0[j], w0, acc[r][j]));
             }
           }
         }
